@@ -1,0 +1,241 @@
+"""MI355X-native DNS + droplet time step (HIP kernels behind a C ABI).
+
+The reference's per-timestep hot path -- the pseudo-spectral Navier-Stokes step of
+turb.f90 / mpifft.f90 / flowlib.f90 and the droplet step of partlib.f90 -- runs
+on one GPU in ``libddns.so``.  This module is the Python host mirror of that C ABI
+(include/ddns.h); the Fortran host driver in ``fortran/`` is the other one.
+
+Arrays cross the boundary in the reference's own layouts:
+  * spectral fields v(N+2, N+2, N), Fortran order: kz, kx re/im pairs, ky;
+  * droplets yp(8, n), fvp(3, 0:2, n), fdp(3, 0:3, n), pertvel(3, n).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+from . import _abi
+from ._abi import (  # noqa: F401  (re-exported)
+    DDNS_ERR_ARG, DDNS_ERR_CAPACITY, DDNS_ERR_DOMAIN, DDNS_ERR_DROPLET_LOSS, DDNS_ERR_HIP,
+    DDNS_ERR_IO, DDNS_ERR_NEIGHBOUR_OVERFLOW, DDNS_ERR_NO_DEVICE, DDNS_ERR_OVERLAP, DDNS_OK,
+    DDNS_WARN_GMRES, FP32_MIXED, FP64, MAXSET, NBIN, RADII_LOGNORMAL, RADII_SETS,
+    SPEC_K4_GAUSS, SPEC_KOLMOGOROV, CollisionCounts, Params, Timing, default_params,
+)
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libddns.so"
+
+_lib = None
+
+
+class DDNSError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"ddns error {code}: {msg}")
+        self.code = code
+
+
+def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
+    """Load libddns.so (built in-tree by build.py).  There is no fallback: without
+    the HIP library nothing in this package can run."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise ImportError(f"{p} is missing: build it with `python -m mpi_2d_domain_decomp_turb_particles_amd.build`")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in _abi.SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _dp(a):
+    if a is None:
+        return None
+    if a.dtype != np.float64 or not (a.flags["F_CONTIGUOUS"] or a.flags["C_CONTIGUOUS"]):
+        raise TypeError("expected a contiguous float64 array")
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def spectral_zeros(N: int) -> np.ndarray:
+    """A spectral array in the reference layout v(N+2, N+2, N) (Fortran order)."""
+    return np.zeros((N + 2, N + 2, N), dtype=np.float64, order="F")
+
+
+def read_parameter_file(path: str | os.PathLike) -> Params:
+    """Positional parameter.d reader (turb.f90:101-166)."""
+    lib = load_library()
+    p = Params()
+    st = lib.ddns_read_parameter_file(str(path).encode(), C.byref(p))
+    if st != 0:
+        raise DDNSError(st, lib.ddns_last_error(None).decode())
+    return p
+
+
+class Solver:
+    """One periodic domain resident on the current GPU."""
+
+    def __init__(self, params: Params):
+        self.lib = load_library()
+        self.p = params
+        self.N = params.nx
+        h = C.c_void_p()
+        st = self.lib.ddns_create(C.byref(params), C.byref(h))
+        if st != 0:
+            raise DDNSError(st, self.lib.ddns_last_error(None).decode())
+        self.h = h
+        self.nek = self.lib.ddns_nek(h)
+
+    # -- lifetime
+    def close(self):
+        if self.h:
+            self.lib.ddns_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st: int) -> int:
+        if st < 0:
+            raise DDNSError(st, self.lib.ddns_last_error(self.h).decode())
+        return st
+
+    # -- flow
+    def set_flow(self, v, o=None):
+        v = [np.asfortranarray(x, dtype=np.float64) for x in v]
+        o = [np.asfortranarray(x, dtype=np.float64) for x in o] if o is not None else [None] * 3
+        self._check(self.lib.ddns_set_flow(self.h, *[_dp(x) for x in v], *[_dp(x) for x in o]))
+
+    def get_flow(self):
+        arrs = [spectral_zeros(self.N) for _ in range(6)]
+        self._check(self.lib.ddns_get_flow(self.h, *[_dp(a) for a in arrs]))
+        return arrs[:3], arrs[3:]
+
+    def init_flow(self, kind: int = SPEC_KOLMOGOROV, seed: int = 123, k0: float = 4.0, urms: float = 1.0):
+        self._check(self.lib.ddns_init_flow(self.h, kind, seed, k0, urms))
+
+    def set_forcing_state(self, b, iseedf: int, iyf: int, ivf):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        ivf = np.ascontiguousarray(ivf, dtype=np.int32)
+        self._check(self.lib.ddns_set_forcing_state(self.h, _dp(b), iseedf, iyf, _ip(ivf)))
+
+    def get_forcing_state(self):
+        b = np.zeros(450)
+        s, y = C.c_int(), C.c_int()
+        iv = np.zeros(32, dtype=np.int32)
+        self._check(self.lib.ddns_get_forcing_state(self.h, _dp(b), C.byref(s), C.byref(y), _ip(iv)))
+        return b, s.value, y.value, iv
+
+    def physical_velocity(self):
+        n3 = self.N ** 3
+        u = [np.zeros(n3) for _ in range(3)]
+        self._check(self.lib.ddns_get_physical_velocity(self.h, *[_dp(a) for a in u]))
+        return [a.reshape(self.N, self.N, self.N) for a in u]   # [z][y][x]
+
+    def spectrum(self):
+        ek = np.zeros(self.nek)
+        self._check(self.lib.ddns_get_spectrum(self.h, _dp(ek)))
+        return ek
+
+    # -- droplets
+    def set_droplets(self, yp, fvp=None, fdp=None, pertvel=None):
+        yp = np.asfortranarray(yp, dtype=np.float64)
+        n = yp.shape[1]
+        fvp = None if fvp is None else np.asfortranarray(fvp, dtype=np.float64)
+        fdp = None if fdp is None else np.asfortranarray(fdp, dtype=np.float64)
+        pertvel = None if pertvel is None else np.asfortranarray(pertvel, dtype=np.float64)
+        self._check(self.lib.ddns_set_droplets(self.h, n, _dp(yp), _dp(fvp), _dp(fdp), _dp(pertvel)))
+
+    def get_droplets(self):
+        cap = max(self.p.capacity if self.p.capacity > 0 else self.p.npart, 1)
+        n = C.c_int()
+        yp = np.zeros((8, cap), order="F")
+        fvp = np.zeros((3, 3, cap), order="F")
+        fdp = np.zeros((3, 4, cap), order="F")
+        pv = np.zeros((3, cap), order="F")
+        vp = np.zeros((3, cap), order="F")
+        dm = np.zeros((3, cap), order="F")
+        self._check(self.lib.ddns_get_droplets(self.h, C.byref(n), _dp(yp), _dp(fvp), _dp(fdp), _dp(pv), _dp(vp), _dp(dm)))
+        m = n.value
+        return dict(yp=yp[:, :m], fvp=fvp[:, :, :m], fdp=fdp[:, :, :m], pertvel=pv[:, :m], vp=vp[:, :m], dmove=dm[:, :m])
+
+    def init_droplets(self, seed=290710, radii=RADII_SETS, median_um=25.0, sigma_ln=0.4, rmin_um=5.0, rmax_um=60.0):
+        self._check(self.lib.ddns_init_droplets(self.h, seed, radii, median_um, sigma_ln, rmin_um, rmax_um))
+
+    def overlap_fix(self):
+        self._check(self.lib.ddns_overlap_fix(self.h))
+
+    # -- step
+    def step(self, nsteps: int = 1) -> int:
+        return self._check(self.lib.ddns_step(self.h, nsteps))
+
+    def collision_counts(self) -> CollisionCounts:
+        c = CollisionCounts()
+        self._check(self.lib.ddns_get_collision_counts(self.h, C.byref(c)))
+        return c
+
+    def collision_bins(self):
+        ns = self.p.nset
+        nb = ns * ns * NBIN
+        nir = np.zeros(nb, dtype=np.int32)
+        b = [np.zeros(nb) for _ in range(7)]
+        self._check(self.lib.ddns_get_collision_bins(self.h, _ip(nir), *[_dp(x) for x in b]))
+        shape = (ns, ns, NBIN)
+        out = {"nirbin": nir.reshape(shape, order="F")}
+        for k, a in zip(("wrbin", "wxbin", "wybin", "wzbin", "urbin", "thetabin", "wrtanbin"), b):
+            out[k] = a.reshape(shape, order="F")
+        return out
+
+    def collision_pairs(self):
+        n = C.c_int()
+        self._check(self.lib.ddns_get_collision_pairs(self.h, None, 0, C.byref(n)))
+        if n.value == 0:
+            return np.zeros((0, 3), dtype=np.int64)
+        t = np.zeros((n.value, 3), dtype=np.int64)
+        self._check(self.lib.ddns_get_collision_pairs(self.h, t.ctypes.data_as(C.POINTER(C.c_int64)), n.value, C.byref(n)))
+        return t
+
+    def timing(self) -> Timing:
+        t = Timing()
+        self._check(self.lib.ddns_get_timing(self.h, C.byref(t)))
+        return t
+
+    # -- kernel level
+    def fft_r2c(self, v):
+        v = np.array(v, dtype=np.float64, order="F", copy=True)
+        self._check(self.lib.ddns_fft_r2c(self.h, _dp(v)))
+        return v
+
+    def fft_c2r(self, v):
+        v = np.array(v, dtype=np.float64, order="F", copy=True)
+        self._check(self.lib.ddns_fft_c2r(self.h, _dp(v)))
+        return v
+
+    def collision_detect(self, yp, dmove, fvp0, vnew, vp):
+        a = [np.asfortranarray(x, dtype=np.float64) for x in (yp, dmove, fvp0, vnew, vp)]
+        self._check(self.lib.ddns_collision_detect(self.h, a[0].shape[1], *[_dp(x) for x in a]))
+
+    def profile_flow(self, k: int = 5):
+        ms = np.zeros(7)
+        self._check(self.lib.ddns_profile_flow(self.h, k, _dp(ms), 7))
+        return dict(zip(("y_inv", "x_fused", "y_fwd", "pre", "z_fused", "sym", "step"), ms))

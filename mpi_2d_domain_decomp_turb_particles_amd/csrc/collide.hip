@@ -1,0 +1,539 @@
+// collide.hip -- droplet pair searches: COLLISION_DET (partlib.f90:1166-1846) with
+// its RDF / relative-velocity statistics, and the overlap sweep of OVERLAP_DET
+// (2047-2167).  Compiled with -ffp-contract=off: every double operation of the
+// pair tests is the same IEEE operation, in the same order, as the reference, so
+// in FP64 the detected pairs and their types are bit-identical to it.
+//
+// Search: one lane per droplet scans the 27 neighbouring cells of the coarse grid
+// (cell width >= the largest distance that can matter).  For each candidate the
+// lane decides whether the reference visits this pair from this droplet and with
+// which periodic image, replaying the reference's rules exactly:
+//   - the reference scans cells around ih1 and takes only ih2 > ih1 (1325), where
+//     halo images of other droplets carry indices above every local droplet;
+//   - y/z periodicity comes from images made by HEADLIST within `dist` of a face
+//     or edge, shifted by +-2pi (2212-2349); the double-count filter (1309-1310)
+//     keeps only the images in the upper z halo and in the upper y halo inside the
+//     z range, i.e. image shifts (sy,sz) in {(+1,0),(+1,+1),(0,+1),(-1,+1)};
+//   - x periodicity is a cell wrap with xb = x + sx*2pi (1312-1321, 1330);
+//   - the cell indices, windows and clamps are the reference's (1288-1306, 2407-2429).
+// The droplet that plays ih1 is therefore the one that sees its partner through an
+// upper image, or the lower slot number for a pair without y/z image.
+#include "ddns_internal.h"
+
+namespace ddns {
+
+constexpr int CT = 128;
+
+struct PairGeom {
+    int N;            // reference cell grid ncdx = ncdy = ncdz
+    double h;         // hx = hy = hz = 2 pi
+    int imax;         // scan half-width (d_stat or d_overl cells)
+    int range;        // |iy2|, |iz2| admissible halo range (ny_coll or jmax)
+    int nx_max;       // HEADLIST clamp
+    double dist;      // HEADLIST image distance
+    int G;            // coarse grid
+    double hc;
+};
+
+__device__ __forceinline__ int ref_cell_local(double x, double h, int N)
+{
+    int c = 1 + (int)(x / h * (double)N);             // 1288-1294
+    return c > N ? N : c;
+}
+
+__device__ __forceinline__ int ref_cell_halo(double y, double h, int N, int nmax)
+{
+    const double d = y - 0.0 * h;                      // 2409-2429
+    int c = d > 0 ? 1 + (int)(d / h * (double)N) : -(int)(fabs(d) / h * (double)N);
+    if (c > N + nmax) c = N + nmax;
+    if (c < 1 - nmax) c = 1 - nmax;
+    return c;
+}
+
+// Does the reference evaluate (ih1 = a, ih2 = image of b with shifts (wx,wy,wz))?
+// On success returns the partner coordinates and the reference cell offsets.
+__device__ __forceinline__ bool ref_visit(const PairGeom& g, int ia, int ib, double xa, double ya, double za,
+                                         double xb0, double yb0, double zb0, int wx, int wy, int wz,
+                                         double* xb, double* yb, double* zb, int* di, int* dj, int* dk)
+{
+    const double h = g.h;
+    const bool image = (wy != 0 || wz != 0);
+    if (image) {
+        if (!(wz == 1 || (wz == 0 && wy == 1))) return false;
+        const double d_e = fabs(1.0 * h - yb0), d_w = fabs(0.0 * h - yb0);
+        const double d_s = fabs(0.0 * h - zb0);
+        bool in;
+        if (wy == 1 && wz == 0) in = d_w <= g.dist;                              // direction 5
+        else if (wy == 1 && wz == 1) in = sqrt(d_w * d_w + d_s * d_s) <= g.dist;  // 6
+        else if (wy == 0 && wz == 1) in = d_s <= g.dist;                          // 7
+        else in = sqrt(d_e * d_e + d_s * d_s) <= g.dist;                          // 8
+        if (!in) return false;
+    } else if (ib <= ia) {
+        return false;
+    }
+    const double pi2 = h;
+    const double ys = wy == 0 ? yb0 : (wy > 0 ? yb0 + pi2 : yb0 - pi2);
+    const double zs = wz == 0 ? zb0 : (wz > 0 ? zb0 + pi2 : zb0 - pi2);
+    const int N = g.N;
+    const int ixa = ref_cell_local(xa, h, N), iya = ref_cell_local(ya, h, N), iza = ref_cell_local(za, h, N);
+    int ixb = 1 + (int)(xb0 / h * (double)N);
+    if (ixb > N) ixb = N;
+    const int iyb = ref_cell_halo(ys, h, N, g.nx_max), izb = ref_cell_halo(zs, h, N, g.nx_max);
+    const int j = iyb - iya, k = izb - iza;
+    if (j < -g.imax || j > g.imax || k < -g.imax || k > g.imax) return false;
+    if (iyb < 1 - g.range || iyb > N + g.range || izb < 1 - g.range || izb > N + g.range) return false;
+    if (izb < 1) return false;
+    if (izb <= N && iyb < 1) return false;
+    const int i0 = ixb - ixa;
+    int i, sx;
+    if (i0 >= -g.imax && i0 <= g.imax) { i = i0; sx = 0; }
+    else if (i0 + N >= -g.imax && i0 + N <= g.imax) { i = i0 + N; sx = 1; }
+    else if (i0 - N >= -g.imax && i0 - N <= g.imax) { i = i0 - N; sx = -1; }
+    else return false;
+    if (sx != wx) return false;
+    *xb = xb0 + (double)sx * h;
+    *yb = ys;
+    *zb = zs;
+    *di = i; *dj = j; *dk = k;
+    return true;
+}
+
+__device__ __forceinline__ void findindex(const double* radius, int nset, double r1, double r2, int* i1, int* i2)
+{
+    double diff1 = 1000000., diff2 = 1000000.;
+    int a = 1, b = 1;
+    for (int il = 1; il <= nset; ++il) {                // 1857-1882
+        const double d1 = fabs(radius[il - 1] - r1), d2 = fabs(radius[il - 1] - r2);
+        if (d1 <= diff1) { diff1 = d1; a = il; }
+        if (d2 <= diff2) { diff2 = d2; b = il; }
+    }
+    if (b < a) { const int t = b; b = a; a = t; }
+    *i1 = a; *i2 = b;
+}
+
+struct CollArgs {
+    const double *pos, *vel, *rad, *gid, *fvp0, *dmove, *vp;
+    const int *cell_start, *sorted;
+    int n, cap, nset;
+    double radius[DDNS_MAXSET];
+    double dt, st;
+    int hdi;
+    PairGeom g;
+    // outputs
+    int* nirbin;
+    double* bins;          // wr, wx, wy, wz, ur, theta, wrtan (each nset*nset*NBIN)
+    int* counts;           // [0..2] icc, [8..107] colltype, [108..207] nir, [256] npairs
+    long long* tested;     // [0] evaluated, [1] within 10 rcoll
+    long long* pairs;      // 4 per pair
+    int pair_cap;
+    int* mult;
+};
+
+__global__ void __launch_bounds__(CT) k_collide(CollArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n) return;
+    const int ia = a.sorted[t];
+    const size_t C = (size_t)a.cap;
+    const PairGeom& g = a.g;
+    const double h = g.h;
+    double xa[3], xan[3], va[3], vna[3], ax1[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        xa[c] = a.pos[c * C + ia];
+        ax1[c] = a.dmove[c * C + ia];
+        xan[c] = xa[c] + ax1[c];
+        va[c] = a.fvp0[c * C + ia];
+        vna[c] = a.vel[c * C + ia];
+    }
+    const double ra = a.rad[ia];
+    const double dmove_a = sqrt(ax1[0] * ax1[0] + ax1[1] * ax1[1] + ax1[2] * ax1[2]);
+    int cc[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        int q = (int)(xa[c] / g.hc);
+        cc[c] = q < 0 ? 0 : (q >= g.G ? g.G - 1 : q);
+    }
+    int ncollip1 = 0;
+    long long tested = 0, instat = 0;
+    const int nbin = DDNS_NBIN, ns = a.nset;
+    for (int ox = -1; ox <= 1; ++ox)
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int oz = -1; oz <= 1; ++oz) {
+                int qx = cc[0] + ox, qy = cc[1] + oy, qz = cc[2] + oz;
+                const int wx = qx < 0 ? -1 : (qx >= g.G ? 1 : 0);
+                const int wy = qy < 0 ? -1 : (qy >= g.G ? 1 : 0);
+                const int wz = qz < 0 ? -1 : (qz >= g.G ? 1 : 0);
+                qx -= wx * g.G; qy -= wy * g.G; qz -= wz * g.G;
+                const int cell = (qx * g.G + qy) * g.G + qz;
+                const int e = a.cell_start[cell + 1];
+                for (int p = a.cell_start[cell]; p < e; ++p) {
+                    const int ib = a.sorted[p];
+                    if (ib == ia) continue;
+                    double xb[3];
+                    int di, dj, dk;
+                    if (!ref_visit(g, ia, ib, xa[0], xa[1], xa[2], a.pos[ib], a.pos[C + ib], a.pos[2 * C + ib],
+                                   wx, wy, wz, &xb[0], &xb[1], &xb[2], &di, &dj, &dk))
+                        continue;
+                    // -------- partlib.f90:1330-1353
+                    double bx1[3], xbn[3], vb[3], xd[3];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        bx1[c] = a.dmove[c * C + ib];
+                        xbn[c] = xb[c] + bx1[c];
+                        vb[c] = a.fvp0[c * C + ib];
+                        xd[c] = xa[c] - xb[c];
+                    }
+                    const double dnij0 = sqrt(xd[0] * xd[0] + xd[1] * xd[1] + xd[2] * xd[2]);
+                    ++tested;
+                    const double rb = a.rad[ib];
+                    const double rcoll = ra + rb;
+                    const double rcolla = a.hdi ? rcoll : rcoll * (1. - a.st / 100.);
+                    const double rcollb = rcoll * (1. + a.st / 100.);
+                    if (dnij0 > rcolla && dnij0 <= 10 * rcoll) {      // 1358-1441
+                        ++instat;
+                        int i1 = 1, i2 = 1;
+                        if (dnij0 > rcoll) {
+                            double xh[3];
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) xh[c] = xd[c] / dnij0;
+                            const double Vr1 = va[0] * xh[0] + va[1] * xh[1] + va[2] * xh[2];
+                            const double Vr2 = vb[0] * xh[0] + vb[1] * xh[1] + vb[2] * xh[2];
+                            const double Wr = Vr1 - Vr2;
+                            int ir = (int)((dnij0 - rcoll) / ((9. / (double)nbin) * rcoll));
+                            if (ir > nbin - 1) ir = nbin - 1;
+                            if (ir < 0) ir = 0;
+                            findindex(a.radius, ns, ra, rb, &i1, &i2);
+                            const int bi = (i1 - 1) + ns * ((i2 - 1) + ns * ir);
+                            const size_t nb = (size_t)ns * ns * nbin;
+                            atomicAdd(a.nirbin + bi, 1);
+                            if (i2 == i1) {
+                                atomicAdd(a.bins + 1 * nb + bi, va[0] - vb[0]);
+                                atomicAdd(a.bins + 2 * nb + bi, va[1] - vb[1]);
+                                atomicAdd(a.bins + 3 * nb + bi, va[2] - vb[2]);
+                            }
+                            if (ra > rb) {
+                                atomicAdd(a.bins + 1 * nb + bi, va[0] - vb[0]);
+                                atomicAdd(a.bins + 2 * nb + bi, va[1] - vb[1]);
+                                atomicAdd(a.bins + 3 * nb + bi, va[2] - vb[2]);
+                            } else if (ra < rb) {
+                                atomicAdd(a.bins + 1 * nb + bi, vb[0] - va[0]);
+                                atomicAdd(a.bins + 2 * nb + bi, vb[1] - va[1]);
+                                atomicAdd(a.bins + 3 * nb + bi, vb[2] - va[2]);
+                            }
+                            atomicAdd(a.bins + 0 * nb + bi, Wr);
+                            const double du0 = a.vp[ia] - a.vp[ib], du1 = a.vp[C + ia] - a.vp[C + ib], du2 = a.vp[2 * C + ia] - a.vp[2 * C + ib];
+                            atomicAdd(a.bins + 4 * nb + bi, du0 * xh[0] + du1 * xh[1] + du2 * xh[2]);
+                            const double aaa = (va[0] - vb[0]) * xh[0] + (va[1] - vb[1]) * xh[1] + (va[2] - vb[2]) * xh[2];
+                            const double bbb = sqrt((va[0] - vb[0]) * (va[0] - vb[0]) + (va[1] - vb[1]) * (va[1] - vb[1]) +
+                                                    (va[2] - vb[2]) * (va[2] - vb[2]));
+                            if (bbb != 0.) {
+                                const double vangle = acos(fabs(aaa / bbb));
+                                atomicAdd(a.bins + 5 * nb + bi, vangle);
+                                const double sb = bbb * sin(vangle);
+                                atomicAdd(a.bins + 6 * nb + bi, sb * sb / dnij0);
+                            }
+                        }
+                        if (dnij0 <= rcollb) {
+                            if (dnij0 <= rcoll) findindex(a.radius, ns, ra, rb, &i1, &i2);
+                            atomicAdd(a.counts + 108 + (i1 - 1) + DDNS_MAXSET * (i2 - 1), 1);
+                        }
+                    }
+                    if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) continue;   // 1445
+                    // -------- coarse check and types I/II/III, 1451-1516
+                    const double dmove_b = sqrt(bx1[0] * bx1[0] + bx1[1] * bx1[1] + bx1[2] * bx1[2]);
+                    const double tmove = rcoll + dmove_a + dmove_b;
+                    if (dnij0 > tmove) continue;
+                    double xe[3];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) xe[c] = xan[c] - xbn[c];
+                    const double dnij1 = sqrt(xe[0] * xe[0] + xe[1] * xe[1] + xe[2] * xe[2]);
+                    if (dnij0 < rcoll && dnij1 > rcoll) continue;
+                    int type = 0;
+                    if (dnij0 > rcoll && dnij1 <= rcoll) {
+                        type = 1;
+                    } else {
+                        double vbn[3];
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) vbn[c] = a.vel[c * C + ib];
+                        const double dt = a.dt;
+                        for (int icheck = 1; icheck <= 19 && type == 0; ++icheck) {
+                            const double tfac = 0.05 * (double)icheck;
+                            const double tfac1 = dt * tfac, tfac2 = tfac * tfac, tfac3 = tfac2 * tfac;
+                            double xt[3];
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) {
+                                const double xta = xa[c] + va[c] * tfac1 + (3. * ax1[c] - (2. * va[c] + vna[c]) * dt) * tfac2 +
+                                                   ((vna[c] + va[c]) * dt - 2. * ax1[c]) * tfac3;
+                                const double xtb = xb[c] + vb[c] * tfac1 + (3. * bx1[c] - (2. * vb[c] + vbn[c]) * dt) * tfac2 +
+                                                   ((vbn[c] + vb[c]) * dt - 2. * bx1[c]) * tfac3;
+                                xt[c] = xta - xtb;
+                            }
+                            const double dnij = sqrt(xt[0] * xt[0] + xt[1] * xt[1] + xt[2] * xt[2]);
+                            if (dnij0 > rcoll && dnij < rcoll) type = 2;
+                            else if (dnij1 <= rcoll && dnij > rcoll) type = 3;
+                        }
+                    }
+                    if (type == 0) continue;
+                    // -------- count and record, 1518-1562
+                    ++ncollip1;
+                    atomicAdd(a.counts + (type - 1), 1);
+                    int i1, i2;
+                    findindex(a.radius, ns, ra, rb, &i1, &i2);
+                    atomicAdd(a.counts + 8 + (i1 - 1) + DDNS_MAXSET * (i2 - 1), 1);
+                    atomicAdd(a.mult + ib, 1);
+                    const int slot = atomicAdd(a.counts + 256, 1);
+                    if (slot < a.pair_cap) {
+                        const long long ga = (long long)a.gid[ia], gb = (long long)a.gid[ib];
+                        const long long lo = ga < gb ? ga : gb, hi = ga < gb ? gb : ga;
+                        a.pairs[4 * (size_t)slot + 0] = (lo << 32) | (hi << 2) | type;
+                        a.pairs[4 * (size_t)slot + 1] = ia;
+                        a.pairs[4 * (size_t)slot + 2] = ib;
+                        a.pairs[4 * (size_t)slot + 3] = ((long long)(di + 8) << 16) | ((dj + 8) << 8) | (dk + 8);
+                    }
+                }
+            }
+    if (ncollip1 != 0) {                                              // 1578-1583
+        atomicAdd(a.mult + ia, 1);
+        atomicAdd(a.counts + 257, 1);
+    }
+    if (tested) atomicAdd((unsigned long long*)a.tested, (unsigned long long)tested);
+    if (instat) atomicAdd((unsigned long long*)(a.tested + 1), (unsigned long long)instat);
+}
+
+// OVERLAP_DET sweep (2076-2157): count, per droplet as ih1, the overlapping pairs
+// the reference finds; each one costs three relocation draws.
+struct OverArgs {
+    const double *pos, *rad;
+    const int *cell_start, *sorted;
+    int n, cap;
+    PairGeom g;
+    int* cnt;
+    int* total;
+};
+
+__global__ void __launch_bounds__(CT) k_overlap(OverArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n) return;
+    const int ia = a.sorted[t];
+    const size_t C = (size_t)a.cap;
+    const PairGeom& g = a.g;
+    const double xa = a.pos[ia], ya = a.pos[C + ia], za = a.pos[2 * C + ia];
+    const double ra = a.rad[ia];
+    int cc[3];
+    const double xs[3] = {xa, ya, za};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        int q = (int)(xs[c] / g.hc);
+        cc[c] = q < 0 ? 0 : (q >= g.G ? g.G - 1 : q);
+    }
+    int cnt = 0;
+    for (int ox = -1; ox <= 1; ++ox)
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int oz = -1; oz <= 1; ++oz) {
+                int qx = cc[0] + ox, qy = cc[1] + oy, qz = cc[2] + oz;
+                const int wx = qx < 0 ? -1 : (qx >= g.G ? 1 : 0);
+                const int wy = qy < 0 ? -1 : (qy >= g.G ? 1 : 0);
+                const int wz = qz < 0 ? -1 : (qz >= g.G ? 1 : 0);
+                qx -= wx * g.G; qy -= wy * g.G; qz -= wz * g.G;
+                const int cell = (qx * g.G + qy) * g.G + qz;
+                const int e = a.cell_start[cell + 1];
+                for (int p = a.cell_start[cell]; p < e; ++p) {
+                    const int ib = a.sorted[p];
+                    if (ib == ia) continue;
+                    double xb, yb, zb;
+                    int di, dj, dk;
+                    if (!ref_visit(g, ia, ib, xa, ya, za, a.pos[ib], a.pos[C + ib], a.pos[2 * C + ib], wx, wy, wz,
+                                   &xb, &yb, &zb, &di, &dj, &dk))
+                        continue;
+                    if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) continue;   // 2091
+                    const double d0 = xa - xb, d1 = ya - yb, d2 = za - zb;
+                    const double dnij = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+                    const double rcoll = ra + a.rad[ib];
+                    if (dnij <= rcoll) ++cnt;
+                }
+            }
+    a.cnt[ia] = cnt;
+    if (cnt) atomicAdd(a.total, cnt);
+}
+
+// ------------------------------------------------------------------ host side
+int exclusive_scan(const int* in, int* out, int n, int* tmp, hipStream_t s);
+int relocate_by_mult(PartBuffers& pb, const PartConst& pc, int* total_host, hipStream_t s);
+
+static PairGeom make_geom(const PartBuffers& pb, const PartConst& pc, double dstat, double dist, int range)
+{
+    PairGeom g;
+    g.N = pc.N;
+    g.h = pc.pi2;
+    g.imax = 1 + (int)(dstat * (double)pc.N / pc.pi2);
+    g.range = range;
+    g.nx_max = pc.nx_max;
+    g.dist = dist;
+    g.G = pb.G;
+    g.hc = pb.hc;
+    return g;
+}
+
+int part_collide(PartBuffers& pb, const PartConst& pc, hipStream_t s)
+{
+    const size_t nb = (size_t)pc.nset * pc.nset * DDNS_NBIN;
+    DDNS_HIP(hipMemsetAsync(pb.nirbin, 0, nb * sizeof(int), s));
+    DDNS_HIP(hipMemsetAsync(pb.bins, 0, 7 * nb * sizeof(double), s));
+    DDNS_HIP(hipMemsetAsync(pb.counts, 0, 512 * sizeof(int), s));
+    DDNS_HIP(hipMemsetAsync(pb.tested, 0, 4 * sizeof(long long), s));
+    DDNS_HIP(hipMemsetAsync(pb.mult, 0, (size_t)(pc.n + 1) * sizeof(int), s));
+    if (pc.n == 0) return 0;
+    CollArgs a{};
+    a.pos = pb.pos; a.vel = pb.vel; a.rad = pb.rad; a.gid = pb.gid; a.fvp0 = pb.fvp; a.dmove = pb.dmove; a.vp = pb.vp;
+    a.cell_start = pb.cell_start; a.sorted = pb.sorted;
+    a.n = pc.n; a.cap = pb.cap; a.nset = pc.nset;
+    for (int i = 0; i < DDNS_MAXSET; ++i) a.radius[i] = pc.radius[i];
+    a.dt = pc.dt; a.st = pc.st; a.hdi = pc.hdi;
+    const double dstat = 20.0 * pc.radmax;                                 // 1264
+    a.g = make_geom(pb, pc, dstat, pc.hdi ? pc.d_hdi : pc.d_coll, pc.nx_coll);
+    a.nirbin = pb.nirbin; a.bins = pb.bins; a.counts = pb.counts; a.tested = pb.tested;
+    a.pairs = pb.pairs; a.pair_cap = pb.pair_cap; a.mult = pb.mult;
+    hipLaunchKernelGGL(k_collide, dim3((pc.n + CT - 1) / CT), dim3(CT), 0, s, a);
+    DDNS_HIP(hipGetLastError());
+    return 0;
+}
+
+int part_overlap(PartBuffers& pb, const PartConst& pc, int* sweeps, int* relocated, hipStream_t s)
+{
+    *sweeps = 0;
+    *relocated = 0;
+    if (pc.n == 0) return 0;
+    const double d_overl = 2.0 * pc.radmax;                                // 2063
+    for (int it = 0; it < 10000; ++it) {
+        int st = part_build_cells(pb, pc, std::max(d_overl, pc.pi2 / pc.N), s);
+        if (st) return st;
+        OverArgs a{};
+        a.pos = pb.pos; a.rad = pb.rad; a.cell_start = pb.cell_start; a.sorted = pb.sorted;
+        a.n = pc.n; a.cap = pb.cap;
+        const int jmax = 1 + (int)(d_overl * (double)pc.N / pc.pi2);
+        a.g = make_geom(pb, pc, d_overl, d_overl, jmax);
+        a.cnt = pb.mult;
+        a.total = pb.flags + 1;
+        DDNS_HIP(hipMemsetAsync(pb.flags + 1, 0, sizeof(int), s));
+        hipLaunchKernelGGL(k_overlap, dim3((pc.n + CT - 1) / CT), dim3(CT), 0, s, a);
+        DDNS_HIP(hipGetLastError());
+        int total = 0;
+        st = relocate_by_mult(pb, pc, &total, s);
+        if (st) return st;
+        if (total == 0) return 0;
+        *sweeps += 1;
+        *relocated += total;
+    }
+    set_error("OVERLAP_DET did not terminate after 10000 sweeps");
+    return DDNS_ERR_OVERLAP;
+}
+
+// canonical pair list: sort the packed keys (min gid, max gid, type)
+__global__ void k_sort_keys(long long* keys, int n)
+{
+    // single work-group odd-even merge via bitonic network in LDS (n <= 4096)
+    __shared__ long long s[4096];
+    int P = 1;
+    while (P < n) P <<= 1;
+    for (int i = threadIdx.x; i < P; i += blockDim.x) s[i] = i < n ? keys[i] : 0x7fffffffffffffffLL;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < P; i += blockDim.x) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const long long x = s[i], y = s[l];
+                    const bool up = (i & k) == 0;
+                    if ((x > y) == up) { s[i] = y; s[l] = x; }
+                }
+            }
+            __syncthreads();
+        }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) keys[i] = s[i];
+}
+
+__global__ void k_bitonic_step(long long* s, int P, int j, int k)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const int l = i ^ j;
+    if (l > i) {
+        const long long x = s[i], y = s[l];
+        const bool up = (i & k) == 0;
+        if ((x > y) == up) { s[i] = y; s[l] = x; }
+    }
+}
+
+__global__ void k_gather_keys(const long long* pairs, long long* keys, int n, int P)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    keys[i] = i < n ? pairs[4 * (size_t)i] : 0x7fffffffffffffffLL;
+}
+
+int part_read_pairs(PartBuffers& pb, const PartConst& pc, int64_t* triples, int cap, int* npairs, hipStream_t s)
+{
+    int np = 0;
+    DDNS_HIP(hipMemcpyAsync(&np, pb.counts + 256, sizeof(int), hipMemcpyDeviceToHost, s));
+    DDNS_HIP(hipStreamSynchronize(s));
+    *npairs = np;
+    if (np > pb.pair_cap) { set_error("collision pair list overflow"); return DDNS_ERR_CAPACITY; }
+    if (np == 0 || !triples || cap <= 0) return 0;
+    int P = 1;
+    while (P < np) P <<= 1;
+    long long* keys = nullptr;
+    DDNS_HIP(hipMalloc(&keys, (size_t)P * sizeof(long long)));
+    hipLaunchKernelGGL(k_gather_keys, dim3((P + 255) / 256), dim3(256), 0, s, pb.pairs, keys, np, P);
+    if (P <= 4096) {
+        hipLaunchKernelGGL(k_sort_keys, dim3(1), dim3(1024), 0, s, keys, P);
+    } else {
+        for (int k = 2; k <= P; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1)
+                hipLaunchKernelGGL(k_bitonic_step, dim3((P + 255) / 256), dim3(256), 0, s, keys, P, j, k);
+    }
+    DDNS_HIP(hipGetLastError());
+    std::vector<long long> h(np);
+    DDNS_HIP(hipMemcpyAsync(h.data(), keys, (size_t)np * sizeof(long long), hipMemcpyDeviceToHost, s));
+    DDNS_HIP(hipStreamSynchronize(s));
+    DDNS_HIP(hipFree(keys));
+    const int m = np < cap ? np : cap;
+    for (int i = 0; i < m; ++i) {
+        triples[3 * i + 0] = h[i] >> 32;
+        triples[3 * i + 1] = (h[i] >> 2) & 0x3fffffff;
+        triples[3 * i + 2] = h[i] & 3;
+    }
+    return 0;
+}
+
+int part_read_counts(const PartBuffers& pb, const PartConst& pc, ddns_collision_counts* cc, hipStream_t s)
+{
+    int h[512];
+    long long t[4];
+    DDNS_HIP(hipMemcpyAsync(h, pb.counts, sizeof(h), hipMemcpyDeviceToHost, s));
+    DDNS_HIP(hipMemcpyAsync(t, pb.tested, sizeof(t), hipMemcpyDeviceToHost, s));
+    int ncoll = 0;
+    DDNS_HIP(hipStreamSynchronize(s));
+    cc->icc[0] = h[0]; cc->icc[1] = h[1]; cc->icc[2] = h[2];
+    for (int i = 0; i < DDNS_MAXSET * DDNS_MAXSET; ++i) { cc->colltype[i] = h[8 + i]; cc->nir[i] = h[108 + i]; }
+    cc->npairs = h[256];
+    cc->ncoll = h[256] + h[257];
+    cc->nset = pc.nset;
+    cc->pairs_tested = t[0];
+    cc->pairs_in_dstat = t[1];
+    (void)ncoll;
+    return 0;
+}
+
+int part_read_bins(const PartBuffers& pb, const PartConst& pc, int* nirbin, double* const bins[7], hipStream_t s)
+{
+    const size_t nb = (size_t)pc.nset * pc.nset * DDNS_NBIN;
+    if (nirbin) DDNS_HIP(hipMemcpyAsync(nirbin, pb.nirbin, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+    for (int k = 0; k < 7; ++k)
+        if (bins[k]) DDNS_HIP(hipMemcpyAsync(bins[k], pb.bins + k * nb, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+    DDNS_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+}  // namespace ddns

@@ -1,0 +1,851 @@
+// ddns_api.hip -- the C ABI of libddns.so (declared in include/ddns.h).
+//
+// A handle owns one periodic domain on the current GPU and one HIP stream.  The
+// step follows the reference driver loop turb.f90:492-707: the spectral step is
+// split around the droplet block exactly where the reference places it (the
+// droplets see the physical velocity of the inverse FFTs of this step).
+#include "ddns_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace ddns {
+
+static thread_local std::string g_err = "no error";
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line)
+{
+    char buf[512];
+    std::snprintf(buf, sizeof(buf), "HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file, line);
+    g_err = buf;
+    return DDNS_ERR_HIP;
+}
+
+int relocate_by_mult(PartBuffers& pb, const PartConst& pc, int* total_host, hipStream_t s);
+
+}  // namespace ddns
+
+using namespace ddns;
+
+struct ddns_handle {
+    ddns_params p;
+    int N = 0, nek = 0, prec = 0;
+    hipStream_t stream = nullptr;
+    FlowBuffers fb;
+    FlowConst fc{};
+    bool a_valid = false;        // inverse pipeline holds the start of the next step
+    bool phys_valid = false;
+    int istep = 0;
+    double time = 0;
+    // droplets
+    PartBuffers pb;
+    PartConst pc{};
+    int n = 0;
+    int new_particle = 0;
+    ddns_collision_counts cc{};
+    ddns_timing timing{};
+    hipEvent_t ev[16] = {};
+    std::string err;
+};
+
+// ------------------------------------------------------------------ helpers
+static double pi2_ref() { return 4. * std::atan(1.) * 2.0; }                 // turb.f90:296-297
+
+// PART_INIT scaling, partlib.f90:104-150
+static void droplet_scaling(const ddns_params& p, PartConst& pc)
+{
+    const double etk_CLOUD = std::sqrt(std::sqrt(p.viscosity_CLOUD * p.viscosity_CLOUD * p.viscosity_CLOUD / p.ediss_CLOUD));
+    const double tauk_CLOUD = std::sqrt(p.viscosity_CLOUD / p.ediss_CLOUD);
+    const double vk_CLOUD = etk_CLOUD / tauk_CLOUD;
+    const double vk_DNS = p.etk_DNS / p.tauk_DNS;
+    const double vk_ratio = vk_CLOUD / vk_DNS;
+    const double etk_ratio = etk_CLOUD / p.etk_DNS;
+    const double tauk_ratio = tauk_CLOUD / p.tauk_DNS;
+    pc.radmax = 0.0;
+    for (int il = 0; il < DDNS_MAXSET; ++il) pc.radius[il] = 0.0;
+    for (int il = 0; il < p.nset; ++il) {
+        pc.radius[il] = p.radius[il] * (p.etk_DNS / etk_CLOUD) / 10000.0;
+        if (il == 0 || pc.radius[il] > pc.radmax) pc.radmax = pc.radius[il];
+    }
+    const double tau_CLOUD_const = 2. * p.rho_water * (etk_ratio * etk_ratio) / (9. * p.rho_air * p.viscosity_CLOUD);
+    pc.Sv_const = tau_CLOUD_const * p.gravity / vk_ratio;
+    pc.tau2 = tau_CLOUD_const / tauk_ratio;
+    pc.pi2 = pi2_ref();
+    pc.d_coll = 20.0 * pc.radmax;
+    pc.nx_coll = 1 + (int)(pc.d_coll * (double)pc.N / pc.pi2);
+    pc.nx_max = pc.nx_coll;
+    pc.d_hdi = 0.0;
+    if (p.hdi_included) {
+        pc.d_hdi = p.hdi_trunc * pc.radmax;
+        const int nx_hdi = 1 + (int)(pc.d_hdi * (double)pc.N / pc.pi2);
+        pc.nx_max = std::max(pc.nx_max, nx_hdi);
+    }
+    pc.imax_stat = pc.nx_coll;
+}
+
+static int fail(ddns_handle* h, int code, const std::string& msg)
+{
+    set_error(msg);
+    if (h) h->err = msg;
+    return code;
+}
+
+static int keep(ddns_handle* h, int st)
+{
+    if (st < 0 && h) h->err = g_err;
+    return st;
+}
+
+// ranff (accesories.f90:38-68) on the host, the portable generator of the setup
+// routines (in place of IFPORT rand in flowlib.f90:163-164).
+static double host_ranff(int* idum, int* iy, int* iv)
+{
+    const int IA = 16807, IM = 2147483647, IQ = 127773, IR = 2836, NTAB = 32;
+    const int NDIV = 1 + (IM - 1) / NTAB;
+    const double AM = 1.0 / IM, EPS = 1.2e-7, RNMX = 1.0 - EPS;
+    int j, k;
+    if (*idum <= 0 || *iy == 0) {
+        *idum = (-*idum > 1) ? -*idum : 1;
+        for (j = NTAB + 8; j >= 1; --j) {
+            k = *idum / IQ;
+            *idum = IA * (*idum - k * IQ) - IR * k;
+            if (*idum < 0) *idum += IM;
+            if (j <= NTAB) iv[j - 1] = *idum;
+        }
+        *iy = iv[0];
+    }
+    k = *idum / IQ;
+    *idum = IA * (*idum - k * IQ) - IR * k;
+    if (*idum < 0) *idum += IM;
+    j = 1 + *iy / NDIV;
+    *iy = iv[j - 1];
+    iv[j - 1] = *idum;
+    const double r = AM * (double)(*iy);
+    return r < RNMX ? r : RNMX;
+}
+
+// ------------------------------------------------------------------ parameters
+extern "C" int ddns_params_default(ddns_params* p)
+{
+    if (!p) return DDNS_ERR_ARG;
+    std::memset(p, 0, sizeof(*p));
+    p->nx = p->ny = p->nz = 64;
+    p->nprocY = 1;
+    p->nstep = 10;
+    p->dt = 0.001;
+    p->rnu = 0.026;
+    p->time = 0.0;
+    p->new_flow = 1;
+    p->idp = 0;
+    p->iseed_m0 = 123;
+    p->forcing = 1;
+    p->force[0] = 1.242477;
+    p->force[1] = 0.391356;
+    p->iseedf = 232300;
+    p->ieout = 10;
+    p->ispec = 1000;
+    p->divstat = 1;
+    p->rho_water = 1.0;
+    p->rho_air = 0.001;
+    p->etk_DNS = 0.0755;
+    p->tauk_DNS = 0.2195;
+    p->st = 2.0;
+    p->ediss_CLOUD = 400.0;
+    p->viscosity_CLOUD = 0.17;
+    p->gravity = 980.0;
+    p->new_particle = 1;
+    p->nonoverlap = 0;
+    p->hdi_included = 0;
+    p->hdi_trunc = 50.0;
+    p->npart = 0;
+    p->ndrag = 0;
+    p->nset = 1;
+    p->radius[0] = 20.0;
+    p->precision = DDNS_FP64;
+    p->nstep_grow = 40000;
+    p->ou_tf = 0.038;
+    p->ou_var = 447.31;
+    p->gmres_maxiter = 50;
+    p->capacity = 0;
+    p->gmres_tol = 1.0e-6;
+    p->relocate_seed = 0;
+    p->istep0 = 0;
+    return DDNS_OK;
+}
+
+static bool parse_logical(const std::string& s, int* out)
+{
+    std::string t;
+    for (char c : s) if (!std::isspace((unsigned char)c)) t += (char)std::toupper((unsigned char)c);
+    if (t.find("TRUE") != std::string::npos || t == "T" || t == ".T.") { *out = 1; return true; }
+    if (t.find("FALSE") != std::string::npos || t == "F" || t == ".F.") { *out = 0; return true; }
+    return false;
+}
+
+// Positional reader: value lines at the positions turb.f90:103-164 reads them; every
+// value line is followed by one label line.  Commas separate list values.
+extern "C" int ddns_read_parameter_file(const char* path, ddns_params* p)
+{
+    if (!path || !p) return DDNS_ERR_ARG;
+    std::ifstream f(path);
+    if (!f) { set_error(std::string("cannot open ") + path); return DDNS_ERR_IO; }
+    std::vector<std::string> lines;
+    std::string line;
+    while (std::getline(f, line)) lines.push_back(line);
+    ddns_params_default(p);
+    size_t li = 0;
+    auto next_value = [&](std::string& out) -> bool {
+        if (li >= lines.size()) return false;
+        out = lines[li++];
+        for (char& c : out) if (c == ',') c = ' ';
+        return true;
+    };
+    auto skip_label = [&]() { if (li < lines.size()) ++li; };
+    std::string v;
+    auto rd_int = [&](int* x) { if (!next_value(v)) return false; std::istringstream is(v); return (bool)(is >> *x); };
+    auto rd_dbl = [&](double* x) { if (!next_value(v)) return false; std::istringstream is(v); std::string tok; if (!(is >> tok)) return false; for (char& c : tok) if (c == 'd' || c == 'D') c = 'e'; *x = std::strtod(tok.c_str(), nullptr); return true; };
+    auto rd_log = [&](int* x) { if (!next_value(v)) return false; return parse_logical(v, x); };
+    bool ok = true;
+    {
+        if (!next_value(v)) ok = false;
+        std::istringstream is(v);
+        ok = ok && (bool)(is >> p->nx >> p->ny >> p->nz);
+    }
+    skip_label(); ok = ok && rd_int(&p->nprocY);
+    skip_label(); ok = ok && rd_int(&p->nstep);
+    skip_label(); ok = ok && rd_dbl(&p->dt);
+    skip_label(); ok = ok && rd_dbl(&p->rnu);
+    skip_label(); ok = ok && rd_dbl(&p->time);
+    skip_label(); ok = ok && rd_log(&p->new_flow);
+    skip_label(); ok = ok && rd_int(&p->idp);
+    skip_label(); ok = ok && rd_int(&p->iseed_m0);
+    skip_label(); ok = ok && rd_int(&p->forcing);
+    skip_label();
+    if (ok && next_value(v)) {
+        std::istringstream is(v);
+        ok = (bool)(is >> p->force[0] >> p->force[1]);
+    } else ok = false;
+    skip_label(); ok = ok && rd_int(&p->iseedf);
+    skip_label(); ok = ok && rd_int(&p->ieout);
+    skip_label(); ok = ok && rd_int(&p->ispec);
+    skip_label(); ok = ok && rd_log(&p->divstat);
+    skip_label(); ok = ok && rd_dbl(&p->rho_water);
+    skip_label(); ok = ok && rd_dbl(&p->rho_air);
+    skip_label();
+    if (ok && next_value(v)) {
+        std::istringstream is(v);
+        ok = (bool)(is >> p->etk_DNS >> p->tauk_DNS);
+    } else ok = false;
+    skip_label(); ok = ok && rd_dbl(&p->st);
+    skip_label(); ok = ok && rd_dbl(&p->ediss_CLOUD);
+    skip_label(); ok = ok && rd_dbl(&p->viscosity_CLOUD);
+    skip_label(); ok = ok && rd_dbl(&p->gravity);
+    skip_label(); ok = ok && rd_log(&p->new_particle);
+    skip_label(); ok = ok && rd_log(&p->nonoverlap);
+    skip_label(); ok = ok && rd_log(&p->hdi_included);
+    skip_label(); ok = ok && rd_dbl(&p->hdi_trunc);
+    skip_label(); ok = ok && rd_int(&p->npart);
+    skip_label(); ok = ok && rd_int(&p->ndrag);
+    skip_label(); ok = ok && rd_int(&p->nset);
+    skip_label();
+    if (ok && (p->nset < 1 || p->nset > DDNS_MAXSET)) ok = false;
+    for (int i = 0; ok && i < p->nset; ++i) ok = rd_dbl(&p->radius[i]);
+    if (!ok) { set_error(std::string("malformed parameter file ") + path); return DDNS_ERR_IO; }
+    if (p->hdi_included) p->nonoverlap = 1;                                 // turb.f90:171
+    return DDNS_OK;
+}
+
+// ------------------------------------------------------------------ lifetime
+extern "C" int ddns_create(const ddns_params* pin, ddns_handle** out)
+{
+    if (!pin || !out) return fail(nullptr, DDNS_ERR_ARG, "null argument");
+    *out = nullptr;
+    const int N = pin->nx;
+    if (N != pin->ny || N != pin->nz || N < 32 || N > 1024 || (N & (N - 1)))
+        return fail(nullptr, DDNS_ERR_ARG, "grid must be cubic, a power of two, 32..1024");
+    if (pin->precision != DDNS_FP64 && pin->precision != DDNS_FP32_MIXED)
+        return fail(nullptr, DDNS_ERR_ARG, "precision must be DDNS_FP64 or DDNS_FP32_MIXED");
+    if (pin->nset < 1 || pin->nset > DDNS_MAXSET) return fail(nullptr, DDNS_ERR_ARG, "nset must be 1..10");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, DDNS_ERR_NO_DEVICE, "no GPU visible");
+    ddns_handle* h = new ddns_handle();
+    h->p = *pin;
+    if (h->p.hdi_included) h->p.nonoverlap = 1;
+    if (h->p.gmres_maxiter <= 0) h->p.gmres_maxiter = 50;
+    if (h->p.gmres_tol <= 0) h->p.gmres_tol = 1.0e-6;
+    if (h->p.ou_tf <= 0) h->p.ou_tf = 0.038;
+    if (h->p.ou_var <= 0) h->p.ou_var = 447.31;
+    if (h->p.nstep_grow <= 0) h->p.nstep_grow = 40000;
+    h->N = N;
+    h->nek = (int)(N / 2 - 1.5);
+    h->prec = pin->precision;
+    h->istep = pin->istep0;
+    h->time = pin->time;
+    h->new_particle = pin->new_particle;
+    int st = 0;
+    if (hipStreamCreate(&h->stream) != hipSuccess) { delete h; return fail(nullptr, DDNS_ERR_HIP, "stream"); }
+    st = flow_alloc(h->fb, N, h->prec);
+    if (st) { ddns_destroy(h); return st; }
+    h->fc.N = N; h->fc.nek = h->nek; h->fc.forcing = h->p.forcing; h->fc.dt = h->p.dt; h->fc.rnu = h->p.rnu;
+    h->fc.force0 = h->p.force[0]; h->fc.force1 = h->p.force[1];
+    h->fc.ou_tf = h->p.ou_tf; h->fc.ou_var = h->p.ou_var; h->fc.pi2 = pi2_ref();
+    // stochastic forcing starts from rest with iseedf = -iseedf (turb.f90:441-447)
+    {
+        int stt[34] = {0};
+        stt[0] = -h->p.iseedf;
+        if (hipMemcpy(h->fb.sth_state, stt, sizeof(stt), hipMemcpyHostToDevice) != hipSuccess) { ddns_destroy(h); return fail(nullptr, DDNS_ERR_HIP, "copy"); }
+    }
+    // droplets
+    h->pc.N = N;
+    h->pc.nset = h->p.nset;
+    h->pc.dt = h->p.dt; h->pc.rnu = h->p.rnu; h->pc.st = h->p.st;
+    h->pc.hdi = h->p.hdi_included; h->pc.nonoverlap = h->p.nonoverlap; h->pc.ndrag = h->p.ndrag;
+    h->pc.hdi_trunc = h->p.hdi_trunc; h->pc.precision = h->prec;
+    droplet_scaling(h->p, h->pc);
+    const int cap = std::max(1, h->p.capacity > 0 ? h->p.capacity : h->p.npart);
+    if (h->p.npart > 0) {
+        const double pi2 = h->pc.pi2;
+        if (pi2 <= h->pc.d_coll || pi2 <= 2.0 * h->pc.radmax || (h->p.hdi_included && pi2 <= h->pc.d_hdi) ||
+            2 * h->pc.nx_max + 1 >= N) {
+            ddns_destroy(h);
+            return fail(nullptr, DDNS_ERR_DOMAIN, "cell/halo too thin for the collision or HI radius (partlib.f90:181-220)");
+        }
+    }
+    st = part_alloc(h->pb, cap, h->p.nset, 1);
+    if (st) { ddns_destroy(h); return st; }
+    st = part_set_rng(h->pb, h->p.relocate_seed, h->stream);
+    if (st) { ddns_destroy(h); return st; }
+    for (int i = 0; i < 16; ++i)
+        if (hipEventCreate(&h->ev[i]) != hipSuccess) { ddns_destroy(h); return fail(nullptr, DDNS_ERR_HIP, "event"); }
+    if (hipStreamSynchronize(h->stream) != hipSuccess) { ddns_destroy(h); return fail(nullptr, DDNS_ERR_HIP, "sync"); }
+    *out = h;
+    return DDNS_OK;
+}
+
+extern "C" int ddns_destroy(ddns_handle* h)
+{
+    if (!h) return DDNS_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    flow_free(h->fb);
+    part_free(h->pb);
+    for (int i = 0; i < 16; ++i) if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return DDNS_OK;
+}
+
+extern "C" const char* ddns_last_error(const ddns_handle* h)
+{
+    if (h && !h->err.empty()) return h->err.c_str();
+    return g_err.c_str();
+}
+
+extern "C" int ddns_nek(const ddns_handle* h) { return h ? h->nek : 0; }
+
+// ------------------------------------------------------------------ flow state
+extern "C" int ddns_set_flow(ddns_handle* h, const double* vx, const double* vy, const double* vz,
+                             const double* ox, const double* oy, const double* oz)
+{
+    if (!h) return DDNS_ERR_ARG;
+    const double* src[6] = {vx, vy, vz, ox, oy, oz};
+    const int st = flow_upload_state(h->fb, h->N, h->prec, src, h->stream);
+    h->a_valid = false;
+    h->phys_valid = false;
+    return keep(h, st);
+}
+
+extern "C" int ddns_get_flow(ddns_handle* h, double* vx, double* vy, double* vz, double* ox, double* oy, double* oz)
+{
+    if (!h) return DDNS_ERR_ARG;
+    double* dst[6] = {vx, vy, vz, ox, oy, oz};
+    return keep(h, flow_download_state(h->fb, h->N, h->prec, dst, h->stream));
+}
+
+extern "C" int ddns_set_forcing_state(ddns_handle* h, const double* b, int iseedf, int iyf, const int* ivf)
+{
+    if (!h) return DDNS_ERR_ARG;
+    int stt[34] = {0};
+    stt[0] = iseedf; stt[1] = iyf;
+    for (int i = 0; i < 32; ++i) stt[2 + i] = ivf ? ivf[i] : 0;
+    DDNS_HIP(hipMemcpy(h->fb.sth_state, stt, sizeof(stt), hipMemcpyHostToDevice));
+    if (b) DDNS_HIP(hipMemcpy(h->fb.sth_b, b, 450 * sizeof(double), hipMemcpyHostToDevice));
+    return DDNS_OK;
+}
+
+extern "C" int ddns_get_forcing_state(ddns_handle* h, double* b, int* iseedf, int* iyf, int* ivf)
+{
+    if (!h) return DDNS_ERR_ARG;
+    int stt[34];
+    DDNS_HIP(hipStreamSynchronize(h->stream));
+    DDNS_HIP(hipMemcpy(stt, h->fb.sth_state, sizeof(stt), hipMemcpyDeviceToHost));
+    if (b) DDNS_HIP(hipMemcpy(b, h->fb.sth_b, 450 * sizeof(double), hipMemcpyDeviceToHost));
+    if (iseedf) *iseedf = stt[0];
+    if (iyf) *iyf = stt[1];
+    if (ivf) for (int i = 0; i < 32; ++i) ivf[i] = stt[2 + i];
+    return DDNS_OK;
+}
+
+// Host restatement of the setup part of initial_flow (flowlib.f90:53-144) on the
+// single-domain layout; the generator is ranff seeded with -seed.
+extern "C" int ddns_init_flow(ddns_handle* h, int kind, int seed, double k0, double urms)
+{
+    if (!h) return DDNS_ERR_ARG;
+    const int N = h->N, L = N + 2, nek = h->nek;
+    const size_t fs = (size_t)L * L * N;
+    std::vector<double> u[3] = {std::vector<double>(fs, 0.0), std::vector<double>(fs, 0.0), std::vector<double>(fs, 0.0)};
+    if (kind >= 0) {
+        const double pi = 4. * std::atan(1.);
+        int idum = -seed, iy = 0, iv[32] = {0};
+        auto I = [&](int i, int j, int k) { return (size_t)i + (size_t)L * ((size_t)j + (size_t)L * k); };
+        for (int c = 0; c < 3; ++c)                                           // gaussian, 158-172
+            for (int i = 0; i < N; ++i)
+                for (int j = 0; j < L; j += 2)
+                    for (int k = 0; k < N; ++k) {
+                        double t1 = host_ranff(&idum, &iy, iv), t2 = host_ranff(&idum, &iy, iv);
+                        if (t1 <= 1.e-10) t1 = 1.e-10;
+                        if (t2 <= 1.e-10) t2 = 1.e-10;
+                        t2 = 2 * pi * t2;
+                        u[c][I(i, j, k)] = std::sqrt(-2.0 * std::log(t1)) * std::cos(t2);
+                        u[c][I(i, j + 1, k)] = std::sqrt(-2.0 * std::log(t1)) * std::sin(t2);
+                    }
+        auto kxv = [&](int j) { return (double)(j / 2); };
+        auto kyv = [&](int k) { return (double)(k <= N / 2 ? k : k - N); };
+        auto kzv = [&](int i) { return (double)(i <= N / 2 ? i : i - N); };
+        auto k2v = [&](int i, int j, int k) { return (i == 0 && j <= 1 && k == 0) ? 1.e-5 : kxv(j) * kxv(j) + kyv(k) * kyv(k) + kzv(i) * kzv(i); };
+        auto ik2v = [&](int i, int j, int k) { return (int)(std::sqrt(k2v(i, j, k)) + 0.5); };
+        // symmetrize (intended form), 94-96
+        for (int c = 0; c < 3; ++c) {
+            std::vector<double>& v = u[c];
+            v[I(0, 0, 0)] = 0.0; v[I(0, 1, 0)] = 0.0;
+            for (int k = 0; k < N; ++k) for (int i = 0; i < L; ++i) { v[I(i, N, k)] = 0.0; v[I(i, N + 1, k)] = 0.0; }
+            for (int j = 0; j < L; ++j) for (int i = 0; i < L; ++i) v[I(i, j, N / 2)] = 0.0;
+            for (int k = 0; k < N; ++k) for (int j = 0; j < L; ++j) v[I(N / 2, j, k)] = 0.0;
+            std::vector<double> re(N * N), im(N * N);
+            for (int k = 0; k < N; ++k) for (int i = 0; i < N; ++i) { re[i + N * k] = v[I(i, 0, k)]; im[i + N * k] = v[I(i, 1, k)]; }
+            for (int k = 0; k < N; ++k)
+                for (int i = 0; i < N; ++i) {
+                    const int kk = (N - k) % N, ii = (N - i) % N;
+                    v[I(i, 0, k)] = 0.5 * (re[i + N * k] + re[ii + N * kk]);
+                    v[I(i, 1, k)] = 0.5 * (im[i + N * k] - im[ii + N * kk]);
+                }
+            v[I(0, 0, 0)] = 0.0; v[I(0, 1, 0)] = 0.0;
+        }
+        // projection, 98-101
+        for (int k = 0; k < N; ++k)
+            for (int j = 0; j < L; ++j)
+                for (int i = 0; i < N; ++i) {
+                    const size_t o = I(i, j, k);
+                    const double kx = kxv(j), ky = kyv(k), kz = kzv(i), k2 = k2v(i, j, k);
+                    const double t = (kx * u[0][o] + ky * u[1][o] + kz * u[2][o]) / k2;
+                    u[0][o] -= kx * t; u[1][o] -= ky * t; u[2][o] -= kz * t;
+                }
+        // shell energies and scaling, 107-137
+        std::vector<double> e_t(nek + 1, 0.0), enumv(nek + 1, 0.0);
+        for (int k = 0; k < N; ++k)
+            for (int j = 0; j < L; ++j)
+                for (int i = 0; i < N; ++i) {
+                    const int s = ik2v(i, j, k);
+                    if (s < 1 || s > nek) continue;
+                    const size_t o = I(i, j, k);
+                    double t = u[0][o] * u[0][o] + u[1][o] * u[1][o] + u[2][o] * u[2][o];
+                    if (j <= 1) t *= 0.5;
+                    e_t[s] += t;
+                    enumv[s] += (kxv(j) > 0.1) ? 1.0 : 0.5;                   // turb.f90:408-415
+                }
+        std::vector<double> target(nek + 1, 0.0);
+        if (kind == DDNS_SPEC_K4_GAUSS) {
+            double tot = 0.0;
+            for (int s = 1; s <= nek; ++s) {
+                const double q = (double)s / k0;
+                target[s] = q * q * q * q * std::exp(-2.0 * q * q);
+                tot += target[s];
+            }
+            for (int s = 1; s <= nek; ++s) target[s] *= 1.5 * urms * urms / tot;
+        }
+        for (int s = 1; s <= nek; ++s) {
+            double e1;
+            if (kind == DDNS_SPEC_K4_GAUSS) {
+                e1 = e_t[s] > 0 ? std::sqrt(target[s] / e_t[s]) : 0.0;
+            } else {
+                const double vol = 4. / 3. * pi * ((s + 0.5) * (s + 0.5) * (s + 0.5) - (s - 0.5) * (s - 0.5) * (s - 0.5));
+                const double ccc = h->p.force[0] * 4. / 3. * pi * (1.5 * 1.5 * 1.5 - 0.5 * 0.5 * 0.5) / enumv[1];
+                e1 = std::sqrt(ccc / e_t[s] * enumv[s] / vol) / std::pow((double)s, 0.833333);
+            }
+            for (int k = 0; k < N; ++k)
+                for (int j = 0; j < L; ++j)
+                    for (int i = 0; i < N; ++i)
+                        if (ik2v(i, j, k) == s) { const size_t o = I(i, j, k); u[0][o] *= e1; u[1][o] *= e1; u[2][o] *= e1; }
+        }
+        for (int c = 0; c < 3; ++c)                                           // dealiasing, 139-141
+            for (int k = 0; k < N; ++k)
+                for (int j = 0; j < L; ++j)
+                    for (int i = 0; i < L; ++i)
+                        if (i >= N || ik2v(i, j, k) > nek) u[c][I(i, j, k)] = 0.0;
+    }
+    // the history starts equal to the velocity (turb.f90:452-454)
+    const int st = ddns_set_flow(h, u[0].data(), u[1].data(), u[2].data(), u[0].data(), u[1].data(), u[2].data());
+    return st;
+}
+
+extern "C" int ddns_get_physical_velocity(ddns_handle* h, double* ux, double* uy, double* uz)
+{
+    if (!h) return DDNS_ERR_ARG;
+    if (!h->phys_valid) return fail(h, DDNS_ERR_ARG, "no step has run since the flow was set");
+    return keep(h, flow_download_phys(h->fb, h->N, h->prec, ux, uy, uz, h->stream));
+}
+
+extern "C" int ddns_get_spectrum(ddns_handle* h, double* ek)
+{
+    if (!h || !ek) return DDNS_ERR_ARG;
+    return keep(h, flow_spectrum(h->fb, h->N, h->nek, h->prec, ek, h->stream));
+}
+
+// ------------------------------------------------------------------ droplets
+extern "C" int ddns_set_droplets(ddns_handle* h, int n, const double* yp, const double* fvp, const double* fdp,
+                                 const double* pertvel)
+{
+    if (!h || n < 0) return DDNS_ERR_ARG;
+    if (n > h->pb.cap) return fail(h, DDNS_ERR_CAPACITY, "more droplets than capacity");
+    const int cap = h->pb.cap;
+    std::vector<double> pos(3 * (size_t)cap, 0.0), vel(3 * (size_t)cap, 0.0), rad(cap, 0.0), gid(cap, 0.0);
+    std::vector<double> fv(9 * (size_t)cap, 0.0), fd(12 * (size_t)cap, 0.0), pv(3 * (size_t)cap, 0.0);
+    for (int i = 0; i < n; ++i) {
+        for (int c = 0; c < 3; ++c) {
+            pos[(size_t)c * cap + i] = yp[8 * (size_t)i + c];
+            vel[(size_t)c * cap + i] = yp[8 * (size_t)i + 3 + c];
+            if (pertvel) pv[(size_t)c * cap + i] = pertvel[3 * (size_t)i + c];
+            for (int t = 0; t < 3; ++t) if (fvp) fv[(size_t)(3 * t + c) * cap + i] = fvp[9 * (size_t)i + 3 * t + c];
+            for (int t = 0; t < 4; ++t) if (fdp) fd[(size_t)(3 * t + c) * cap + i] = fdp[12 * (size_t)i + 3 * t + c];
+        }
+        rad[i] = yp[8 * (size_t)i + 6];
+        gid[i] = yp[8 * (size_t)i + 7];
+    }
+    DDNS_HIP(hipMemcpy(h->pb.pos, pos.data(), pos.size() * 8, hipMemcpyHostToDevice));
+    DDNS_HIP(hipMemcpy(h->pb.vel, vel.data(), vel.size() * 8, hipMemcpyHostToDevice));
+    DDNS_HIP(hipMemcpy(h->pb.rad, rad.data(), rad.size() * 8, hipMemcpyHostToDevice));
+    DDNS_HIP(hipMemcpy(h->pb.gid, gid.data(), gid.size() * 8, hipMemcpyHostToDevice));
+    DDNS_HIP(hipMemcpy(h->pb.fvp, fv.data(), fv.size() * 8, hipMemcpyHostToDevice));
+    DDNS_HIP(hipMemcpy(h->pb.fdp, fd.data(), fd.size() * 8, hipMemcpyHostToDevice));
+    DDNS_HIP(hipMemcpy(h->pb.pert, pv.data(), pv.size() * 8, hipMemcpyHostToDevice));
+    h->n = n;
+    h->pc.n = n;
+    return DDNS_OK;
+}
+
+extern "C" int ddns_get_droplets(ddns_handle* h, int* n, double* yp, double* fvp, double* fdp, double* pertvel,
+                                 double* vp, double* dmove)
+{
+    if (!h) return DDNS_ERR_ARG;
+    DDNS_HIP(hipStreamSynchronize(h->stream));
+    const int cap = h->pb.cap, m = h->n;
+    if (n) *n = m;
+    std::vector<double> pos(3 * (size_t)cap), vel(3 * (size_t)cap), rad(cap), gid(cap);
+    std::vector<double> fv(9 * (size_t)cap), fd(12 * (size_t)cap), pv(3 * (size_t)cap), vv(3 * (size_t)cap), dm(3 * (size_t)cap);
+    DDNS_HIP(hipMemcpy(pos.data(), h->pb.pos, pos.size() * 8, hipMemcpyDeviceToHost));
+    DDNS_HIP(hipMemcpy(vel.data(), h->pb.vel, vel.size() * 8, hipMemcpyDeviceToHost));
+    DDNS_HIP(hipMemcpy(rad.data(), h->pb.rad, rad.size() * 8, hipMemcpyDeviceToHost));
+    DDNS_HIP(hipMemcpy(gid.data(), h->pb.gid, gid.size() * 8, hipMemcpyDeviceToHost));
+    DDNS_HIP(hipMemcpy(fv.data(), h->pb.fvp, fv.size() * 8, hipMemcpyDeviceToHost));
+    DDNS_HIP(hipMemcpy(fd.data(), h->pb.fdp, fd.size() * 8, hipMemcpyDeviceToHost));
+    DDNS_HIP(hipMemcpy(pv.data(), h->pb.pert, pv.size() * 8, hipMemcpyDeviceToHost));
+    DDNS_HIP(hipMemcpy(vv.data(), h->pb.vp, vv.size() * 8, hipMemcpyDeviceToHost));
+    DDNS_HIP(hipMemcpy(dm.data(), h->pb.dmove, dm.size() * 8, hipMemcpyDeviceToHost));
+    for (int i = 0; i < m; ++i) {
+        for (int c = 0; c < 3; ++c) {
+            if (yp) { yp[8 * (size_t)i + c] = pos[(size_t)c * cap + i]; yp[8 * (size_t)i + 3 + c] = vel[(size_t)c * cap + i]; }
+            if (pertvel) pertvel[3 * (size_t)i + c] = pv[(size_t)c * cap + i];
+            if (vp) vp[3 * (size_t)i + c] = vv[(size_t)c * cap + i];
+            if (dmove) dmove[3 * (size_t)i + c] = dm[(size_t)c * cap + i];
+            for (int t = 0; t < 3; ++t) if (fvp) fvp[9 * (size_t)i + 3 * t + c] = fv[(size_t)(3 * t + c) * cap + i];
+            for (int t = 0; t < 4; ++t) if (fdp) fdp[12 * (size_t)i + 3 * t + c] = fd[(size_t)(3 * t + c) * cap + i];
+        }
+        if (yp) { yp[8 * (size_t)i + 6] = rad[i]; yp[8 * (size_t)i + 7] = gid[i]; }
+    }
+    return DDNS_OK;
+}
+
+extern "C" int ddns_overlap_fix(ddns_handle* h)
+{
+    if (!h) return DDNS_ERR_ARG;
+    int sw = 0, rl = 0;
+    const int st = part_overlap(h->pb, h->pc, &sw, &rl, h->stream);
+    h->cc.noverlap_sweeps = sw;
+    h->cc.noverlap_relocated = rl;
+    return keep(h, st);
+}
+
+// PART_INIT positions and radii (partlib.f90:227-281), then PERIODIC + OVERLAP_DET.
+extern "C" int ddns_init_droplets(ddns_handle* h, int seed, int radii_kind, double median_um, double sigma_ln,
+                                  double rmin_um, double rmax_um)
+{
+    if (!h) return DDNS_ERR_ARG;
+    const int npart = h->p.npart;
+    if (npart > h->pb.cap) return fail(h, DDNS_ERR_CAPACITY, "npart exceeds capacity");
+    const double pi2 = h->pc.pi2;
+    const double etk_CLOUD = std::sqrt(std::sqrt(h->p.viscosity_CLOUD * h->p.viscosity_CLOUD * h->p.viscosity_CLOUD / h->p.ediss_CLOUD));
+    const double scale = (h->p.etk_DNS / etk_CLOUD) / 10000.0;
+    std::vector<double> yp(8 * (size_t)std::max(npart, 1), 0.0);
+    int idum = -seed, iy = 0, iv[32] = {0};
+    for (int ih = 0; ih < npart; ++ih) {
+        const double l1 = host_ranff(&idum, &iy, iv), l2 = host_ranff(&idum, &iy, iv), l3 = host_ranff(&idum, &iy, iv);
+        yp[8 * (size_t)ih + 0] = pi2 * l1;
+        yp[8 * (size_t)ih + 1] = pi2 * l2;
+        yp[8 * (size_t)ih + 2] = pi2 * l3;
+        yp[8 * (size_t)ih + 7] = (double)(ih + 1);
+    }
+    if (radii_kind == DDNS_RADII_LOGNORMAL) {
+        int is2 = -(seed + 1), iy2 = 0, iv2[32] = {0};
+        for (int ih = 0; ih < npart; ih += 2) {
+            const double u1 = host_ranff(&is2, &iy2, iv2), u2 = host_ranff(&is2, &iy2, iv2);
+            const double rr = std::sqrt(-2.0 * std::log(u1)), th = 2.0 * M_PI * u2;
+            const double g[2] = {rr * std::cos(th), rr * std::sin(th)};
+            for (int q = 0; q < 2 && ih + q < npart; ++q) {
+                double r = median_um * std::exp(sigma_ln * g[q]);
+                r = std::min(std::max(r, rmin_um), rmax_um);
+                yp[8 * (size_t)(ih + q) + 6] = r * scale;
+            }
+        }
+    } else {
+        for (int ih = 0; ih < npart; ++ih) {
+            const int i1 = (int)std::floor((double)ih * (double)h->p.nset / (double)npart) + 1;
+            yp[8 * (size_t)ih + 6] = h->pc.radius[i1 - 1];
+        }
+    }
+    for (int ih = 0; ih < npart; ++ih)                                        // PERIODIC, turb.f90:427
+        for (int c = 0; c < 3; ++c) {
+            double& y = yp[8 * (size_t)ih + c];
+            if (y >= pi2) y = y - pi2;
+            if (y < 0.0) y = y + pi2;
+        }
+    int st = ddns_set_droplets(h, npart, yp.data(), nullptr, nullptr, nullptr);
+    if (st) return st;
+    if (h->p.nonoverlap) return ddns_overlap_fix(h);                          // turb.f90:429
+    return DDNS_OK;
+}
+
+// ------------------------------------------------------------------ the step
+static double cell_width(const ddns_handle* h)
+{
+    // widest distance a pair test can need: the statistics radius 20 a_max, the HI
+    // truncation, and two grid cells (the collision test's cell window)
+    const double dx = h->pc.pi2 / h->N;
+    double w = std::max(20.0 * h->pc.radmax, 2.0 * dx);
+    if (h->p.hdi_included) w = std::max(w, h->pc.d_hdi);
+    return w;
+}
+
+static int droplet_block(ddns_handle* h, float* ms)
+{
+    PartConst& pc = h->pc;
+    pc.n = h->n;
+    pc.new_particle = h->new_particle;
+    hipStream_t s = h->stream;
+    int status = 0, st;
+    DDNS_HIP(hipEventRecord(h->ev[8], s));
+    st = part_build_cells(h->pb, pc, cell_width(h), s);                       // HEADLIST (binning)
+    if (st) return st;
+    DDNS_HIP(hipEventRecord(h->ev[9], s));
+    st = part_interp(h->pb, pc, h->fb.uphys, s);                              // 553-556
+    if (st) return st;
+    st = part_history_fvp(h->pb, pc, s);                                      // 564
+    if (st) return st;
+    DDNS_HIP(hipEventRecord(h->ev[10], s));
+    h->cc = ddns_collision_counts{};
+    if (h->p.hdi_included) {                                                  // 571-572
+        int it = 0; double res = 0;
+        st = part_hdi(h->pb, pc, h->p.gmres_maxiter, h->p.gmres_tol, &it, &res, s);
+        if (st < 0) return st;
+        if (st > 0) status = st;
+        h->cc.gmres_iters = it;
+        h->cc.gmres_residual = res;
+    }
+    DDNS_HIP(hipEventRecord(h->ev[11], s));
+    st = part_advance(h->pb, pc, s);                                          // 574-576
+    if (st) return st;
+    h->new_particle = 0;
+    pc.new_particle = 0;
+    DDNS_HIP(hipEventRecord(h->ev[12], s));
+    st = part_collide(h->pb, pc, s);                                          // 584
+    if (st) return st;
+    const bool reloc = h->p.nonoverlap && h->istep < h->p.nstep_grow;         // 590
+    st = part_update_and_wrap(h->pb, pc, reloc, s);                           // 586, 590, 608
+    if (st) return st;
+    DDNS_HIP(hipEventRecord(h->ev[13], s));
+    if (h->p.nonoverlap) {                                                    // 610
+        int sw = 0, rl = 0;
+        st = part_overlap(h->pb, pc, &sw, &rl, s);
+        if (st) return st;
+        h->cc.noverlap_sweeps = sw;
+        h->cc.noverlap_relocated = rl;
+    }
+    DDNS_HIP(hipEventRecord(h->ev[14], s));
+    DDNS_HIP(hipEventSynchronize(h->ev[14]));
+    float t;
+    DDNS_HIP(hipEventElapsedTime(&t, h->ev[8], h->ev[9])); ms[0] += t;
+    DDNS_HIP(hipEventElapsedTime(&t, h->ev[9], h->ev[10])); ms[1] += t;
+    DDNS_HIP(hipEventElapsedTime(&t, h->ev[10], h->ev[11])); ms[2] += t;
+    DDNS_HIP(hipEventElapsedTime(&t, h->ev[11], h->ev[12])); ms[3] += t;
+    DDNS_HIP(hipEventElapsedTime(&t, h->ev[12], h->ev[13])); ms[4] += t;
+    DDNS_HIP(hipEventElapsedTime(&t, h->ev[13], h->ev[14])); ms[5] += t;
+    ddns_collision_counts c2{};
+    st = part_read_counts(h->pb, pc, &c2, s);
+    if (st) return st;
+    for (int k = 0; k < 3; ++k) h->cc.icc[k] = c2.icc[k];
+    std::memcpy(h->cc.colltype, c2.colltype, sizeof(c2.colltype));
+    std::memcpy(h->cc.nir, c2.nir, sizeof(c2.nir));
+    h->cc.npairs = c2.npairs;
+    h->cc.ncoll = c2.ncoll;
+    h->cc.nset = pc.nset;
+    h->cc.pairs_tested = c2.pairs_tested;
+    h->cc.pairs_in_dstat = c2.pairs_in_dstat;
+    return status;
+}
+
+extern "C" int ddns_step(ddns_handle* h, int nsteps)
+{
+    if (!h || nsteps < 0) return DDNS_ERR_ARG;
+    hipStream_t s = h->stream;
+    int status = 0, st;
+    float pms[6] = {0, 0, 0, 0, 0, 0};
+    double flow_ms = 0;
+    DDNS_HIP(hipEventRecord(h->ev[15], s));
+    for (int k = 0; k < nsteps; ++k) {
+        h->istep += 1;                                                        // turb.f90:492
+        DDNS_HIP(hipEventRecord(h->ev[0], s));
+        if (!h->a_valid) {
+            st = flow_prologue(h->fb, h->fc, h->prec, s);                     // curl + inverse z, 506-526
+            if (st) return keep(h, st);
+            h->a_valid = true;
+        }
+        st = flow_step(h->fb, h->fc, h->prec, false, s, nullptr);             // inverse y, x stage
+        if (st) return keep(h, st);
+        DDNS_HIP(hipEventRecord(h->ev[1], s));
+        h->phys_valid = true;
+        if (h->p.npart > 0 && h->n > 0) {
+            st = droplet_block(h, pms);
+            if (st < 0) return keep(h, st);
+            if (st > 0 && status == 0) status = st;
+            if (h->n < h->p.npart / 2) return fail(h, DDNS_ERR_DROPLET_LOSS, "npart_t < npart/2 (turb.f90:600-604)");
+        }
+        DDNS_HIP(hipEventRecord(h->ev[2], s));
+        st = flow_step(h->fb, h->fc, h->prec, true, s, nullptr);              // forward, update, 644-691
+        if (st) return keep(h, st);
+        DDNS_HIP(hipEventRecord(h->ev[3], s));
+        DDNS_HIP(hipEventSynchronize(h->ev[3]));
+        float t1, t2;
+        DDNS_HIP(hipEventElapsedTime(&t1, h->ev[0], h->ev[1]));
+        DDNS_HIP(hipEventElapsedTime(&t2, h->ev[2], h->ev[3]));
+        flow_ms += t1 + t2;
+        h->time += h->p.dt;                                                   // 693
+    }
+    DDNS_HIP(hipEventRecord(h->ev[7], s));
+    DDNS_HIP(hipEventSynchronize(h->ev[7]));
+    float tot = 0;
+    DDNS_HIP(hipEventElapsedTime(&tot, h->ev[15], h->ev[7]));
+    h->timing = ddns_timing{};
+    h->timing.flow_ms = flow_ms;
+    h->timing.cells_ms = pms[0];
+    h->timing.interp_ms = pms[1];
+    h->timing.hdi_ms = pms[2];
+    h->timing.advance_ms = pms[3];
+    h->timing.colldet_ms = pms[4];
+    h->timing.overlap_ms = pms[5];
+    h->timing.total_ms = tot;
+    h->timing.nsteps = nsteps;
+    return status;
+}
+
+extern "C" int ddns_get_collision_counts(ddns_handle* h, ddns_collision_counts* out)
+{
+    if (!h || !out) return DDNS_ERR_ARG;
+    *out = h->cc;
+    return DDNS_OK;
+}
+
+extern "C" int ddns_get_collision_bins(ddns_handle* h, int* nirbin, double* wrbin, double* wxbin, double* wybin,
+                                       double* wzbin, double* urbin, double* thetabin, double* wrtanbin)
+{
+    if (!h) return DDNS_ERR_ARG;
+    double* const b[7] = {wrbin, wxbin, wybin, wzbin, urbin, thetabin, wrtanbin};
+    return keep(h, part_read_bins(h->pb, h->pc, nirbin, b, h->stream));
+}
+
+extern "C" int ddns_get_collision_pairs(ddns_handle* h, int64_t* triples, int cap, int* npairs)
+{
+    if (!h || !npairs) return DDNS_ERR_ARG;
+    return keep(h, part_read_pairs(h->pb, h->pc, triples, cap, npairs, h->stream));
+}
+
+extern "C" int ddns_get_timing(ddns_handle* h, ddns_timing* out)
+{
+    if (!h || !out) return DDNS_ERR_ARG;
+    *out = h->timing;
+    return DDNS_OK;
+}
+
+// ------------------------------------------------------------------ kernel-level entry points
+extern "C" int ddns_fft_r2c(ddns_handle* h, double* v)
+{
+    if (!h || !v) return DDNS_ERR_ARG;
+    h->a_valid = false;
+    return keep(h, fft_r2c_host(h->fb, h->N, h->prec, v, h->stream));
+}
+
+extern "C" int ddns_fft_c2r(ddns_handle* h, double* v)
+{
+    if (!h || !v) return DDNS_ERR_ARG;
+    h->a_valid = false;
+    return keep(h, fft_c2r_host(h->fb, h->N, h->prec, v, h->stream));
+}
+
+extern "C" int ddns_collision_detect(ddns_handle* h, int n, const double* yp, const double* dmove,
+                                     const double* fvp0, const double* vnew, const double* vp)
+{
+    if (!h || n < 0 || !yp || !dmove || !fvp0 || !vnew || !vp) return DDNS_ERR_ARG;
+    if (n > h->pb.cap) return fail(h, DDNS_ERR_CAPACITY, "more droplets than capacity");
+    int st = ddns_set_droplets(h, n, yp, nullptr, nullptr, nullptr);
+    if (st) return st;
+    const int cap = h->pb.cap;
+    std::vector<double> a(3 * (size_t)cap, 0.0), b(3 * (size_t)cap, 0.0), c(3 * (size_t)cap, 0.0), d(3 * (size_t)cap, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            a[(size_t)k * cap + i] = dmove[3 * (size_t)i + k];
+            b[(size_t)k * cap + i] = fvp0[3 * (size_t)i + k];
+            c[(size_t)k * cap + i] = vnew[3 * (size_t)i + k];
+            d[(size_t)k * cap + i] = vp[3 * (size_t)i + k];
+        }
+    DDNS_HIP(hipMemcpy(h->pb.dmove, a.data(), a.size() * 8, hipMemcpyHostToDevice));
+    DDNS_HIP(hipMemcpy(h->pb.fvp, b.data(), b.size() * 8, hipMemcpyHostToDevice));
+    DDNS_HIP(hipMemcpy(h->pb.vel, c.data(), c.size() * 8, hipMemcpyHostToDevice));
+    DDNS_HIP(hipMemcpy(h->pb.vp, d.data(), d.size() * 8, hipMemcpyHostToDevice));
+    h->pc.n = n;
+    st = part_build_cells(h->pb, h->pc, cell_width(h), h->stream);
+    if (st) return keep(h, st);
+    st = part_collide(h->pb, h->pc, h->stream);
+    if (st) return keep(h, st);
+    ddns_collision_counts c2{};
+    st = part_read_counts(h->pb, h->pc, &c2, h->stream);
+    if (st) return keep(h, st);
+    h->cc = c2;
+    return DDNS_OK;
+}
+
+extern "C" int ddns_profile_flow(ddns_handle* h, int k, double* ms_per_kernel, int nkernels)
+{
+    if (!h || k <= 0 || !ms_per_kernel) return DDNS_ERR_ARG;
+    if (!h->a_valid) {
+        const int st = flow_prologue(h->fb, h->fc, h->prec, h->stream);
+        if (st) return keep(h, st);
+        h->a_valid = true;
+    }
+    return keep(h, flow_profile(h->fb, h->fc, h->prec, k, ms_per_kernel, nkernels, h->stream));
+}

@@ -1,0 +1,164 @@
+// ddns_internal.h -- internal declarations shared by the HIP translation units of
+// libddns.so: precision traits, complex helpers, device buffers of a handle and the
+// host-side launch wrappers of every kernel family.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/ddns.h"
+
+#define DDNS_HIP(call)                                                              \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess) return ddns::hip_fail(e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+namespace ddns {
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+void set_error(const std::string& msg);
+
+// ---------------------------------------------------------------- complex helpers
+template <typename T> struct cpx_t;
+template <> struct cpx_t<float>  { using type = float2; };
+template <> struct cpx_t<double> { using type = double2; };
+template <typename T> using cpx = typename cpx_t<T>::type;
+
+template <typename C> __host__ __device__ __forceinline__ C mk(decltype(C{}.x) a, decltype(C{}.x) b) { C r; r.x = a; r.y = b; return r; }
+template <typename C> __host__ __device__ __forceinline__ C cadd(C a, C b) { return mk<C>(a.x + b.x, a.y + b.y); }
+template <typename C> __host__ __device__ __forceinline__ C csub(C a, C b) { return mk<C>(a.x - b.x, a.y - b.y); }
+template <typename C> __host__ __device__ __forceinline__ C cmul(C a, C b) { return mk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+template <typename C> __host__ __device__ __forceinline__ C cconj(C a) { return mk<C>(a.x, -a.y); }
+template <typename C> __host__ __device__ __forceinline__ C cscale(C a, decltype(C{}.x) s) { return mk<C>(a.x * s, a.y * s); }
+// multiply by -i (forward) or +i (inverse)
+template <typename C, bool INV> __host__ __device__ __forceinline__ C cmul_mi(C a) { return INV ? mk<C>(-a.y, a.x) : mk<C>(a.y, -a.x); }
+
+// ---------------------------------------------------------------- flow layout
+// Spectral and intermediate fields: complex arrays [s][r][kx] with kx fastest,
+// kx in [0, N/2) (the kx = N/2 plane is identically zero and not stored), r the
+// y-or-ky index and s the z-or-kz index.  One field = N*N*(N/2) complex values.
+// Physical velocity for the droplets: real [z][y][x][3] interleaved.
+struct FlowBuffers {
+    void* U[3] = {nullptr, nullptr, nullptr};  // state u-hat [kz][ky][kx]
+    void* O[3] = {nullptr, nullptr, nullptr};  // AB2 history [kz][ky][kx]
+    void* A[6] = {};                            // inverse pipeline: u(3), w(3)
+    void* B[3] = {};                            // forward pipeline: (u x w)
+    void* uphys = nullptr;                      // [z][y][x][3]
+    void* tw = nullptr;                         // N twiddles exp(-2 pi i m / N)
+    void* twh = nullptr;                        // N/2 twiddles for the half-length x FFT
+    double* low = nullptr;                      // pre-kernel scratch: ff[2], energies, forcing table
+    int* sth_state = nullptr;                   // ranff state on the device: iseedf, iyf, ivf[32]
+    double* sth_b = nullptr;                    // b1r..b3r (450)
+};
+
+struct FlowConst {
+    int N;
+    int nek;
+    int forcing;
+    double dt, rnu;
+    double force0, force1;
+    double ou_tf, ou_var;
+    double pi2;
+};
+
+// launch wrappers (flow.hip)
+int flow_alloc(FlowBuffers& fb, int N, int precision);
+void flow_free(FlowBuffers& fb);
+int flow_prologue(const FlowBuffers& fb, const FlowConst& fc, int precision, hipStream_t s);
+int flow_step(const FlowBuffers& fb, const FlowConst& fc, int precision, bool last, hipStream_t s,
+              hipEvent_t* ev /* 7 events or nullptr */);
+int flow_upload_state(const FlowBuffers& fb, int N, int precision, const double* const src[6], hipStream_t s);
+int flow_download_state(const FlowBuffers& fb, int N, int precision, double* const dst[6], hipStream_t s);
+int flow_download_phys(const FlowBuffers& fb, int N, int precision, double* ux, double* uy, double* uz, hipStream_t s);
+int flow_spectrum(const FlowBuffers& fb, int N, int nek, int precision, double* ek, hipStream_t s);
+int fft_r2c_host(const FlowBuffers& fb, int N, int precision, double* v, hipStream_t s);
+int fft_c2r_host(const FlowBuffers& fb, int N, int precision, double* v, hipStream_t s);
+int flow_profile(const FlowBuffers& fb, const FlowConst& fc, int precision, int k, double* ms, int nk, hipStream_t s);
+
+// ---------------------------------------------------------------- droplets
+struct PartBuffers {
+    int cap = 0;
+    // state, SoA FP64, slot order = the reference's droplet index order
+    double* pos = nullptr;      // 3*cap
+    double* vel = nullptr;      // 3*cap   yp(4:6)
+    double* rad = nullptr;      // cap     yp(7)
+    double* gid = nullptr;      // cap     yp(8)
+    double* fvp = nullptr;      // 9*cap   fvp(:,0:2) as [t][c][cap]
+    double* fdp = nullptr;      // 12*cap  fdp(:,0:3) as [t][c][cap]
+    double* dmove = nullptr;    // 3*cap
+    double* vp = nullptr;       // 3*cap
+    double* pert = nullptr;     // 3*cap   pertvel
+    // cell grid
+    int G = 0;                  // coarse cells per dimension
+    double hc = 0;
+    int* cell_count = nullptr;  // G^3 + 1
+    int* cell_start = nullptr;  // G^3 + 1
+    int* cell_of = nullptr;     // cap
+    int* sorted = nullptr;      // cap
+    int* cursor = nullptr;      // G^3
+    void* scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
+    // HI (CSR)
+    int* nei_cnt = nullptr;     // cap + 1
+    int* nei_ptr = nullptr;     // cap + 1
+    int* nei_idx = nullptr;     // nnz_cap
+    double* nei_m = nullptr;    // 6 * nnz_cap
+    long long nnz_cap = 0;
+    void* gm_Q = nullptr;       // (maxiter+1) * 3 cap
+    void* gm_v = nullptr;       // 3 cap
+    double* gm_x0 = nullptr;    // 3 cap
+    double* gm_b = nullptr;     // 3 cap
+    double* gm_red = nullptr;   // reduction partials
+    double* gm_small = nullptr; // Hessenberg etc
+    // collisions
+    int* mult = nullptr;        // cap: removal-list multiplicity
+    int* flags = nullptr;       // device flags / counters (64 ints)
+    long long* tested = nullptr;
+    int* nirbin = nullptr;
+    double* bins = nullptr;     // 7 * nset*nset*NBIN
+    int* counts = nullptr;      // icc[3], colltype[100], nir[100]
+    long long* pairs = nullptr; // pair_cap * 4: key fields
+    int pair_cap = 0;
+    double* draws = nullptr;    // relocation draws
+    int draw_cap = 0;
+    int* draw_off = nullptr;    // cap + 1
+    int* rng_state = nullptr;   // ranff2 state: idum, iy, idum2, iv[32]
+    double* red = nullptr;      // generic reduction scratch
+};
+
+struct PartConst {
+    int N;            // flow grid (reference cell grid: ncdx = N)
+    int n;            // droplets
+    int nset;
+    double radius[DDNS_MAXSET];
+    double radmax;
+    double dt, rnu, st;
+    double Sv_const, tau2;
+    double pi2;
+    double d_coll, d_hdi, hdi_trunc;
+    int nx_coll, nx_max, imax_stat;
+    int hdi, nonoverlap, ndrag;
+    int new_particle;
+    int precision;
+};
+
+int part_alloc(PartBuffers& pb, int cap, int nset, int G);
+void part_free(PartBuffers& pb);
+int part_interp(const PartBuffers& pb, const PartConst& pc, const void* uphys, hipStream_t s);
+int part_history_fvp(const PartBuffers& pb, const PartConst& pc, hipStream_t s);
+int part_build_cells(PartBuffers& pb, const PartConst& pc, double hmin, hipStream_t s);
+int part_hdi(PartBuffers& pb, const PartConst& pc, int maxiter, double tol, int* iters, double* res, hipStream_t s);
+int part_advance(const PartBuffers& pb, const PartConst& pc, hipStream_t s);
+int part_collide(PartBuffers& pb, const PartConst& pc, hipStream_t s);
+int part_update_and_wrap(const PartBuffers& pb, const PartConst& pc, bool relocate, hipStream_t s);
+int part_overlap(PartBuffers& pb, const PartConst& pc, int* sweeps, int* relocated, hipStream_t s);
+int part_read_counts(const PartBuffers& pb, const PartConst& pc, ddns_collision_counts* cc, hipStream_t s);
+int part_read_bins(const PartBuffers& pb, const PartConst& pc, int* nirbin, double* const bins[7], hipStream_t s);
+int part_read_pairs(PartBuffers& pb, const PartConst& pc, int64_t* triples, int cap, int* npairs, hipStream_t s);
+int part_set_rng(const PartBuffers& pb, int iseedr, hipStream_t s);
+
+}  // namespace ddns

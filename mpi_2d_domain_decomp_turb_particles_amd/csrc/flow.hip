@@ -1,0 +1,1080 @@
+// flow.hip -- spectral Navier-Stokes step on one MI355X.
+//
+// Reference: turb.f90:506-691 (curl, inverse FFTs, u x w, forward FFTs, dealiasing,
+// AB2/Crank-Nicolson update, projection, forcing, symmetrize), mpifft.f90:81-198
+// (the 3-D real FFT as x / y / z line transforms with a 1/N scale after each
+// forward stage), flowlib.f90:180-562 (dealiasing, supforDET, symmetrize,
+// supforSTH).
+//
+// Layout: every complex field is [s][r][kx], kx fastest with kx < N/2, so the x
+// stage reads contiguous lines and the y and z stages read tiles of TC adjacent kx
+// columns (one 128-256 byte segment per row).  The step runs as five launches:
+//   Ky(inverse, 6 fields, kx = 0 column symmetrised)  ->  Kx(6 C2R, u x w, 3 R2C,
+//   physical u for the droplets)  ->  Ky(forward, 3 fields)  ->  Kpre (one block:
+//   deterministic-forcing shell energies or the OU forcing table)  ->  Kz(forward z
+//   FFT, update, curl, inverse z FFT of u and w)  ->  Ksym (kx = 0 plane of u-hat).
+// The z stage therefore closes step n and opens step n + 1 in one pass over HBM.
+#include "fft_engine.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace ddns {
+
+constexpr int NT = 256;
+
+enum { KZ_FULL = 0, KZ_CURL_INV = 1, KZ_PLAIN_FWD = 2, KZ_PLAIN_INV = 3 };
+
+// tile widths (kx columns per work-group) for the y and z stages
+template <typename T, int N> constexpr int ky_tc() { return (sizeof(T) == 4) ? (N >= 1024 ? 8 : 16) : (N >= 1024 ? 4 : 8); }
+template <typename T, int N> constexpr int kz_tc()
+{
+    if constexpr (sizeof(T) == 4) return N <= 256 ? 16 : (N == 512 ? 8 : 4);
+    else return N <= 256 ? 8 : (N == 512 ? 4 : 2);
+}
+template <int TC> constexpr int ldr_of() { return (TC >= 8) ? TC + 1 : TC; }
+template <typename T, int N> constexpr int kz_ldr()
+{
+    constexpr int TC = kz_tc<T, N>();
+    return (4 * N * (TC + 1) * (int)sizeof(cpx<T>) < 160 * 1024) ? TC + 1 : TC;
+}
+
+__device__ __forceinline__ int wave_index(int i, int N) { return i <= N / 2 ? i : i - N; }
+
+struct KzArgs {
+    const void* B[3];
+    void* U[3];
+    void* O[3];
+    void* A[6];
+    const void* tw;
+    const double* low;
+    double dt, rnu;
+    int nek, forcing;
+    double scale;      // plain modes: output scale
+};
+
+struct KyArgs {
+    void* F[6];
+    const void* tw;
+    double scale;
+};
+
+struct KxArgs {
+    const void* A[6];
+    void* B[3];
+    void* uphys;
+    const void* tw;    // N entries (post/pre-processing twiddles)
+    const void* twh;   // N/2 entries (half-length FFT)
+};
+
+// ------------------------------------------------------------------ spectral update
+// One mode of turb.f90:662-687 for the three components held in n[3] (forward FFT
+// output, already scaled) and the state u[3], o[3]; returns the new u-hat in v[3]
+// and the dealiased nonlinear term in nd[3].  ff: supforDET factors; fa: the
+// supforSTH table a(6,5,5) per component.
+template <typename T>
+__device__ __forceinline__ void update_mode(int kx, int ky, int kz, int N, int nek, int forcing,
+                                            T dt, T rnu, const double* __restrict__ low,
+                                            const cpx<T>* n, const cpx<T>* u, const cpx<T>* o,
+                                            cpx<T>* v, cpx<T>* nd)
+{
+    using C = cpx<T>;
+    const int kyw = wave_index(ky, N), kzw = wave_index(kz, N);
+    const T fx = (T)kx, fy = (T)kyw, fz = (T)kzw;
+    T k2 = fx * fx + fy * fy + fz * fz;
+    const bool zero_mode = (kx == 0 && kyw == 0 && kzw == 0);
+    if (zero_mode) k2 = (T)1.e-5;                                   // flowlib.f90:36-38
+    const int ik2 = (int)(sqrt(k2) + (T)0.5);                       // turb.f90:405
+    const bool keep = ik2 <= nek && !zero_mode;
+    const T dt_h = (T)0.5 * dt, dtr = (T)1.5 * dt;                   // turb.f90:483-484
+    const T ac = rnu * dt_h * k2;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        nd[c] = (ik2 <= nek) ? n[c] : mk<C>((T)0, (T)0);             // dealiasing, 648-650
+        v[c] = csub(cadd(cscale(u[c], (T)1 - ac), cscale(nd[c], dtr)), cscale(o[c], dt_h));
+    }
+    if (forcing == 2 && kx < 3) {                                    // supforSTH add, flowlib.f90:529-559
+        const int js = ky <= 2 ? ky : (ky >= N - 2 ? ky - (N - 5) : -1);
+        const int ks = kz <= 2 ? kz : (kz >= N - 2 ? kz - (N - 5) : -1);
+        if (js >= 0 && ks >= 0) {
+            const double* fa = low + 8;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int o0 = 2 * kx + 6 * (js + 5 * ks);
+                v[c].x = v[c].x + dt * (T)fa[150 * c + o0];
+                v[c].y = v[c].y + dt * (T)fa[150 * c + o0 + 1];
+            }
+        }
+    }
+    // projection, 669-672
+    const C t = cscale(cadd(cadd(cscale(v[0], fx), cscale(v[1], fy)), cscale(v[2], fz)), (T)1 / k2);
+    v[0] = csub(v[0], cscale(t, fx));
+    v[1] = csub(v[1], cscale(t, fy));
+    v[2] = csub(v[2], cscale(t, fz));
+    const T den = (T)1 + ac;                                         // 674-676
+    T f = (T)1 / den;
+    if (forcing == 1 && (ik2 == 1 || ik2 == 2)) f = f * (T)low[ik2 - 1];   // supforDET
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        v[c] = keep ? cscale(v[c], f) : mk<C>((T)0, (T)0);
+    }
+}
+
+// ------------------------------------------------------------------ z stage
+template <typename T, int N, int MODE>
+__global__ void __launch_bounds__(NT) k_z(KzArgs a)
+{
+    using C = cpx<T>;
+    constexpr int M = N / 2;
+    constexpr int TC = kz_tc<T, N>();
+    constexpr int LDR = kz_ldr<T, N>();
+    constexpr int NTILE = (MODE == KZ_FULL || MODE == KZ_CURL_INV) ? 4 : 1;
+    __shared__ C lds[NTILE][N * LDR];
+    const int tid = threadIdx.x;
+    const int kx0 = blockIdx.x * TC;
+    const int ky = blockIdx.y;
+    const C* tw = (const C*)a.tw;
+    const size_t rowstride = (size_t)N * M;
+    const size_t base = (size_t)ky * M + kx0;
+
+    if constexpr (MODE == KZ_PLAIN_FWD || MODE == KZ_PLAIN_INV) {
+        const C* src = (const C*)a.B[0];
+        C* dst = (C*)a.U[0];
+        for (int e = tid; e < TC * N; e += NT) {
+            const int n = e / TC, c = e % TC;
+            lds[0][n * LDR + c] = src[base + n * rowstride + c];
+        }
+        __syncthreads();
+        fft_tile<T, N, MODE == KZ_PLAIN_INV, TC, LDR, NT>(lds[0], tw, tid);
+        const T sc = (T)a.scale;
+        for (int e = tid; e < TC * N; e += NT) {
+            const int n = e / TC, c = e % TC;
+            dst[base + n * rowstride + c] = cscale(lds[0][n * LDR + c], sc);
+        }
+        return;
+    } else {
+        if constexpr (MODE == KZ_FULL) {
+            for (int e = tid; e < TC * N; e += NT) {
+                const int n = e / TC, c = e % TC;
+                const size_t g = base + n * rowstride + c;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) lds[q][n * LDR + c] = ((const C*)a.B[q])[g];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 3; ++q) fft_tile<T, N, false, TC, LDR, NT>(lds[q], tw, tid);
+            const T inv_n = (T)1 / (T)N;
+            for (int e = tid; e < TC * N; e += NT) {
+                const int n = e / TC, c = e % TC;
+                const size_t g = base + n * rowstride + c;
+                C nn[3], uu[3], oo[3], vv[3], nd[3];
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    nn[q] = cscale(lds[q][n * LDR + c], inv_n);
+                    uu[q] = ((const C*)a.U[q])[g];
+                    oo[q] = ((const C*)a.O[q])[g];
+                }
+                update_mode<T>(kx0 + c, ky, n, N, a.nek, a.forcing, (T)a.dt, (T)a.rnu, a.low, nn, uu, oo, vv, nd);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    ((C*)a.U[q])[g] = vv[q];
+                    ((C*)a.O[q])[g] = nd[q];
+                    lds[q][n * LDR + c] = vv[q];
+                }
+            }
+        } else {
+            for (int e = tid; e < TC * N; e += NT) {
+                const int n = e / TC, c = e % TC;
+                const size_t g = base + n * rowstride + c;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) lds[q][n * LDR + c] = ((const C*)a.U[q])[g];
+            }
+        }
+        __syncthreads();
+        // vorticity w = i k x u (turb.f90:506-517), one component at a time in tile 3
+#pragma unroll 1
+        for (int q = 0; q < 3; ++q) {
+            for (int e = tid; e < TC * N; e += NT) {
+                const int n = e / TC, c = e % TC;
+                const T fx = (T)(kx0 + c), fy = (T)wave_index(ky, N), fz = (T)wave_index(n, N);
+                const C ux = lds[0][n * LDR + c], uy = lds[1][n * LDR + c], uz = lds[2][n * LDR + c];
+                C w;
+                if (q == 0) w = csub(cscale(uz, fy), cscale(uy, fz));
+                else if (q == 1) w = csub(cscale(ux, fz), cscale(uz, fx));
+                else w = csub(cscale(uy, fx), cscale(ux, fy));
+                lds[3][n * LDR + c] = mk<C>(-w.y, w.x);
+            }
+            __syncthreads();
+            fft_tile<T, N, true, TC, LDR, NT>(lds[3], tw, tid);
+            C* dst = (C*)a.A[3 + q];
+            for (int e = tid; e < TC * N; e += NT) {
+                const int n = e / TC, c = e % TC;
+                dst[base + n * rowstride + c] = lds[3][n * LDR + c];
+            }
+            __syncthreads();
+        }
+#pragma unroll 1
+        for (int q = 0; q < 3; ++q) {
+            fft_tile<T, N, true, TC, LDR, NT>(lds[q], tw, tid);
+            C* dst = (C*)a.A[q];
+            for (int e = tid; e < TC * N; e += NT) {
+                const int n = e / TC, c = e % TC;
+                dst[base + n * rowstride + c] = lds[q][n * LDR + c];
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ y stage
+template <typename T, int N, bool INV, bool SYM>
+__global__ void __launch_bounds__(NT) k_y(KyArgs a)
+{
+    using C = cpx<T>;
+    constexpr int M = N / 2;
+    constexpr int TC = ky_tc<T, N>();
+    constexpr int LDR = ldr_of<TC>();
+    __shared__ C lds[N * LDR];
+    const int tid = threadIdx.x;
+    const int kx0 = blockIdx.x * TC;
+    const int z = blockIdx.y;
+    C* f = (C*)a.F[blockIdx.z];
+    const size_t base = (size_t)z * N * M + kx0;
+    for (int e = tid; e < TC * N; e += NT) {
+        const int n = e / TC, c = e % TC;
+        lds[n * LDR + c] = f[base + (size_t)n * M + c];
+    }
+    __syncthreads();
+    if constexpr (SYM) {
+        // kx = 0 column: (c(ky) + conj c(-ky)) / 2, the z-transformed form of the
+        // kx = 0 plane symmetrisation of flowlib.f90:264-329
+        if (kx0 == 0) {
+            for (int k = tid; k <= N / 2; k += NT) {
+                const int km = (N - k) & (N - 1);
+                const C p = lds[k * LDR], m = lds[km * LDR];
+                const C s = mk<C>((T)0.5 * (p.x + m.x), (T)0.5 * (p.y - m.y));
+                lds[k * LDR] = s;
+                lds[km * LDR] = cconj(s);
+            }
+            __syncthreads();
+        }
+    }
+    fft_tile<T, N, INV, TC, LDR, NT>(lds, (const C*)a.tw, tid);
+    const T sc = (T)a.scale;
+    for (int e = tid; e < TC * N; e += NT) {
+        const int n = e / TC, c = e % TC;
+        f[base + (size_t)n * M + c] = INV ? lds[n * LDR + c] : cscale(lds[n * LDR + c], sc);
+    }
+}
+
+// ------------------------------------------------------------------ x stage
+// Half-length packing of a real line of N points as N/2 complex points.
+template <int N> constexpr int kx_lines() { return 2048 / N; }   // 4 complex points per thread
+
+// C2R pre-processing of the tile (in place, pairs k and M-k of every line).
+template <typename T, int N, int TL, int LDR>
+__device__ __forceinline__ void c2r_pack(cpx<T>* lds, const cpx<T>* tw, int tid)
+{
+    using C = cpx<T>;
+    constexpr int M = N / 2;
+    for (int e = tid; e < TL * (M / 2 + 1); e += NT) {
+        const int line = e / (M / 2 + 1), k = e % (M / 2 + 1);
+        if (k == 0) {
+            const C x0 = lds[line];
+            lds[line] = mk<C>(x0.x, x0.x);               // X(N/2) = 0, Im X(0) ignored
+        } else {
+            const int k2 = M - k;
+            const C xa = lds[k * LDR + line], xb = lds[k2 * LDR + line];
+            C w = tw[k];                                  // exp(-2 pi i k / N)
+            w.y = -w.y;                                   // exp(+2 pi i k / N)
+            const C sa = cadd(xa, cconj(xb)), da = csub(xa, cconj(xb));
+            const C za = cadd(sa, cmul(mk<C>(-w.y, w.x), da));       // + i w (xa - conj xb)
+            if (k2 != k) {
+                const C wb = mk<C>(-w.x, w.y);            // exp(+2 pi i (M-k)/N) = -conj(w)
+                const C sb = cadd(xb, cconj(xa)), db = csub(xb, cconj(xa));
+                const C zb = cadd(sb, cmul(mk<C>(-wb.y, wb.x), db));
+                lds[k2 * LDR + line] = zb;
+            }
+            lds[k * LDR + line] = za;
+        }
+    }
+}
+
+// R2C post-processing: from Z = FFT_{N/2}(x even + i x odd) to X(k), k < N/2, scaled.
+template <typename T, int N, int TL, int LDR>
+__device__ __forceinline__ void r2c_unpack_store(const cpx<T>* lds, const cpx<T>* tw, cpx<T>* dst,
+                                                 size_t line0, T scale, int tid)
+{
+    using C = cpx<T>;
+    constexpr int M = N / 2;
+    for (int e = tid; e < TL * (M / 2 + 1); e += NT) {
+        const int line = e / (M / 2 + 1), k = e % (M / 2 + 1);
+        C* out = dst + (line0 + line) * M;
+        if (k == 0) {
+            const C z0 = lds[line];
+            out[0] = mk<C>((z0.x + z0.y) * scale, (T)0);
+        } else {
+            const int k2 = M - k;
+            const C za = lds[k * LDR + line], zb = lds[k2 * LDR + line];
+            const C w = tw[k];                            // exp(-2 pi i k / N)
+            const C e1 = cadd(za, cconj(zb)), d1 = csub(za, cconj(zb));
+            // X = e1/2 - (i/2) w d1
+            const C wd = cmul(w, d1);
+            const C xa = mk<C>((T)0.5 * (e1.x + wd.y), (T)0.5 * (e1.y - wd.x));
+            out[k] = cscale(xa, scale);
+            if (k2 != k) {
+                const C wb = mk<C>(-w.x, w.y);            // exp(-2 pi i (M-k)/N) = -conj(w)
+                const C e2 = cadd(zb, cconj(za)), d2 = csub(zb, cconj(za));
+                const C wd2 = cmul(wb, d2);
+                const C xb = mk<C>((T)0.5 * (e2.x + wd2.y), (T)0.5 * (e2.y - wd2.x));
+                out[k2] = cscale(xb, scale);
+            }
+        }
+    }
+}
+
+// Fused x stage: six C2R (u, w), physical u for the droplets, u x w, three R2C.
+template <typename T, int N>
+__global__ void __launch_bounds__(NT) k_x_fused(KxArgs a)
+{
+    using C = cpx<T>;
+    constexpr int M = N / 2;
+    constexpr int TL = kx_lines<N>();
+    constexpr int LDR = TL + 1;
+    constexpr int QP = TL * M / NT;
+    __shared__ C lds[M * LDR];
+    const int tid = threadIdx.x;
+    const size_t line0 = (size_t)blockIdx.x * TL;
+    const C* tw = (const C*)a.tw;
+    const C* twh = (const C*)a.twh;
+    C ph[6][QP];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        const C* src = (const C*)a.A[f] + line0 * M;
+        for (int e = tid; e < TL * M; e += NT) {
+            const int line = e / M, m = e % M;
+            lds[m * LDR + line] = src[(size_t)line * M + m];
+        }
+        __syncthreads();
+        c2r_pack<T, N, TL, LDR>(lds, tw, tid);
+        __syncthreads();
+        fft_tile<T, M, true, TL, LDR, NT>(lds, twh, tid);
+#pragma unroll
+        for (int q = 0; q < QP; ++q) {
+            const int e = tid + NT * q;
+            const int line = e / M, m = e % M;
+            ph[f][q] = lds[m * LDR + line];
+        }
+        __syncthreads();
+    }
+    // physical velocity for the droplet gather: [line][x][3]
+    {
+        T* up = (T*)a.uphys + line0 * (size_t)N * 3;
+#pragma unroll
+        for (int q = 0; q < QP; ++q) {
+            const int e = tid + NT * q;
+            const int line = e / M, m = e % M;
+            T* p = up + ((size_t)line * N + 2 * m) * 3;
+            p[0] = ph[0][q].x; p[1] = ph[1][q].x; p[2] = ph[2][q].x;
+            p[3] = ph[0][q].y; p[4] = ph[1][q].y; p[5] = ph[2][q].y;
+        }
+    }
+    // nonlinear term u x w (turb.f90:634-637) in natural component order
+    C nl[3][QP];
+#pragma unroll
+    for (int q = 0; q < QP; ++q) {
+        const C ux = ph[0][q], uy = ph[1][q], uz = ph[2][q];
+        const C wx = ph[3][q], wy = ph[4][q], wz = ph[5][q];
+        nl[0][q] = mk<C>(uy.x * wz.x - uz.x * wy.x, uy.y * wz.y - uz.y * wy.y);
+        nl[1][q] = mk<C>(uz.x * wx.x - ux.x * wz.x, uz.y * wx.y - ux.y * wz.y);
+        nl[2][q] = mk<C>(ux.x * wy.x - uy.x * wx.x, ux.y * wy.y - uy.y * wx.y);
+    }
+    const T scale = (T)1 / (T)N;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int q = 0; q < QP; ++q) {
+            const int e = tid + NT * q;
+            const int line = e / M, m = e % M;
+            lds[m * LDR + line] = nl[c][q];
+        }
+        __syncthreads();
+        fft_tile<T, M, false, TL, LDR, NT>(lds, twh, tid);
+        r2c_unpack_store<T, N, TL, LDR>(lds, tw, (C*)a.B[c], line0, scale, tid);
+        __syncthreads();
+    }
+}
+
+// Stand-alone x transforms of one field (kernel-level parity of mpifft3DRC/CR).
+template <typename T, int N, bool INV>
+__global__ void __launch_bounds__(NT) k_x_plain(const void* src, void* dst, const void* tw_, const void* twh_)
+{
+    using C = cpx<T>;
+    constexpr int M = N / 2;
+    constexpr int TL = kx_lines<N>();
+    constexpr int LDR = TL + 1;
+    __shared__ C lds[M * LDR];
+    const int tid = threadIdx.x;
+    const size_t line0 = (size_t)blockIdx.x * TL;
+    const C* tw = (const C*)tw_;
+    const C* twh = (const C*)twh_;
+    // both directions see the line as M complex values (real pairs for the C2R output / R2C input)
+    const C* s = (const C*)src + line0 * M;
+    for (int e = tid; e < TL * M; e += NT) {
+        const int line = e / M, m = e % M;
+        lds[m * LDR + line] = s[(size_t)line * M + m];
+    }
+    __syncthreads();
+    if constexpr (INV) {
+        c2r_pack<T, N, TL, LDR>(lds, tw, tid);
+        __syncthreads();
+        fft_tile<T, M, true, TL, LDR, NT>(lds, twh, tid);
+        C* d = (C*)dst + line0 * M;
+        for (int e = tid; e < TL * M; e += NT) {
+            const int line = e / M, m = e % M;
+            d[(size_t)line * M + m] = lds[m * LDR + line];
+        }
+    } else {
+        fft_tile<T, M, false, TL, LDR, NT>(lds, twh, tid);
+        r2c_unpack_store<T, N, TL, LDR>(lds, tw, (C*)dst, line0, (T)1 / (T)N, tid);
+    }
+}
+
+// ------------------------------------------------------------------ low-mode pre-kernel
+// DET (forcing = 1): the new value of every mode with |k| < 2.5 is computed from a
+// direct z-DFT of the y-transformed nonlinear term, then the shell-1 and shell-2
+// energies (kx = 0 halved, flowlib.f90:207-217) give ff = sqrt(force/E); low[0..1].
+// STH (forcing = 2): ranff white noise, OU step, projection and Hermitian fix-up
+// (flowlib.f90:399-510) into low[8 .. 8+450).
+struct PreArgs {
+    const void* B[3];
+    const void* U[3];
+    const void* O[3];
+    const void* tw;
+    double* low;
+    int* sth_state;
+    double* sth_b;
+    double dt, rnu, force0, force1, ou_tf, ou_var, pi2;
+    int nek, forcing;
+};
+
+__device__ double dev_ranff(int* idum, int* iy, int* iv)
+{
+    const int IA = 16807, IM = 2147483647, IQ = 127773, IR = 2836, NTAB_ = 32;
+    const int NDIV = 1 + (IM - 1) / NTAB_;
+    const double AM = 1.0 / IM, EPS = 1.2e-7, RNMX = 1.0 - EPS;
+    int j, k;
+    if (*idum <= 0 || *iy == 0) {
+        *idum = (-*idum > 1) ? -*idum : 1;
+        for (j = NTAB_ + 8; j >= 1; --j) {
+            k = *idum / IQ;
+            *idum = IA * (*idum - k * IQ) - IR * k;
+            if (*idum < 0) *idum += IM;
+            if (j <= NTAB_) iv[j - 1] = *idum;
+        }
+        *iy = iv[0];
+    }
+    k = *idum / IQ;
+    *idum = IA * (*idum - k * IQ) - IR * k;
+    if (*idum < 0) *idum += IM;
+    j = 1 + *iy / NDIV;
+    *iy = iv[j - 1];
+    iv[j - 1] = *idum;
+    const double r = AM * (double)(*iy);
+    return r < RNMX ? r : RNMX;
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__(NT) k_pre(PreArgs a)
+{
+    using C = cpx<T>;
+    constexpr int M = N / 2;
+    const int tid = threadIdx.x;
+    if (a.forcing == 2) {
+        if (tid != 0) return;
+        int* st = a.sth_state;
+        int idum = st[0], iy = st[1];
+        int iv[32];
+        for (int i = 0; i < 32; ++i) iv[i] = st[2 + i];
+        const double dt = a.dt, tf = a.ou_tf, var = a.ou_var, pi2 = a.pi2;
+        double fr[3][150];
+        for (int mc = 0; mc < 3; ++mc)                                   // flowlib.f90:405-419
+            for (int k = 0; k < 5; ++k)
+                for (int j = 0; j < 5; ++j)
+                    for (int i = 0; i < 6; i += 2) {
+                        double cont1 = dev_ranff(&idum, &iy, iv);
+                        if (cont1 < 1.e-6) cont1 = 1.e-6;
+                        const double rmag = sqrt(-4.0 * var * dt * log(cont1) / tf);
+                        const double rph = dev_ranff(&idum, &iy, iv);
+                        fr[mc][i + 6 * (j + 5 * k)] = rmag * cos(pi2 * rph);
+                        fr[mc][i + 1 + 6 * (j + 5 * k)] = rmag * sin(pi2 * rph);
+                    }
+        for (int mc = 0; mc < 3; ++mc) {                                 // 424-440
+            for (int k = 1; k <= 5; ++k) {
+                const int kk = (k == 1) ? 1 : 7 - k;
+                for (int j = 2; j <= 3; ++j) {
+                    const int jj = 7 - j;
+                    fr[mc][0 + 6 * ((jj - 1) + 5 * (kk - 1))] = fr[mc][0 + 6 * ((j - 1) + 5 * (k - 1))];
+                    fr[mc][1 + 6 * ((jj - 1) + 5 * (kk - 1))] = -fr[mc][1 + 6 * ((j - 1) + 5 * (k - 1))];
+                }
+            }
+            for (int k = 2; k <= 3; ++k) {
+                const int kk = 7 - k;
+                fr[mc][0 + 6 * (5 * (kk - 1))] = fr[mc][0 + 6 * (5 * (k - 1))];
+                fr[mc][1 + 6 * (5 * (kk - 1))] = -fr[mc][1 + 6 * (5 * (k - 1))];
+            }
+            fr[mc][0] = 0.0;
+            fr[mc][1] = 0.0;
+        }
+        double* b = a.sth_b;
+        double* fa = a.low + 8;
+        const double smallx = 1.0e-18, xkf = sqrt(8.0);
+        for (int k = 0; k < 5; ++k)                                      // 447-478
+            for (int j = 0; j < 5; ++j)
+                for (int i = 0; i < 6; ++i) {
+                    const int o = i + 6 * (j + 5 * k);
+                    const double kx_ = (double)(i / 2);
+                    const double ky_ = (double)(j <= 2 ? j : j - 5);
+                    const double kz_ = (double)(k <= 2 ? k : k - 5);
+                    const double radm = sqrt(kx_ * kx_ + ky_ * ky_ + kz_ * kz_);
+                    if (radm < xkf) {
+                        b[o] = b[o] * (1.0 - dt / tf) + fr[0][o];
+                        b[150 + o] = b[150 + o] * (1.0 - dt / tf) + fr[1][o];
+                        b[300 + o] = b[300 + o] * (1.0 - dt / tf) + fr[2][o];
+                        const double t1 = kx_ * b[o], t2 = ky_ * b[150 + o], t3 = kz_ * b[300 + o];
+                        double divb = (t1 + t2 + t3) / ((radm + smallx) * (radm + smallx));
+                        if (radm == 0.) divb = 0.0;
+                        fa[o] = b[o] - kx_ * divb;
+                        fa[150 + o] = b[150 + o] - ky_ * divb;
+                        fa[300 + o] = b[300 + o] - kz_ * divb;
+                    } else {
+                        fa[o] = 0.0; fa[150 + o] = 0.0; fa[300 + o] = 0.0;
+                    }
+                }
+        for (int mc = 0; mc < 3; ++mc) { fa[150 * mc] = 0.0; fa[150 * mc + 1] = 0.0; }   // 481-483
+        for (int k = 1; k <= 5; ++k) {                                   // 488-500
+            const int kk = (k == 1) ? 1 : 7 - k;
+            for (int j = 2; j <= 3; ++j) {
+                const int jj = 7 - j;
+                for (int mc = 0; mc < 3; ++mc) {
+                    fa[150 * mc + 6 * ((jj - 1) + 5 * (kk - 1))] = fa[150 * mc + 6 * ((j - 1) + 5 * (k - 1))];
+                    fa[150 * mc + 1 + 6 * ((jj - 1) + 5 * (kk - 1))] = -fa[150 * mc + 1 + 6 * ((j - 1) + 5 * (k - 1))];
+                }
+            }
+        }
+        for (int k = 2; k <= 3; ++k) {                                   // 502-510
+            const int kk = 7 - k;
+            for (int mc = 0; mc < 3; ++mc) {
+                fa[150 * mc + 6 * (5 * (kk - 1))] = fa[150 * mc + 6 * (5 * (k - 1))];
+                fa[150 * mc + 1 + 6 * (5 * (kk - 1))] = -fa[150 * mc + 1 + 6 * (5 * (k - 1))];
+            }
+        }
+        st[0] = idum; st[1] = iy;
+        for (int i = 0; i < 32; ++i) st[2 + i] = iv[i];
+        return;
+    }
+    if (a.forcing != 1) return;
+    // 75 low modes: kx in 0..2, ky, kz in -2..2; 225 (mode, component) DFT sums
+    __shared__ double nre[225], nim[225];
+    __shared__ double emode[75];
+    const C* tw = (const C*)a.tw;
+    if (tid < 225) {
+        const int comp = tid / 75, mi = tid % 75;
+        const int kx = mi % 3, kyw = (mi / 3) % 5 - 2, kzw = mi / 15 - 2;
+        const int ky = (kyw + N) % N;
+        const int kzi = (kzw + N) % N;
+        const C* b = (const C*)a.B[comp];
+        double sr = 0.0, si = 0.0;
+        for (int z = 0; z < N; ++z) {
+            const C v = b[((size_t)z * N + ky) * M + kx];
+            const C w = tw[(kzi * z) & (N - 1)];
+            sr += (double)v.x * (double)w.x - (double)v.y * (double)w.y;
+            si += (double)v.x * (double)w.y + (double)v.y * (double)w.x;
+        }
+        nre[tid] = sr / N;
+        nim[tid] = si / N;
+    }
+    __syncthreads();
+    if (tid < 75) {
+        const int kx = tid % 3, kyw = (tid / 3) % 5 - 2, kzw = tid / 15 - 2;
+        const int ky = (kyw + N) % N, kz = (kzw + N) % N;
+        const size_t g = ((size_t)kz * N + ky) * M + kx;
+        C nn[3], uu[3], oo[3], vv[3], nd[3];
+        for (int q = 0; q < 3; ++q) {
+            nn[q] = mk<C>((T)nre[q * 75 + tid], (T)nim[q * 75 + tid]);
+            uu[q] = ((const C*)a.U[q])[g];
+            oo[q] = ((const C*)a.O[q])[g];
+        }
+        // the update without the forcing factor (ff = 1 here)
+        double one[2] = {1.0, 1.0};
+        update_mode<T>(kx, ky, kz, N, a.nek, 0, (T)a.dt, (T)a.rnu, one, nn, uu, oo, vv, nd);
+        const double k2 = (double)(kx * kx + kyw * kyw + kzw * kzw);
+        const int ik2 = (int)(sqrt(k2 == 0.0 ? 1.e-5 : k2) + 0.5);
+        double e = 0.0;
+        if (ik2 == 1 || ik2 == 2) {
+            for (int q = 0; q < 3; ++q) e += (double)vv[q].x * (double)vv[q].x + (double)vv[q].y * (double)vv[q].y;
+            if (kx == 0) e *= 0.5;
+            e = (ik2 == 1) ? e : -e;      // sign tags the shell
+        }
+        emode[tid] = e;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double e1 = 0.0, e2 = 0.0;
+        for (int i = 0; i < 75; ++i) {
+            if (emode[i] > 0) e1 += emode[i];
+            else e2 -= emode[i];
+        }
+        a.low[0] = sqrt(a.force0 / e1);
+        a.low[1] = sqrt(a.force1 / e2);
+        a.low[2] = e1;
+        a.low[3] = e2;
+    }
+}
+
+// ------------------------------------------------------------------ kx = 0 plane of u-hat
+template <typename T, int N>
+__global__ void k_sym(void* u0, void* u1, void* u2)
+{
+    using C = cpx<T>;
+    constexpr int M = N / 2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * N) return;
+    const int kz = t / N, ky = t % N;
+    const int mz = (N - kz) & (N - 1), my = (N - ky) & (N - 1);
+    if (mz * N + my < t) return;           // the partner with the smaller index does the pair
+    void* fs[3] = {u0, u1, u2};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        C* f = (C*)fs[q];
+        const size_t gp = ((size_t)kz * N + ky) * M, gm = ((size_t)mz * N + my) * M;
+        const C p = f[gp], m = f[gm];
+        const C s = mk<C>((T)0.5 * (p.x + m.x), (T)0.5 * (p.y - m.y));
+        f[gp] = s;
+        f[gm] = cconj(s);
+    }
+}
+
+// ------------------------------------------------------------------ energy spectrum
+template <typename T, int N>
+__global__ void k_spectrum(const void* u0, const void* u1, const void* u2, int nek, double* ek)
+{
+    using C = cpx<T>;
+    constexpr int M = N / 2;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)N * N * M) return;
+    const int kx = (int)(t % M), ky = (int)((t / M) % N), kz = (int)(t / ((size_t)N * M));
+    const int kyw = wave_index(ky, N), kzw = wave_index(kz, N);
+    double k2 = (double)(kx * kx + kyw * kyw + kzw * kzw);
+    if (k2 == 0.0) k2 = 1.e-5;
+    const int ik2 = (int)(sqrt(k2) + 0.5);
+    if (ik2 < 1 || ik2 > nek) return;
+    const C a = ((const C*)u0)[t], b = ((const C*)u1)[t], c = ((const C*)u2)[t];
+    double e = (double)a.x * a.x + (double)a.y * a.y + (double)b.x * b.x + (double)b.y * b.y +
+               (double)c.x * c.x + (double)c.y * c.y;
+    if (kx == 0) e *= 0.5;
+    atomicAdd(ek + ik2 - 1, e);
+}
+
+// ================================================================== host side
+static size_t field_elems(int N) { return (size_t)N * N * (N / 2); }
+
+int flow_alloc(FlowBuffers& fb, int N, int precision)
+{
+    const size_t cs = (precision == DDNS_FP64) ? sizeof(double2) : sizeof(float2);
+    const size_t rs = (precision == DDNS_FP64) ? sizeof(double) : sizeof(float);
+    const size_t fe = field_elems(N);
+    for (int i = 0; i < 3; ++i) {
+        DDNS_HIP(hipMalloc(&fb.U[i], fe * cs));
+        DDNS_HIP(hipMalloc(&fb.O[i], fe * cs));
+        DDNS_HIP(hipMalloc(&fb.B[i], fe * cs));
+        DDNS_HIP(hipMemset(fb.U[i], 0, fe * cs));
+        DDNS_HIP(hipMemset(fb.O[i], 0, fe * cs));
+    }
+    for (int i = 0; i < 6; ++i) DDNS_HIP(hipMalloc(&fb.A[i], fe * cs));
+    DDNS_HIP(hipMalloc(&fb.uphys, (size_t)N * N * N * 3 * rs));
+    DDNS_HIP(hipMemset(fb.uphys, 0, (size_t)N * N * N * 3 * rs));
+    // twiddles exp(-2 pi i m / L), computed in double and rounded once
+    for (int pass = 0; pass < 2; ++pass) {
+        const int L = pass == 0 ? N : N / 2;
+        std::vector<double> d(2 * (size_t)L);
+        for (int m = 0; m < L; ++m) {
+            const double ang = -2.0 * M_PI * (double)m / (double)L;
+            d[2 * m] = std::cos(ang);
+            d[2 * m + 1] = std::sin(ang);
+        }
+        void** dst = pass == 0 ? &fb.tw : &fb.twh;
+        if (precision == DDNS_FP64) {
+            DDNS_HIP(hipMalloc(dst, d.size() * sizeof(double)));
+            DDNS_HIP(hipMemcpy(*dst, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> f(d.begin(), d.end());
+            DDNS_HIP(hipMalloc(dst, f.size() * sizeof(float)));
+            DDNS_HIP(hipMemcpy(*dst, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+        }
+    }
+    DDNS_HIP(hipMalloc(&fb.low, 512 * sizeof(double)));
+    DDNS_HIP(hipMemset(fb.low, 0, 512 * sizeof(double)));
+    DDNS_HIP(hipMalloc(&fb.sth_state, 64 * sizeof(int)));
+    DDNS_HIP(hipMemset(fb.sth_state, 0, 64 * sizeof(int)));
+    DDNS_HIP(hipMalloc(&fb.sth_b, 450 * sizeof(double)));
+    DDNS_HIP(hipMemset(fb.sth_b, 0, 450 * sizeof(double)));
+    return 0;
+}
+
+void flow_free(FlowBuffers& fb)
+{
+    for (int i = 0; i < 3; ++i) { (void)hipFree(fb.U[i]); (void)hipFree(fb.O[i]); (void)hipFree(fb.B[i]); }
+    for (int i = 0; i < 6; ++i) (void)hipFree(fb.A[i]);
+    (void)hipFree(fb.uphys); (void)hipFree(fb.tw); (void)hipFree(fb.twh);
+    (void)hipFree(fb.low); (void)hipFree(fb.sth_state); (void)hipFree(fb.sth_b);
+    fb = FlowBuffers{};
+}
+
+template <typename T, int N>
+static int launch_z(int mode, const FlowBuffers& fb, const FlowConst& fc, hipStream_t s,
+                    const void* src = nullptr, void* dst = nullptr, double scale = 1.0)
+{
+    KzArgs a{};
+    for (int i = 0; i < 3; ++i) { a.B[i] = fb.B[i]; a.U[i] = fb.U[i]; a.O[i] = fb.O[i]; }
+    for (int i = 0; i < 6; ++i) a.A[i] = fb.A[i];
+    a.tw = fb.tw; a.low = fb.low; a.dt = fc.dt; a.rnu = fc.rnu; a.nek = fc.nek; a.forcing = fc.forcing;
+    a.scale = scale;
+    if (mode == KZ_PLAIN_FWD || mode == KZ_PLAIN_INV) { a.B[0] = src; a.U[0] = dst; }
+    constexpr int TC = kz_tc<T, N>();
+    dim3 grid(N / 2 / TC, N);
+    switch (mode) {
+    case KZ_FULL: hipLaunchKernelGGL((k_z<T, N, KZ_FULL>), grid, dim3(NT), 0, s, a); break;
+    case KZ_CURL_INV: hipLaunchKernelGGL((k_z<T, N, KZ_CURL_INV>), grid, dim3(NT), 0, s, a); break;
+    case KZ_PLAIN_FWD: hipLaunchKernelGGL((k_z<T, N, KZ_PLAIN_FWD>), grid, dim3(NT), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_z<T, N, KZ_PLAIN_INV>), grid, dim3(NT), 0, s, a); break;
+    }
+    DDNS_HIP(hipGetLastError());
+    return 0;
+}
+
+template <typename T, int N>
+static int launch_y(bool inv, bool sym, void* const* f, int nf, const FlowBuffers& fb, hipStream_t s)
+{
+    KyArgs a{};
+    for (int i = 0; i < nf; ++i) a.F[i] = f[i];
+    a.tw = fb.tw;
+    a.scale = 1.0 / N;
+    constexpr int TC = ky_tc<T, N>();
+    dim3 grid(N / 2 / TC, N, nf);
+    if (inv && sym) hipLaunchKernelGGL((k_y<T, N, true, true>), grid, dim3(NT), 0, s, a);
+    else if (inv) hipLaunchKernelGGL((k_y<T, N, true, false>), grid, dim3(NT), 0, s, a);
+    else hipLaunchKernelGGL((k_y<T, N, false, false>), grid, dim3(NT), 0, s, a);
+    DDNS_HIP(hipGetLastError());
+    return 0;
+}
+
+template <typename T, int N>
+static int launch_x(const FlowBuffers& fb, hipStream_t s)
+{
+    KxArgs a{};
+    for (int i = 0; i < 6; ++i) a.A[i] = fb.A[i];
+    for (int i = 0; i < 3; ++i) a.B[i] = fb.B[i];
+    a.uphys = fb.uphys; a.tw = fb.tw; a.twh = fb.twh;
+    dim3 grid((unsigned)((size_t)N * N / kx_lines<N>()));
+    hipLaunchKernelGGL((k_x_fused<T, N>), grid, dim3(NT), 0, s, a);
+    DDNS_HIP(hipGetLastError());
+    return 0;
+}
+
+template <typename T, int N>
+static int launch_pre(const FlowBuffers& fb, const FlowConst& fc, hipStream_t s)
+{
+    if (fc.forcing != 1 && fc.forcing != 2) return 0;
+    PreArgs a{};
+    for (int i = 0; i < 3; ++i) { a.B[i] = fb.B[i]; a.U[i] = fb.U[i]; a.O[i] = fb.O[i]; }
+    a.tw = fb.tw; a.low = fb.low; a.sth_state = fb.sth_state; a.sth_b = fb.sth_b;
+    a.dt = fc.dt; a.rnu = fc.rnu; a.force0 = fc.force0; a.force1 = fc.force1;
+    a.ou_tf = fc.ou_tf; a.ou_var = fc.ou_var; a.pi2 = fc.pi2; a.nek = fc.nek; a.forcing = fc.forcing;
+    hipLaunchKernelGGL((k_pre<T, N>), dim3(1), dim3(NT), 0, s, a);
+    DDNS_HIP(hipGetLastError());
+    return 0;
+}
+
+template <typename T, int N>
+static int launch_sym(const FlowBuffers& fb, hipStream_t s)
+{
+    const int tot = N * N;
+    hipLaunchKernelGGL((k_sym<T, N>), dim3((tot + 255) / 256), dim3(256), 0, s, fb.U[0], fb.U[1], fb.U[2]);
+    DDNS_HIP(hipGetLastError());
+    return 0;
+}
+
+template <typename T, int N>
+static int step_impl(const FlowBuffers& fb, const FlowConst& fc, hipStream_t s, hipEvent_t* ev, int part)
+{
+    // part 0: inverse y + fused x (produces the physical velocity);
+    // part 1: forward y + pre + z + sym (closes the step, opens the next)
+    if (part == 0) {
+        if (ev) DDNS_HIP(hipEventRecord(ev[0], s));
+        int st = launch_y<T, N>(true, true, fb.A, 6, fb, s);
+        if (st) return st;
+        if (ev) DDNS_HIP(hipEventRecord(ev[1], s));
+        st = launch_x<T, N>(fb, s);
+        if (st) return st;
+        if (ev) DDNS_HIP(hipEventRecord(ev[2], s));
+        return 0;
+    }
+    if (ev) DDNS_HIP(hipEventRecord(ev[3], s));
+    void* bb[3] = {fb.B[0], fb.B[1], fb.B[2]};
+    int st = launch_y<T, N>(false, false, bb, 3, fb, s);
+    if (st) return st;
+    if (ev) DDNS_HIP(hipEventRecord(ev[4], s));
+    st = launch_pre<T, N>(fb, fc, s);
+    if (st) return st;
+    if (ev) DDNS_HIP(hipEventRecord(ev[5], s));
+    st = launch_z<T, N>(KZ_FULL, fb, fc, s);
+    if (st) return st;
+    if (ev) DDNS_HIP(hipEventRecord(ev[6], s));
+    st = launch_sym<T, N>(fb, s);
+    if (st) return st;
+    if (ev) DDNS_HIP(hipEventRecord(ev[7], s));
+    return 0;
+}
+
+#define DDNS_DISPATCH_N(N_, CALL)                         \
+    switch (N_) {                                          \
+    case 32: { constexpr int NN = 32; CALL; } break;       \
+    case 64: { constexpr int NN = 64; CALL; } break;       \
+    case 128: { constexpr int NN = 128; CALL; } break;     \
+    case 256: { constexpr int NN = 256; CALL; } break;     \
+    case 512: { constexpr int NN = 512; CALL; } break;     \
+    case 1024: { constexpr int NN = 1024; CALL; } break;   \
+    default: set_error("unsupported grid size"); return DDNS_ERR_ARG; \
+    }
+
+int flow_prologue(const FlowBuffers& fb, const FlowConst& fc, int precision, hipStream_t s)
+{
+    int st = 0;
+    if (precision == DDNS_FP64) { DDNS_DISPATCH_N(fc.N, st = (launch_z<double, NN>(KZ_CURL_INV, fb, fc, s))) }
+    else { DDNS_DISPATCH_N(fc.N, st = (launch_z<float, NN>(KZ_CURL_INV, fb, fc, s))) }
+    return st;
+}
+
+int flow_step(const FlowBuffers& fb, const FlowConst& fc, int precision, bool last, hipStream_t s, hipEvent_t* ev)
+{
+    // 'last' selects the part: false = first half (inverse), true = second half (forward + update)
+    const int part = last ? 1 : 0;
+    int st = 0;
+    if (precision == DDNS_FP64) { DDNS_DISPATCH_N(fc.N, st = (step_impl<double, NN>(fb, fc, s, ev, part))) }
+    else { DDNS_DISPATCH_N(fc.N, st = (step_impl<float, NN>(fb, fc, s, ev, part))) }
+    return st;
+}
+
+// ---------------------------------------------------------------- host conversion
+template <typename T>
+static int upload_spectral(void* dev, int N, const double* v, hipStream_t s)
+{
+    using C = cpx<T>;
+    const int L = N + 2, M = N / 2;
+    std::vector<C> h(field_elems(N));
+    for (int kz = 0; kz < N; ++kz)
+        for (int ky = 0; ky < N; ++ky)
+            for (int kx = 0; kx < M; ++kx) {
+                const size_t o = (size_t)kz + (size_t)L * ((size_t)(2 * kx) + (size_t)L * ky);
+                h[((size_t)kz * N + ky) * M + kx] = mk<C>((T)v[o], (T)v[o + L]);
+            }
+    DDNS_HIP(hipMemcpyAsync(dev, h.data(), h.size() * sizeof(C), hipMemcpyHostToDevice, s));
+    DDNS_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+template <typename T>
+static int download_spectral(const void* dev, int N, double* v, hipStream_t s)
+{
+    using C = cpx<T>;
+    const int L = N + 2, M = N / 2;
+    std::vector<C> h(field_elems(N));
+    DDNS_HIP(hipMemcpyAsync(h.data(), dev, h.size() * sizeof(C), hipMemcpyDeviceToHost, s));
+    DDNS_HIP(hipStreamSynchronize(s));
+    std::memset(v, 0, sizeof(double) * (size_t)L * L * N);
+    for (int kz = 0; kz < N; ++kz)
+        for (int ky = 0; ky < N; ++ky)
+            for (int kx = 0; kx < M; ++kx) {
+                const size_t o = (size_t)kz + (size_t)L * ((size_t)(2 * kx) + (size_t)L * ky);
+                const C c = h[((size_t)kz * N + ky) * M + kx];
+                v[o] = (double)c.x;
+                v[o + L] = (double)c.y;
+            }
+    return 0;
+}
+
+int flow_upload_state(const FlowBuffers& fb, int N, int precision, const double* const src[6], hipStream_t s)
+{
+    for (int f = 0; f < 6; ++f) {
+        if (!src[f]) continue;
+        void* dev = f < 3 ? fb.U[f] : fb.O[f - 3];
+        const int st = precision == DDNS_FP64 ? upload_spectral<double>(dev, N, src[f], s)
+                                              : upload_spectral<float>(dev, N, src[f], s);
+        if (st) return st;
+    }
+    return 0;
+}
+
+int flow_download_state(const FlowBuffers& fb, int N, int precision, double* const dst[6], hipStream_t s)
+{
+    for (int f = 0; f < 6; ++f) {
+        if (!dst[f]) continue;
+        const void* dev = f < 3 ? fb.U[f] : fb.O[f - 3];
+        const int st = precision == DDNS_FP64 ? download_spectral<double>(dev, N, dst[f], s)
+                                              : download_spectral<float>(dev, N, dst[f], s);
+        if (st) return st;
+    }
+    return 0;
+}
+
+template <typename T>
+static int download_phys_t(const void* dev, int N, double* ux, double* uy, double* uz, hipStream_t s)
+{
+    const size_t n3 = (size_t)N * N * N;
+    std::vector<T> h(n3 * 3);
+    DDNS_HIP(hipMemcpyAsync(h.data(), dev, h.size() * sizeof(T), hipMemcpyDeviceToHost, s));
+    DDNS_HIP(hipStreamSynchronize(s));
+    double* d[3] = {ux, uy, uz};
+    for (int c = 0; c < 3; ++c)
+        if (d[c])
+            for (size_t i = 0; i < n3; ++i) d[c][i] = (double)h[3 * i + c];
+    return 0;
+}
+
+int flow_download_phys(const FlowBuffers& fb, int N, int precision, double* ux, double* uy, double* uz, hipStream_t s)
+{
+    return precision == DDNS_FP64 ? download_phys_t<double>(fb.uphys, N, ux, uy, uz, s)
+                                  : download_phys_t<float>(fb.uphys, N, ux, uy, uz, s);
+}
+
+template <typename T, int N>
+static int spectrum_impl(const FlowBuffers& fb, int nek, double* ek, hipStream_t s)
+{
+    double* d = nullptr;
+    DDNS_HIP(hipMalloc(&d, sizeof(double) * nek));
+    DDNS_HIP(hipMemsetAsync(d, 0, sizeof(double) * nek, s));
+    const size_t tot = field_elems(N);
+    hipLaunchKernelGGL((k_spectrum<T, N>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                       fb.U[0], fb.U[1], fb.U[2], nek, d);
+    DDNS_HIP(hipGetLastError());
+    DDNS_HIP(hipMemcpyAsync(ek, d, sizeof(double) * nek, hipMemcpyDeviceToHost, s));
+    DDNS_HIP(hipStreamSynchronize(s));
+    DDNS_HIP(hipFree(d));
+    return 0;
+}
+
+int flow_spectrum(const FlowBuffers& fb, int N, int nek, int precision, double* ek, hipStream_t s)
+{
+    int st = 0;
+    if (precision == DDNS_FP64) { DDNS_DISPATCH_N(N, st = (spectrum_impl<double, NN>(fb, nek, ek, s))) }
+    else { DDNS_DISPATCH_N(N, st = (spectrum_impl<float, NN>(fb, nek, ek, s))) }
+    return st;
+}
+
+// ---------------------------------------------------------------- stand-alone FFTs
+template <typename T, int N>
+static int r2c_impl(const FlowBuffers& fb, double* v, hipStream_t s)
+{
+    using C = cpx<T>;
+    const int L = N + 2;
+    const size_t n3 = (size_t)N * N * N;
+    std::vector<T> h(n3);
+    for (int z = 0; z < N; ++z)
+        for (int y = 0; y < N; ++y)
+            for (int x = 0; x < N; ++x)
+                h[((size_t)z * N + y) * N + x] = (T)v[(size_t)x + (size_t)L * ((size_t)y + (size_t)L * z)];
+    // A[0] holds the real input (same byte size as one complex field), B[0] the output
+    DDNS_HIP(hipMemcpyAsync(fb.A[0], h.data(), n3 * sizeof(T), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL((k_x_plain<T, N, false>), dim3((unsigned)((size_t)N * N / kx_lines<N>())), dim3(NT), 0, s,
+                       fb.A[0], fb.B[0], fb.tw, fb.twh);
+    DDNS_HIP(hipGetLastError());
+    void* f[1] = {fb.B[0]};
+    int st = launch_y<T, N>(false, false, f, 1, fb, s);
+    if (st) return st;
+    FlowConst fc{};
+    fc.N = N;
+    st = launch_z<T, N>(KZ_PLAIN_FWD, fb, fc, s, fb.B[0], fb.A[1], 1.0 / N);
+    if (st) return st;
+    return download_spectral<T>(fb.A[1], N, v, s);
+}
+
+template <typename T, int N>
+static int c2r_impl(const FlowBuffers& fb, double* v, hipStream_t s)
+{
+    const int L = N + 2;
+    const size_t n3 = (size_t)N * N * N;
+    int st = upload_spectral<T>(fb.A[1], N, v, s);
+    if (st) return st;
+    FlowConst fc{};
+    fc.N = N;
+    st = launch_z<T, N>(KZ_PLAIN_INV, fb, fc, s, fb.A[1], fb.B[0], 1.0);
+    if (st) return st;
+    void* f[1] = {fb.B[0]};
+    st = launch_y<T, N>(true, false, f, 1, fb, s);
+    if (st) return st;
+    hipLaunchKernelGGL((k_x_plain<T, N, true>), dim3((unsigned)((size_t)N * N / kx_lines<N>())), dim3(NT), 0, s,
+                       fb.B[0], fb.A[0], fb.tw, fb.twh);
+    DDNS_HIP(hipGetLastError());
+    std::vector<T> h(n3);
+    DDNS_HIP(hipMemcpyAsync(h.data(), fb.A[0], n3 * sizeof(T), hipMemcpyDeviceToHost, s));
+    DDNS_HIP(hipStreamSynchronize(s));
+    std::memset(v, 0, sizeof(double) * (size_t)L * L * N);
+    for (int z = 0; z < N; ++z)
+        for (int y = 0; y < N; ++y)
+            for (int x = 0; x < N; ++x)
+                v[(size_t)x + (size_t)L * ((size_t)y + (size_t)L * z)] = (double)h[((size_t)z * N + y) * N + x];
+    return 0;
+}
+
+int fft_r2c_host(const FlowBuffers& fb, int N, int precision, double* v, hipStream_t s)
+{
+    int st = 0;
+    if (precision == DDNS_FP64) { DDNS_DISPATCH_N(N, st = (r2c_impl<double, NN>(fb, v, s))) }
+    else { DDNS_DISPATCH_N(N, st = (r2c_impl<float, NN>(fb, v, s))) }
+    return st;
+}
+
+int fft_c2r_host(const FlowBuffers& fb, int N, int precision, double* v, hipStream_t s)
+{
+    int st = 0;
+    if (precision == DDNS_FP64) { DDNS_DISPATCH_N(N, st = (c2r_impl<double, NN>(fb, v, s))) }
+    else { DDNS_DISPATCH_N(N, st = (c2r_impl<float, NN>(fb, v, s))) }
+    return st;
+}
+
+// ---------------------------------------------------------------- per-kernel timing
+template <typename T, int N>
+static int profile_impl(const FlowBuffers& fb, const FlowConst& fc, int k, double* ms, int nk, hipStream_t s)
+{
+    hipEvent_t ev[8];
+    for (int i = 0; i < 8; ++i) DDNS_HIP(hipEventCreate(&ev[i]));
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int it = 0; it < k; ++it) {
+        int st = step_impl<T, N>(fb, fc, s, ev, 0);
+        if (st) return st;
+        st = step_impl<T, N>(fb, fc, s, ev, 1);
+        if (st) return st;
+        DDNS_HIP(hipEventSynchronize(ev[7]));
+        float t;
+        DDNS_HIP(hipEventElapsedTime(&t, ev[0], ev[1])); acc[0] += t;   // y inverse
+        DDNS_HIP(hipEventElapsedTime(&t, ev[1], ev[2])); acc[1] += t;   // x fused
+        DDNS_HIP(hipEventElapsedTime(&t, ev[3], ev[4])); acc[2] += t;   // y forward
+        DDNS_HIP(hipEventElapsedTime(&t, ev[4], ev[5])); acc[3] += t;   // pre
+        DDNS_HIP(hipEventElapsedTime(&t, ev[5], ev[6])); acc[4] += t;   // z fused
+        DDNS_HIP(hipEventElapsedTime(&t, ev[6], ev[7])); acc[5] += t;   // sym
+        DDNS_HIP(hipEventElapsedTime(&t, ev[0], ev[7])); acc[6] += t;   // whole
+    }
+    for (int i = 0; i < nk && i < 7; ++i) ms[i] = acc[i] / k;
+    for (int i = 0; i < 8; ++i) (void)hipEventDestroy(ev[i]);
+    return 0;
+}
+
+int flow_profile(const FlowBuffers& fb, const FlowConst& fc, int precision, int k, double* ms, int nk, hipStream_t s)
+{
+    int st = 0;
+    if (precision == DDNS_FP64) { DDNS_DISPATCH_N(fc.N, st = (profile_impl<double, NN>(fb, fc, k, ms, nk, s))) }
+    else { DDNS_DISPATCH_N(fc.N, st = (profile_impl<float, NN>(fb, fc, k, ms, nk, s))) }
+    return st;
+}
+
+}  // namespace ddns

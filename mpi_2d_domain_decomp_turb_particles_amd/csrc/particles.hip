@@ -1,0 +1,524 @@
+// particles.hip -- Lagrangian droplet step on one MI355X (everything but the pair
+// searches, which live in collide.hip, and the HI solve in hdi.hip).
+//
+// Reference: PREPARE_INTERP / INTERPOLATE partlib.f90:305-520 (6-point Lagrange
+// gather), PART_HISTORY_FVP/FDP 553-625, PART_ADVANCE 629-726, UPDATE_PART_POSITION
+// 730-757, PERIODIC wrap 1142-1151, RELOCATE 1984-2012.  The reference's linked
+// cell list (HEADLIST 2171-2444) is replaced by a counting sort of droplets into a
+// coarse periodic grid; periodic images are formed on the fly by the pair kernels.
+//
+// Droplet state is structure-of-arrays in double, slot i = the reference's array
+// index i+1, so every index-order rule of the reference (ih2 > ih1, the order of the
+// removal list and of the relocation draws) is evaluated on slot numbers.
+#include "ddns_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace ddns {
+
+constexpr int PT = 256;
+
+static inline unsigned nblk(long long n, int t = PT) { return (unsigned)((n + t - 1) / t); }
+
+// ------------------------------------------------------------------ scans
+// exclusive scan of n ints (in may alias out); writes the total to out[n]
+__global__ void k_scan_block(const int* in, int* out, int* bsum, int n)
+{
+    __shared__ int s[1024];
+    const int base = blockIdx.x * 1024;
+    const int t = threadIdx.x;
+    int v[4];
+    int tot = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = base + t * 4 + q;
+        v[q] = i < n ? in[i] : 0;
+        tot += v[q];
+    }
+    s[t] = tot;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int x = t >= off ? s[t - off] : 0;
+        __syncthreads();
+        s[t] += x;
+        __syncthreads();
+    }
+    int run = s[t] - tot;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = base + t * 4 + q;
+        if (i < n) out[i] = run;
+        run += v[q];
+    }
+    if (t == 255) bsum[blockIdx.x] = s[255];
+}
+
+__global__ void k_scan_sums(int* bsum, int nb, int* total_out)
+{
+    if (threadIdx.x != 0) return;
+    int run = 0;
+    for (int b = 0; b < nb; ++b) {
+        const int v = bsum[b];
+        bsum[b] = run;
+        run += v;
+    }
+    *total_out = run;
+}
+
+__global__ void k_scan_add(int* out, const int* bsum, int n)
+{
+    const int i = blockIdx.x * 1024 + threadIdx.x * 4;
+    const int add = bsum[blockIdx.x];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (i + q < n) out[i + q] += add;
+}
+
+// out[0..n-1] = exclusive scan, out[n] = total; tmp holds ceil(n/1024) ints
+int exclusive_scan(const int* in, int* out, int n, int* tmp, hipStream_t s)
+{
+    const int nb = (n + 1023) / 1024;
+    hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(256), 0, s, in, out, tmp, n);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1), 0, s, tmp, nb, out + n);
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, s, out, tmp, n);
+    DDNS_HIP(hipGetLastError());
+    return 0;
+}
+
+// ------------------------------------------------------------------ allocation
+int part_alloc(PartBuffers& pb, int cap, int nset, int G)
+{
+    pb.cap = cap;
+    const size_t c = (size_t)cap;
+    DDNS_HIP(hipMalloc(&pb.pos, 3 * c * 8));
+    DDNS_HIP(hipMalloc(&pb.vel, 3 * c * 8));
+    DDNS_HIP(hipMalloc(&pb.rad, c * 8));
+    DDNS_HIP(hipMalloc(&pb.gid, c * 8));
+    DDNS_HIP(hipMalloc(&pb.fvp, 9 * c * 8));
+    DDNS_HIP(hipMalloc(&pb.fdp, 12 * c * 8));
+    DDNS_HIP(hipMalloc(&pb.dmove, 3 * c * 8));
+    DDNS_HIP(hipMalloc(&pb.vp, 3 * c * 8));
+    DDNS_HIP(hipMalloc(&pb.pert, 3 * c * 8));
+    DDNS_HIP(hipMemset(pb.pert, 0, 3 * c * 8));
+    DDNS_HIP(hipMemset(pb.dmove, 0, 3 * c * 8));
+    DDNS_HIP(hipMemset(pb.vp, 0, 3 * c * 8));
+    DDNS_HIP(hipMalloc(&pb.cell_of, c * sizeof(int)));
+    DDNS_HIP(hipMalloc(&pb.sorted, c * sizeof(int)));
+    DDNS_HIP(hipMalloc(&pb.mult, (c + 1) * sizeof(int)));
+    DDNS_HIP(hipMalloc(&pb.draw_off, (c + 1) * sizeof(int)));
+    DDNS_HIP(hipMalloc(&pb.nei_cnt, (c + 1) * sizeof(int)));
+    DDNS_HIP(hipMalloc(&pb.nei_ptr, (c + 1) * sizeof(int)));
+    DDNS_HIP(hipMalloc(&pb.flags, 64 * sizeof(int)));
+    DDNS_HIP(hipMemset(pb.flags, 0, 64 * sizeof(int)));
+    DDNS_HIP(hipMalloc(&pb.tested, 4 * sizeof(long long)));
+    const size_t nb = (size_t)nset * nset * DDNS_NBIN;
+    DDNS_HIP(hipMalloc(&pb.nirbin, nb * sizeof(int)));
+    DDNS_HIP(hipMalloc(&pb.bins, 7 * nb * sizeof(double)));
+    DDNS_HIP(hipMalloc(&pb.counts, 512 * sizeof(int)));
+    pb.pair_cap = 1 << 16;
+    DDNS_HIP(hipMalloc(&pb.pairs, (size_t)pb.pair_cap * 4 * sizeof(long long)));
+    pb.draw_cap = 1 << 16;
+    DDNS_HIP(hipMalloc(&pb.draws, (size_t)pb.draw_cap * sizeof(double)));
+    DDNS_HIP(hipMalloc(&pb.rng_state, 64 * sizeof(int)));
+    DDNS_HIP(hipMemset(pb.rng_state, 0, 64 * sizeof(int)));
+    DDNS_HIP(hipMalloc(&pb.red, 8192 * sizeof(double)));
+    pb.scan_tmp_bytes = (size_t)((std::max((long long)G * G * G + 1, (long long)c + 1) + 1023) / 1024 + 8) * sizeof(int);
+    DDNS_HIP(hipMalloc(&pb.scan_tmp, pb.scan_tmp_bytes));
+    pb.G = 0;
+    return 0;
+}
+
+void part_free(PartBuffers& pb)
+{
+    void* ps[] = {pb.pos, pb.vel, pb.rad, pb.gid, pb.fvp, pb.fdp, pb.dmove, pb.vp, pb.pert,
+                  pb.cell_count, pb.cell_start, pb.cell_of, pb.sorted, pb.cursor, pb.scan_tmp,
+                  pb.nei_cnt, pb.nei_ptr, pb.nei_idx, pb.nei_m, pb.gm_Q, pb.gm_v, pb.gm_x0, pb.gm_b,
+                  pb.gm_red, pb.gm_small, pb.mult, pb.flags, pb.tested, pb.nirbin, pb.bins, pb.counts,
+                  pb.pairs, pb.draws, pb.draw_off, pb.rng_state, pb.red};
+    for (void* p : ps) if (p) (void)hipFree(p);
+    pb = PartBuffers{};
+}
+
+// ------------------------------------------------------------------ interpolation
+// Lagrange weights for nodes node-2 .. node+3 at fractional position z (partlib.f90:333-338)
+__device__ __forceinline__ void lagrange6(double z, double* bg)
+{
+    const double z2 = z * z, z3 = z2 * z, z4 = z3 * z, z5 = z4 * z;
+    bg[0] = (6.0 * z - 5.0 * z2 - 5.0 * z3 + 5.0 * z4 - z5) / 120.0;
+    bg[1] = (-12.0 * z + 16.0 * z2 - z3 - 4.0 * z4 + z5) / 24.0;
+    bg[2] = (12.0 - 4.0 * z - 15.0 * z2 + 5.0 * z3 + 3.0 * z4 - z5) / 12.0;
+    bg[3] = (12.0 * z + 8.0 * z2 - 7.0 * z3 - 2.0 * z4 + z5) / 12.0;
+    bg[4] = (-6.0 * z - z2 + 7.0 * z3 + z4 - z5) / 24.0;
+    bg[5] = (4.0 * z - 5.0 * z3 + z5) / 120.0;
+}
+
+// One wavefront lane per droplet; droplets are visited in cell order (sorted[])
+// so that neighbouring lanes gather overlapping stencils from the same L2 lines.
+template <typename T>
+__global__ void __launch_bounds__(PT) k_interp(const T* __restrict__ u, int N, double pi2,
+                                               const double* __restrict__ pos, const int* __restrict__ order,
+                                               double* __restrict__ vp, int n, int cap)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = order ? order[t] : t;
+    const double dx = pi2 / (double)N;
+    double bg[3][6];
+    int nd[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double y = pos[(size_t)c * cap + i];
+        const int node = (int)floor(y / dx);
+        const double z = (y - (double)node * dx) / dx;
+        nd[c] = node;
+        lagrange6(z, bg[c]);
+    }
+    int ix[6], iy[6], iz[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+        ix[m] = ((nd[0] - 2 + m) % N + N) % N;
+        iy[m] = ((nd[1] - 2 + m) % N + N) % N;
+        iz[m] = ((nd[2] - 2 + m) % N + N) % N;
+    }
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+#pragma unroll 1
+    for (int a = 0; a < 6; ++a) {
+#pragma unroll
+        for (int b = 0; b < 6; ++b) {
+            const double wab = bg[0][a] * bg[1][b];
+#pragma unroll
+            for (int d = 0; d < 6; ++d) {
+                const T* p = u + (((size_t)iz[d] * N + iy[b]) * N + ix[a]) * 3;
+                const double w = wab * bg[2][d];
+                acc0 += (double)p[0] * w;
+                acc1 += (double)p[1] * w;
+                acc2 += (double)p[2] * w;
+            }
+        }
+    }
+    vp[i] = acc0;
+    vp[(size_t)cap + i] = acc1;
+    vp[2 * (size_t)cap + i] = acc2;
+}
+
+int part_interp(const PartBuffers& pb, const PartConst& pc, const void* uphys, hipStream_t s)
+{
+    if (pc.n == 0) return 0;
+    const int* order = pb.G > 0 ? pb.sorted : nullptr;
+    if (pc.precision == DDNS_FP64)
+        hipLaunchKernelGGL(k_interp<double>, dim3(nblk(pc.n)), dim3(PT), 0, s, (const double*)uphys, pc.N, pc.pi2,
+                           pb.pos, order, pb.vp, pc.n, pb.cap);
+    else
+        hipLaunchKernelGGL(k_interp<float>, dim3(nblk(pc.n)), dim3(PT), 0, s, (const float*)uphys, pc.N, pc.pi2,
+                           pb.pos, order, pb.vp, pc.n, pb.cap);
+    DDNS_HIP(hipGetLastError());
+    return 0;
+}
+
+// ------------------------------------------------------------------ histories
+// PART_HISTORY_FVP, partlib.f90:562-592
+__global__ void k_hist_fvp(double* vel, const double* rad, double* fvp, int n, int cap, int newp, double Sv)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (newp) {
+        const double a = rad[i];
+        vel[i] = Sv * (a * a);
+        vel[(size_t)cap + i] = 0.0;
+        vel[2 * (size_t)cap + i] = 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double v = vel[(size_t)c * cap + i];
+        fvp[(size_t)c * cap + i] = v;
+        if (newp) { fvp[(size_t)(3 + c) * cap + i] = v; fvp[(size_t)(6 + c) * cap + i] = v; }
+    }
+}
+
+int part_history_fvp(const PartBuffers& pb, const PartConst& pc, hipStream_t s)
+{
+    if (pc.n == 0) return 0;
+    hipLaunchKernelGGL(k_hist_fvp, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.vel, pb.rad, pb.fvp, pc.n, pb.cap,
+                       pc.new_particle, pc.Sv_const);
+    DDNS_HIP(hipGetLastError());
+    return 0;
+}
+
+// ------------------------------------------------------------------ advance
+// PART_HISTORY_FDP (599-625) + PART_ADVANCE (629-726): implicit Adams-Moulton
+// velocity with Adams-Bashforth fluid history, Stokes or Schiller-Naumann drag,
+// gravity along +x, displacement dmove.
+__global__ void k_advance(double* vel, const double* rad, double* fvp, double* fdp, const double* vp,
+                          const double* pert, double* dmove, int n, int cap, int newp, int hdi,
+                          int ndrag, double dt, double rnu, double Sv, double tau2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t C = (size_t)cap;
+    double f0[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        f0[c] = hdi ? vp[c * C + i] + pert[c * C + i] : vp[c * C + i];
+        fdp[c * C + i] = f0[c];
+        if (newp) { fdp[(3 + c) * C + i] = f0[c]; fdp[(6 + c) * C + i] = f0[c]; fdp[(9 + c) * C + i] = f0[c]; }
+    }
+    const double a = rad[i];
+    double FSN = 1.;
+    if (ndrag == 1) {
+        double d[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            d[c] = hdi ? vel[c * C + i] - vp[c * C + i] - pert[c * C + i] : vel[c * C + i] - vp[c * C + i];
+        const double Vrel = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        FSN = 1. + 0.15 * pow(Vrel * 2. * a / rnu, 0.687);
+    }
+    const double taup = tau2 * (a * a);
+    const double dtau = dt / (24.0 * taup) * FSN;
+    const double wp_vel = Sv * (a * a);
+    const double dtc = dt / 24.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double unitv = c == 0 ? 1.0 : 0.0;
+        const double fv0 = fvp[c * C + i], fv1 = fvp[(3 + c) * C + i], fv2 = fvp[(6 + c) * C + i];
+        const double tempv = 19.0 * fv0 - 5.0 * fv1 + fv2;
+        const double tempd = 55.0 * f0[c] - 59.0 * fdp[(3 + c) * C + i] + 37.0 * fdp[(6 + c) * C + i] - 9.0 * fdp[(9 + c) * C + i];
+        const double vnew = (vel[c * C + i] + dt * unitv * wp_vel / taup * FSN - dtau * tempv + dtau * tempd) / (1.0 + 9.0 * dtau);
+        vel[c * C + i] = vnew;
+        dmove[c * C + i] = dtc * (9.0 * vnew + 19.0 * fv0 - 5.0 * fv1 + fv2);
+    }
+}
+
+int part_advance(const PartBuffers& pb, const PartConst& pc, hipStream_t s)
+{
+    if (pc.n == 0) return 0;
+    hipLaunchKernelGGL(k_advance, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.vel, pb.rad, pb.fvp, pb.fdp, pb.vp, pb.pert,
+                       pb.dmove, pc.n, pb.cap, pc.new_particle, pc.hdi, pc.ndrag, pc.dt, pc.rnu, pc.Sv_const, pc.tau2);
+    DDNS_HIP(hipGetLastError());
+    return 0;
+}
+
+// ------------------------------------------------------------------ position update, relocation, wrap
+// UPDATE_PART_POSITION (741-751)
+__global__ void k_update_pos(double* pos, const double* dmove, double* fvp, double* fdp, int n, int cap)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t C = (size_t)cap;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        fvp[(6 + c) * C + i] = fvp[(3 + c) * C + i];
+        fvp[(3 + c) * C + i] = fvp[c * C + i];
+        fdp[(9 + c) * C + i] = fdp[(6 + c) * C + i];
+        fdp[(6 + c) * C + i] = fdp[(3 + c) * C + i];
+        fdp[(3 + c) * C + i] = fdp[c * C + i];
+        pos[c * C + i] = pos[c * C + i] + dmove[c * C + i];
+    }
+}
+
+// PERIODIC wrap (1142-1151)
+__global__ void k_wrap(double* pos, int n, int cap, double pi2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double y = pos[(size_t)c * cap + i];
+        if (y >= pi2) y = y - pi2;
+        if (y < 0.0) y = y + pi2;
+        pos[(size_t)c * cap + i] = y;
+    }
+}
+
+// ranff2, accesories.f90:107-147, sequential on one lane: the relocation draws of
+// RELOCATE / OVERLAP_DET come from one stream in the reference's order.
+__device__ double dev_ranff2(int* idum, int* iv, int* iy, int* idum2)
+{
+    const int IM1 = 2147483563, IM2 = 2147483399, IMM1 = IM1 - 1;
+    const int IA1 = 40014, IA2 = 40692, IQ1 = 53668, IQ2 = 52774, IR1 = 12211, IR2 = 3791;
+    const int NDIV = 1 + IMM1 / 32;
+    const double AM = 1.0 / IM1, EPS = 1.2e-7, RNMX = 1.0 - EPS;
+    int j, k;
+    if (*idum <= 0) {
+        *idum = (-*idum > 1) ? -*idum : 1;
+        *idum2 = *idum;
+        for (j = 32 + 8; j >= 1; --j) {
+            k = *idum / IQ1;
+            *idum = IA1 * (*idum - k * IQ1) - k * IR1;
+            if (*idum < 0) *idum += IM1;
+            if (j <= 32) iv[j - 1] = *idum;
+        }
+        *iy = iv[0];
+    }
+    k = *idum / IQ1;
+    *idum = IA1 * (*idum - k * IQ1) - k * IR1;
+    if (*idum < 0) *idum += IM1;
+    k = *idum2 / IQ2;
+    *idum2 = IA2 * (*idum2 - k * IQ2) - k * IR2;
+    if (*idum2 < 0) *idum2 += IM2;
+    j = 1 + *iy / NDIV;
+    *iy = iv[j - 1] - *idum2;
+    iv[j - 1] = *idum;
+    if (*iy < 1) *iy += IMM1;
+    const double r = AM * (double)(*iy);
+    return r < RNMX ? r : RNMX;
+}
+
+// state layout: [0] idum, [1] iy, [2] idum2, [3..34] iv
+__global__ void k_draws(int* st, double* draws, const int* total)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int idum = st[0], iy = st[1], idum2 = st[2];
+    int iv[32];
+    for (int q = 0; q < 32; ++q) iv[q] = st[3 + q];
+    const int nd = 3 * (*total);
+    for (int q = 0; q < nd; ++q) draws[q] = dev_ranff2(&idum, iv, &iy, &idum2);
+    st[0] = idum; st[1] = iy; st[2] = idum2;
+    for (int q = 0; q < 32; ++q) st[3 + q] = iv[q];
+}
+
+// A droplet that appears m times in the sorted removal list consumes 3m draws at
+// offset off; the last triple is the position it keeps (RELOCATE, 1994-2006).
+__global__ void k_apply_reloc(double* pos, const int* mult, const int* off, const double* draws, int n, int cap, double pi2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int m = mult[i];
+    if (m <= 0) return;
+    const double* d = draws + 3 * (size_t)(off[i] + m - 1);
+    pos[i] = d[0] * pi2;
+    pos[(size_t)cap + i] = d[1] * pi2 + 0.0 * pi2;
+    pos[2 * (size_t)cap + i] = d[2] * pi2 + 0.0 * pi2;
+}
+
+__global__ void k_set_rng(int* st, int seed)
+{
+    if (threadIdx.x != 0) return;
+    st[0] = seed; st[1] = 0; st[2] = 0;
+    for (int q = 0; q < 32; ++q) st[3 + q] = 0;
+}
+
+int part_set_rng(const PartBuffers& pb, int iseedr, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_rng, dim3(1), dim3(1), 0, s, pb.rng_state, iseedr);
+    DDNS_HIP(hipGetLastError());
+    return 0;
+}
+
+// relocate every droplet with mult > 0; returns the number of draws triples via *total_host
+int relocate_by_mult(PartBuffers& pb, const PartConst& pc, int* total_host, hipStream_t s)
+{
+    int st = exclusive_scan(pb.mult, pb.draw_off, pc.n, (int*)pb.scan_tmp, s);
+    if (st) return st;
+    int total = 0;
+    DDNS_HIP(hipMemcpyAsync(&total, pb.draw_off + pc.n, sizeof(int), hipMemcpyDeviceToHost, s));
+    DDNS_HIP(hipStreamSynchronize(s));
+    *total_host = total;
+    if (total == 0) return 0;
+    if (3 * total > pb.draw_cap) {
+        DDNS_HIP(hipFree(pb.draws));
+        pb.draw_cap = 3 * total + 1024;
+        DDNS_HIP(hipMalloc(&pb.draws, (size_t)pb.draw_cap * sizeof(double)));
+    }
+    hipLaunchKernelGGL(k_draws, dim3(1), dim3(1), 0, s, pb.rng_state, pb.draws, pb.draw_off + pc.n);
+    hipLaunchKernelGGL(k_apply_reloc, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.pos, pb.mult, pb.draw_off, pb.draws,
+                       pc.n, pb.cap, pc.pi2);
+    DDNS_HIP(hipGetLastError());
+    return 0;
+}
+
+int part_update_and_wrap(const PartBuffers& pb, const PartConst& pc, bool relocate, hipStream_t s)
+{
+    if (pc.n == 0) return 0;
+    hipLaunchKernelGGL(k_update_pos, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.pos, pb.dmove, pb.fvp, pb.fdp, pc.n, pb.cap);
+    DDNS_HIP(hipGetLastError());
+    if (relocate) {
+        int tot = 0;
+        int st = relocate_by_mult(const_cast<PartBuffers&>(pb), pc, &tot, s);
+        if (st) return st;
+    }
+    hipLaunchKernelGGL(k_wrap, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.pos, pc.n, pb.cap, pc.pi2);
+    DDNS_HIP(hipGetLastError());
+    return 0;
+}
+
+// ------------------------------------------------------------------ coarse cell grid
+// A periodic grid of G^3 cells of width hc = 2pi/G >= the largest pair distance the
+// caller needs, so a 27-cell neighbourhood holds every candidate.  Counting sort by
+// cell, then each cell's slice is ordered by slot number (deterministic).
+__global__ void k_cell_key(const double* pos, int* cell_of, int* cnt, int n, int cap, int G, double hc)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int c3[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        int q = (int)(pos[(size_t)c * cap + i] / hc);
+        q = q < 0 ? 0 : (q >= G ? G - 1 : q);
+        c3[c] = q;
+    }
+    const int cell = (c3[0] * G + c3[1]) * G + c3[2];
+    cell_of[i] = cell;
+    atomicAdd(cnt + cell, 1);
+}
+
+__global__ void k_cell_fill(const int* cell_of, const int* start, int* cursor, int* sorted, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = cell_of[i];
+    const int p = start[c] + atomicAdd(cursor + c, 1);
+    sorted[p] = i;
+}
+
+__global__ void k_cell_order(const int* start, int* sorted, int ncell)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int b = start[c], e = start[c + 1];
+    for (int i = b + 1; i < e; ++i) {
+        const int v = sorted[i];
+        int j = i - 1;
+        while (j >= b && sorted[j] > v) { sorted[j + 1] = sorted[j]; --j; }
+        sorted[j + 1] = v;
+    }
+}
+
+int part_build_cells(PartBuffers& pb, const PartConst& pc, double hmin, hipStream_t s)
+{
+    constexpr int GMAX = 256;
+    int G = (int)floor(pc.pi2 / hmin);
+    if (G > GMAX) G = GMAX;
+    if (G < 3) { set_error("domain too small for the pair search grid (need 3 cells of the interaction radius)"); return DDNS_ERR_DOMAIN; }
+    const long long ncell = (long long)G * G * G;
+    if (!pb.cell_count) {
+        // sized once for the finest grid, so switching between the collision and the
+        // overlap grids never reallocates
+        const long long nmax = (long long)GMAX * GMAX * GMAX + 1;
+        DDNS_HIP(hipMalloc(&pb.cell_count, nmax * sizeof(int)));
+        DDNS_HIP(hipMalloc(&pb.cell_start, nmax * sizeof(int)));
+        DDNS_HIP(hipMalloc(&pb.cursor, nmax * sizeof(int)));
+        const size_t need = (size_t)((std::max(nmax, (long long)pb.cap + 1) + 1023) / 1024 + 8) * sizeof(int);
+        if (need > pb.scan_tmp_bytes) {
+            (void)hipFree(pb.scan_tmp);
+            DDNS_HIP(hipMalloc(&pb.scan_tmp, need));
+            pb.scan_tmp_bytes = need;
+        }
+    }
+    pb.G = G;
+    pb.hc = pc.pi2 / G;
+    DDNS_HIP(hipMemsetAsync(pb.cell_count, 0, (ncell + 1) * sizeof(int), s));
+    DDNS_HIP(hipMemsetAsync(pb.cursor, 0, (ncell + 1) * sizeof(int), s));
+    if (pc.n > 0)
+        hipLaunchKernelGGL(k_cell_key, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.pos, pb.cell_of, pb.cell_count, pc.n, pb.cap, G, pb.hc);
+    int st = exclusive_scan(pb.cell_count, pb.cell_start, (int)ncell, (int*)pb.scan_tmp, s);
+    if (st) return st;
+    if (pc.n > 0) {
+        hipLaunchKernelGGL(k_cell_fill, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.cell_of, pb.cell_start, pb.cursor, pb.sorted, pc.n);
+        hipLaunchKernelGGL(k_cell_order, dim3(nblk(ncell)), dim3(PT), 0, s, pb.cell_start, pb.sorted, (int)ncell);
+    }
+    DDNS_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace ddns
