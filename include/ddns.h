@@ -132,6 +132,9 @@ typedef struct ddns_timing {
     double total_ms;          /* whole call                                            */
     int    nsteps;
     int    pad;
+    /* flow kernels, summed over the call: inverse y stage, fused x stage, forward y
+       stage, low-mode pre-kernel, fused z stage, kx = 0 symmetrisation */
+    double flow_kernel_ms[6];
 } ddns_timing;
 
 typedef struct ddns_handle ddns_handle;

@@ -59,8 +59,9 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
 def _dp(a):
     if a is None:
         return None
-    if a.dtype != np.float64 or not (a.flags["F_CONTIGUOUS"] or a.flags["C_CONTIGUOUS"]):
-        raise TypeError("expected a contiguous float64 array")
+    if a.dtype != np.float64 or not a.flags["F_CONTIGUOUS"]:
+        # the C ABI takes the reference's column-major arrays
+        raise TypeError("expected a Fortran-contiguous float64 array")
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 

@@ -68,6 +68,7 @@ class Timing(C.Structure):
         ("flow_ms", C.c_double), ("interp_ms", C.c_double), ("cells_ms", C.c_double),
         ("hdi_ms", C.c_double), ("advance_ms", C.c_double), ("colldet_ms", C.c_double),
         ("overlap_ms", C.c_double), ("total_ms", C.c_double), ("nsteps", C.c_int), ("pad", C.c_int),
+        ("flow_kernel_ms", C.c_double * 6),
     ]
 
 

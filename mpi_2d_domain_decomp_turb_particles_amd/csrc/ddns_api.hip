@@ -54,6 +54,7 @@ struct ddns_handle {
     ddns_collision_counts cc{};
     ddns_timing timing{};
     hipEvent_t ev[16] = {};
+    hipEvent_t fev[8] = {};      // flow kernel boundaries
     std::string err;
 };
 
@@ -326,6 +327,8 @@ extern "C" int ddns_create(const ddns_params* pin, ddns_handle** out)
     if (st) { ddns_destroy(h); return st; }
     for (int i = 0; i < 16; ++i)
         if (hipEventCreate(&h->ev[i]) != hipSuccess) { ddns_destroy(h); return fail(nullptr, DDNS_ERR_HIP, "event"); }
+    for (int i = 0; i < 8; ++i)
+        if (hipEventCreate(&h->fev[i]) != hipSuccess) { ddns_destroy(h); return fail(nullptr, DDNS_ERR_HIP, "event"); }
     if (hipStreamSynchronize(h->stream) != hipSuccess) { ddns_destroy(h); return fail(nullptr, DDNS_ERR_HIP, "sync"); }
     *out = h;
     return DDNS_OK;
@@ -338,6 +341,7 @@ extern "C" int ddns_destroy(ddns_handle* h)
     flow_free(h->fb);
     part_free(h->pb);
     for (int i = 0; i < 16; ++i) if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
+    for (int i = 0; i < 8; ++i) if (h->fev[i]) (void)hipEventDestroy(h->fev[i]);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return DDNS_OK;
@@ -471,20 +475,25 @@ extern "C" int ddns_init_flow(ddns_handle* h, int kind, int seed, double k0, dou
             }
             for (int s = 1; s <= nek; ++s) target[s] *= 1.5 * urms * urms / tot;
         }
+        std::vector<double> fac(nek + 2, 1.0);
         for (int s = 1; s <= nek; ++s) {
-            double e1;
             if (kind == DDNS_SPEC_K4_GAUSS) {
-                e1 = e_t[s] > 0 ? std::sqrt(target[s] / e_t[s]) : 0.0;
+                fac[s] = e_t[s] > 0 ? std::sqrt(target[s] / e_t[s]) : 0.0;
             } else {
                 const double vol = 4. / 3. * pi * ((s + 0.5) * (s + 0.5) * (s + 0.5) - (s - 0.5) * (s - 0.5) * (s - 0.5));
                 const double ccc = h->p.force[0] * 4. / 3. * pi * (1.5 * 1.5 * 1.5 - 0.5 * 0.5 * 0.5) / enumv[1];
-                e1 = std::sqrt(ccc / e_t[s] * enumv[s] / vol) / std::pow((double)s, 0.833333);
+                fac[s] = std::sqrt(ccc / e_t[s] * enumv[s] / vol) / std::pow((double)s, 0.833333);
             }
-            for (int k = 0; k < N; ++k)
-                for (int j = 0; j < L; ++j)
-                    for (int i = 0; i < N; ++i)
-                        if (ik2v(i, j, k) == s) { const size_t o = I(i, j, k); u[0][o] *= e1; u[1][o] *= e1; u[2][o] *= e1; }
         }
+        // every mode belongs to exactly one shell: one pass applies all the shell factors
+        for (int k = 0; k < N; ++k)
+            for (int j = 0; j < L; ++j)
+                for (int i = 0; i < N; ++i) {
+                    const int s = ik2v(i, j, k);
+                    if (s < 1 || s > nek) continue;
+                    const size_t o = I(i, j, k);
+                    u[0][o] *= fac[s]; u[1][o] *= fac[s]; u[2][o] *= fac[s];
+                }
         for (int c = 0; c < 3; ++c)                                           // dealiasing, 139-141
             for (int k = 0; k < N; ++k)
                 for (int j = 0; j < L; ++j)
@@ -717,6 +726,7 @@ extern "C" int ddns_step(ddns_handle* h, int nsteps)
     int status = 0, st;
     float pms[6] = {0, 0, 0, 0, 0, 0};
     double flow_ms = 0;
+    double kms[6] = {0, 0, 0, 0, 0, 0};
     DDNS_HIP(hipEventRecord(h->ev[15], s));
     for (int k = 0; k < nsteps; ++k) {
         h->istep += 1;                                                        // turb.f90:492
@@ -726,7 +736,7 @@ extern "C" int ddns_step(ddns_handle* h, int nsteps)
             if (st) return keep(h, st);
             h->a_valid = true;
         }
-        st = flow_step(h->fb, h->fc, h->prec, false, s, nullptr);             // inverse y, x stage
+        st = flow_step(h->fb, h->fc, h->prec, false, s, h->fev);              // inverse y, x stage
         if (st) return keep(h, st);
         DDNS_HIP(hipEventRecord(h->ev[1], s));
         h->phys_valid = true;
@@ -737,10 +747,18 @@ extern "C" int ddns_step(ddns_handle* h, int nsteps)
             if (h->n < h->p.npart / 2) return fail(h, DDNS_ERR_DROPLET_LOSS, "npart_t < npart/2 (turb.f90:600-604)");
         }
         DDNS_HIP(hipEventRecord(h->ev[2], s));
-        st = flow_step(h->fb, h->fc, h->prec, true, s, nullptr);              // forward, update, 644-691
+        st = flow_step(h->fb, h->fc, h->prec, true, s, h->fev);               // forward, update, 644-691
         if (st) return keep(h, st);
         DDNS_HIP(hipEventRecord(h->ev[3], s));
         DDNS_HIP(hipEventSynchronize(h->ev[3]));
+        {
+            const int pairs[6][2] = {{0, 1}, {1, 2}, {3, 4}, {4, 5}, {5, 6}, {6, 7}};
+            for (int q = 0; q < 6; ++q) {
+                float t;
+                DDNS_HIP(hipEventElapsedTime(&t, h->fev[pairs[q][0]], h->fev[pairs[q][1]]));
+                kms[q] += t;
+            }
+        }
         float t1, t2;
         DDNS_HIP(hipEventElapsedTime(&t1, h->ev[0], h->ev[1]));
         DDNS_HIP(hipEventElapsedTime(&t2, h->ev[2], h->ev[3]));
@@ -761,6 +779,7 @@ extern "C" int ddns_step(ddns_handle* h, int nsteps)
     h->timing.overlap_ms = pms[5];
     h->timing.total_ms = tot;
     h->timing.nsteps = nsteps;
+    for (int q = 0; q < 6; ++q) h->timing.flow_kernel_ms[q] = kms[q];
     return status;
 }
 

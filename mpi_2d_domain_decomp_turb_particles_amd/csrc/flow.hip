@@ -465,11 +465,19 @@ __device__ double dev_ranff(int* idum, int* iy, int* iv)
     int j, k;
     if (*idum <= 0 || *iy == 0) {
         *idum = (-*idum > 1) ? -*idum : 1;
-        for (j = NTAB_ + 8; j >= 1; --j) {
+        // 8 warm-up draws, then the table from iv[31] down to iv[0]: the same
+        // sequence as one loop j = 40..1 storing at j <= 32, written so that no
+        // store index is ever outside the table
+        for (j = 0; j < 8; ++j) {
             k = *idum / IQ;
             *idum = IA * (*idum - k * IQ) - IR * k;
             if (*idum < 0) *idum += IM;
-            if (j <= NTAB_) iv[j - 1] = *idum;
+        }
+        for (j = NTAB_ - 1; j >= 0; --j) {
+            k = *idum / IQ;
+            *idum = IA * (*idum - k * IQ) - IR * k;
+            if (*idum < 0) *idum += IM;
+            iv[j] = *idum;
         }
         *iy = iv[0];
     }
@@ -493,7 +501,10 @@ __global__ void __launch_bounds__(NT) k_pre(PreArgs a)
         if (tid != 0) return;
         int* st = a.sth_state;
         int idum = st[0], iy = st[1];
-        int iv[32];
+        // shuffle table in LDS: a private array indexed by a data-dependent j is
+        // lowered to indexed register moves, which gives no protection at all
+        // against a stray index
+        __shared__ int iv[32];
         for (int i = 0; i < 32; ++i) iv[i] = st[2 + i];
         const double dt = a.dt, tf = a.ou_tf, var = a.ou_var, pi2 = a.pi2;
         double fr[3][150];

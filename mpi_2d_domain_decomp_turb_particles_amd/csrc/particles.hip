@@ -342,11 +342,18 @@ __device__ double dev_ranff2(int* idum, int* iv, int* iy, int* idum2)
     if (*idum <= 0) {
         *idum = (-*idum > 1) ? -*idum : 1;
         *idum2 = *idum;
-        for (j = 32 + 8; j >= 1; --j) {
+        // 8 warm-up draws, then iv[31] .. iv[0]: the loop j = 40..1 with a store at
+        // j <= 32, split so that no store index is ever outside the table
+        for (j = 0; j < 8; ++j) {
             k = *idum / IQ1;
             *idum = IA1 * (*idum - k * IQ1) - k * IR1;
             if (*idum < 0) *idum += IM1;
-            if (j <= 32) iv[j - 1] = *idum;
+        }
+        for (j = 31; j >= 0; --j) {
+            k = *idum / IQ1;
+            *idum = IA1 * (*idum - k * IQ1) - k * IR1;
+            if (*idum < 0) *idum += IM1;
+            iv[j] = *idum;
         }
         *iy = iv[0];
     }
@@ -369,7 +376,9 @@ __global__ void k_draws(int* st, double* draws, const int* total)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     int idum = st[0], iy = st[1], idum2 = st[2];
-    int iv[32];
+    // shuffle table in LDS, not in a private array that the compiler turns into
+    // indexed register moves
+    __shared__ int iv[32];
     for (int q = 0; q < 32; ++q) iv[q] = st[3 + q];
     const int nd = 3 * (*total);
     for (int q = 0; q < nd; ++q) draws[q] = dev_ranff2(&idum, iv, &iy, &idum2);

@@ -92,8 +92,9 @@ def lib() -> C.CDLL:
 def dp(a):
     if a is None:
         return None
-    if a.dtype != np.float64 or not (a.flags["F_CONTIGUOUS"] or a.flags["C_CONTIGUOUS"]):
-        raise TypeError("expected contiguous float64")
+    if a.dtype != np.float64 or not a.flags["F_CONTIGUOUS"]:
+        # every multi-dimensional array crosses in Fortran (column-major) order
+        raise TypeError("expected a Fortran-contiguous float64 array")
     return a.ctypes.data_as(_D)
 
 

@@ -66,3 +66,31 @@ def random_droplets(n, seed, radius, domain=2 * np.pi):
     yp[6, :] = r if r.ndim == 0 else r[rng.integers(0, len(r), size=n)]
     yp[7, :] = np.arange(1, n + 1)
     return yp
+
+
+def kgrid(N):
+    """Wavenumbers of every entry of the reference spectral array (N+2, N+2, N):
+    row i -> kz, column j -> kx = j // 2 (re/im pair), plane k -> ky.  Padding rows
+    i >= N carry k = 0."""
+    L = N + 2
+    i = np.arange(L)
+    kz = np.where(i < N, np.where(i <= N // 2, i, i - N), 0).astype(float)
+    kx = (np.arange(L) // 2).astype(float)
+    k3 = np.arange(N)
+    ky = np.where(k3 <= N // 2, k3, k3 - N).astype(float)
+    KZ, KX, KY = np.meshgrid(kz, kx, ky, indexing="ij")
+    KX = KX.copy(); KY = KY.copy()
+    KX[N:, :, :] = 0
+    KY[N:, :, :] = 0
+    return KZ, KX, KY
+
+
+def to_spectral(oracle_mod, f, N):
+    """Physical field f[x, y, z] (N^3) -> reference spectral array via the oracle FFT."""
+    phys = np.zeros((N + 2, N + 2, N), order="F")
+    phys[:N, :N, :] = f
+    return oracle_mod.fft_r2c(phys)
+
+
+def to_physical(oracle_mod, c, N):
+    return oracle_mod.fft_c2r(c)[:N, :N, :]

@@ -140,3 +140,47 @@ def test_droplet_steps_match_oracle(ddns, oracle_mod):
         gv, _ = s.get_flow()
         ov, _ = o.get_flow()
         assert rel_l2(gv, ov) < 1e-11
+
+
+def test_overlap_relocation_matches_oracle(ddns, oracle_mod):
+    """OVERLAP_DET sweeps that really relocate droplets (partlib.f90:2047-2167,
+    RELOCATE 1984-2012): the relocation draws come from one ranff2 stream, so the
+    relocated positions and the sweep/relocation counts must be identical."""
+    N = 32
+    p = _params(ddns, N=N, npart=20000, nset=2, radius=[60.0, 90.0], nonoverlap=1)
+    with oracle_mod.Oracle(p) as o, ddns.Solver(p) as s:
+        rad = o.scaling()["radius"][:2]
+        yp = random_droplets(20000, 13, rad)
+        o.set_droplets(yp)
+        s.set_droplets(yp)
+        o.overlap_fix()
+        s.overlap_fix()
+        co, cg = o.collision_counts(), s.collision_counts()
+        assert co.noverlap_relocated > 5, co.noverlap_relocated
+        assert (cg.noverlap_sweeps, cg.noverlap_relocated) == (co.noverlap_sweeps, co.noverlap_relocated)
+        dg, do = s.get_droplets(), o.get_droplets()
+        assert np.array_equal(dg["yp"], do["yp"])
+
+
+def test_droplet_steps_with_collisions_match_oracle(ddns, oracle_mod):
+    """Three full steps with enough droplets that collisions occur and the
+    colliders are relocated (turb.f90:590, istep < nstep_grow)."""
+    N = 32
+    p = _params(ddns, N=N, npart=40000, nset=2, radius=[60.0, 90.0], forcing=1, dt=0.004, rnu=0.02, nonoverlap=1)
+    v = random_spectral(N, seed=9, kmax=6, amp=4.0)
+    with ddns.Solver(p) as s, oracle_mod.Oracle(p) as o:
+        s.set_flow(v, v)
+        o.set_flow(v, v)
+        s.init_droplets(seed=4711)
+        o.init_droplets(seed=4711)
+        assert np.array_equal(s.get_droplets()["yp"], o.get_droplets()["yp"])
+        ncoll = 0
+        for _ in range(3):
+            s.step(1)
+            o.step(1)
+            cg, co = s.collision_counts(), o.collision_counts()
+            assert cg.ncoll == co.ncoll and list(cg.icc) == list(co.icc)
+            ncoll += co.ncoll
+            dg, do = s.get_droplets(), o.get_droplets()
+            assert np.allclose(dg["yp"], do["yp"], rtol=1e-12, atol=1e-12)
+        assert ncoll > 0
