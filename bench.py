@@ -69,6 +69,20 @@ def kernel_bytes(N: int, prec: int):
     }
 
 
+def measured_traffic(kernel, N, prec):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            tab = json.load(f)
+    except (OSError, ValueError):
+        return None
+    key = f"{kernel}/N{N}/{'fp64' if prec == dd.FP64 else 'fp32'}"
+    ent = tab.get(key)
+    return None if ent is None else ent.get("bytes_per_launch")
+
+
 def timed_run(s, steps, warmup):
     if warmup:
         s.step(warmup)
@@ -105,6 +119,8 @@ def main(argv=None):
     ap.add_argument("--npart", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--only-fp32", action="store_true",
+                    help="time the FP32-mixed path only (for profiler runs)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU: the whole periodic domain lives on a single MI355X; --gpus {a.gpus} is not supported", file=sys.stderr)
@@ -127,7 +143,15 @@ def main(argv=None):
     achieved = kb[dom] / (per_launch[dom] * 1e-3) / 1e9
     flow_bytes = sum(kb.values())
     flow_ms = t32.flow_ms / a.steps
+    copy_ms, copy_bytes = s32.profile_copy(20)
     s32.close()
+    if a.only_fp32:
+        print(json.dumps({"metric": "fp32-mixed steps/s (profiling run)", "value": steps_per_s,
+                          "flow_kernel_ms_per_launch": per_launch, "phase_ms_per_step": {
+                              "flow": flow_ms, "cells": t32.cells_ms / a.steps, "interp": t32.interp_ms / a.steps,
+                              "hdi": t32.hdi_ms / a.steps, "advance": t32.advance_ms / a.steps,
+                              "colldet": t32.colldet_ms / a.steps, "overlap": t32.overlap_ms / a.steps}}))
+        return 0
 
     # ---- FP64 plain path, same initial state
     log(f"FP32-mixed {steps_per_s:.3f} steps/s; FP64 timed run")
@@ -200,7 +224,9 @@ def main(argv=None):
         },
         "flow_kernel_ms_per_launch": per_launch,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, N, dd.FP32_MIXED),
+                     "algorithmic_bytes": kb[dom]},
+        "hbm_copy_GBps": copy_bytes / (copy_ms * 1e-3) / 1e9,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

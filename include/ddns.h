@@ -251,9 +251,17 @@ int ddns_fft_c2r(ddns_handle* h, double* v);
 int ddns_collision_detect(ddns_handle* h, int n, const double* yp, const double* dmove,
                           const double* fvp0, const double* vnew, const double* vp);
 
-/* Kernel timing helper: run the flow step alone k times and return the mean time of
- * each kernel class (ms), for roofline reporting. names: "x", "yf", "z", "yi", "pre", "sym". */
+/* Kernel timing helper (no counterpart in the reference): run the flow step alone k
+ * times and return the mean time (ms) of, in order: inverse y stage, fused x stage,
+ * forward y stage, low-mode pre-kernel, fused z stage, kx = 0 symmetrisation, whole
+ * flow step.  nkernels <= 7 values are written. */
 int ddns_profile_flow(ddns_handle* h, int k, double* ms_per_kernel, int nkernels);
+
+/* HBM calibration helper (no counterpart in the reference): a plain streaming copy
+ * between two of the handle's spectral work buffers, run reps times.  Returns the mean
+ * time per launch (ms) and the bytes the copy moves per launch (read + write), so
+ * that profiler byte counters can be scaled against a kernel of known traffic. */
+int ddns_profile_copy(ddns_handle* h, int reps, double* ms_per_launch, double* bytes_per_launch);
 
 #ifdef __cplusplus
 }

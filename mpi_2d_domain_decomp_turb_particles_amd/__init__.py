@@ -240,3 +240,9 @@ class Solver:
         ms = np.zeros(7)
         self._check(self.lib.ddns_profile_flow(self.h, k, _dp(ms), 7))
         return dict(zip(("y_inv", "x_fused", "y_fwd", "pre", "z_fused", "sym", "step"), ms))
+
+    def profile_copy(self, reps: int = 20):
+        """Mean time (ms) and bytes moved per launch of a plain HBM copy kernel."""
+        ms, by = C.c_double(), C.c_double()
+        self._check(self.lib.ddns_profile_copy(self.h, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value

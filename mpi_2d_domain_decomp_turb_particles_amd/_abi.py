@@ -166,4 +166,5 @@ SIGNATURES = {
     "ddns_fft_c2r": (C.c_int, [_P, _D]),
     "ddns_collision_detect": (C.c_int, [_P, C.c_int, _D, _D, _D, _D, _D]),
     "ddns_profile_flow": (C.c_int, [_P, C.c_int, _D, C.c_int]),
+    "ddns_profile_copy": (C.c_int, [_P, C.c_int, _D, _D]),
 }

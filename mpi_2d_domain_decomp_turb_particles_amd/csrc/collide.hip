@@ -124,7 +124,7 @@ struct CollArgs {
     double* bins;          // wr, wx, wy, wz, ur, theta, wrtan (each nset*nset*NBIN)
     int* counts;           // [0..2] icc, [8..107] colltype, [108..207] nir, [256] npairs
     long long* tested;     // [0] evaluated, [1] within 10 rcoll
-    long long* pairs;      // 4 per pair
+    long long* pairs;      // DDNS_PAIR_REC per pair, see k_collide
     int pair_cap;
     int* mult;
 };
@@ -284,12 +284,26 @@ __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
                     atomicAdd(a.mult + ib, 1);
                     const int slot = atomicAdd(a.counts + 256, 1);
                     if (slot < a.pair_cap) {
+                        // the collist record (1555-1562) plus where the reference meets
+                        // the pair: ih1, the cell offset (i, j, k) of its loop and the
+                        // partner's ypn index (halo images follow the locals, by HEADLIST
+                        // direction 5..8, then by droplet index)
                         const long long ga = (long long)a.gid[ia], gb = (long long)a.gid[ib];
                         const long long lo = ga < gb ? ga : gb, hi = ga < gb ? gb : ga;
-                        a.pairs[4 * (size_t)slot + 0] = (lo << 32) | (hi << 2) | type;
-                        a.pairs[4 * (size_t)slot + 1] = ia;
-                        a.pairs[4 * (size_t)slot + 2] = ib;
-                        a.pairs[4 * (size_t)slot + 3] = ((long long)(di + 8) << 16) | ((dj + 8) << 8) | (dk + 8);
+                        const int dir = (wy == 0 && wz == 0) ? 0 : (wz == 0 ? 5 : (wy == 1 ? 6 : (wy == 0 ? 7 : 8)));
+                        const long long ypn = (long long)dir * a.cap + ib;
+                        const int off = ((di + 1) * 3 + (dj + 1)) * 3 + (dk + 1);
+                        long long* r = a.pairs + DDNS_PAIR_REC * (size_t)slot;
+                        r[0] = (lo << 32) | (hi << 2) | type;
+                        r[1] = ia;
+                        r[2] = ib;
+                        r[3] = ((long long)(di + 8) << 16) | ((dj + 8) << 8) | (dk + 8);
+                        r[4] = dir;
+                        r[5] = __double_as_longlong(ra);
+                        r[6] = __double_as_longlong(rb);
+                        // visiting order: ih1 ascending, then the (i, j, k) loop, then the
+                        // linked list of the cell, which runs from high to low ypn index
+                        r[7] = ((long long)ia << 32) | ((long long)off << 27) | (((1LL << 27) - 1) - ypn);
                     }
                 }
             }
@@ -470,7 +484,7 @@ __global__ void k_gather_keys(const long long* pairs, long long* keys, int n, in
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
-    keys[i] = i < n ? pairs[4 * (size_t)i] : 0x7fffffffffffffffLL;
+    keys[i] = i < n ? pairs[DDNS_PAIR_REC * (size_t)i] : 0x7fffffffffffffffLL;
 }
 
 int part_read_pairs(PartBuffers& pb, const PartConst& pc, int64_t* triples, int cap, int* npairs, hipStream_t s)
@@ -504,6 +518,144 @@ int part_read_pairs(PartBuffers& pb, const PartConst& pc, int64_t* triples, int 
         triples[3 * i + 1] = (h[i] >> 2) & 0x3fffffff;
         triples[3 * i + 2] = h[i] & 3;
     }
+    return 0;
+}
+
+// ------------------------------------------------------------------ coalescence
+// (key, value) sort of up to 4096 entries in one work-group
+__global__ void __launch_bounds__(1024) k_sort_kv_block(long long* keys, int* vals, int P)
+{
+    __shared__ long long sk[4096];
+    __shared__ int sv[4096];
+    for (int i = threadIdx.x; i < P; i += blockDim.x) { sk[i] = keys[i]; sv[i] = vals[i]; }
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < P; i += blockDim.x) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const long long x = sk[i], y = sk[l];
+                    const bool up = (i & k) == 0;
+                    if ((x > y) == up) { sk[i] = y; sk[l] = x; const int t = sv[i]; sv[i] = sv[l]; sv[l] = t; }
+                }
+            }
+            __syncthreads();
+        }
+    for (int i = threadIdx.x; i < P; i += blockDim.x) { keys[i] = sk[i]; vals[i] = sv[i]; }
+}
+
+__global__ void k_bitonic_kv_step(long long* keys, int* vals, int P, int j, int k)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const int l = i ^ j;
+    if (l > i) {
+        const long long x = keys[i], y = keys[l];
+        const bool up = (i & k) == 0;
+        if ((x > y) == up) { keys[i] = y; keys[l] = x; const int t = vals[i]; vals[i] = vals[l]; vals[l] = t; }
+    }
+}
+
+__global__ void k_gather_order(const long long* pairs, long long* keys, int* vals, int n, int P)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    keys[i] = i < n ? pairs[DDNS_PAIR_REC * (size_t)i + 7] : 0x7fffffffffffffffLL;
+    vals[i] = i < n ? i : -1;
+}
+
+struct CoalArgs {
+    double *pos, *vel, *rad, *gid, *vp, *dmove, *fvp, *fdp;
+    const long long* pairs;
+    const int* order;      // record slots in the reference's visiting order
+    int npairs, n, cap;
+    double Sv;
+    int* removed;
+};
+
+// COALESCE, partlib.f90:3642-3720, applied pair by pair in the reference's order by
+// one wavefront: ih1 takes the volume-equivalent radius and the terminal velocity;
+// a partner that is a droplet of this domain (not a halo image) is overwritten by
+// the current last droplet, whose slot is cleared.  Indices are the ones recorded at
+// detection, as in the reference, so a droplet moved by an earlier pair is addressed
+// by its old slot.  35 state columns are copied, one per lane.
+__global__ void __launch_bounds__(64) k_coalesce(CoalArgs a)
+{
+    const int lane = threadIdx.x;
+    const size_t C = (size_t)a.cap;
+    int jj = 0;
+    for (int q = 0; q < a.npairs; ++q) {
+        const long long* r = a.pairs + DDNS_PAIR_REC * (size_t)a.order[q];
+        const int p1 = (int)r[1], p2 = (int)r[2], dir = (int)r[4];
+        if (lane == 0) {
+            const double r1 = __longlong_as_double(r[5]), r2 = __longlong_as_double(r[6]);
+            const double an = pow(pow(r1, 3.0) + pow(r2, 3.0), 1.0 / 3.0);
+            a.rad[p1] = an;
+            a.vel[p1] = a.Sv * (an * an);
+            a.vel[C + p1] = 0.0;
+            a.vel[2 * C + p1] = 0.0;
+        }
+        __threadfence();
+        __syncthreads();
+        if (dir == 0) {
+            if (jj >= a.n) { jj = -1; break; }       // more removals than droplets
+            ++jj;
+            const size_t last = (size_t)(a.n - jj);
+            double* col = nullptr;
+            if (lane < 3) col = a.pos + lane * C;
+            else if (lane < 6) col = a.vel + (lane - 3) * C;
+            else if (lane == 6) col = a.rad;
+            else if (lane == 7) col = a.gid;
+            else if (lane < 11) col = a.vp + (lane - 8) * C;
+            else if (lane < 14) col = a.dmove + (lane - 11) * C;
+            else if (lane < 23) col = a.fvp + (lane - 14) * C;
+            else if (lane < 35) col = a.fdp + (lane - 23) * C;
+            if (col) {
+                col[p2] = col[last];
+                col[last] = 0.0;
+            }
+            __threadfence();
+            __syncthreads();
+        }
+    }
+    if (lane == 0) *a.removed = jj;
+}
+
+int part_coalesce(PartBuffers& pb, const PartConst& pc, int* removed, hipStream_t s)
+{
+    *removed = 0;
+    int np = 0;
+    DDNS_HIP(hipMemcpyAsync(&np, pb.counts + 256, sizeof(int), hipMemcpyDeviceToHost, s));
+    DDNS_HIP(hipStreamSynchronize(s));
+    if (np == 0) return 0;
+    if (np > pb.pair_cap) { set_error("collision pair list overflow"); return DDNS_ERR_CAPACITY; }
+    int P = 1;
+    while (P < np) P <<= 1;
+    long long* keys = nullptr;
+    int* vals = nullptr;
+    DDNS_HIP(hipMalloc(&keys, (size_t)P * sizeof(long long)));
+    DDNS_HIP(hipMalloc(&vals, (size_t)P * sizeof(int)));
+    hipLaunchKernelGGL(k_gather_order, dim3((P + 255) / 256), dim3(256), 0, s, pb.pairs, keys, vals, np, P);
+    if (P <= 4096) {
+        hipLaunchKernelGGL(k_sort_kv_block, dim3(1), dim3(1024), 0, s, keys, vals, P);
+    } else {
+        for (int k = 2; k <= P; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1)
+                hipLaunchKernelGGL(k_bitonic_kv_step, dim3((P + 255) / 256), dim3(256), 0, s, keys, vals, P, j, k);
+    }
+    CoalArgs a{};
+    a.pos = pb.pos; a.vel = pb.vel; a.rad = pb.rad; a.gid = pb.gid; a.vp = pb.vp; a.dmove = pb.dmove;
+    a.fvp = pb.fvp; a.fdp = pb.fdp; a.pairs = pb.pairs; a.order = vals; a.npairs = np; a.n = pc.n;
+    a.cap = pb.cap; a.Sv = pc.Sv_const; a.removed = pb.flags + 3;
+    hipLaunchKernelGGL(k_coalesce, dim3(1), dim3(64), 0, s, a);
+    DDNS_HIP(hipGetLastError());
+    int rm = 0;
+    DDNS_HIP(hipMemcpyAsync(&rm, pb.flags + 3, sizeof(int), hipMemcpyDeviceToHost, s));
+    DDNS_HIP(hipStreamSynchronize(s));
+    DDNS_HIP(hipFree(keys));
+    DDNS_HIP(hipFree(vals));
+    if (rm < 0) { set_error("coalescence would remove every droplet"); return DDNS_ERR_DROPLET_LOSS; }
+    *removed = rm;
     return 0;
 }
 

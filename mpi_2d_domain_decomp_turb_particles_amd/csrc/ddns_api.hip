@@ -688,6 +688,16 @@ static int droplet_block(ddns_handle* h, float* ms)
     const bool reloc = h->p.nonoverlap && h->istep < h->p.nstep_grow;         // 590
     st = part_update_and_wrap(h->pb, pc, reloc, s);                           // 586, 590, 608
     if (st) return st;
+    if (h->p.nonoverlap && h->istep >= h->p.nstep_grow) {                     // 591: COALESCE
+        // the wrap is per droplet and coalescence only copies whole droplets, so
+        // the two commute; the removed droplets leave the end of the arrays
+        int removed = 0;
+        st = part_coalesce(h->pb, pc, &removed, s);
+        if (st) return st;
+        h->n -= removed;
+        pc.n = h->n;
+        if (h->n < h->p.npart / 2) return fail(h, DDNS_ERR_DROPLET_LOSS, "npart_t < npart/2 (turb.f90:600-604)");
+    }
     DDNS_HIP(hipEventRecord(h->ev[13], s));
     if (h->p.nonoverlap) {                                                    // 610
         int sw = 0, rl = 0;
@@ -867,4 +877,10 @@ extern "C" int ddns_profile_flow(ddns_handle* h, int k, double* ms_per_kernel, i
         h->a_valid = true;
     }
     return keep(h, flow_profile(h->fb, h->fc, h->prec, k, ms_per_kernel, nkernels, h->stream));
+}
+
+extern "C" int ddns_profile_copy(ddns_handle* h, int reps, double* ms_per_launch, double* bytes_per_launch)
+{
+    if (!h || reps <= 0 || !ms_per_launch || !bytes_per_launch) return DDNS_ERR_ARG;
+    return keep(h, flow_profile_copy(h->fb, h->N, h->prec, reps, ms_per_launch, bytes_per_launch, h->stream));
 }

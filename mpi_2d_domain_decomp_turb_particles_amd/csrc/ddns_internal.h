@@ -19,6 +19,13 @@
 
 namespace ddns {
 
+// One detected collision pair on the device, in 64-bit words:
+//   [0] canonical key (min gid << 32 | max gid << 2 | type)   [1] ih1 slot   [2] ih2 slot
+//   [3] packed cell offsets   [4] ih2 image direction (0: the droplet itself, 5..8:
+//   HEADLIST halo directions)   [5] radius of ih1, [6] radius of ih2 (bit patterns)
+//   [7] the reference's visiting order of the pair (ih1, cell loop, linked list)
+constexpr int DDNS_PAIR_REC = 8;
+
 int hip_fail(hipError_t e, const char* what, const char* file, int line);
 void set_error(const std::string& msg);
 
@@ -78,6 +85,7 @@ int flow_spectrum(const FlowBuffers& fb, int N, int nek, int precision, double* 
 int fft_r2c_host(const FlowBuffers& fb, int N, int precision, double* v, hipStream_t s);
 int fft_c2r_host(const FlowBuffers& fb, int N, int precision, double* v, hipStream_t s);
 int flow_profile(const FlowBuffers& fb, const FlowConst& fc, int precision, int k, double* ms, int nk, hipStream_t s);
+int flow_profile_copy(const FlowBuffers& fb, int N, int precision, int reps, double* ms, double* bytes, hipStream_t s);
 
 // ---------------------------------------------------------------- droplets
 struct PartBuffers {
@@ -121,7 +129,7 @@ struct PartBuffers {
     int* nirbin = nullptr;
     double* bins = nullptr;     // 7 * nset*nset*NBIN
     int* counts = nullptr;      // icc[3], colltype[100], nir[100]
-    long long* pairs = nullptr; // pair_cap * 4: key fields
+    long long* pairs = nullptr; // pair_cap * DDNS_PAIR_REC: one record per detected pair
     int pair_cap = 0;
     double* draws = nullptr;    // relocation draws
     int draw_cap = 0;
@@ -159,6 +167,7 @@ int part_overlap(PartBuffers& pb, const PartConst& pc, int* sweeps, int* relocat
 int part_read_counts(const PartBuffers& pb, const PartConst& pc, ddns_collision_counts* cc, hipStream_t s);
 int part_read_bins(const PartBuffers& pb, const PartConst& pc, int* nirbin, double* const bins[7], hipStream_t s);
 int part_read_pairs(PartBuffers& pb, const PartConst& pc, int64_t* triples, int cap, int* npairs, hipStream_t s);
+int part_coalesce(PartBuffers& pb, const PartConst& pc, int* removed, hipStream_t s);
 int part_set_rng(const PartBuffers& pb, int iseedr, hipStream_t s);
 
 }  // namespace ddns

@@ -1080,6 +1080,37 @@ static int profile_impl(const FlowBuffers& fb, const FlowConst& fc, int k, doubl
     return 0;
 }
 
+// plain streaming copy, 16 B per lane per iteration: the HBM reference kernel
+__global__ void __launch_bounds__(256) k_copy_calibration(const float4* __restrict__ src, float4* __restrict__ dst, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+int flow_profile_copy(const FlowBuffers& fb, int N, int precision, int reps, double* ms, double* bytes, hipStream_t s)
+{
+    const size_t cs = (precision == DDNS_FP64) ? sizeof(double2) : sizeof(float2);
+    const size_t n4 = field_elems(N) * cs / sizeof(float4);
+    hipEvent_t e0, e1;
+    DDNS_HIP(hipEventCreate(&e0));
+    DDNS_HIP(hipEventCreate(&e1));
+    const unsigned grid = 256 * 32;
+    hipLaunchKernelGGL(k_copy_calibration, dim3(grid), dim3(256), 0, s, (const float4*)fb.A[0], (float4*)fb.A[1], n4);
+    DDNS_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(k_copy_calibration, dim3(grid), dim3(256), 0, s, (const float4*)fb.A[0], (float4*)fb.A[1], n4);
+    DDNS_HIP(hipGetLastError());
+    DDNS_HIP(hipEventRecord(e1, s));
+    DDNS_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    DDNS_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = (double)t / reps;
+    *bytes = 2.0 * (double)n4 * sizeof(float4);
+    return 0;
+}
+
 int flow_profile(const FlowBuffers& fb, const FlowConst& fc, int precision, int k, double* ms, int nk, hipStream_t s)
 {
     int st = 0;

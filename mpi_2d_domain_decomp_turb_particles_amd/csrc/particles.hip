@@ -117,7 +117,7 @@ int part_alloc(PartBuffers& pb, int cap, int nset, int G)
     DDNS_HIP(hipMalloc(&pb.bins, 7 * nb * sizeof(double)));
     DDNS_HIP(hipMalloc(&pb.counts, 512 * sizeof(int)));
     pb.pair_cap = 1 << 16;
-    DDNS_HIP(hipMalloc(&pb.pairs, (size_t)pb.pair_cap * 4 * sizeof(long long)));
+    DDNS_HIP(hipMalloc(&pb.pairs, (size_t)pb.pair_cap * DDNS_PAIR_REC * sizeof(long long)));
     pb.draw_cap = 1 << 16;
     DDNS_HIP(hipMalloc(&pb.draws, (size_t)pb.draw_cap * sizeof(double)));
     DDNS_HIP(hipMalloc(&pb.rng_state, 64 * sizeof(int)));

@@ -184,3 +184,33 @@ def test_droplet_steps_with_collisions_match_oracle(ddns, oracle_mod):
             dg, do = s.get_droplets(), o.get_droplets()
             assert np.allclose(dg["yp"], do["yp"], rtol=1e-12, atol=1e-12)
         assert ncoll > 0
+
+
+def test_droplet_steps_with_coalescence_match_oracle(ddns, oracle_mod):
+    """COALESCE (partlib.f90:3642-3720) from the first step (nstep_grow = 0 would mean
+    the default; 1 makes every step a growth step): merged radii, terminal velocity,
+    removal by swapping in the last droplet, in the reference's pair order."""
+    N = 32
+    p = _params(ddns, N=N, npart=40000, nset=2, radius=[60.0, 90.0], forcing=1, dt=0.004, rnu=0.02,
+                nonoverlap=1, nstep_grow=1)
+    v = random_spectral(N, seed=9, kmax=6, amp=4.0)
+    with ddns.Solver(p) as s, oracle_mod.Oracle(p) as o:
+        s.set_flow(v, v)
+        o.set_flow(v, v)
+        s.init_droplets(seed=4711)
+        o.init_droplets(seed=4711)
+        removed = 0
+        vol0 = float(np.sum(o.get_droplets()["yp"][6] ** 3))
+        for _ in range(3):
+            s.step(1)
+            o.step(1)
+            dg, do = s.get_droplets(), o.get_droplets()
+            assert dg["yp"].shape == do["yp"].shape
+            removed = 40000 - do["yp"].shape[1]
+            assert np.array_equal(dg["yp"][7], do["yp"][7])          # same droplets in the same slots
+            assert np.allclose(dg["yp"], do["yp"], rtol=1e-13, atol=1e-13)
+            assert np.allclose(dg["fvp"], do["fvp"], rtol=1e-12, atol=1e-12)
+        assert removed > 0
+        # merged pairs keep their volume; only pairs with a halo-image partner (whose
+        # partner is not removed, as in the reference) add volume
+        assert float(np.sum(do["yp"][6] ** 3)) >= vol0 * (1 - 1e-12)
