@@ -113,6 +113,7 @@ __device__ __forceinline__ void findindex(const double* radius, int nset, double
 
 struct CollArgs {
     const double *pos, *vel, *rad, *gid, *fvp0, *dmove, *vp;
+    const double4* pk;     // (x, y, z, radius) in cell order
     const int *cell_start, *sorted;
     int n, cap, nset;
     double radius[DDNS_MAXSET];
@@ -165,14 +166,15 @@ __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
                 const int wy = qy < 0 ? -1 : (qy >= g.G ? 1 : 0);
                 const int wz = qz < 0 ? -1 : (qz >= g.G ? 1 : 0);
                 qx -= wx * g.G; qy -= wy * g.G; qz -= wz * g.G;
-                const int cell = (qx * g.G + qy) * g.G + qz;
+                const int cell = (qz * g.G + qy) * g.G + qx;
                 const int e = a.cell_start[cell + 1];
                 for (int p = a.cell_start[cell]; p < e; ++p) {
+                    if (p == t) continue;
                     const int ib = a.sorted[p];
-                    if (ib == ia) continue;
+                    const double4 cb = a.pk[p];
                     double xb[3];
                     int di, dj, dk;
-                    if (!ref_visit(g, ia, ib, xa[0], xa[1], xa[2], a.pos[ib], a.pos[C + ib], a.pos[2 * C + ib],
+                    if (!ref_visit(g, ia, ib, xa[0], xa[1], xa[2], cb.x, cb.y, cb.z,
                                    wx, wy, wz, &xb[0], &xb[1], &xb[2], &di, &dj, &dk))
                         continue;
                     // -------- partlib.f90:1330-1353
@@ -186,7 +188,7 @@ __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
                     }
                     const double dnij0 = sqrt(xd[0] * xd[0] + xd[1] * xd[1] + xd[2] * xd[2]);
                     ++tested;
-                    const double rb = a.rad[ib];
+                    const double rb = cb.w;
                     const double rcoll = ra + rb;
                     const double rcolla = a.hdi ? rcoll : rcoll * (1. - a.st / 100.);
                     const double rcollb = rcoll * (1. + a.st / 100.);
@@ -319,6 +321,7 @@ __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
 // the reference finds; each one costs three relocation draws.
 struct OverArgs {
     const double *pos, *rad;
+    const double4* pk;
     const int *cell_start, *sorted;
     int n, cap;
     PairGeom g;
@@ -331,10 +334,10 @@ __global__ void __launch_bounds__(CT) k_overlap(OverArgs a)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     const int ia = a.sorted[t];
-    const size_t C = (size_t)a.cap;
     const PairGeom& g = a.g;
-    const double xa = a.pos[ia], ya = a.pos[C + ia], za = a.pos[2 * C + ia];
-    const double ra = a.rad[ia];
+    const double4 ca = a.pk[t];
+    const double xa = ca.x, ya = ca.y, za = ca.z;
+    const double ra = ca.w;
     int cc[3];
     const double xs[3] = {xa, ya, za};
 #pragma unroll
@@ -351,20 +354,21 @@ __global__ void __launch_bounds__(CT) k_overlap(OverArgs a)
                 const int wy = qy < 0 ? -1 : (qy >= g.G ? 1 : 0);
                 const int wz = qz < 0 ? -1 : (qz >= g.G ? 1 : 0);
                 qx -= wx * g.G; qy -= wy * g.G; qz -= wz * g.G;
-                const int cell = (qx * g.G + qy) * g.G + qz;
+                const int cell = (qz * g.G + qy) * g.G + qx;
                 const int e = a.cell_start[cell + 1];
                 for (int p = a.cell_start[cell]; p < e; ++p) {
+                    if (p == t) continue;
                     const int ib = a.sorted[p];
-                    if (ib == ia) continue;
+                    const double4 cb = a.pk[p];
                     double xb, yb, zb;
                     int di, dj, dk;
-                    if (!ref_visit(g, ia, ib, xa, ya, za, a.pos[ib], a.pos[C + ib], a.pos[2 * C + ib], wx, wy, wz,
+                    if (!ref_visit(g, ia, ib, xa, ya, za, cb.x, cb.y, cb.z, wx, wy, wz,
                                    &xb, &yb, &zb, &di, &dj, &dk))
                         continue;
                     if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) continue;   // 2091
                     const double d0 = xa - xb, d1 = ya - yb, d2 = za - zb;
                     const double dnij = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-                    const double rcoll = ra + a.rad[ib];
+                    const double rcoll = ra + cb.w;
                     if (dnij <= rcoll) ++cnt;
                 }
             }
@@ -401,7 +405,7 @@ int part_collide(PartBuffers& pb, const PartConst& pc, hipStream_t s)
     if (pc.n == 0) return 0;
     CollArgs a{};
     a.pos = pb.pos; a.vel = pb.vel; a.rad = pb.rad; a.gid = pb.gid; a.fvp0 = pb.fvp; a.dmove = pb.dmove; a.vp = pb.vp;
-    a.cell_start = pb.cell_start; a.sorted = pb.sorted;
+    a.cell_start = pb.cell_start; a.sorted = pb.sorted; a.pk = pb.pk;
     a.n = pc.n; a.cap = pb.cap; a.nset = pc.nset;
     for (int i = 0; i < DDNS_MAXSET; ++i) a.radius[i] = pc.radius[i];
     a.dt = pc.dt; a.st = pc.st; a.hdi = pc.hdi;
@@ -424,7 +428,7 @@ int part_overlap(PartBuffers& pb, const PartConst& pc, int* sweeps, int* relocat
         int st = part_build_cells(pb, pc, std::max(d_overl, pc.pi2 / pc.N), s);
         if (st) return st;
         OverArgs a{};
-        a.pos = pb.pos; a.rad = pb.rad; a.cell_start = pb.cell_start; a.sorted = pb.sorted;
+        a.pos = pb.pos; a.rad = pb.rad; a.pk = pb.pk; a.cell_start = pb.cell_start; a.sorted = pb.sorted;
         a.n = pc.n; a.cap = pb.cap;
         const int jmax = 1 + (int)(d_overl * (double)pc.N / pc.pi2);
         a.g = make_geom(pb, pc, d_overl, d_overl, jmax);

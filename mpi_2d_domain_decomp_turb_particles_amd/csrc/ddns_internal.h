@@ -54,7 +54,7 @@ struct FlowBuffers {
     void* O[3] = {nullptr, nullptr, nullptr};  // AB2 history [kz][ky][kx]
     void* A[6] = {};                            // inverse pipeline: u(3), w(3)
     void* B[3] = {};                            // forward pipeline: (u x w)
-    void* uphys = nullptr;                      // [z][y][x][3]
+    void* uphys = nullptr;                      // [z][y][x][4]: u, v, w, unused
     void* tw = nullptr;                         // N twiddles exp(-2 pi i m / N)
     void* twh = nullptr;                        // N/2 twiddles for the half-length x FFT
     double* low = nullptr;                      // pre-kernel scratch: ff[2], energies, forcing table
@@ -107,6 +107,7 @@ struct PartBuffers {
     int* cell_start = nullptr;  // G^3 + 1
     int* cell_of = nullptr;     // cap
     int* sorted = nullptr;      // cap
+    double4* pk = nullptr;      // cap: (x, y, z, radius) of sorted[p], refreshed with the cells
     int* cursor = nullptr;      // G^3
     void* scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;

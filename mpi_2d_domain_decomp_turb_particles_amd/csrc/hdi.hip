@@ -27,7 +27,7 @@ int exclusive_scan(const int* in, int* out, int n, int* tmp, hipStream_t s);
 
 // ------------------------------------------------------------------ neighbour lists
 template <typename T>
-__global__ void __launch_bounds__(HT) k_nei(int pass, const double* __restrict__ pos, const double* __restrict__ rad,
+__global__ void __launch_bounds__(HT) k_nei(int pass, const double4* __restrict__ pk,
                                             const int* __restrict__ cell_start, const int* __restrict__ sorted,
                                             int n, int cap, int G, double hc, double pi2, double trunc,
                                             int* cnt, const int* ptr, int* idx, T* mm, int* flags)
@@ -35,8 +35,8 @@ __global__ void __launch_bounds__(HT) k_nei(int pass, const double* __restrict__
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int i = sorted[t];
-    const size_t C = (size_t)cap;
-    const double xi = pos[i], yi = pos[C + i], zi = pos[2 * C + i], ri = rad[i];
+    const double4 me = pk[t];
+    const double xi = me.x, yi = me.y, zi = me.z, ri = me.w;
     int cc[3];
     const double xs[3] = {xi, yi, zi};
 #pragma unroll
@@ -54,16 +54,16 @@ __global__ void __launch_bounds__(HT) k_nei(int pass, const double* __restrict__
                 const int wy = qy < 0 ? -1 : (qy >= G ? 1 : 0);
                 const int wz = qz < 0 ? -1 : (qz >= G ? 1 : 0);
                 qx -= wx * G; qy -= wy * G; qz -= wz * G;
-                const int cell = (qx * G + qy) * G + qz;
+                const int cell = (qz * G + qy) * G + qx;
                 const int e = cell_start[cell + 1];
                 for (int p = cell_start[cell]; p < e; ++p) {
-                    const int j = sorted[p];
-                    if (j == i) continue;
-                    const double d0 = xi - (pos[j] + (double)wx * pi2);
-                    const double d1 = yi - (pos[C + j] + (double)wy * pi2);
-                    const double d2 = zi - (pos[2 * C + j] + (double)wz * pi2);
+                    if (p == t) continue;
+                    const double4 o = pk[p];
+                    const double d0 = xi - (o.x + (double)wx * pi2);
+                    const double d1 = yi - (o.y + (double)wy * pi2);
+                    const double d2 = zi - (o.z + (double)wz * pi2);
                     const double dnij = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-                    const double rj = rad[j];
+                    const double rj = o.w;
                     if (dnij < ri + rj) atomicOr(flags, 1);                 // 3294-3299
                     if (!(dnij < trunc * rj)) continue;                     // 3302-3305
                     if (pass) {
@@ -73,14 +73,14 @@ __global__ void __launch_bounds__(HT) k_nei(int pass, const double* __restrict__
                             fact1 = 0.75 * fact1;
                             const double gamma = (fact1 - 0.75 * fact2) / (dnij * dnij);
                             const double beta = fact1 + 0.25 * fact2;
-                            const size_t o = (size_t)base + k;
-                            idx[o] = j;
-                            mm[6 * o + 0] = (T)(gamma * (d0 * d0) + beta);
-                            mm[6 * o + 1] = (T)(gamma * d0 * d1);
-                            mm[6 * o + 2] = (T)(gamma * d0 * d2);
-                            mm[6 * o + 3] = (T)(gamma * (d1 * d1) + beta);
-                            mm[6 * o + 4] = (T)(gamma * d1 * d2);
-                            mm[6 * o + 5] = (T)(gamma * (d2 * d2) + beta);
+                            const size_t q = (size_t)base + k;
+                            idx[q] = sorted[p];
+                            mm[6 * q + 0] = (T)(gamma * (d0 * d0) + beta);
+                            mm[6 * q + 1] = (T)(gamma * d0 * d1);
+                            mm[6 * q + 2] = (T)(gamma * d0 * d2);
+                            mm[6 * q + 3] = (T)(gamma * (d1 * d1) + beta);
+                            mm[6 * q + 4] = (T)(gamma * d1 * d2);
+                            mm[6 * q + 5] = (T)(gamma * (d2 * d2) + beta);
                         }
                     }
                     ++k;
@@ -206,6 +206,91 @@ __global__ void k_div_dev(T* qo, const T* v, const double* h, long long n)
     if (i < n) qo[i] = (T)((double)v[i] / *h);
 }
 
+// ---- block-Gram-Schmidt kernels (Arnoldi step with classical Gram-Schmidt applied
+// twice).  Every reduction uses a fixed grid of RED_BLOCKS work-groups and a fixed
+// summation tree, so the result does not depend on scheduling.
+constexpr int MD_J = 8;   // Krylov columns per pass of k_mdot
+
+// part[j * gridDim.x + block] = sum over this block's share of Q_j . v, j < k
+template <typename T>
+__global__ void __launch_bounds__(HT) k_mdot(const T* __restrict__ Q, const T* __restrict__ v, int k, long long n3,
+                                             double* __restrict__ part)
+{
+    __shared__ double sh[MD_J][HT];
+    const int tid = threadIdx.x;
+    for (int j0 = 0; j0 < k; j0 += MD_J) {
+        double acc[MD_J];
+#pragma unroll
+        for (int jj = 0; jj < MD_J; ++jj) acc[jj] = 0.0;
+        for (long long i = (long long)blockIdx.x * HT + tid; i < n3; i += (long long)gridDim.x * HT) {
+            const double vi = (double)v[i];
+#pragma unroll
+            for (int jj = 0; jj < MD_J; ++jj)
+                if (j0 + jj < k) acc[jj] += (double)Q[(size_t)(j0 + jj) * n3 + i] * vi;
+        }
+#pragma unroll
+        for (int jj = 0; jj < MD_J; ++jj) sh[jj][tid] = acc[jj];
+        __syncthreads();
+        for (int o = HT / 2; o > 0; o >>= 1) {
+            if (tid < o) {
+#pragma unroll
+                for (int jj = 0; jj < MD_J; ++jj) sh[jj][tid] += sh[jj][tid + o];
+            }
+            __syncthreads();
+        }
+        if (tid < MD_J && j0 + tid < k) part[(size_t)(j0 + tid) * gridDim.x + blockIdx.x] = sh[tid][0];
+        __syncthreads();
+    }
+}
+
+// one work-group per column j: h[j] = sum of its partials; the Hessenberg column
+// entry is set (first pass) or incremented (second pass)
+__global__ void __launch_bounds__(HT) k_mfinal(const double* __restrict__ part, int np, double* h, double* hcol, int add)
+{
+    __shared__ double sh[HT];
+    const int j = blockIdx.x, tid = threadIdx.x;
+    double acc = 0.0;
+    for (int i = tid; i < np; i += HT) acc += part[(size_t)j * np + i];
+    sh[tid] = acc;
+    __syncthreads();
+    for (int o = HT / 2; o > 0; o >>= 1) {
+        if (tid < o) sh[tid] += sh[tid + o];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        h[j] = sh[0];
+        hcol[j] = add ? hcol[j] + sh[0] : sh[0];
+    }
+}
+
+// v -= sum_j h[j] Q_j; with normpart, also the block partials of |v|^2
+template <typename T>
+__global__ void __launch_bounds__(HT) k_maxpy(T* __restrict__ v, const T* __restrict__ Q, const double* __restrict__ h,
+                                              int k, long long n3, double* normpart)
+{
+    __shared__ double hs[128];
+    __shared__ double sh[HT];
+    const int tid = threadIdx.x;
+    for (int j = tid; j < k; j += HT) hs[j] = h[j];
+    __syncthreads();
+    double nacc = 0.0;
+    for (long long i = (long long)blockIdx.x * HT + tid; i < n3; i += (long long)gridDim.x * HT) {
+        double x = (double)v[i];
+        for (int j = 0; j < k; ++j) x -= hs[j] * (double)Q[(size_t)j * n3 + i];
+        const T xt = (T)x;
+        v[i] = xt;
+        nacc += (double)xt * (double)xt;
+    }
+    if (!normpart) return;
+    sh[tid] = nacc;
+    __syncthreads();
+    for (int o = HT / 2; o > 0; o >>= 1) {
+        if (tid < o) sh[tid] += sh[tid + o];
+        __syncthreads();
+    }
+    if (tid == 0) normpart[blockIdx.x] = sh[0];
+}
+
 // pertvel = x0 + Q y (2951-2956)
 template <typename T>
 __global__ void k_solution(const double* x0, const T* Q, const double* y, int iters, double* pert, int n, int cap)
@@ -328,7 +413,7 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
     if (n == 0) return 0;
     DDNS_HIP(hipMemsetAsync(pb.flags, 0, 4 * sizeof(int), s));
     // neighbour lists, two passes
-    hipLaunchKernelGGL((k_nei<T>), dim3((n + HT - 1) / HT), dim3(HT), 0, s, 0, pb.pos, pb.rad, pb.cell_start, pb.sorted,
+    hipLaunchKernelGGL((k_nei<T>), dim3((n + HT - 1) / HT), dim3(HT), 0, s, 0, (const double4*)pb.pk, pb.cell_start, pb.sorted,
                        n, pb.cap, pb.G, pb.hc, pc.pi2, pc.hdi_trunc, pb.nei_cnt, (const int*)nullptr, (int*)nullptr,
                        (T*)nullptr, pb.flags);
     int st = exclusive_scan(pb.nei_cnt, pb.nei_ptr, n, (int*)pb.scan_tmp, s);
@@ -345,7 +430,7 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
         DDNS_HIP(hipMalloc(&pb.nei_idx, (size_t)pb.nnz_cap * sizeof(int)));
         DDNS_HIP(hipMalloc(&pb.nei_m, (size_t)pb.nnz_cap * 6 * sizeof(double)));
     }
-    hipLaunchKernelGGL((k_nei<T>), dim3((n + HT - 1) / HT), dim3(HT), 0, s, 1, pb.pos, pb.rad, pb.cell_start, pb.sorted,
+    hipLaunchKernelGGL((k_nei<T>), dim3((n + HT - 1) / HT), dim3(HT), 0, s, 1, (const double4*)pb.pk, pb.cell_start, pb.sorted,
                        n, pb.cap, pb.G, pb.hc, pc.pi2, pc.hdi_trunc, pb.nei_cnt, (const int*)pb.nei_ptr, pb.nei_idx,
                        (T*)pb.nei_m, pb.flags);
     DDNS_HIP(hipGetLastError());
@@ -353,13 +438,15 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
     GmSmall L{maxiter};
     if (!pb.gm_small) {
         DDNS_HIP(hipMalloc(&pb.gm_small, 16384 * sizeof(double)));
-        DDNS_HIP(hipMalloc(&pb.gm_red, RED_BLOCKS * sizeof(double)));
+        DDNS_HIP(hipMalloc(&pb.gm_red, (size_t)(128 + 2) * RED_BLOCKS * sizeof(double)));
         DDNS_HIP(hipMalloc(&pb.gm_x0, 3 * (size_t)pb.cap * sizeof(double)));
         DDNS_HIP(hipMalloc(&pb.gm_b, 3 * (size_t)pb.cap * sizeof(double)));
         DDNS_HIP(hipMalloc(&pb.gm_v, 3 * (size_t)pb.cap * sizeof(double)));
         DDNS_HIP(hipMalloc(&pb.gm_Q, (size_t)(maxiter + 1) * 3 * pb.cap * sizeof(T)));
     }
-    if (L.total() > 16384) { set_error("GMRES maxiter too large (at most 120)"); return DDNS_ERR_ARG; }
+    if (L.total() > 16384 || maxiter > 120) { set_error("GMRES maxiter too large (at most 120)"); return DDNS_ERR_ARG; }
+    double* normpart = pb.gm_red + (size_t)128 * RED_BLOCKS;
+    double* hbuf = normpart + RED_BLOCKS;
     double* sm = pb.gm_small;
     double* ax0 = (double*)pb.gm_v;       // reuse as double scratch for A x0
     T* Q = (T*)pb.gm_Q;
@@ -392,16 +479,17 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
         hipLaunchKernelGGL((k_spmv<T>), dim3((n + HT - 1) / HT), dim3(HT), 0, s, pb.nei_ptr, pb.nei_idx,
                            (const T*)pb.nei_m, (const T*)qi, v, n, (const double*)nullptr, (double*)nullptr,
                            (const double*)nullptr, (const double*)nullptr, (double*)nullptr, pb.cap);
-        for (int j = 1; j <= iter; ++j) {
-            double* hj = Hd + (j - 1) + (size_t)M1 * (iter - 1);
-            const T* qj = Q + (size_t)(j - 1) * n3;
-            st = dot(qj, (const T*)v, n3, pb.gm_red, hj, 0, s);
-            if (st) return st;
-            hipLaunchKernelGGL((k_axpy_dev<T>), dim3(nb3), dim3(HT), 0, s, v, qj, hj, n3);
+        // orthogonalise v against q_1..q_iter: classical Gram-Schmidt, twice (the
+        // reference's modified Gram-Schmidt, 2886-2894, in two block passes)
+        double* hcol = Hd + (size_t)M1 * (iter - 1);
+        double* hn = hcol + iter;
+        for (int pass = 0; pass < 2; ++pass) {
+            hipLaunchKernelGGL((k_mdot<T>), dim3(RED_BLOCKS), dim3(HT), 0, s, (const T*)Q, (const T*)v, iter, n3, pb.gm_red);
+            hipLaunchKernelGGL(k_mfinal, dim3(iter), dim3(HT), 0, s, (const double*)pb.gm_red, RED_BLOCKS, hbuf, hcol, pass);
+            hipLaunchKernelGGL((k_maxpy<T>), dim3(RED_BLOCKS), dim3(HT), 0, s, v, (const T*)Q, (const double*)hbuf, iter, n3,
+                               pass ? normpart : (double*)nullptr);
         }
-        double* hn = Hd + iter + (size_t)M1 * (iter - 1);
-        st = dot((const T*)v, (const T*)v, n3, pb.gm_red, hn, 1, s);
-        if (st) return st;
+        hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(HT), 0, s, (const double*)normpart, RED_BLOCKS, hn, 1);
         if (iter < maxiter) hipLaunchKernelGGL((k_div_dev<T>), dim3(nb3), dim3(HT), 0, s, Q + (size_t)iter * n3, v, hn, n3);
         hipLaunchKernelGGL(k_ls, dim3(1), dim3(1), 0, s, sm, L, iter, done_dev);
         DDNS_HIP(hipGetLastError());

@@ -54,36 +54,49 @@ __global__ void k_scan_block(const int* in, int* out, int* bsum, int n)
     if (t == 255) bsum[blockIdx.x] = s[255];
 }
 
-__global__ void k_scan_sums(int* bsum, int nb, int* total_out)
-{
-    if (threadIdx.x != 0) return;
-    int run = 0;
-    for (int b = 0; b < nb; ++b) {
-        const int v = bsum[b];
-        bsum[b] = run;
-        run += v;
-    }
-    *total_out = run;
-}
-
-__global__ void k_scan_add(int* out, const int* bsum, int n)
+__global__ void k_scan_add(int* out, const int* bsum, int n, int* total_out, const int* total_in)
 {
     const int i = blockIdx.x * 1024 + threadIdx.x * 4;
     const int add = bsum[blockIdx.x];
 #pragma unroll
     for (int q = 0; q < 4; ++q)
         if (i + q < n) out[i + q] += add;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *total_out = *total_in;
 }
 
-// out[0..n-1] = exclusive scan, out[n] = total; tmp holds ceil(n/1024) ints
+__global__ void k_scan_total(const int* bsum, int* total_out)
+{
+    if (threadIdx.x == 0) *total_out = bsum[0];
+}
+
+// out[0..n-1] = exclusive scan of in, out[n] = total (in may alias out).  Block sums
+// are scanned recursively, so every level is a full-width launch; tmp holds
+// scan_tmp_ints(n) ints.
 int exclusive_scan(const int* in, int* out, int n, int* tmp, hipStream_t s)
 {
     const int nb = (n + 1023) / 1024;
     hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(256), 0, s, in, out, tmp, n);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1), 0, s, tmp, nb, out + n);
-    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, s, out, tmp, n);
+    if (nb == 1) {
+        hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, s, (const int*)tmp, out + n);
+    } else {
+        const int st = exclusive_scan(tmp, tmp, nb, tmp + nb + 1, s);
+        if (st) return st;
+        hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, s, out, (const int*)tmp, n, out + n, (const int*)(tmp + nb));
+    }
     DDNS_HIP(hipGetLastError());
     return 0;
+}
+
+// ints of scratch exclusive_scan needs for n elements
+size_t scan_tmp_ints(long long n)
+{
+    size_t t = 0;
+    long long m = n;
+    do {
+        m = (m + 1023) / 1024;
+        t += (size_t)m + 1;
+    } while (m > 1);
+    return t + 16;
 }
 
 // ------------------------------------------------------------------ allocation
@@ -105,6 +118,7 @@ int part_alloc(PartBuffers& pb, int cap, int nset, int G)
     DDNS_HIP(hipMemset(pb.vp, 0, 3 * c * 8));
     DDNS_HIP(hipMalloc(&pb.cell_of, c * sizeof(int)));
     DDNS_HIP(hipMalloc(&pb.sorted, c * sizeof(int)));
+    DDNS_HIP(hipMalloc(&pb.pk, c * sizeof(double4)));
     DDNS_HIP(hipMalloc(&pb.mult, (c + 1) * sizeof(int)));
     DDNS_HIP(hipMalloc(&pb.draw_off, (c + 1) * sizeof(int)));
     DDNS_HIP(hipMalloc(&pb.nei_cnt, (c + 1) * sizeof(int)));
@@ -123,7 +137,7 @@ int part_alloc(PartBuffers& pb, int cap, int nset, int G)
     DDNS_HIP(hipMalloc(&pb.rng_state, 64 * sizeof(int)));
     DDNS_HIP(hipMemset(pb.rng_state, 0, 64 * sizeof(int)));
     DDNS_HIP(hipMalloc(&pb.red, 8192 * sizeof(double)));
-    pb.scan_tmp_bytes = (size_t)((std::max((long long)G * G * G + 1, (long long)c + 1) + 1023) / 1024 + 8) * sizeof(int);
+    pb.scan_tmp_bytes = scan_tmp_ints(std::max((long long)G * G * G + 1, (long long)c + 1)) * sizeof(int);
     DDNS_HIP(hipMalloc(&pb.scan_tmp, pb.scan_tmp_bytes));
     pb.G = 0;
     return 0;
@@ -132,7 +146,7 @@ int part_alloc(PartBuffers& pb, int cap, int nset, int G)
 void part_free(PartBuffers& pb)
 {
     void* ps[] = {pb.pos, pb.vel, pb.rad, pb.gid, pb.fvp, pb.fdp, pb.dmove, pb.vp, pb.pert,
-                  pb.cell_count, pb.cell_start, pb.cell_of, pb.sorted, pb.cursor, pb.scan_tmp,
+                  pb.cell_count, pb.cell_start, pb.cell_of, pb.sorted, pb.pk, pb.cursor, pb.scan_tmp,
                   pb.nei_cnt, pb.nei_ptr, pb.nei_idx, pb.nei_m, pb.gm_Q, pb.gm_v, pb.gm_x0, pb.gm_b,
                   pb.gm_red, pb.gm_small, pb.mult, pb.flags, pb.tested, pb.nirbin, pb.bins, pb.counts,
                   pb.pairs, pb.draws, pb.draw_off, pb.rng_state, pb.red};
@@ -153,8 +167,22 @@ __device__ __forceinline__ void lagrange6(double z, double* bg)
     bg[5] = (4.0 * z - 5.0 * z3 + z5) / 120.0;
 }
 
-// One wavefront lane per droplet; droplets are visited in cell order (sorted[])
-// so that neighbouring lanes gather overlapping stencils from the same L2 lines.
+// The physical velocity is stored node-major, [z][y][x][4] (u, v, w, unused), so one
+// node is one 16-byte (float) or 32-byte (double) access.
+__device__ __forceinline__ void load_node(const float* p, double& a, double& b, double& c)
+{
+    const float4 v = *(const float4*)p;
+    a = v.x; b = v.y; c = v.z;
+}
+__device__ __forceinline__ void load_node(const double* p, double& a, double& b, double& c)
+{
+    const double2 v = *(const double2*)p;
+    a = v.x; b = v.y; c = p[2];
+}
+
+// One lane per droplet, droplets taken in cell order (sorted[], cells x-fastest like
+// the field) so that neighbouring lanes read overlapping stencil rows.  The 6 x 6 x 6
+// stencil is fully unrolled: weights and node indices stay in registers.
 template <typename T>
 __global__ void __launch_bounds__(PT) k_interp(const T* __restrict__ u, int N, double pi2,
                                                const double* __restrict__ pos, const int* __restrict__ order,
@@ -164,36 +192,39 @@ __global__ void __launch_bounds__(PT) k_interp(const T* __restrict__ u, int N, d
     if (t >= n) return;
     const int i = order ? order[t] : t;
     const double dx = pi2 / (double)N;
-    double bg[3][6];
-    int nd[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const double y = pos[(size_t)c * cap + i];
-        const int node = (int)floor(y / dx);
-        const double z = (y - (double)node * dx) / dx;
-        nd[c] = node;
-        lagrange6(z, bg[c]);
+    double wx[6], wy[6], wz[6];
+    int nx, ny, nz;
+    {
+        const double y0 = pos[i], y1 = pos[(size_t)cap + i], y2 = pos[2 * (size_t)cap + i];
+        nx = (int)floor(y0 / dx);
+        ny = (int)floor(y1 / dx);
+        nz = (int)floor(y2 / dx);
+        lagrange6((y0 - (double)nx * dx) / dx, wx);
+        lagrange6((y1 - (double)ny * dx) / dx, wy);
+        lagrange6((y2 - (double)nz * dx) / dx, wz);
     }
-    int ix[6], iy[6], iz[6];
+    int ix[6];
+    size_t rowy[6], rowz[6];
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
-        ix[m] = ((nd[0] - 2 + m) % N + N) % N;
-        iy[m] = ((nd[1] - 2 + m) % N + N) % N;
-        iz[m] = ((nd[2] - 2 + m) % N + N) % N;
+        ix[m] = ((nx - 2 + m) % N + N) % N;
+        rowy[m] = (size_t)(((ny - 2 + m) % N + N) % N) * N;
+        rowz[m] = (size_t)(((nz - 2 + m) % N + N) % N) * N * N;
     }
     double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-#pragma unroll 1
-    for (int a = 0; a < 6; ++a) {
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
 #pragma unroll
         for (int b = 0; b < 6; ++b) {
-            const double wab = bg[0][a] * bg[1][b];
+            const T* row = u + (rowz[d] + rowy[b]) * 4;
 #pragma unroll
-            for (int d = 0; d < 6; ++d) {
-                const T* p = u + (((size_t)iz[d] * N + iy[b]) * N + ix[a]) * 3;
-                const double w = wab * bg[2][d];
-                acc0 += (double)p[0] * w;
-                acc1 += (double)p[1] * w;
-                acc2 += (double)p[2] * w;
+            for (int a = 0; a < 6; ++a) {
+                double u0, u1, u2;
+                load_node(row + (size_t)ix[a] * 4, u0, u1, u2);
+                const double w = wx[a] * wy[b] * wz[d];      // bg(x) bg(y) bg(z), partlib.f90:509
+                acc0 += u0 * w;
+                acc1 += u1 * w;
+                acc2 += u2 * w;
             }
         }
     }
@@ -466,7 +497,7 @@ __global__ void k_cell_key(const double* pos, int* cell_of, int* cnt, int n, int
         q = q < 0 ? 0 : (q >= G ? G - 1 : q);
         c3[c] = q;
     }
-    const int cell = (c3[0] * G + c3[1]) * G + c3[2];
+    const int cell = (c3[2] * G + c3[1]) * G + c3[0];       // x fastest, as the field layout
     cell_of[i] = cell;
     atomicAdd(cnt + cell, 1);
 }
@@ -493,6 +524,17 @@ __global__ void k_cell_order(const int* start, int* sorted, int ncell)
     }
 }
 
+// positions and radii in cell order, one 32-byte record per droplet, so the pair
+// searches read each candidate cell as one contiguous run
+__global__ void k_pack_sorted(const int* __restrict__ sorted, const double* __restrict__ pos,
+                              const double* __restrict__ rad, double4* __restrict__ pk, int n, int cap)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int i = sorted[p];
+    pk[p] = make_double4(pos[i], pos[(size_t)cap + i], pos[2 * (size_t)cap + i], rad[i]);
+}
+
 int part_build_cells(PartBuffers& pb, const PartConst& pc, double hmin, hipStream_t s)
 {
     constexpr int GMAX = 256;
@@ -507,7 +549,7 @@ int part_build_cells(PartBuffers& pb, const PartConst& pc, double hmin, hipStrea
         DDNS_HIP(hipMalloc(&pb.cell_count, nmax * sizeof(int)));
         DDNS_HIP(hipMalloc(&pb.cell_start, nmax * sizeof(int)));
         DDNS_HIP(hipMalloc(&pb.cursor, nmax * sizeof(int)));
-        const size_t need = (size_t)((std::max(nmax, (long long)pb.cap + 1) + 1023) / 1024 + 8) * sizeof(int);
+        const size_t need = scan_tmp_ints(std::max(nmax, (long long)pb.cap + 1)) * sizeof(int);
         if (need > pb.scan_tmp_bytes) {
             (void)hipFree(pb.scan_tmp);
             DDNS_HIP(hipMalloc(&pb.scan_tmp, need));
@@ -525,6 +567,7 @@ int part_build_cells(PartBuffers& pb, const PartConst& pc, double hmin, hipStrea
     if (pc.n > 0) {
         hipLaunchKernelGGL(k_cell_fill, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.cell_of, pb.cell_start, pb.cursor, pb.sorted, pc.n);
         hipLaunchKernelGGL(k_cell_order, dim3(nblk(ncell)), dim3(PT), 0, s, pb.cell_start, pb.sorted, (int)ncell);
+        hipLaunchKernelGGL(k_pack_sorted, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.sorted, pb.pos, pb.rad, pb.pk, pc.n, pb.cap);
     }
     DDNS_HIP(hipGetLastError());
     return 0;

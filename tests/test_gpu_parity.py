@@ -214,3 +214,34 @@ def test_droplet_steps_with_coalescence_match_oracle(ddns, oracle_mod):
         # merged pairs keep their volume; only pairs with a halo-image partner (whose
         # partner is not removed, as in the reference) add volume
         assert float(np.sum(do["yp"][6] ** 3)) >= vol0 * (1 - 1e-12)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_hdi_steps_match_oracle(ddns, oracle_mod, prec):
+    """Hydrodynamic interactions (NEI_LIST_PRECOMP, AFUN_PRECOMP, HDI_GMRES,
+    partlib.f90:2756-3510) through full steps: the disturbance velocities, the GMRES
+    iteration count and the droplet state against the oracle.  FP64: Gram-Schmidt
+    order differs, so 1e-9; FP32-mixed: Krylov vectors in float, 1e-4."""
+    N = 32
+    kw = dict(N=N, npart=4000, nset=2, radius=[20.0, 40.0], forcing=1, dt=0.002, rnu=0.02,
+              hdi_included=1, hdi_trunc=10.0)
+    v = random_spectral(N, seed=6, kmax=6)
+    with ddns.Solver(_params(ddns, precision=prec, **kw)) as s, oracle_mod.Oracle(_params(ddns, **kw)) as o:
+        s.set_flow(v, v)
+        o.set_flow(v, v)
+        s.init_droplets(seed=77)
+        o.init_droplets(seed=77)
+        for _ in range(2):
+            s.step(1)
+            o.step(1)
+            cg, co = s.collision_counts(), o.collision_counts()
+            assert co.gmres_iters > 1
+            if prec == 0:
+                assert cg.gmres_iters == co.gmres_iters
+        dg, do = s.get_droplets(), o.get_droplets()
+        pg, po = dg["pertvel"], do["pertvel"]
+        scale = np.abs(po).max()
+        assert scale > 0
+        tol = 1e-9 if prec == 0 else 1e-4
+        assert np.abs(pg - po).max() < tol * scale
+        assert np.allclose(dg["yp"][:3], do["yp"][:3], rtol=0, atol=1e-9 if prec == 0 else 1e-6)
