@@ -425,7 +425,10 @@ int part_overlap(PartBuffers& pb, const PartConst& pc, int* sweeps, int* relocat
     if (pc.n == 0) return 0;
     const double d_overl = 2.0 * pc.radmax;                                // 2063
     for (int it = 0; it < 10000; ++it) {
-        int st = part_build_cells(pb, pc, std::max(d_overl, pc.pi2 / pc.N), s);
+        // two droplets in adjacent reference cells (the window of partlib.f90:2091) are
+        // less than 2 dx apart per component, so cells of width >= 2 dx keep every such
+        // pair within the 27-cell neighbourhood
+        int st = part_build_cells(pb, pc, std::max(d_overl, 2.0 * pc.pi2 / pc.N), s);
         if (st) return st;
         OverArgs a{};
         a.pos = pb.pos; a.rad = pb.rad; a.pk = pb.pk; a.cell_start = pb.cell_start; a.sorted = pb.sorted;

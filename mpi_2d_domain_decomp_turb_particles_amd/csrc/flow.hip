@@ -27,10 +27,32 @@ enum { KZ_FULL = 0, KZ_CURL_INV = 1, KZ_PLAIN_FWD = 2, KZ_PLAIN_INV = 3 };
 
 // tile widths (kx columns per work-group) for the y and z stages
 template <typename T, int N> constexpr int ky_tc() { return (sizeof(T) == 4) ? (N >= 1024 ? 8 : 16) : (N >= 1024 ? 4 : 8); }
+// z-stage tile widths for N <= 256, overridable at build time for tuning sweeps
+#ifndef DDNS_KZ_TC_F32
+#define DDNS_KZ_TC_F32 16
+#endif
+#ifndef DDNS_KZ_TC_F64
+#define DDNS_KZ_TC_F64 8
+#endif
 template <typename T, int N> constexpr int kz_tc()
 {
-    if constexpr (sizeof(T) == 4) return N <= 256 ? 16 : (N == 512 ? 8 : 4);
-    else return N <= 256 ? 8 : (N == 512 ? 4 : 2);
+    if constexpr (sizeof(T) == 4) return N <= 256 ? DDNS_KZ_TC_F32 : (N == 512 ? 8 : 4);
+    else return N <= 256 ? DDNS_KZ_TC_F64 : (N == 512 ? 4 : 2);
+}
+
+// Work-groups are dispatched to the 8 XCDs round-robin by linear id, and each XCD has
+// its own L2.  When a tile row is narrower than a 128-byte line, the P tiles that share
+// a line get linear ids L, L+8, ..., so they land on the same XCD and meet in its L2.
+// Returns the tile index of linear work-group id L (total a multiple of 8 P).
+template <int P>
+__device__ __forceinline__ int xcd_tile(int L)
+{
+    if constexpr (P <= 1) {
+        return L;
+    } else {
+        const int g = L / (8 * P), r = L % (8 * P);
+        return g * (8 * P) + (r % 8) * P + r / 8;
+    }
 }
 template <int TC> constexpr int ldr_of() { return (TC >= 8) ? TC + 1 : TC; }
 template <typename T, int N> constexpr int kz_ldr()
@@ -131,8 +153,11 @@ __global__ void __launch_bounds__(NT) k_z(KzArgs a)
     constexpr int NTILE = (MODE == KZ_FULL || MODE == KZ_CURL_INV) ? 4 : 1;
     __shared__ C lds[NTILE][N * LDR];
     const int tid = threadIdx.x;
-    const int kx0 = blockIdx.x * TC;
-    const int ky = blockIdx.y;
+    constexpr int NBX = M / TC;
+    constexpr int P = (TC * (int)sizeof(C) >= 128) ? 1 : 128 / (TC * (int)sizeof(C));
+    const int tile = xcd_tile<(NBX % P == 0) ? P : 1>(blockIdx.x);
+    const int kx0 = (tile % NBX) * TC;
+    const int ky = tile / NBX;
     const C* tw = (const C*)a.tw;
     const size_t rowstride = (size_t)N * M;
     const size_t base = (size_t)ky * M + kx0;
@@ -762,7 +787,7 @@ static int launch_z(int mode, const FlowBuffers& fb, const FlowConst& fc, hipStr
     a.scale = scale;
     if (mode == KZ_PLAIN_FWD || mode == KZ_PLAIN_INV) { a.B[0] = src; a.U[0] = dst; }
     constexpr int TC = kz_tc<T, N>();
-    dim3 grid(N / 2 / TC, N);
+    dim3 grid((N / 2 / TC) * N);
     switch (mode) {
     case KZ_FULL: hipLaunchKernelGGL((k_z<T, N, KZ_FULL>), grid, dim3(NT), 0, s, a); break;
     case KZ_CURL_INV: hipLaunchKernelGGL((k_z<T, N, KZ_CURL_INV>), grid, dim3(NT), 0, s, a); break;
