@@ -8,7 +8,12 @@
 ! The whole step runs inside libddns.so; this program only moves parameters and
 ! diagnostics across the C ABI.
 !
-! Usage: turb_gpu [parameter-file] [fp64|fp32]
+! Restarts use the reference's files (load_save.f90): vel2D<idp>.<rank> (u, omega-hat
+! history, six unformatted records of the rank's spectral block), force.<idp> (OU state)
+! and part.<rank> (droplets).  A run written by P ranks (nprocY from parameter.d) is
+! assembled block by block; this driver writes one-rank files, index idp+1.
+!
+! Usage: turb_gpu [parameter-file] [fp64|fp32] [ranks-of-the-restart-files]
 program turb_gpu
   use, intrinsic :: iso_c_binding
   use ddns_mod
@@ -26,7 +31,11 @@ program turb_gpu
   double precision :: ediss_CLOUD, viscosity_CLOUD, gravity, hdi_trunc, radius(10)
   double precision :: t_total
   double precision, allocatable :: ek(:)
-  integer(c_int) :: stat, prec
+  double precision, allocatable :: vel(:,:,:,:), blk(:,:,:), yp(:,:), fvp(:,:,:), fdp(:,:,:), pert(:,:)
+  double precision, allocatable :: vpd(:,:), dmv(:,:)
+  double precision :: bforce(450)
+  integer :: ivf(32), iyf, isf, nin, nprocY_in, nps_r, nps_tot, ir, c
+  integer(c_int) :: stat, prec, np_c
   integer(8) :: c0, c1, crate
 
   pfile = 'parameter.d'
@@ -35,6 +44,11 @@ program turb_gpu
   if (command_argument_count() >= 2) then
     call get_command_argument(2, arg)
     if (trim(arg) == 'fp32') prec = DDNS_FP32_MIXED
+  end if
+  nin = 1
+  if (command_argument_count() >= 3) then
+    call get_command_argument(3, arg)
+    read(arg, *) nin
   end if
 
   ! ---- parameter.d, turb.f90:102-166
@@ -124,25 +138,57 @@ program turb_gpu
   allocate(ek(nek))
 
   ! ---- initial flow, turb.f90:438-454
-  if (.not. new_flow) then
-    write(*,*) 'new_flow = .FALSE.: restart files are written by the caller through ddns_set_flow'
-    stop 1
-  end if
-  if (forcing == 1) then
-    stat = ddns_init_flow(h, DDNS_SPEC_KOLMOGOROV, iseed_m0, 4.d0, 1.d0)
-  else
-    stat = ddns_init_flow(h, -1, 0, 0.d0, 0.d0)                ! start from rest
-  end if
-  call ddns_check(stat, h, 'ddns_init_flow')
-
-  ! ---- droplets: PART_INIT (seed 290710, partlib.f90:96), PERIODIC, OVERLAP_DET
-  if (npart > 0) then
-    if (.not. new_particle) then
-      write(*,*) 'new_particle = .FALSE.: droplet files are loaded by the caller through ddns_set_droplets'
-      stop 1
+  nprocY_in = 1
+  if (nin > 1) nprocY_in = nprocY
+  if (new_flow) then
+    if (forcing == 1) then
+      stat = ddns_init_flow(h, DDNS_SPEC_KOLMOGOROV, iseed_m0, 4.d0, 1.d0)
+    else
+      stat = ddns_init_flow(h, -1, 0, 0.d0, 0.d0)              ! start from rest
     end if
-    stat = ddns_init_droplets(h, 290710_c_int, DDNS_RADII_SETS, 0.d0, 0.d0, 0.d0, 0.d0)
-    call ddns_check(stat, h, 'ddns_init_droplets')
+    call ddns_check(stat, h, 'ddns_init_flow')
+  else
+    allocate(vel(nx + 2, nx + 2, nx, 6))
+    call read_velocity(idp, nin, nprocY_in, nx, vel)
+    stat = ddns_set_flow(h, vel(:,:,:,1), vel(:,:,:,2), vel(:,:,:,3), vel(:,:,:,4), vel(:,:,:,5), vel(:,:,:,6))
+    call ddns_check(stat, h, 'ddns_set_flow')
+    if (forcing == 2) then
+      call read_forcing(idp, isf, ivf, iyf, bforce)
+      stat = ddns_set_forcing_state(h, bforce, isf, iyf, ivf)
+      call ddns_check(stat, h, 'ddns_set_forcing_state')
+    end if
+    deallocate(vel)
+  end if
+
+  ! ---- droplets: PART_INIT (seed 290710, partlib.f90:96), PERIODIC, OVERLAP_DET, or
+  !      LOAD_PARTICLE (load_save.f90:99-124) from every rank's file
+  if (npart > 0) then
+    if (new_particle) then
+      stat = ddns_init_droplets(h, 290710_c_int, DDNS_RADII_SETS, 0.d0, 0.d0, 0.d0, 0.d0)
+      call ddns_check(stat, h, 'ddns_init_droplets')
+    else
+      nps_tot = 0
+      do ir = 0, nin - 1
+        open(12, file=rank_name('part.', ir), status='old', form='unformatted')
+        read(12) nps_r
+        close(12)
+        nps_tot = nps_tot + nps_r
+      end do
+      allocate(yp(8, nps_tot), fvp(3, 0:2, nps_tot), fdp(3, 0:3, nps_tot), pert(3, nps_tot))
+      pert = 0.d0
+      nps_tot = 0
+      do ir = 0, nin - 1
+        open(12, file=rank_name('part.', ir), status='old', form='unformatted')
+        read(12) nps_r
+        read(12) yp(:, nps_tot + 1:nps_tot + nps_r), fvp(:, :, nps_tot + 1:nps_tot + nps_r), &
+                 fdp(:, :, nps_tot + 1:nps_tot + nps_r)
+        close(12)
+        nps_tot = nps_tot + nps_r
+      end do
+      stat = ddns_set_droplets(h, int(nps_tot, c_int), yp, fvp, fdp, pert)
+      call ddns_check(stat, h, 'ddns_set_droplets')
+      deallocate(yp, fvp, fdp, pert)
+    end if
   end if
 
   open(40, file='collision.dat', status='unknown')
@@ -177,6 +223,109 @@ program turb_gpu
   write(*,*) 'total time              =', dble(c1 - c0) / dble(crate)
   write(*,*) 'GPU time in ddns_step   =', t_total / 1000.d0
   write(*,*) 'Next run starts at time =', time
+
+  ! ---- final velocity, forcing and droplets as one-rank restart files (turb.f90:722-729)
+  allocate(vel(nx + 2, nx + 2, nx, 6))
+  stat = ddns_get_flow(h, vel(:,:,:,1), vel(:,:,:,2), vel(:,:,:,3), vel(:,:,:,4), vel(:,:,:,5), vel(:,:,:,6))
+  call ddns_check(stat, h, 'ddns_get_flow')
+  open(10, file=dump_name('vel2D', idp + 1, 0), status='unknown', form='unformatted')
+  do c = 1, 6
+    write(10) vel(:,:,:,c)
+  end do
+  close(10)
+  deallocate(vel)
+  if (forcing == 2) then
+    stat = ddns_get_forcing_state(h, bforce, isf, iyf, ivf)
+    open(10, file=force_name(idp + 1), status='unknown', form='unformatted')
+    write(10) isf
+    write(10) ivf
+    write(10) iyf
+    write(10) bforce
+    close(10)
+  end if
+  if (npart > 0) then
+    allocate(yp(8, npart), fvp(3, 0:2, npart), fdp(3, 0:3, npart), pert(3, npart), vpd(3, npart), dmv(3, npart))
+    stat = ddns_get_droplets(h, np_c, yp, fvp, fdp, pert, vpd, dmv)
+    call ddns_check(stat, h, 'ddns_get_droplets')
+    open(12, file=rank_name('part.', 0), status='unknown', form='unformatted')
+    write(12) int(np_c)
+    write(12) yp(:, 1:np_c), fvp(:, :, 1:np_c), fdp(:, :, 1:np_c)
+    close(12)
+  end if
   stat = ddns_destroy(h)
   write(*,*) 'Successfully completed'
+
+contains
+
+  ! 'part.0007', load_save.f90:107-116
+  function rank_name(stem, id) result(f)
+    character(len=*), intent(in) :: stem
+    integer, intent(in) :: id
+    character(len=64) :: f
+    write(f, '(a, i4.4)') stem, id
+  end function rank_name
+
+  ! 'vel2D3.0007' / 'vel2D12.0007', load_save.f90:15-31
+  function dump_name(stem, idump, id) result(f)
+    character(len=*), intent(in) :: stem
+    integer, intent(in) :: idump, id
+    character(len=64) :: f
+    if (idump < 10) then
+      write(f, '(a, i1, a, i4.4)') stem, idump, '.', id
+    else
+      write(f, '(a, i2, a, i4.4)') stem, idump, '.', id
+    end if
+  end function dump_name
+
+  ! 'force.3' / 'force.12', load_save.f90:71-77
+  function force_name(idump) result(f)
+    integer, intent(in) :: idump
+    character(len=64) :: f
+    if (idump < 10) then
+      write(f, '(a, i1)') 'force.', idump
+    else
+      write(f, '(a, i2)') 'force.', idump
+    end if
+  end function force_name
+
+  ! input_output (load_save.f90:2-56) over the ranks of a run: rank id holds the kx
+  ! pair columns of Y-rank mod(id, nprocY) and the ky planes of Z-rank id / nprocY
+  ! (turb.f90:282-295), each as six records of v(nx+2, ly+lyext, lz)
+  subroutine read_velocity(idump, nranks, npy, n, v)
+    integer, intent(in) :: idump, nranks, npy, n
+    double precision, intent(out) :: v(n + 2, n + 2, n, 6)
+    integer :: id, npz, ly, lz, lyx, iy0, iz0, f
+    npz = nranks / npy
+    ly = n / npy
+    lz = n / npz
+    v = 0.d0
+    do id = 0, nranks - 1
+      lyx = 0
+      if (mod(id, npy) + 1 == npy) lyx = 2
+      iy0 = mod(id, npy) * ly
+      iz0 = (id / npy) * lz
+      allocate(blk(n + 2, ly + lyx, lz))
+      open(10, file=dump_name('vel2D', idump, id), status='old', form='unformatted')
+      do f = 1, 6
+        read(10) blk
+        v(:, iy0 + 1:iy0 + ly + lyx, iz0 + 1:iz0 + lz, f) = blk
+      end do
+      close(10)
+      deallocate(blk)
+    end do
+  end subroutine read_velocity
+
+  ! input_outputf (load_save.f90:59-95)
+  subroutine read_forcing(idump, iseed, iv, iy, b)
+    integer, intent(in) :: idump
+    integer, intent(out) :: iseed, iv(32), iy
+    double precision, intent(out) :: b(450)
+    open(10, file=force_name(idump), status='old', form='unformatted')
+    read(10) iseed
+    read(10) iv
+    read(10) iy
+    read(10) b
+    close(10)
+  end subroutine read_forcing
+
 end program turb_gpu

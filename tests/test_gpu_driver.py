@@ -1,0 +1,66 @@
+"""The Fortran host driver (parameter.d in, reference restart files out) on the GPU:
+10 steps in one run equal 5 steps, a restart through vel2D<idp>.0000 / part.0000
+(load_save.f90 formats), and 5 more steps."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parents[1]
+DRIVER = ROOT / "mpi_2d_domain_decomp_turb_particles_amd" / "turb_gpu"
+BASE = ROOT / "tests" / "golden" / "parameter.d_64_short"
+
+
+def _param(path, nstep, new_flow, idp, new_particle):
+    lines = BASE.read_text().split("\n")
+    lines[4] = str(nstep)
+    lines[12] = ".TRUE." if new_flow else ".FALSE."
+    lines[14] = str(idp)
+    lines[44] = ".TRUE." if new_particle else ".FALSE."
+    lines[48] = ".FALSE."          # no HI: the restart files carry no disturbance velocity
+    lines[52] = "5000"
+    path.write_text("\n".join(lines))
+
+
+def _read_records(path, count):
+    """Fortran unformatted sequential records (4-byte length markers)."""
+    out = []
+    with open(path, "rb") as f:
+        for _ in range(count):
+            n = int(np.frombuffer(f.read(4), dtype=np.int32)[0])
+            out.append(f.read(n))
+            assert int(np.frombuffer(f.read(4), dtype=np.int32)[0]) == n
+    return out
+
+
+def _run(cwd, param):
+    r = subprocess.run([str(DRIVER), str(param)], cwd=cwd, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0 and "Successfully completed" in r.stdout
+
+
+@pytest.mark.skipif(not DRIVER.exists(), reason="Fortran driver not built")
+def test_restart_through_reference_files(tmp_path):
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir(); b.mkdir()
+    _param(a / "p10", 10, True, 0, True)
+    _run(a, a / "p10")
+    _param(b / "p5", 5, True, 0, True)
+    _run(b, b / "p5")
+    _param(b / "p5r", 5, False, 1, False)
+    _run(b, b / "p5r")
+    va = _read_records(a / "vel2D1.0000", 6)
+    vb = _read_records(b / "vel2D2.0000", 6)
+    for x, y in zip(va, vb):
+        assert x == y                                   # the flow is deterministic in FP64
+    pa = _read_records(a / "part.0000", 2)
+    pb = _read_records(b / "part.0000", 2)
+    assert pa[0] == pb[0]                               # same droplet count
+    n = int(np.frombuffer(pa[0], dtype=np.int32)[0])
+    ya = np.frombuffer(pa[1], dtype=np.float64)[: 8 * n].reshape(n, 8)
+    yb = np.frombuffer(pb[1], dtype=np.float64)[: 8 * n].reshape(n, 8)
+    assert np.array_equal(ya[:, 7], yb[:, 7])
+    assert np.array_equal(ya, yb)                        # no HI, no relocation: deterministic too
