@@ -642,12 +642,18 @@ extern "C" int ddns_init_droplets(ddns_handle* h, int seed, int radii_kind, doub
 }
 
 // ------------------------------------------------------------------ the step
+// Cell widths of the pair searches.  Collisions need the statistics radius 20 a_max
+// and two grid cells (the reference's adjacent-cell window); HI needs its truncation
+// radius, usually much wider, so it gets its own, coarser grid.
+static double coll_cell_width(const ddns_handle* h)
+{
+    const double dx = h->pc.pi2 / h->N;
+    return std::max(20.0 * h->pc.radmax, 2.0 * dx);
+}
+
 static double cell_width(const ddns_handle* h)
 {
-    // widest distance a pair test can need: the statistics radius 20 a_max, the HI
-    // truncation, and two grid cells (the collision test's cell window)
-    const double dx = h->pc.pi2 / h->N;
-    double w = std::max(20.0 * h->pc.radmax, 2.0 * dx);
+    double w = coll_cell_width(h);
     if (h->p.hdi_included) w = std::max(w, h->pc.d_hdi);
     return w;
 }
@@ -683,6 +689,10 @@ static int droplet_block(ddns_handle* h, float* ms)
     h->new_particle = 0;
     pc.new_particle = 0;
     DDNS_HIP(hipEventRecord(h->ev[12], s));
+    if (h->p.hdi_included && std::min(256, (int)std::floor(pc.pi2 / coll_cell_width(h))) > h->pb.G) {
+        st = part_build_cells(h->pb, pc, coll_cell_width(h), s);              // finer grid for the collision search
+        if (st) return st;
+    }
     st = part_collide(h->pb, pc, s);                                          // 584
     if (st) return st;
     const bool reloc = h->p.nonoverlap && h->istep < h->p.nstep_grow;         // 590

@@ -21,7 +21,7 @@
 namespace ddns {
 
 constexpr int HT = 256;
-constexpr int RED_BLOCKS = 512;
+constexpr int RED_BLOCKS = 2048;   // fixed reduction grid: 8 work-groups per CU
 
 int exclusive_scan(const int* in, int* out, int n, int* tmp, hipStream_t s);
 
@@ -139,6 +139,7 @@ __global__ void __launch_bounds__(HT) k_dot_part(const TA* a, const TB* b, long 
 {
     __shared__ double s[HT];
     double acc = 0.0;
+#pragma unroll 4
     for (long long i = (long long)blockIdx.x * HT + threadIdx.x; i < n; i += (long long)gridDim.x * HT)
         acc += (double)a[i] * (double)b[i];
     s[threadIdx.x] = acc;
@@ -222,6 +223,7 @@ __global__ void __launch_bounds__(HT) k_mdot(const T* __restrict__ Q, const T* _
         double acc[MD_J];
 #pragma unroll
         for (int jj = 0; jj < MD_J; ++jj) acc[jj] = 0.0;
+#pragma unroll 2
         for (long long i = (long long)blockIdx.x * HT + tid; i < n3; i += (long long)gridDim.x * HT) {
             const double vi = (double)v[i];
 #pragma unroll
@@ -274,8 +276,10 @@ __global__ void __launch_bounds__(HT) k_maxpy(T* __restrict__ v, const T* __rest
     for (int j = tid; j < k; j += HT) hs[j] = h[j];
     __syncthreads();
     double nacc = 0.0;
+#pragma unroll 2
     for (long long i = (long long)blockIdx.x * HT + tid; i < n3; i += (long long)gridDim.x * HT) {
         double x = (double)v[i];
+#pragma unroll 4
         for (int j = 0; j < k; ++j) x -= hs[j] * (double)Q[(size_t)j * n3 + i];
         const T xt = (T)x;
         v[i] = xt;
@@ -329,9 +333,24 @@ struct GmSmall {
     __host__ __device__ size_t total() const { return RV() + M1(); }
 };
 
-__global__ void k_ls(double* sm, GmSmall L, int iter, int* done_host)
+__device__ void ls_solve(double* sm, GmSmall L, int iter, int* done_host);
+constexpr int LS_MAX = 8192;   // GmSmall::total() for maxiter <= 50, the reference's value
+
+__global__ void __launch_bounds__(HT) k_ls(double* gsm, GmSmall L, int iter, int* done_host)
 {
-    if (threadIdx.x != 0) return;
+    // the whole small problem (at most LS_MAX doubles, 64 KB) is copied to LDS, solved
+    // by one lane at LDS latency, and copied back
+    __shared__ double sm[LS_MAX];
+    const size_t tot = L.total();
+    for (size_t i = threadIdx.x; i < tot; i += blockDim.x) sm[i] = gsm[i];
+    __syncthreads();
+    if (threadIdx.x == 0) ls_solve(sm, L, iter, done_host);
+    __syncthreads();
+    for (size_t i = threadIdx.x; i < tot; i += blockDim.x) gsm[i] = sm[i];
+}
+
+__device__ void ls_solve(double* sm, GmSmall L, int iter, int* done_host)
+{
     const int M1 = L.M1();
     double* H = sm + L.H0();
     double* Rn = sm + L.R0();
@@ -444,7 +463,7 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
         DDNS_HIP(hipMalloc(&pb.gm_v, 3 * (size_t)pb.cap * sizeof(double)));
         DDNS_HIP(hipMalloc(&pb.gm_Q, (size_t)(maxiter + 1) * 3 * pb.cap * sizeof(T)));
     }
-    if (L.total() > 16384 || maxiter > 120) { set_error("GMRES maxiter too large (at most 120)"); return DDNS_ERR_ARG; }
+    if (L.total() > (size_t)LS_MAX) { set_error("GMRES maxiter too large (at most 50)"); return DDNS_ERR_ARG; }
     double* normpart = pb.gm_red + (size_t)128 * RED_BLOCKS;
     double* hbuf = normpart + RED_BLOCKS;
     double* sm = pb.gm_small;
@@ -491,7 +510,7 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
         }
         hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(HT), 0, s, (const double*)normpart, RED_BLOCKS, hn, 1);
         if (iter < maxiter) hipLaunchKernelGGL((k_div_dev<T>), dim3(nb3), dim3(HT), 0, s, Q + (size_t)iter * n3, v, hn, n3);
-        hipLaunchKernelGGL(k_ls, dim3(1), dim3(1), 0, s, sm, L, iter, done_dev);
+        hipLaunchKernelGGL(k_ls, dim3(1), dim3(HT), 0, s, sm, L, iter, done_dev);
         DDNS_HIP(hipGetLastError());
         DDNS_HIP(hipMemcpyAsync(&done, done_dev, sizeof(int), hipMemcpyDeviceToHost, s));
         DDNS_HIP(hipStreamSynchronize(s));

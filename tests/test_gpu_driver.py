@@ -54,8 +54,12 @@ def test_restart_through_reference_files(tmp_path):
     _run(b, b / "p5r")
     va = _read_records(a / "vel2D1.0000", 6)
     vb = _read_records(b / "vel2D2.0000", 6)
+    # a restart rebuilds the z-inverse fields from the stored (symmetrised) u-hat, the
+    # continuous run symmetrises the kx = 0 plane inside the y stage: the same
+    # operations in another order, so the fields agree to rounding
     for x, y in zip(va, vb):
-        assert x == y                                   # the flow is deterministic in FP64
+        fx, fy = np.frombuffer(x, dtype=np.float64), np.frombuffer(y, dtype=np.float64)
+        assert np.abs(fx - fy).max() <= 1e-12 * np.abs(fx).max()
     pa = _read_records(a / "part.0000", 2)
     pb = _read_records(b / "part.0000", 2)
     assert pa[0] == pb[0]                               # same droplet count
@@ -63,4 +67,4 @@ def test_restart_through_reference_files(tmp_path):
     ya = np.frombuffer(pa[1], dtype=np.float64)[: 8 * n].reshape(n, 8)
     yb = np.frombuffer(pb[1], dtype=np.float64)[: 8 * n].reshape(n, 8)
     assert np.array_equal(ya[:, 7], yb[:, 7])
-    assert np.array_equal(ya, yb)                        # no HI, no relocation: deterministic too
+    assert np.abs(ya - yb).max() < 1e-10
