@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
                 const int wy = qy < 0 ? -1 : (qy >= g.G ? 1 : 0);
                 const int wz = qz < 0 ? -1 : (qz >= g.G ? 1 : 0);
                 qx -= wx * g.G; qy -= wy * g.G; qz -= wz * g.G;
-                const int cell = (qz * g.G + qy) * g.G + qx;
+                const int cell = cell_code(qx, qy, qz);
                 const int e = a.cell_start[cell + 1];
                 for (int p = a.cell_start[cell]; p < e; ++p) {
                     if (p == t) continue;
@@ -354,7 +354,7 @@ __global__ void __launch_bounds__(CT) k_overlap(OverArgs a)
                 const int wy = qy < 0 ? -1 : (qy >= g.G ? 1 : 0);
                 const int wz = qz < 0 ? -1 : (qz >= g.G ? 1 : 0);
                 qx -= wx * g.G; qy -= wy * g.G; qz -= wz * g.G;
-                const int cell = (qz * g.G + qy) * g.G + qx;
+                const int cell = cell_code(qx, qy, qz);
                 const int e = a.cell_start[cell + 1];
                 for (int p = a.cell_start[cell]; p < e; ++p) {
                     if (p == t) continue;

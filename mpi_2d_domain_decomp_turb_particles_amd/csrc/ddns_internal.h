@@ -26,6 +26,29 @@ namespace ddns {
 //   [7] the reference's visiting order of the pair (ih1, cell loop, linked list)
 constexpr int DDNS_PAIR_REC = 8;
 
+// Coarse-cell number of cell (x, y, z), 0 <= x, y, z < 256: the Morton (Z-curve)
+// interleave of the three indices, x in the lowest bit.  Droplets sorted by it form
+// compact 3-D blocks, so the lanes of a wavefront touch overlapping field stencils and
+// neighbouring cells.  The numbers of a G^3 grid lie below 8^cell_bits(G).
+__host__ __device__ __forceinline__ unsigned cell_spread(unsigned v)
+{
+    v &= 0xffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__host__ __device__ __forceinline__ int cell_code(int x, int y, int z)
+{
+    return (int)(cell_spread((unsigned)x) | (cell_spread((unsigned)y) << 1) | (cell_spread((unsigned)z) << 2));
+}
+inline int cell_bits(int G)
+{
+    int b = 0;
+    while ((1 << b) < G) ++b;
+    return b;
+}
+
 int hip_fail(hipError_t e, const char* what, const char* file, int line);
 void set_error(const std::string& msg);
 

@@ -12,6 +12,86 @@
 
 namespace ddns {
 
+// ---------------------------------------------------------------- tile transfers
+// A tile is R rows of W adjacent complex values in HBM (row r at src + r * rstride)
+// and R * LDR values in LDS (row-major, or transposed when TRANS: lds[c * LDR + r]).
+// Every lane moves 16 bytes per access and the trip count is a compile-time
+// constant, so a lane issues all of its loads before the first wait.
+template <typename C> struct vec16;
+template <> struct vec16<float2> {
+    using V = float4;
+    static constexpr int VW = 2;
+    __device__ static void split(const V& v, float2* o) { o[0] = make_float2(v.x, v.y); o[1] = make_float2(v.z, v.w); }
+    __device__ static V join(const float2* c) { return make_float4(c[0].x, c[0].y, c[1].x, c[1].y); }
+};
+template <> struct vec16<double2> {
+    using V = double2;
+    static constexpr int VW = 1;
+    __device__ static void split(const V& v, double2* o) { o[0] = v; }
+    __device__ static V join(const double2* c) { return c[0]; }
+};
+
+// the registers one lane holds for a tile in flight
+template <typename C, int R, int W, int NT>
+struct TileRegs {
+    using VT = vec16<C>;
+    static constexpr int VW = VT::VW, PER_ROW = W / VW, TOT = R * PER_ROW, Q = TOT / NT;
+    static_assert(W % VW == 0 && TOT % NT == 0, "tile must split evenly over the work-group");
+    typename VT::V v[Q];
+};
+
+// issue the loads of a tile into registers (returns without waiting)
+template <typename C, int R, int W, int NT>
+__device__ __forceinline__ void tile_fetch(TileRegs<C, R, W, NT>& t, const C* __restrict__ src, size_t rstride, int tid)
+{
+    using TR = TileRegs<C, R, W, NT>;
+#pragma unroll
+    for (int q = 0; q < TR::Q; ++q) {
+        const int e = tid + q * NT, r = e / TR::PER_ROW, c = (e % TR::PER_ROW) * TR::VW;
+        t.v[q] = *(const typename TR::VT::V*)(src + (size_t)r * rstride + c);
+    }
+}
+
+// write a fetched tile to LDS
+template <typename C, int R, int W, int LDR, int NT, bool TRANS = false>
+__device__ __forceinline__ void tile_put(C* __restrict__ lds, const TileRegs<C, R, W, NT>& t, int tid)
+{
+    using TR = TileRegs<C, R, W, NT>;
+#pragma unroll
+    for (int q = 0; q < TR::Q; ++q) {
+        const int e = tid + q * NT, r = e / TR::PER_ROW, c = (e % TR::PER_ROW) * TR::VW;
+        C v[TR::VW];
+        TR::VT::split(t.v[q], v);
+#pragma unroll
+        for (int k = 0; k < TR::VW; ++k) lds[TRANS ? (c + k) * LDR + r : r * LDR + c + k] = v[k];
+    }
+}
+
+template <typename C, int R, int W, int LDR, int NT, bool TRANS = false>
+__device__ __forceinline__ void tile_load(C* __restrict__ lds, const C* __restrict__ src, size_t rstride, int tid)
+{
+    TileRegs<C, R, W, NT> t;
+    tile_fetch<C, R, W, NT>(t, src, rstride, tid);
+    tile_put<C, R, W, LDR, NT, TRANS>(lds, t, tid);
+}
+
+template <typename C, int R, int W, int LDR, int NT>
+__device__ __forceinline__ void tile_store(C* __restrict__ dst, const C* __restrict__ lds, size_t rstride, int tid,
+                                           decltype(C{}.x) scale = 1)
+{
+    using VT = vec16<C>;
+    constexpr int VW = VT::VW, PER_ROW = W / VW, TOT = R * PER_ROW;
+    static_assert(W % VW == 0 && TOT % NT == 0, "tile must split evenly over the work-group");
+#pragma unroll
+    for (int q = 0; q < TOT / NT; ++q) {
+        const int e = tid + q * NT, r = e / PER_ROW, c = (e % PER_ROW) * VW;
+        C v[VW];
+#pragma unroll
+        for (int k = 0; k < VW; ++k) v[k] = cscale(lds[r * LDR + c + k], scale);
+        *(typename VT::V*)(dst + (size_t)r * rstride + c) = VT::join(v);
+    }
+}
+
 // ---------------------------------------------------------------- register DFTs
 template <typename C, bool INV>
 __device__ __forceinline__ void dft2(C* v)

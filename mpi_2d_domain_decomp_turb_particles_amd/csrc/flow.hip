@@ -165,30 +165,20 @@ __global__ void __launch_bounds__(NT) k_z(KzArgs a)
     if constexpr (MODE == KZ_PLAIN_FWD || MODE == KZ_PLAIN_INV) {
         const C* src = (const C*)a.B[0];
         C* dst = (C*)a.U[0];
-        for (int e = tid; e < TC * N; e += NT) {
-            const int n = e / TC, c = e % TC;
-            lds[0][n * LDR + c] = src[base + n * rowstride + c];
-        }
+        tile_load<C, N, TC, LDR, NT>(lds[0], src + base, rowstride, tid);
         __syncthreads();
         fft_tile<T, N, MODE == KZ_PLAIN_INV, TC, LDR, NT>(lds[0], tw, tid);
-        const T sc = (T)a.scale;
-        for (int e = tid; e < TC * N; e += NT) {
-            const int n = e / TC, c = e % TC;
-            dst[base + n * rowstride + c] = cscale(lds[0][n * LDR + c], sc);
-        }
+        tile_store<C, N, TC, LDR, NT>(dst + base, lds[0], rowstride, tid, (T)a.scale);
         return;
     } else {
         if constexpr (MODE == KZ_FULL) {
-            for (int e = tid; e < TC * N; e += NT) {
-                const int n = e / TC, c = e % TC;
-                const size_t g = base + n * rowstride + c;
 #pragma unroll
-                for (int q = 0; q < 3; ++q) lds[q][n * LDR + c] = ((const C*)a.B[q])[g];
-            }
+            for (int q = 0; q < 3; ++q) tile_load<C, N, TC, LDR, NT>(lds[q], (const C*)a.B[q] + base, rowstride, tid);
             __syncthreads();
 #pragma unroll
             for (int q = 0; q < 3; ++q) fft_tile<T, N, false, TC, LDR, NT>(lds[q], tw, tid);
             const T inv_n = (T)1 / (T)N;
+#pragma unroll 2
             for (int e = tid; e < TC * N; e += NT) {
                 const int n = e / TC, c = e % TC;
                 const size_t g = base + n * rowstride + c;
@@ -208,12 +198,8 @@ __global__ void __launch_bounds__(NT) k_z(KzArgs a)
                 }
             }
         } else {
-            for (int e = tid; e < TC * N; e += NT) {
-                const int n = e / TC, c = e % TC;
-                const size_t g = base + n * rowstride + c;
 #pragma unroll
-                for (int q = 0; q < 3; ++q) lds[q][n * LDR + c] = ((const C*)a.U[q])[g];
-            }
+            for (int q = 0; q < 3; ++q) tile_load<C, N, TC, LDR, NT>(lds[q], (const C*)a.U[q] + base, rowstride, tid);
         }
         __syncthreads();
         // vorticity w = i k x u (turb.f90:506-517), one component at a time in tile 3
@@ -231,21 +217,13 @@ __global__ void __launch_bounds__(NT) k_z(KzArgs a)
             }
             __syncthreads();
             fft_tile<T, N, true, TC, LDR, NT>(lds[3], tw, tid);
-            C* dst = (C*)a.A[3 + q];
-            for (int e = tid; e < TC * N; e += NT) {
-                const int n = e / TC, c = e % TC;
-                dst[base + n * rowstride + c] = lds[3][n * LDR + c];
-            }
+            tile_store<C, N, TC, LDR, NT>((C*)a.A[3 + q] + base, lds[3], rowstride, tid);
             __syncthreads();
         }
 #pragma unroll 1
         for (int q = 0; q < 3; ++q) {
             fft_tile<T, N, true, TC, LDR, NT>(lds[q], tw, tid);
-            C* dst = (C*)a.A[q];
-            for (int e = tid; e < TC * N; e += NT) {
-                const int n = e / TC, c = e % TC;
-                dst[base + n * rowstride + c] = lds[q][n * LDR + c];
-            }
+            tile_store<C, N, TC, LDR, NT>((C*)a.A[q] + base, lds[q], rowstride, tid);
         }
     }
 }
@@ -264,10 +242,7 @@ __global__ void __launch_bounds__(NT) k_y(KyArgs a)
     const int z = blockIdx.y;
     C* f = (C*)a.F[blockIdx.z];
     const size_t base = (size_t)z * N * M + kx0;
-    for (int e = tid; e < TC * N; e += NT) {
-        const int n = e / TC, c = e % TC;
-        lds[n * LDR + c] = f[base + (size_t)n * M + c];
-    }
+    tile_load<C, N, TC, LDR, NT>(lds, f + base, M, tid);
     __syncthreads();
     if constexpr (SYM) {
         // kx = 0 column: (c(ky) + conj c(-ky)) / 2, the z-transformed form of the
@@ -284,11 +259,7 @@ __global__ void __launch_bounds__(NT) k_y(KyArgs a)
         }
     }
     fft_tile<T, N, INV, TC, LDR, NT>(lds, (const C*)a.tw, tid);
-    const T sc = (T)a.scale;
-    for (int e = tid; e < TC * N; e += NT) {
-        const int n = e / TC, c = e % TC;
-        f[base + (size_t)n * M + c] = INV ? lds[n * LDR + c] : cscale(lds[n * LDR + c], sc);
-    }
+    tile_store<C, N, TC, LDR, NT>(f + base, lds, M, tid, INV ? (T)1 : (T)a.scale);
 }
 
 // ------------------------------------------------------------------ x stage
@@ -382,13 +353,13 @@ __global__ void __launch_bounds__(NT) k_x_fused(KxArgs a)
     const C* tw = (const C*)a.tw;
     const C* twh = (const C*)a.twh;
     C ph[6][QP];
+    // the loads of field f + 1 are in flight while field f is transformed
+    TileRegs<C, TL, M, NT> next;
+    tile_fetch<C, TL, M, NT>(next, (const C*)a.A[0] + line0 * M, M, tid);
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
-        const C* src = (const C*)a.A[f] + line0 * M;
-        for (int e = tid; e < TL * M; e += NT) {
-            const int line = e / M, m = e % M;
-            lds[m * LDR + line] = src[(size_t)line * M + m];
-        }
+        tile_put<C, TL, M, LDR, NT, true>(lds, next, tid);
+        if (f < 5) tile_fetch<C, TL, M, NT>(next, (const C*)a.A[f + 1] + line0 * M, M, tid);
         __syncthreads();
         c2r_pack<T, N, TL, LDR>(lds, tw, tid);
         __syncthreads();
@@ -453,11 +424,7 @@ __global__ void __launch_bounds__(NT) k_x_plain(const void* src, void* dst, cons
     const C* tw = (const C*)tw_;
     const C* twh = (const C*)twh_;
     // both directions see the line as M complex values (real pairs for the C2R output / R2C input)
-    const C* s = (const C*)src + line0 * M;
-    for (int e = tid; e < TL * M; e += NT) {
-        const int line = e / M, m = e % M;
-        lds[m * LDR + line] = s[(size_t)line * M + m];
-    }
+    tile_load<C, TL, M, LDR, NT, true>(lds, (const C*)src + line0 * M, M, tid);
     __syncthreads();
     if constexpr (INV) {
         c2r_pack<T, N, TL, LDR>(lds, tw, tid);

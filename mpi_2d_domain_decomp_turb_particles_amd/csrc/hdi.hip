@@ -54,7 +54,7 @@ __global__ void __launch_bounds__(HT) k_nei(int pass, const double4* __restrict_
                 const int wy = qy < 0 ? -1 : (qy >= G ? 1 : 0);
                 const int wz = qz < 0 ? -1 : (qz >= G ? 1 : 0);
                 qx -= wx * G; qy -= wy * G; qz -= wz * G;
-                const int cell = (qz * G + qy) * G + qx;
+                const int cell = cell_code(qx, qy, qz);
                 const int e = cell_start[cell + 1];
                 for (int p = cell_start[cell]; p < e; ++p) {
                     if (p == t) continue;

@@ -497,7 +497,7 @@ __global__ void k_cell_key(const double* pos, int* cell_of, int* cnt, int n, int
         q = q < 0 ? 0 : (q >= G ? G - 1 : q);
         c3[c] = q;
     }
-    const int cell = (c3[2] * G + c3[1]) * G + c3[0];       // x fastest, as the field layout
+    const int cell = cell_code(c3[0], c3[1], c3[2]);
     cell_of[i] = cell;
     atomicAdd(cnt + cell, 1);
 }
@@ -541,7 +541,7 @@ int part_build_cells(PartBuffers& pb, const PartConst& pc, double hmin, hipStrea
     int G = (int)floor(pc.pi2 / hmin);
     if (G > GMAX) G = GMAX;
     if (G < 3) { set_error("domain too small for the pair search grid (need 3 cells of the interaction radius)"); return DDNS_ERR_DOMAIN; }
-    const long long ncell = (long long)G * G * G;
+    const long long ncell = 1LL << (3 * cell_bits(G));      // Morton numbers of the G^3 cells (G <= 256)
     if (!pb.cell_count) {
         // sized once for the finest grid, so switching between the collision and the
         // overlap grids never reallocates
