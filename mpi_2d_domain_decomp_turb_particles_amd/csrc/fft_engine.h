@@ -184,24 +184,28 @@ template <> struct Plan<256>  { static constexpr int P = 2, R0 = 16, R1 = 16, R2
 template <> struct Plan<512>  { static constexpr int P = 3, R0 = 8,  R1 = 8,  R2 = 8; };
 template <> struct Plan<1024> { static constexpr int P = 3, R0 = 16, R1 = 16, R2 = 4; };
 
-// One Stockham pass of stride NS (product of the radices already applied).
-// Caller guarantees the LDS tile is complete (barrier) before the call; the pass
-// ends with a barrier.
-template <typename T, int N, int R, int NS, bool INV, int TC, int LDR, int NT>
+// One Stockham pass of stride NS (product of the radices already applied) over K
+// tiles at once (tile k at lds + k * TS): each work item runs one radix-R butterfly
+// per tile, so a lane has K independent butterflies in flight.  Caller guarantees the
+// tiles are complete (barrier) before the call; the pass ends with a barrier.  tw may
+// point to LDS or to global memory.
+template <typename T, int N, int R, int NS, bool INV, int TC, int LDR, int NT, int K = 1, int TS = 0>
 __device__ __forceinline__ void fft_pass(cpx<T>* __restrict__ lds, const cpx<T>* __restrict__ tw, int tid)
 {
     using C = cpx<T>;
     constexpr int NJ = N / R;
     constexpr int ITEMS = TC * NJ;
     constexpr int Q = (ITEMS + NT - 1) / NT;
-    C v[Q][R];
+    C v[K][Q][R];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int it = tid + NT * q;
         if (ITEMS % NT == 0 || it < ITEMS) {
             const int c = it % TC, j = it / TC;
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = lds[(j + r * NJ) * LDR + c];
+            for (int k = 0; k < K; ++k)
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[k][q][r] = lds[k * TS + (j + r * NJ) * LDR + c];
         }
     }
     __syncthreads();
@@ -210,32 +214,45 @@ __device__ __forceinline__ void fft_pass(cpx<T>* __restrict__ lds, const cpx<T>*
         const int it = tid + NT * q;
         if (ITEMS % NT == 0 || it < ITEMS) {
             const int c = it % TC, j = it / TC;
-            const int k = j % NS;
+            const int m = j % NS;
             if constexpr (NS > 1) {
                 constexpr int STEP = N / (NS * R);
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    C w = tw[(k * r * STEP) & (N - 1)];
+                    C w = tw[(m * r * STEP) & (N - 1)];
                     if (INV) w.y = -w.y;
-                    v[q][r] = cmul(v[q][r], w);
+#pragma unroll
+                    for (int k = 0; k < K; ++k) v[k][q][r] = cmul(v[k][q][r], w);
                 }
             }
-            dft<R, C, INV>(v[q]);
-            const int base = (j / NS) * NS * R + k;
+            const int base = (j / NS) * NS * R + m;
 #pragma unroll
-            for (int r = 0; r < R; ++r) lds[(base + r * NS) * LDR + c] = v[q][r];
+            for (int k = 0; k < K; ++k) {
+                dft<R, C, INV>(v[k][q]);
+#pragma unroll
+                for (int r = 0; r < R; ++r) lds[k * TS + (base + r * NS) * LDR + c] = v[k][q][r];
+            }
         }
     }
     __syncthreads();
 }
 
-template <typename T, int N, bool INV, int TC, int LDR, int NT>
+// FFT of every column of K tiles (tile k at lds + k * TS) through the plan of N.
+template <typename T, int N, bool INV, int TC, int LDR, int NT, int K = 1, int TS = 0>
 __device__ __forceinline__ void fft_tile(cpx<T>* __restrict__ lds, const cpx<T>* __restrict__ tw, int tid)
 {
     using P = Plan<N>;
-    fft_pass<T, N, P::R0, 1, INV, TC, LDR, NT>(lds, tw, tid);
-    if constexpr (P::P > 1) fft_pass<T, N, P::R1, P::R0, INV, TC, LDR, NT>(lds, tw, tid);
-    if constexpr (P::P > 2) fft_pass<T, N, P::R2, P::R0 * P::R1, INV, TC, LDR, NT>(lds, tw, tid);
+    fft_pass<T, N, P::R0, 1, INV, TC, LDR, NT, K, TS>(lds, tw, tid);
+    if constexpr (P::P > 1) fft_pass<T, N, P::R1, P::R0, INV, TC, LDR, NT, K, TS>(lds, tw, tid);
+    if constexpr (P::P > 2) fft_pass<T, N, P::R2, P::R0 * P::R1, INV, TC, LDR, NT, K, TS>(lds, tw, tid);
+}
+
+// the twiddle table exp(-2 pi i m / N), m < N, copied to LDS by the whole work-group
+template <typename C, int N, int NT>
+__device__ __forceinline__ void stage_twiddles(C* __restrict__ tws, const C* __restrict__ tw, int tid)
+{
+#pragma unroll
+    for (int m = tid; m < N; m += NT) tws[m] = tw[m];
 }
 
 }  // namespace ddns

@@ -58,7 +58,7 @@ template <int TC> constexpr int ldr_of() { return (TC >= 8) ? TC + 1 : TC; }
 template <typename T, int N> constexpr int kz_ldr()
 {
     constexpr int TC = kz_tc<T, N>();
-    return (4 * N * (TC + 1) * (int)sizeof(cpx<T>) < 160 * 1024) ? TC + 1 : TC;
+    return ((4 * N * (TC + 1) + N) * (int)sizeof(cpx<T>) <= 160 * 1024) ? TC + 1 : TC;   // 4 tiles + twiddles
 }
 
 __device__ __forceinline__ int wave_index(int i, int N) { return i <= N / 2 ? i : i - N; }
@@ -152,13 +152,15 @@ __global__ void __launch_bounds__(NT) k_z(KzArgs a)
     constexpr int LDR = kz_ldr<T, N>();
     constexpr int NTILE = (MODE == KZ_FULL || MODE == KZ_CURL_INV) ? 4 : 1;
     __shared__ C lds[NTILE][N * LDR];
+    __shared__ C tws[N];
     const int tid = threadIdx.x;
     constexpr int NBX = M / TC;
     constexpr int P = (TC * (int)sizeof(C) >= 128) ? 1 : 128 / (TC * (int)sizeof(C));
     const int tile = xcd_tile<(NBX % P == 0) ? P : 1>(blockIdx.x);
     const int kx0 = (tile % NBX) * TC;
     const int ky = tile / NBX;
-    const C* tw = (const C*)a.tw;
+    stage_twiddles<C, N, NT>(tws, (const C*)a.tw, tid);
+    const C* tw = tws;
     const size_t rowstride = (size_t)N * M;
     const size_t base = (size_t)ky * M + kx0;
 
@@ -175,8 +177,7 @@ __global__ void __launch_bounds__(NT) k_z(KzArgs a)
 #pragma unroll
             for (int q = 0; q < 3; ++q) tile_load<C, N, TC, LDR, NT>(lds[q], (const C*)a.B[q] + base, rowstride, tid);
             __syncthreads();
-#pragma unroll
-            for (int q = 0; q < 3; ++q) fft_tile<T, N, false, TC, LDR, NT>(lds[q], tw, tid);
+            fft_tile<T, N, false, TC, LDR, NT, 3, N * LDR>(lds[0], tw, tid);     // the three components together
             const T inv_n = (T)1 / (T)N;
 #pragma unroll 2
             for (int e = tid; e < TC * N; e += NT) {
@@ -220,11 +221,9 @@ __global__ void __launch_bounds__(NT) k_z(KzArgs a)
             tile_store<C, N, TC, LDR, NT>((C*)a.A[3 + q] + base, lds[3], rowstride, tid);
             __syncthreads();
         }
-#pragma unroll 1
-        for (int q = 0; q < 3; ++q) {
-            fft_tile<T, N, true, TC, LDR, NT>(lds[q], tw, tid);
-            tile_store<C, N, TC, LDR, NT>((C*)a.A[q] + base, lds[q], rowstride, tid);
-        }
+        fft_tile<T, N, true, TC, LDR, NT, 3, N * LDR>(lds[0], tw, tid);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) tile_store<C, N, TC, LDR, NT>((C*)a.A[q] + base, lds[q], rowstride, tid);
     }
 }
 
@@ -237,9 +236,11 @@ __global__ void __launch_bounds__(NT) k_y(KyArgs a)
     constexpr int TC = ky_tc<T, N>();
     constexpr int LDR = ldr_of<TC>();
     __shared__ C lds[N * LDR];
+    __shared__ C tws[N];
     const int tid = threadIdx.x;
     const int kx0 = blockIdx.x * TC;
     const int z = blockIdx.y;
+    stage_twiddles<C, N, NT>(tws, (const C*)a.tw, tid);
     C* f = (C*)a.F[blockIdx.z];
     const size_t base = (size_t)z * N * M + kx0;
     tile_load<C, N, TC, LDR, NT>(lds, f + base, M, tid);
@@ -258,7 +259,7 @@ __global__ void __launch_bounds__(NT) k_y(KyArgs a)
             __syncthreads();
         }
     }
-    fft_tile<T, N, INV, TC, LDR, NT>(lds, (const C*)a.tw, tid);
+    fft_tile<T, N, INV, TC, LDR, NT>(lds, tws, tid);
     tile_store<C, N, TC, LDR, NT>(f + base, lds, M, tid, INV ? (T)1 : (T)a.scale);
 }
 
@@ -348,10 +349,13 @@ __global__ void __launch_bounds__(NT) k_x_fused(KxArgs a)
     constexpr int LDR = TL + 1;
     constexpr int QP = TL * M / NT;
     __shared__ C lds[M * LDR];
+    __shared__ C tws[N], twhs[M];
     const int tid = threadIdx.x;
     const size_t line0 = (size_t)blockIdx.x * TL;
-    const C* tw = (const C*)a.tw;
-    const C* twh = (const C*)a.twh;
+    stage_twiddles<C, N, NT>(tws, (const C*)a.tw, tid);
+    stage_twiddles<C, M, NT>(twhs, (const C*)a.twh, tid);
+    const C* tw = tws;
+    const C* twh = twhs;
     C ph[6][QP];
     // the loads of field f + 1 are in flight while field f is transformed
     TileRegs<C, TL, M, NT> next;

@@ -245,3 +245,37 @@ def test_hdi_steps_match_oracle(ddns, oracle_mod, prec):
         tol = 1e-9 if prec == 0 else 1e-4
         assert np.abs(pg - po).max() < tol * scale
         assert np.allclose(dg["yp"][:3], do["yp"][:3], rtol=0, atol=1e-9 if prec == 0 else 1e-6)
+
+
+@pytest.mark.slow
+def test_config_128_stochastic_hi_matches_oracle(ddns, oracle_mod):
+    """The 128^3 configuration at its own size: parameter.d_128_stoch (stochastic
+    forcing, its nu, dt and scales) with 100,000 droplets of 20 / 40 um and HI to 50
+    radii.  Two FP64 steps against the oracle."""
+    from pathlib import Path
+    p = ddns.read_parameter_file(Path(__file__).parent / "golden" / "parameter.d_128_stoch")
+    p.npart, p.nset = 100000, 2
+    p.radius[0], p.radius[1] = 20.0, 40.0
+    p.hdi_included, p.nonoverlap, p.hdi_trunc = 1, 1, 50.0
+    N = p.nx
+    v = random_spectral(N, seed=12, kmax=10)
+    with ddns.Solver(p) as s, oracle_mod.Oracle(p) as o:
+        for x in (s, o):
+            x.set_flow(v, v)
+            x.init_droplets(seed=290710)
+        assert np.array_equal(s.get_droplets()["yp"], o.get_droplets()["yp"])
+        for _ in range(2):
+            s.step(1)
+            o.step(1)
+            cg, co = s.collision_counts(), o.collision_counts()
+            assert cg.gmres_iters == co.gmres_iters
+            assert list(cg.icc) == list(co.icc) and cg.ncoll == co.ncoll
+        gv, _ = s.get_flow()
+        ov, _ = o.get_flow()
+        assert rel_l2(gv, ov) < 1e-11
+        bg, sg, yg, ivg = s.get_forcing_state()
+        bo, so, yo, ivo = o.get_forcing_state()
+        assert sg == so and yg == yo and np.array_equal(ivg, ivo)
+        dg, do = s.get_droplets(), o.get_droplets()
+        assert np.allclose(dg["yp"][:6], do["yp"][:6], rtol=1e-9, atol=1e-10)
+        assert np.abs(dg["pertvel"] - do["pertvel"]).max() < 1e-8 * max(1e-30, np.abs(do["pertvel"]).max())
