@@ -340,6 +340,9 @@ __device__ __forceinline__ void store_node(double* p, double a, double b, double
 }
 
 // Fused x stage: six C2R (u, w), physical u for the droplets, u x w, three R2C.
+// The C2R transforms run two fields at a time and the R2C transforms three at a time
+// through the same FFT passes, so every pass has two or three butterflies per lane;
+// the next pair's loads are in flight while the current pair is transformed.
 template <typename T, int N>
 __global__ void __launch_bounds__(NT) k_x_fused(KxArgs a)
 {
@@ -347,8 +350,9 @@ __global__ void __launch_bounds__(NT) k_x_fused(KxArgs a)
     constexpr int M = N / 2;
     constexpr int TL = kx_lines<N>();
     constexpr int LDR = TL + 1;
+    constexpr int TS = M * LDR;
     constexpr int QP = TL * M / NT;
-    __shared__ C lds[M * LDR];
+    __shared__ C lds[3 * TS];
     __shared__ C tws[N], twhs[M];
     const int tid = threadIdx.x;
     const size_t line0 = (size_t)blockIdx.x * TL;
@@ -357,22 +361,28 @@ __global__ void __launch_bounds__(NT) k_x_fused(KxArgs a)
     const C* tw = tws;
     const C* twh = twhs;
     C ph[6][QP];
-    // the loads of field f + 1 are in flight while field f is transformed
-    TileRegs<C, TL, M, NT> next;
-    tile_fetch<C, TL, M, NT>(next, (const C*)a.A[0] + line0 * M, M, tid);
+    TileRegs<C, TL, M, NT> next0, next1;
+    tile_fetch<C, TL, M, NT>(next0, (const C*)a.A[0] + line0 * M, M, tid);
+    tile_fetch<C, TL, M, NT>(next1, (const C*)a.A[1] + line0 * M, M, tid);
 #pragma unroll
-    for (int f = 0; f < 6; ++f) {
-        tile_put<C, TL, M, LDR, NT, true>(lds, next, tid);
-        if (f < 5) tile_fetch<C, TL, M, NT>(next, (const C*)a.A[f + 1] + line0 * M, M, tid);
+    for (int f = 0; f < 6; f += 2) {
+        tile_put<C, TL, M, LDR, NT, true>(lds, next0, tid);
+        tile_put<C, TL, M, LDR, NT, true>(lds + TS, next1, tid);
+        if (f < 4) {
+            tile_fetch<C, TL, M, NT>(next0, (const C*)a.A[f + 2] + line0 * M, M, tid);
+            tile_fetch<C, TL, M, NT>(next1, (const C*)a.A[f + 3] + line0 * M, M, tid);
+        }
         __syncthreads();
         c2r_pack<T, N, TL, LDR>(lds, tw, tid);
+        c2r_pack<T, N, TL, LDR>(lds + TS, tw, tid);
         __syncthreads();
-        fft_tile<T, M, true, TL, LDR, NT>(lds, twh, tid);
+        fft_tile<T, M, true, TL, LDR, NT, 2, TS>(lds, twh, tid);
 #pragma unroll
         for (int q = 0; q < QP; ++q) {
             const int e = tid + NT * q;
             const int line = e / M, m = e % M;
             ph[f][q] = lds[m * LDR + line];
+            ph[f + 1][q] = lds[TS + m * LDR + line];
         }
         __syncthreads();
     }
@@ -388,30 +398,23 @@ __global__ void __launch_bounds__(NT) k_x_fused(KxArgs a)
             store_node(p + 4, ph[0][q].y, ph[1][q].y, ph[2][q].y);
         }
     }
-    // nonlinear term u x w (turb.f90:634-637) in natural component order
-    C nl[3][QP];
+    // nonlinear term u x w (turb.f90:634-637) in natural component order, into the
+    // three tiles
 #pragma unroll
     for (int q = 0; q < QP; ++q) {
+        const int e = tid + NT * q;
+        const int line = e / M, m = e % M;
         const C ux = ph[0][q], uy = ph[1][q], uz = ph[2][q];
         const C wx = ph[3][q], wy = ph[4][q], wz = ph[5][q];
-        nl[0][q] = mk<C>(uy.x * wz.x - uz.x * wy.x, uy.y * wz.y - uz.y * wy.y);
-        nl[1][q] = mk<C>(uz.x * wx.x - ux.x * wz.x, uz.y * wx.y - ux.y * wz.y);
-        nl[2][q] = mk<C>(ux.x * wy.x - uy.x * wx.x, ux.y * wy.y - uy.y * wx.y);
+        lds[m * LDR + line] = mk<C>(uy.x * wz.x - uz.x * wy.x, uy.y * wz.y - uz.y * wy.y);
+        lds[TS + m * LDR + line] = mk<C>(uz.x * wx.x - ux.x * wz.x, uz.y * wx.y - ux.y * wz.y);
+        lds[2 * TS + m * LDR + line] = mk<C>(ux.x * wy.x - uy.x * wx.x, ux.y * wy.y - uy.y * wx.y);
     }
+    __syncthreads();
+    fft_tile<T, M, false, TL, LDR, NT, 3, TS>(lds, twh, tid);
     const T scale = (T)1 / (T)N;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-#pragma unroll
-        for (int q = 0; q < QP; ++q) {
-            const int e = tid + NT * q;
-            const int line = e / M, m = e % M;
-            lds[m * LDR + line] = nl[c][q];
-        }
-        __syncthreads();
-        fft_tile<T, M, false, TL, LDR, NT>(lds, twh, tid);
-        r2c_unpack_store<T, N, TL, LDR>(lds, tw, (C*)a.B[c], line0, scale, tid);
-        __syncthreads();
-    }
+    for (int c = 0; c < 3; ++c) r2c_unpack_store<T, N, TL, LDR>(lds + c * TS, tw, (C*)a.B[c], line0, scale, tid);
 }
 
 // Stand-alone x transforms of one field (kernel-level parity of mpifft3DRC/CR).
