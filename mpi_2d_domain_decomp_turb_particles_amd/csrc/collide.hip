@@ -138,17 +138,17 @@ __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
     const size_t C = (size_t)a.cap;
     const PairGeom& g = a.g;
     const double h = g.h;
-    double xa[3], xan[3], va[3], vna[3], ax1[3];
+    // only the position, radius and displacement length stay in registers; the
+    // velocities and the displacement itself are read in the rare branches that use them
+    double xa[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        xa[c] = a.pos[c * C + ia];
-        ax1[c] = a.dmove[c * C + ia];
-        xan[c] = xa[c] + ax1[c];
-        va[c] = a.fvp0[c * C + ia];
-        vna[c] = a.vel[c * C + ia];
-    }
+    for (int c = 0; c < 3; ++c) xa[c] = a.pos[c * C + ia];
     const double ra = a.rad[ia];
-    const double dmove_a = sqrt(ax1[0] * ax1[0] + ax1[1] * ax1[1] + ax1[2] * ax1[2]);
+    double dmove_a;
+    {
+        const double d0 = a.dmove[ia], d1 = a.dmove[C + ia], d2 = a.dmove[2 * C + ia];
+        dmove_a = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    }
     int cc[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -158,17 +158,15 @@ __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
     int ncollip1 = 0;
     long long tested = 0, instat = 0;
     const int nbin = DDNS_NBIN, ns = a.nset;
-    for (int ox = -1; ox <= 1; ++ox)
-        for (int oy = -1; oy <= 1; ++oy)
-            for (int oz = -1; oz <= 1; ++oz) {
-                int qx = cc[0] + ox, qy = cc[1] + oy, qz = cc[2] + oz;
-                const int wx = qx < 0 ? -1 : (qx >= g.G ? 1 : 0);
-                const int wy = qy < 0 ? -1 : (qy >= g.G ? 1 : 0);
-                const int wz = qz < 0 ? -1 : (qz >= g.G ? 1 : 0);
-                qx -= wx * g.G; qy -= wy * g.G; qz -= wz * g.G;
-                const int cell = cell_code(qx, qy, qz);
-                const int e = a.cell_start[cell + 1];
-                for (int p = a.cell_start[cell]; p < e; ++p) {
+    __shared__ int2 rng[27 * CT];
+    neighbour_ranges(a.cell_start, g.G, cc, rng + threadIdx.x, CT);
+#pragma unroll 1
+    for (int kc = 0; kc < 27; ++kc) {
+                int wx, wy, wz;
+                neighbour_shift(kc, g.G, cc, &wx, &wy, &wz);
+                const int2 range = rng[kc * CT + threadIdx.x];
+                const int e = range.y;
+                for (int p = range.x; p < e; ++p) {
                     if (p == t) continue;
                     const int ib = a.sorted[p];
                     const double4 cb = a.pk[p];
@@ -178,14 +176,9 @@ __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
                                    wx, wy, wz, &xb[0], &xb[1], &xb[2], &di, &dj, &dk))
                         continue;
                     // -------- partlib.f90:1330-1353
-                    double bx1[3], xbn[3], vb[3], xd[3];
+                    double xd[3];
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        bx1[c] = a.dmove[c * C + ib];
-                        xbn[c] = xb[c] + bx1[c];
-                        vb[c] = a.fvp0[c * C + ib];
-                        xd[c] = xa[c] - xb[c];
-                    }
+                    for (int c = 0; c < 3; ++c) xd[c] = xa[c] - xb[c];
                     const double dnij0 = sqrt(xd[0] * xd[0] + xd[1] * xd[1] + xd[2] * xd[2]);
                     ++tested;
                     const double rb = cb.w;
@@ -196,6 +189,12 @@ __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
                         ++instat;
                         int i1 = 1, i2 = 1;
                         if (dnij0 > rcoll) {
+                            double va[3], vb[3];
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) {
+                                va[c] = a.fvp0[c * C + ia];
+                                vb[c] = a.fvp0[c * C + ib];
+                            }
                             double xh[3];
 #pragma unroll
                             for (int c = 0; c < 3; ++c) xh[c] = xd[c] / dnij0;
@@ -230,9 +229,13 @@ __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
                             const double bbb = sqrt((va[0] - vb[0]) * (va[0] - vb[0]) + (va[1] - vb[1]) * (va[1] - vb[1]) +
                                                     (va[2] - vb[2]) * (va[2] - vb[2]));
                             if (bbb != 0.) {
-                                const double vangle = acos(fabs(aaa / bbb));
+                                const double q = fabs(aaa / bbb);
+                                const double vangle = acos(q);
                                 atomicAdd(a.bins + 5 * nb + bi, vangle);
-                                const double sb = bbb * sin(vangle);
+                                // sin(acos q) = sqrt(1 - q^2) for q in [0, 1]: the same value
+                                // as the reference's sin(vangle) (partlib.f90:1400) to rounding,
+                                // without the register cost of a general double sine
+                                const double sb = bbb * sqrt(fmax(0.0, 1.0 - q * q));
                                 atomicAdd(a.bins + 6 * nb + bi, sb * sb / dnij0);
                             }
                         }
@@ -243,22 +246,38 @@ __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
                     }
                     if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) continue;   // 1445
                     // -------- coarse check and types I/II/III, 1451-1516
+                    double bx1[3], xbn[3];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        bx1[c] = a.dmove[c * C + ib];
+                        xbn[c] = xb[c] + bx1[c];
+                    }
                     const double dmove_b = sqrt(bx1[0] * bx1[0] + bx1[1] * bx1[1] + bx1[2] * bx1[2]);
                     const double tmove = rcoll + dmove_a + dmove_b;
                     if (dnij0 > tmove) continue;
-                    double xe[3];
+                    double ax1[3], xan[3], xe[3];
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) xe[c] = xan[c] - xbn[c];
+                    for (int c = 0; c < 3; ++c) {
+                        ax1[c] = a.dmove[c * C + ia];
+                        xan[c] = xa[c] + ax1[c];
+                        xe[c] = xan[c] - xbn[c];
+                    }
                     const double dnij1 = sqrt(xe[0] * xe[0] + xe[1] * xe[1] + xe[2] * xe[2]);
                     if (dnij0 < rcoll && dnij1 > rcoll) continue;
                     int type = 0;
                     if (dnij0 > rcoll && dnij1 <= rcoll) {
                         type = 1;
                     } else {
-                        double vbn[3];
+                        double vbn[3], vb[3], va[3], vna[3];
 #pragma unroll
-                        for (int c = 0; c < 3; ++c) vbn[c] = a.vel[c * C + ib];
+                        for (int c = 0; c < 3; ++c) {
+                            vbn[c] = a.vel[c * C + ib];
+                            vb[c] = a.fvp0[c * C + ib];
+                            va[c] = a.fvp0[c * C + ia];
+                            vna[c] = a.vel[c * C + ia];
+                        }
                         const double dt = a.dt;
+                        #pragma unroll 1
                         for (int icheck = 1; icheck <= 19 && type == 0; ++icheck) {
                             const double tfac = 0.05 * (double)icheck;
                             const double tfac1 = dt * tfac, tfac2 = tfac * tfac, tfac3 = tfac2 * tfac;
@@ -346,17 +365,15 @@ __global__ void __launch_bounds__(CT) k_overlap(OverArgs a)
         cc[c] = q < 0 ? 0 : (q >= g.G ? g.G - 1 : q);
     }
     int cnt = 0;
-    for (int ox = -1; ox <= 1; ++ox)
-        for (int oy = -1; oy <= 1; ++oy)
-            for (int oz = -1; oz <= 1; ++oz) {
-                int qx = cc[0] + ox, qy = cc[1] + oy, qz = cc[2] + oz;
-                const int wx = qx < 0 ? -1 : (qx >= g.G ? 1 : 0);
-                const int wy = qy < 0 ? -1 : (qy >= g.G ? 1 : 0);
-                const int wz = qz < 0 ? -1 : (qz >= g.G ? 1 : 0);
-                qx -= wx * g.G; qy -= wy * g.G; qz -= wz * g.G;
-                const int cell = cell_code(qx, qy, qz);
-                const int e = a.cell_start[cell + 1];
-                for (int p = a.cell_start[cell]; p < e; ++p) {
+    __shared__ int2 rng[27 * CT];
+    neighbour_ranges(a.cell_start, g.G, cc, rng + threadIdx.x, CT);
+#pragma unroll 1
+    for (int kc = 0; kc < 27; ++kc) {
+                int wx, wy, wz;
+                neighbour_shift(kc, g.G, cc, &wx, &wy, &wz);
+                const int2 range = rng[kc * CT + threadIdx.x];
+                const int e = range.y;
+                for (int p = range.x; p < e; ++p) {
                     if (p == t) continue;
                     const int ib = a.sorted[p];
                     const double4 cb = a.pk[p];

@@ -42,6 +42,35 @@ __host__ __device__ __forceinline__ int cell_code(int x, int y, int z)
 {
     return (int)(cell_spread((unsigned)x) | (cell_spread((unsigned)y) << 1) | (cell_spread((unsigned)z) << 2));
 }
+// The [start, end) ranges of the 27 cells around coarse cell cc (periodic), written to
+// rng[k * stride] for k = (ox+1)*9 + (oy+1)*3 + (oz+1).  The loads go out in three
+// batches of 18, so a lane pays the cell-table latency three times instead of 27
+// times without holding all 54 results in registers.
+__device__ __forceinline__ void neighbour_ranges(const int* __restrict__ cell_start, int G, const int cc[3],
+                                                 int2* rng, int stride)
+{
+#pragma unroll 1
+    for (int k0 = 0; k0 < 27; k0 += 9)
+#pragma unroll
+    for (int k = k0; k < k0 + 9; ++k) {
+        int qx = cc[0] + k / 9 - 1, qy = cc[1] + (k / 3) % 3 - 1, qz = cc[2] + k % 3 - 1;
+        qx += qx < 0 ? G : (qx >= G ? -G : 0);
+        qy += qy < 0 ? G : (qy >= G ? -G : 0);
+        qz += qz < 0 ? G : (qz >= G ? -G : 0);
+        const int cell = cell_code(qx, qy, qz);
+        rng[k * stride] = make_int2(cell_start[cell], cell_start[cell + 1]);
+    }
+}
+
+// periodic shift (-1, 0, +1) of neighbour k along each axis for a lane in cell cc
+__device__ __forceinline__ void neighbour_shift(int k, int G, const int cc[3], int* wx, int* wy, int* wz)
+{
+    const int qx = cc[0] + k / 9 - 1, qy = cc[1] + (k / 3) % 3 - 1, qz = cc[2] + k % 3 - 1;
+    *wx = qx < 0 ? -1 : (qx >= G ? 1 : 0);
+    *wy = qy < 0 ? -1 : (qy >= G ? 1 : 0);
+    *wz = qz < 0 ? -1 : (qz >= G ? 1 : 0);
+}
+
 inline int cell_bits(int G)
 {
     int b = 0;
