@@ -132,7 +132,7 @@ struct CollArgs {
 
 __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     const int ia = a.sorted[t];
     const size_t C = (size_t)a.cap;
@@ -350,7 +350,7 @@ struct OverArgs {
 
 __global__ void __launch_bounds__(CT) k_overlap(OverArgs a)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     const int ia = a.sorted[t];
     const PairGeom& g = a.g;
@@ -365,15 +365,15 @@ __global__ void __launch_bounds__(CT) k_overlap(OverArgs a)
         cc[c] = q < 0 ? 0 : (q >= g.G ? g.G - 1 : q);
     }
     int cnt = 0;
-    __shared__ int2 rng[27 * CT];
-    neighbour_ranges(a.cell_start, g.G, cc, rng + threadIdx.x, CT);
-#pragma unroll 1
+    // a light kernel: direct loads of the cell ranges keep its occupancy high
     for (int kc = 0; kc < 27; ++kc) {
                 int wx, wy, wz;
                 neighbour_shift(kc, g.G, cc, &wx, &wy, &wz);
-                const int2 range = rng[kc * CT + threadIdx.x];
-                const int e = range.y;
-                for (int p = range.x; p < e; ++p) {
+                int qx = cc[0] + kc / 9 - 1 - wx * g.G, qy = cc[1] + (kc / 3) % 3 - 1 - wy * g.G;
+                int qz = cc[2] + kc % 3 - 1 - wz * g.G;
+                const int cell = cell_code(qx, qy, qz);
+                const int e = a.cell_start[cell + 1];
+                for (int p = a.cell_start[cell]; p < e; ++p) {
                     if (p == t) continue;
                     const int ib = a.sorted[p];
                     const double4 cb = a.pk[p];

@@ -62,6 +62,16 @@ __device__ __forceinline__ void neighbour_ranges(const int* __restrict__ cell_st
     }
 }
 
+// Work-group b of a launch runs on XCD b % 8, and each of the 8 XCDs has its own L2.
+// This gives the work-groups of one XCD a contiguous range of the nb logical blocks,
+// so droplets that are close in cell order, which read the same field nodes and
+// neighbour cells, share an L2.  A bijection of [0, nb).
+__device__ __forceinline__ int xcd_chunk(int b, int nb)
+{
+    const int x = b & 7, r = b >> 3, q = nb >> 3, m = nb & 7;
+    return x * q + (x < m ? x : m) + r;
+}
+
 // periodic shift (-1, 0, +1) of neighbour k along each axis for a lane in cell cc
 __device__ __forceinline__ void neighbour_shift(int k, int G, const int cc[3], int* wx, int* wy, int* wz)
 {
@@ -163,16 +173,16 @@ struct PartBuffers {
     int* cursor = nullptr;      // G^3
     void* scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
-    // HI (CSR)
+    // HI, in cell-sorted droplet order: ELL neighbour lists, slot-major
     int* nei_cnt = nullptr;     // cap + 1
-    int* nei_ptr = nullptr;     // cap + 1
-    int* nei_idx = nullptr;     // nnz_cap
-    double* nei_m = nullptr;    // 6 * nnz_cap
-    long long nnz_cap = 0;
-    void* gm_Q = nullptr;       // (maxiter+1) * 3 cap
+    int* nei_idx = nullptr;     // ell_width * cap: sorted position of neighbour s of row t at s * n + t
+    void* nei_m = nullptr;      // 6 * ell_width * cap (float or double): entry c at (6 s + c) * n + t
+    int ell_width = 0;
+    void* gm_Q = nullptr;       // (maxiter+2) * 3 cap: Krylov basis, then two work vectors
     void* gm_v = nullptr;       // 3 cap
     double* gm_x0 = nullptr;    // 3 cap
     double* gm_b = nullptr;     // 3 cap
+    double* gm_w = nullptr;     // 3 cap
     double* gm_red = nullptr;   // reduction partials
     double* gm_small = nullptr; // Hessenberg etc
     // collisions

@@ -7,12 +7,18 @@
 // back substitution, tolerance relaxed at iteration maxiter/2 and from maxiter-2,
 // warm start from the previous disturbance velocity) and EUC_NORM 2756-2791.
 //
-// Layout: the neighbour lists are CSR rows (column index + the 6 distinct entries
-// of the symmetric 3x3 block), built in two passes over the coarse cell grid.
+// The system is solved in cell-sorted droplet order: row t is the droplet at sorted
+// position t, so the neighbours a row gathers sit in the few cells around it and
+// their vector entries are close in memory.  The solution is scattered back to the
+// reference's droplet order at the end.  Neighbour lists are ELL, slot-major: slot s
+// of row t holds the neighbour's sorted position and the 6 distinct entries of its
+// symmetric 3x3 block, at s * n + t, so a wavefront reads each slot as one run.  One
+// pass over the coarse cells fills them; the slot width grows when a row needs more.
 // Krylov vectors use the reference's blocked order x(1:n), x(n+1:2n), x(2n+1:3n).
 // In FP32-mixed mode the matrix and the Krylov basis are float, every inner product
 // and norm accumulates in double, and the small least-squares problem is double.
-// Reductions are two-pass with a fixed grid, so results do not depend on timing.
+// Reductions use a fixed grid and a fixed summation tree, and the last work-group to
+// finish does the final sum, so results do not depend on timing.
 #include "ddns_internal.h"
 
 #include <algorithm>
@@ -21,116 +27,144 @@
 namespace ddns {
 
 constexpr int HT = 256;
-constexpr int RED_BLOCKS = 2048;   // fixed reduction grid: 8 work-groups per CU
+constexpr int RED_BLOCKS = 256;    // reduction grids: at most one work-group per CU
+constexpr int ELL_STEP = 8;        // the ELL width grows in multiples of this
 
 int exclusive_scan(const int* in, int* out, int n, int* tmp, hipStream_t s);
 
-// ------------------------------------------------------------------ neighbour lists
-template <typename T>
-__global__ void __launch_bounds__(HT) k_nei(int pass, const double4* __restrict__ pk,
-                                            const int* __restrict__ cell_start, const int* __restrict__ sorted,
-                                            int n, int cap, int G, double hc, double pi2, double trunc,
-                                            int* cnt, const int* ptr, int* idx, T* mm, int* flags)
+// deterministic wavefront sum: lane 0 holds the butterfly result
+__device__ __forceinline__ double wave_sum(double x)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int i = sorted[t];
-    const double4 me = pk[t];
-    const double xi = me.x, yi = me.y, zi = me.z, ri = me.w;
-    int cc[3];
-    const double xs[3] = {xi, yi, zi};
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        int q = (int)(xs[c] / hc);
-        cc[c] = q < 0 ? 0 : (q >= G ? G - 1 : q);
-    }
-    int k = 0;
-    int base = pass ? ptr[i] : 0;
-    for (int ox = -1; ox <= 1; ++ox)
-        for (int oy = -1; oy <= 1; ++oy)
-            for (int oz = -1; oz <= 1; ++oz) {
-                int qx = cc[0] + ox, qy = cc[1] + oy, qz = cc[2] + oz;
-                const int wx = qx < 0 ? -1 : (qx >= G ? 1 : 0);
-                const int wy = qy < 0 ? -1 : (qy >= G ? 1 : 0);
-                const int wz = qz < 0 ? -1 : (qz >= G ? 1 : 0);
-                qx -= wx * G; qy -= wy * G; qz -= wz * G;
-                const int cell = cell_code(qx, qy, qz);
-                const int e = cell_start[cell + 1];
-                for (int p = cell_start[cell]; p < e; ++p) {
-                    if (p == t) continue;
-                    const double4 o = pk[p];
-                    const double d0 = xi - (o.x + (double)wx * pi2);
-                    const double d1 = yi - (o.y + (double)wy * pi2);
-                    const double d2 = zi - (o.z + (double)wz * pi2);
-                    const double dnij = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-                    const double rj = o.w;
-                    if (dnij < ri + rj) atomicOr(flags, 1);                 // 3294-3299
-                    if (!(dnij < trunc * rj)) continue;                     // 3302-3305
-                    if (pass) {
-                        if (k < 100) {
-                            double fact1 = rj / dnij;                       // 3312-3324
-                            const double fact2 = fact1 * fact1 * fact1;
-                            fact1 = 0.75 * fact1;
-                            const double gamma = (fact1 - 0.75 * fact2) / (dnij * dnij);
-                            const double beta = fact1 + 0.25 * fact2;
-                            const size_t q = (size_t)base + k;
-                            idx[q] = sorted[p];
-                            mm[6 * q + 0] = (T)(gamma * (d0 * d0) + beta);
-                            mm[6 * q + 1] = (T)(gamma * d0 * d1);
-                            mm[6 * q + 2] = (T)(gamma * d0 * d2);
-                            mm[6 * q + 3] = (T)(gamma * (d1 * d1) + beta);
-                            mm[6 * q + 4] = (T)(gamma * d1 * d2);
-                            mm[6 * q + 5] = (T)(gamma * (d2 * d2) + beta);
-                        }
-                    }
-                    ++k;
-                }
-            }
-    if (!pass) {
-        if (k > DDNS_MAXNEI) { atomicOr(flags, 2); k = DDNS_MAXNEI; }
-        cnt[i] = k;
-    }
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
 }
 
-// y = x + M x (AFUN_PRECOMP); on first use also b = M (V - U).  x, y blocked.
+// ------------------------------------------------------------------ neighbour lists
+// Row t (sorted position): every droplet p != t in the 27 cells around t with
+// |x_t - x_p| < trunc * r_p, in cell order, as partlib.f90:3280-3330 visits them.
 template <typename T>
-__global__ void __launch_bounds__(HT) k_spmv(const int* __restrict__ ptr, const int* __restrict__ idx,
-                                             const T* __restrict__ mm, const T* __restrict__ x, T* __restrict__ y,
-                                             int n, const double* x0d, double* y0d,
-                                             const double* vel, const double* vp, double* b, int cap)
+__global__ void __launch_bounds__(HT) k_nei(const double4* __restrict__ pk, const int* __restrict__ cell_start,
+                                            int n, int G, double hc, double pi2, double trunc, int width,
+                                            int* __restrict__ cnt, int* __restrict__ idx, T* __restrict__ mm,
+                                            int* flags)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int s = ptr[i], e = ptr[i + 1];
+    const int t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    int k = 0;
+    if (t < n) {
+        const double4 me = pk[t];
+        const double xi = me.x, yi = me.y, zi = me.z, ri = me.w;
+        int cc[3];
+        const double xs[3] = {xi, yi, zi};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            int q = (int)(xs[c] / hc);
+            cc[c] = q < 0 ? 0 : (q >= G ? G - 1 : q);
+        }
+        const size_t ld = (size_t)n;
+#pragma unroll 1
+        for (int kc = 0; kc < 27; ++kc) {
+            int wx, wy, wz;
+            neighbour_shift(kc, G, cc, &wx, &wy, &wz);
+            const int cell = cell_code(cc[0] + kc / 9 - 1 - wx * G, cc[1] + (kc / 3) % 3 - 1 - wy * G,
+                                       cc[2] + kc % 3 - 1 - wz * G);
+            const int e = cell_start[cell + 1];
+            for (int p = cell_start[cell]; p < e; ++p) {
+                if (p == t) continue;
+                const double4 o = pk[p];
+                const double d0 = xi - (o.x + (double)wx * pi2);
+                const double d1 = yi - (o.y + (double)wy * pi2);
+                const double d2 = zi - (o.z + (double)wz * pi2);
+                const double dnij = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+                const double rj = o.w;
+                if (dnij < ri + rj) atomicOr(flags, 1);                         // 3294-3299
+                if (!(dnij < trunc * rj)) continue;                             // 3302-3305
+                if (k < width) {
+                    double fact1 = rj / dnij;                                   // 3312-3324
+                    const double fact2 = fact1 * fact1 * fact1;
+                    fact1 = 0.75 * fact1;
+                    const double gamma = (fact1 - 0.75 * fact2) / (dnij * dnij);
+                    const double beta = fact1 + 0.25 * fact2;
+                    idx[(size_t)k * ld + t] = p;
+                    T* m = mm + (size_t)(6 * k) * ld + t;
+                    m[0] = (T)(gamma * (d0 * d0) + beta);
+                    m[ld] = (T)(gamma * d0 * d1);
+                    m[2 * ld] = (T)(gamma * d0 * d2);
+                    m[3 * ld] = (T)(gamma * (d1 * d1) + beta);
+                    m[4 * ld] = (T)(gamma * d1 * d2);
+                    m[5 * ld] = (T)(gamma * (d2 * d2) + beta);
+                }
+                ++k;
+            }
+        }
+        if (k > DDNS_MAXNEI) atomicOr(flags, 2);
+        cnt[t] = k;
+    }
+    // the widest row of the wavefront, for the width check
+    int km = k;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) km = max(km, __shfl_xor(km, o, 64));
+    if ((threadIdx.x & 63) == 0 && km > width) atomicMax(flags + 4, km);
+}
+
+// x0 = pertvel and w = V - U, gathered into sorted order
+__global__ void k_gather(const int* __restrict__ sorted, const double* __restrict__ pert, const double* __restrict__ vel,
+                         const double* __restrict__ vp, double* __restrict__ x0, double* __restrict__ w, int n, int cap)
+{
+    const long long i = (long long)blockIdx.x * HT + threadIdx.x;
+    if (i >= 3LL * n) return;
+    const int c = (int)(i / n), t = (int)(i - (long long)c * n);
+    const size_t j = (size_t)c * cap + sorted[t];
+    x0[i] = pert[j];
+    w[i] = vel[j] - vp[j];
+}
+
+// y = x + M x (AFUN_PRECOMP).  MODE 0: x = x0 in double, y0d = A x0 and b = M w.
+// MODE 1: x is a basis vector.  MODE 2: x = v / h for the previous work vector v; the
+// row's own entries of x are also stored as the next basis vector (q = v / h,
+// partlib.f90:2897).
+template <typename T, int MODE>
+__global__ void __launch_bounds__(HT) k_spmv(const int* __restrict__ cnt, const int* __restrict__ idx,
+                                             const T* __restrict__ mm, int n, const T* __restrict__ x,
+                                             const double* __restrict__ hx, T* __restrict__ qout, T* __restrict__ y,
+                                             const double* __restrict__ x0d, double* __restrict__ y0d,
+                                             const double* __restrict__ w, double* __restrict__ b)
+{
+    const int t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const size_t N = (size_t)n;
+    const double h = MODE == 2 ? *hx : 1.0;
+    auto xv = [&](size_t q) -> double {
+        if (MODE == 0) return x0d[q];
+        if (MODE == 1) return (double)x[q];
+        return (double)(T)((double)x[q] / h);
+    };
+    const int kk = cnt[t];
     double r0 = 0.0, r1 = 0.0, r2 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
-    const size_t N = (size_t)n, C = (size_t)cap;
-    for (int q = s; q < e; ++q) {
+    for (int s = 0; s < kk; ++s) {
+        const size_t q = (size_t)s * N + t;
         const int j = idx[q];
-        const double m0 = mm[6 * (size_t)q + 0], m1 = mm[6 * (size_t)q + 1], m2 = mm[6 * (size_t)q + 2];
-        const double m3 = mm[6 * (size_t)q + 3], m4 = mm[6 * (size_t)q + 4], m5 = mm[6 * (size_t)q + 5];
-        double v0, v1, v2;
-        if (x0d) { v0 = x0d[j]; v1 = x0d[N + j]; v2 = x0d[2 * N + j]; }
-        else { v0 = (double)x[j]; v1 = (double)x[N + j]; v2 = (double)x[2 * N + j]; }
+        const T* m = mm + (size_t)(6 * s) * N + t;
+        const double m0 = m[0], m1 = m[N], m2 = m[2 * N], m3 = m[3 * N], m4 = m[4 * N], m5 = m[5 * N];
+        const double v0 = xv(j), v1 = xv(N + j), v2 = xv(2 * N + j);
         r0 = r0 + m0 * v0 + m1 * v1 + m2 * v2;
         r1 = r1 + m1 * v0 + m3 * v1 + m4 * v2;
         r2 = r2 + m2 * v0 + m4 * v1 + m5 * v2;
-        if (b) {
-            const double q0 = vel[j] - vp[j], q1 = vel[C + j] - vp[C + j], q2 = vel[2 * C + j] - vp[2 * C + j];
+        if (MODE == 0) {
+            const double q0 = w[j], q1 = w[N + j], q2 = w[2 * N + j];
             b0 = b0 + m0 * q0 + m1 * q1 + m2 * q2;
             b1 = b1 + m1 * q0 + m3 * q1 + m4 * q2;
             b2 = b2 + m2 * q0 + m4 * q1 + m5 * q2;
         }
     }
-    if (x0d) {
-        y0d[i] = r0 + x0d[i];
-        y0d[N + i] = r1 + x0d[N + i];
-        y0d[2 * N + i] = r2 + x0d[2 * N + i];
+    const double s0 = xv(t), s1 = xv(N + t), s2 = xv(2 * N + t);
+    if (MODE == 0) {
+        y0d[t] = r0 + s0; y0d[N + t] = r1 + s1; y0d[2 * N + t] = r2 + s2;
+        b[t] = b0; b[N + t] = b1; b[2 * N + t] = b2;
     } else {
-        y[i] = (T)(r0 + (double)x[i]);
-        y[N + i] = (T)(r1 + (double)x[N + i]);
-        y[2 * N + i] = (T)(r2 + (double)x[2 * N + i]);
+        y[t] = (T)(r0 + s0); y[N + t] = (T)(r1 + s1); y[2 * N + t] = (T)(r2 + s2);
+        if (MODE == 2) { qout[t] = (T)s0; qout[N + t] = (T)s1; qout[2 * N + t] = (T)s2; }
     }
-    if (b) { b[i] = b0; b[N + i] = b1; b[2 * N + i] = b2; }
 }
 
 // ------------------------------------------------------------------ reductions
@@ -191,132 +225,150 @@ __global__ void k_scale(T* q, const double* den, long long n)
     if (i < n) q[i] = (T)((double)q[i] / *den);
 }
 
-// v = v - h * q, h read from device memory (MGS, partlib.f90:2892)
-template <typename T>
-__global__ void k_axpy_dev(T* v, const T* q, const double* h, long long n)
+// ---- Arnoldi step: classical Gram-Schmidt applied twice.  Each projection kernel
+// runs on a fixed grid of P <= RED_BLOCKS work-groups and writes one partial sum per
+// work-group and column (a fixed tree inside the work-group); the kernel that needs
+// the sums adds the P partials itself, in a fixed order, at its start.  Every result
+// is independent of scheduling.
+constexpr int RT = 1024;          // threads per work-group of the projection kernels
+constexpr int RW = RT / 64;       // wavefronts per work-group
+constexpr int MD_J = 8;           // Krylov columns per pass of k_mdot
+
+// Work-group sums of acc[0..J): red[jj] valid for all threads on return.
+template <int J>
+__device__ __forceinline__ void block_sums(double (&acc)[J], double (*sh)[J], double* red)
 {
-    const long long i = (long long)blockIdx.x * HT + threadIdx.x;
-    if (i < n) v[i] = (T)((double)v[i] - *h * (double)q[i]);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int jj = 0; jj < J; ++jj) {
+        const double v = wave_sum(acc[jj]);
+        if (lane == 0) sh[wv][jj] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < J) {
+        double v = sh[0][threadIdx.x];
+#pragma unroll
+        for (int w = 1; w < RW; ++w) v += sh[w][threadIdx.x];
+        red[threadIdx.x] = v;
+    }
+    __syncthreads();
 }
 
-// q_out = v / h
-template <typename T>
-__global__ void k_div_dev(T* qo, const T* v, const double* h, long long n)
+// hs[j] = sum over b < P of part[j * P + b] for j < k (lane-strided in order, then a
+// fixed wavefront tree), the same in every work-group.  Work-group 0 also sets
+// (add = 0) or increments (add = 1) the Hessenberg column hcol.
+__device__ void sum_partials(const double* __restrict__ part, int P, int k, double* hs, double* hcol, int add)
 {
-    const long long i = (long long)blockIdx.x * HT + threadIdx.x;
-    if (i < n) qo[i] = (T)((double)v[i] / *h);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int j = wv; j < k; j += RW) {
+        double acc = 0.0;
+        for (int b = lane; b < P; b += 64) acc += part[(size_t)j * P + b];
+        acc = wave_sum(acc);
+        if (lane == 0) {
+            hs[j] = acc;
+            if (blockIdx.x == 0) hcol[j] = add ? hcol[j] + acc : acc;
+        }
+    }
+    __syncthreads();
 }
 
-// ---- block-Gram-Schmidt kernels (Arnoldi step with classical Gram-Schmidt applied
-// twice).  Every reduction uses a fixed grid of RED_BLOCKS work-groups and a fixed
-// summation tree, so the result does not depend on scheduling.
-constexpr int MD_J = 8;   // Krylov columns per pass of k_mdot
-
-// part[j * gridDim.x + block] = sum over this block's share of Q_j . v, j < k
+// partials of Q_j . v for j < k
 template <typename T>
-__global__ void __launch_bounds__(HT) k_mdot(const T* __restrict__ Q, const T* __restrict__ v, int k, long long n3,
+__global__ void __launch_bounds__(RT) k_mdot(const T* __restrict__ Q, const T* __restrict__ v, int k, long long n3,
                                              double* __restrict__ part)
 {
-    __shared__ double sh[MD_J][HT];
+    __shared__ double sh[RW][MD_J];
+    __shared__ double red[MD_J];
     const int tid = threadIdx.x;
     for (int j0 = 0; j0 < k; j0 += MD_J) {
         double acc[MD_J];
 #pragma unroll
         for (int jj = 0; jj < MD_J; ++jj) acc[jj] = 0.0;
 #pragma unroll 2
-        for (long long i = (long long)blockIdx.x * HT + tid; i < n3; i += (long long)gridDim.x * HT) {
+        for (long long i = (long long)blockIdx.x * RT + tid; i < n3; i += (long long)gridDim.x * RT) {
             const double vi = (double)v[i];
 #pragma unroll
             for (int jj = 0; jj < MD_J; ++jj)
                 if (j0 + jj < k) acc[jj] += (double)Q[(size_t)(j0 + jj) * n3 + i] * vi;
         }
-#pragma unroll
-        for (int jj = 0; jj < MD_J; ++jj) sh[jj][tid] = acc[jj];
-        __syncthreads();
-        for (int o = HT / 2; o > 0; o >>= 1) {
-            if (tid < o) {
-#pragma unroll
-                for (int jj = 0; jj < MD_J; ++jj) sh[jj][tid] += sh[jj][tid + o];
-            }
-            __syncthreads();
-        }
-        if (tid < MD_J && j0 + tid < k) part[(size_t)(j0 + tid) * gridDim.x + blockIdx.x] = sh[tid][0];
-        __syncthreads();
+        block_sums<MD_J>(acc, sh, red);
+        if (tid < MD_J && j0 + tid < k) part[(size_t)(j0 + tid) * gridDim.x + blockIdx.x] = red[tid];
     }
 }
 
-// one work-group per column j: h[j] = sum of its partials; the Hessenberg column
-// entry is set (first pass) or incremented (second pass)
-__global__ void __launch_bounds__(HT) k_mfinal(const double* __restrict__ part, int np, double* h, double* hcol, int add)
+// First pass's update fused with the second pass's projections, for k <= KC columns:
+// h = sum of part_in (first pass), v -= sum_j h_j Q_j, partials of Q_j . v into part_out
+template <typename T, int KC>
+__global__ void __launch_bounds__(RT) k_maxpy_mdot(T* __restrict__ v, const T* __restrict__ Q, int k, long long n3,
+                                                   const double* __restrict__ part_in, double* __restrict__ hcol,
+                                                   double* __restrict__ part_out)
 {
-    __shared__ double sh[HT];
-    const int j = blockIdx.x, tid = threadIdx.x;
-    double acc = 0.0;
-    for (int i = tid; i < np; i += HT) acc += part[(size_t)j * np + i];
-    sh[tid] = acc;
-    __syncthreads();
-    for (int o = HT / 2; o > 0; o >>= 1) {
-        if (tid < o) sh[tid] += sh[tid + o];
-        __syncthreads();
-    }
-    if (tid == 0) {
-        h[j] = sh[0];
-        hcol[j] = add ? hcol[j] + sh[0] : sh[0];
-    }
-}
-
-// v -= sum_j h[j] Q_j; with normpart, also the block partials of |v|^2
-template <typename T>
-__global__ void __launch_bounds__(HT) k_maxpy(T* __restrict__ v, const T* __restrict__ Q, const double* __restrict__ h,
-                                              int k, long long n3, double* normpart)
-{
-    __shared__ double hs[128];
-    __shared__ double sh[HT];
+    __shared__ double hs[KC];
+    __shared__ double sh[RW][KC];
+    __shared__ double red[KC];
     const int tid = threadIdx.x;
-    for (int j = tid; j < k; j += HT) hs[j] = h[j];
-    __syncthreads();
-    double nacc = 0.0;
+    sum_partials(part_in, gridDim.x, k, hs, hcol, 0);
+    double acc[KC];
+#pragma unroll
+    for (int jj = 0; jj < KC; ++jj) acc[jj] = 0.0;
+    for (long long i = (long long)blockIdx.x * RT + tid; i < n3; i += (long long)gridDim.x * RT) {
+        T q[KC];
+        double x = (double)v[i];
+#pragma unroll
+        for (int jj = 0; jj < KC; ++jj)
+            if (jj < k) { q[jj] = Q[(size_t)jj * n3 + i]; x -= hs[jj] * (double)q[jj]; }
+        const T xt = (T)x;
+        v[i] = xt;
+        const double xd = (double)xt;
+#pragma unroll
+        for (int jj = 0; jj < KC; ++jj)
+            if (jj < k) acc[jj] += (double)q[jj] * xd;
+    }
+    block_sums<KC>(acc, sh, red);
+    if (tid < k) part_out[(size_t)tid * gridDim.x + blockIdx.x] = red[tid];
+}
+
+// h = sum of part_in, v -= sum_j h[j] Q_j; with normpart, also the partials of |v|^2
+template <typename T>
+__global__ void __launch_bounds__(RT) k_maxpy(T* __restrict__ v, const T* __restrict__ Q, int k, long long n3,
+                                              const double* __restrict__ part_in, double* __restrict__ hcol, int add,
+                                              double* __restrict__ normpart)
+{
+    __shared__ double hs[64];
+    __shared__ double sh[RW][1];
+    __shared__ double red[1];
+    const int tid = threadIdx.x;
+    sum_partials(part_in, gridDim.x, k, hs, hcol, add);
+    double nacc[1] = {0.0};
 #pragma unroll 2
-    for (long long i = (long long)blockIdx.x * HT + tid; i < n3; i += (long long)gridDim.x * HT) {
+    for (long long i = (long long)blockIdx.x * RT + tid; i < n3; i += (long long)gridDim.x * RT) {
         double x = (double)v[i];
 #pragma unroll 4
         for (int j = 0; j < k; ++j) x -= hs[j] * (double)Q[(size_t)j * n3 + i];
         const T xt = (T)x;
         v[i] = xt;
-        nacc += (double)xt * (double)xt;
+        nacc[0] += (double)xt * (double)xt;
     }
     if (!normpart) return;
-    sh[tid] = nacc;
-    __syncthreads();
-    for (int o = HT / 2; o > 0; o >>= 1) {
-        if (tid < o) sh[tid] += sh[tid + o];
-        __syncthreads();
-    }
-    if (tid == 0) normpart[blockIdx.x] = sh[0];
+    block_sums<1>(nacc, sh, red);
+    if (tid == 0) normpart[blockIdx.x] = red[0];
 }
 
-// pertvel = x0 + Q y (2951-2956)
+// pertvel = x0 + Q y (2951-2956), scattered back to the reference's droplet order
 template <typename T>
-__global__ void k_solution(const double* x0, const T* Q, const double* y, int iters, double* pert, int n, int cap)
+__global__ void k_solution(const double* x0, const T* Q, const double* y, int iters, const int* sorted,
+                           double* pert, int n, int cap)
 {
     const long long i = (long long)blockIdx.x * HT + threadIdx.x;
     const long long n3 = 3LL * n;
     if (i >= n3) return;
     double acc = 0.0;
     for (int j = 0; j < iters; ++j) acc = acc + (double)Q[(size_t)j * n3 + i] * y[j];
-    const int c = (int)(i / n), r = (int)(i % n);
-    pert[(size_t)c * cap + r] = x0[i] + acc;
+    const int c = (int)(i / n), t = (int)(i - (long long)c * n);
+    pert[(size_t)c * cap + sorted[t]] = x0[i] + acc;
 }
 
-__global__ void k_pert_to_x0(const double* pert, double* x0, int n, int cap)
-{
-    const long long i = (long long)blockIdx.x * HT + threadIdx.x;
-    if (i >= 3LL * n) return;
-    const int c = (int)(i / n), r = (int)(i % n);
-    x0[i] = pert[(size_t)c * cap + r];
-}
-
-// Small least-squares update of HDI_GMRES (2904-2971), one lane.
+// Small least-squares update of HDI_GMRES (2904-2971), one wavefront.
 // sm layout: [0] norm_b [1] norm_rhs [2] res [3] maxerr [4] done [5] iter
 //            H at 16 + (r-1) + M1*(c-1), Rn after it, Qn after it, bn, y, v1, v2, resv
 struct GmSmall {
@@ -333,94 +385,166 @@ struct GmSmall {
     __host__ __device__ size_t total() const { return RV() + M1(); }
 };
 
-__device__ void ls_solve(double* sm, GmSmall L, int iter, int* done_host);
+constexpr int LT = 64;         // k_ls: one wavefront; maxiter + 1 <= LT
 constexpr int LS_MAX = 8192;   // GmSmall::total() for maxiter <= 50, the reference's value
 
-__global__ void __launch_bounds__(HT) k_ls(double* gsm, GmSmall L, int iter, int* done_host)
+__device__ void ls_solve(double* sm, GmSmall L, int iter, int* done_host);
+
+__device__ __forceinline__ void copy_range(double* dst, const double* src, size_t a, size_t b)
 {
-    // the whole small problem (at most LS_MAX doubles, 64 KB) is copied to LDS, solved
-    // by one lane at LDS latency, and copied back
-    __shared__ double sm[LS_MAX];
-    const size_t tot = L.total();
-    for (size_t i = threadIdx.x; i < tot; i += blockDim.x) sm[i] = gsm[i];
-    __syncthreads();
-    if (threadIdx.x == 0) ls_solve(sm, L, iter, done_host);
-    __syncthreads();
-    for (size_t i = threadIdx.x; i < tot; i += blockDim.x) gsm[i] = sm[i];
+    for (size_t i = a + threadIdx.x; i < b; i += LT) dst[i] = src[i];
 }
 
+__global__ void __launch_bounds__(LT) k_ls(double* gsm, GmSmall L, int iter, int* done_host,
+                                          const double* __restrict__ normpart, int P)
+{
+    // the part of the small problem this iteration touches is copied to LDS, solved
+    // there, and the changed part copied back
+    __shared__ double sm[LS_MAX];
+    const size_t M1 = L.M1();
+    // H(iter+1, iter) = |v| from the partials of the last projection kernel
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < P; b += LT) acc += normpart[b];
+    acc = wave_sum(acc);
+    copy_range(sm, gsm, 0, 16);
+    copy_range(sm, gsm, L.H0(), L.H0() + M1 * iter);
+    copy_range(sm, gsm, L.R0(), L.R0() + M1 * iter);
+    copy_range(sm, gsm, L.Q0(), L.Q0() + M1 * (iter + 1));
+    copy_range(sm, gsm, L.B0(), L.total());
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const size_t e = L.H0() + iter + M1 * (iter - 1);
+        sm[e] = gsm[e] = sqrt(acc);
+    }
+    __syncthreads();
+    ls_solve(sm, L, iter, done_host);
+    __syncthreads();
+    copy_range(gsm, sm, 0, 16);
+    copy_range(gsm, sm, L.R0(), L.R0() + M1 * iter);
+    copy_range(gsm, sm, L.Q0(), L.Q0() + M1 * (iter + 1));
+    copy_range(gsm, sm, L.B0(), L.total());
+}
+
+// All LT lanes call it.  Lane r owns row r + 1 of the triangular factor and of the
+// residual; the Givens rotation and the back substitution run over the lanes.
 __device__ void ls_solve(double* sm, GmSmall L, int iter, int* done_host)
 {
+    const int lane = threadIdx.x;
     const int M1 = L.M1();
     double* H = sm + L.H0();
     double* Rn = sm + L.R0();
     double* Qn = sm + L.Q0();
     double* bn = sm + L.B0();
     double* y = sm + L.Y0();
-    double* v1 = sm + L.V1();
-    double* v2 = sm + L.V2();
     double* rv = sm + L.RV();
 #define HH(r, c) H[((r) - 1) + (size_t)M1 * ((c) - 1)]
 #define RN(r, c) Rn[((r) - 1) + (size_t)M1 * ((c) - 1)]
 #define QN(r, c) Qn[((r) - 1) + (size_t)M1 * ((c) - 1)]
     const double norm_rhs = sm[1];
     if (iter == 1) {
-        const double nrm = sqrt(HH(1, 1) * HH(1, 1) + HH(2, 1) * HH(2, 1));
-        const double q11 = HH(1, 1) / nrm, q21 = HH(2, 1) / nrm;
-        QN(1, 1) = q11; QN(2, 1) = q21;
-        QN(1, 2) = q21; QN(2, 2) = -q11;
-        RN(1, 1) = sqrt(HH(1, 1) * HH(1, 1) + HH(2, 1) * HH(2, 1));
-        bn[0] = q11 * norm_rhs;
-        y[0] = bn[0] / RN(1, 1);
-    } else {
-        for (int r = 1; r <= iter; ++r) {
-            double acc = 0.0;
-            for (int c = 1; c <= iter; ++c) acc = acc + QN(r, c) * HH(c, iter);
-            RN(r, iter) = acc;
+        if (lane == 0) {
+            const double nrm = sqrt(HH(1, 1) * HH(1, 1) + HH(2, 1) * HH(2, 1));
+            const double q11 = HH(1, 1) / nrm, q21 = HH(2, 1) / nrm;
+            QN(1, 1) = q11; QN(2, 1) = q21;
+            QN(1, 2) = q21; QN(2, 2) = -q11;
+            RN(1, 1) = sqrt(HH(1, 1) * HH(1, 1) + HH(2, 1) * HH(2, 1));
+            bn[0] = q11 * norm_rhs;
+            y[0] = bn[0] / RN(1, 1);
         }
+        __syncthreads();
+    } else {
+        // new column of R: Qn(1:iter, 1:iter) H(1:iter, iter)
+        if (lane < iter) {
+            double acc = 0.0;
+            for (int c = 1; c <= iter; ++c) acc = acc + QN(lane + 1, c) * HH(c, iter);
+            RN(lane + 1, iter) = acc;
+        }
+        __syncthreads();
         const double aa = RN(iter, iter), bb = HH(iter + 1, iter);
         const double dd = sqrt(aa * aa + bb * bb);
         const double cc = aa / dd, ss = bb / dd;
-        RN(iter, iter) = dd;
-        for (int c = 1; c <= iter; ++c) { v1[c - 1] = cc * QN(iter, c); v2[c - 1] = -ss * QN(iter, c); }
-        v1[iter] = ss; v2[iter] = cc;
-        for (int c = 1; c <= iter + 1; ++c) { QN(iter, c) = v1[c - 1]; QN(iter + 1, c) = v2[c - 1]; }
-        bn[iter - 2] = QN(iter - 1, 1) * norm_rhs;
-        bn[iter - 1] = QN(iter, 1) * norm_rhs;
-        y[iter - 1] = bn[iter - 1] / RN(iter, iter);
-        for (int r = iter - 1; r >= 1; --r) {
-            double d = 0.0;
-            for (int c = r + 1; c <= iter; ++c) d = d + y[c - 1] * RN(r, c);
-            y[r - 1] = (bn[r - 1] - d) / RN(r, r);
+        const double qi = lane < iter ? QN(iter, lane + 1) : 0.0;
+        __syncthreads();
+        // rows iter and iter+1 of Qn after the rotation
+        if (lane < iter) { QN(iter, lane + 1) = cc * qi; QN(iter + 1, lane + 1) = -ss * qi; }
+        if (lane == iter) { QN(iter, iter + 1) = ss; QN(iter + 1, iter + 1) = cc; }
+        if (lane == 0) RN(iter, iter) = dd;
+        __syncthreads();
+        if (lane == 0) {
+            bn[iter - 2] = QN(iter - 1, 1) * norm_rhs;
+            bn[iter - 1] = QN(iter, 1) * norm_rhs;
         }
-    }
-    for (int r = 1; r <= iter + 1; ++r) {
+        __syncthreads();
+        // back substitution by columns: lane r accumulates row r + 1
         double acc = 0.0;
-        for (int c = 1; c <= iter; ++c) acc = acc + HH(r, c) * y[c - 1];
-        rv[r - 1] = acc;
+        for (int c = iter; c >= 1; --c) {
+            if (lane == c - 1) y[c - 1] = (bn[c - 1] - acc) / RN(c, c);
+            __syncthreads();
+            const double yc = y[c - 1];
+            if (lane < c - 1) acc = acc + yc * RN(lane + 1, c);
+        }
+        __syncthreads();
     }
-    rv[0] = rv[0] - norm_rhs;
-    double x_c = 0.0;
-    for (int r = 0; r <= iter; ++r) x_c = x_c + rv[r] * rv[r];
-    const double res = sqrt(x_c) / sm[0];
-    sm[2] = res;
-    sm[5] = iter;
-    double maxerr = sm[3];
-    int done = 0;
-    if (res < maxerr) done = 1;
-    else {
-        if (iter == L.maxiter / 2) maxerr = maxerr * 10;
-        if (iter >= L.maxiter - 2) maxerr = maxerr * 10;
+    // residual H y - |r0| e1 (2958-2966)
+    if (lane <= iter) {
+        double acc = 0.0;
+        for (int c = 1; c <= iter; ++c) acc = acc + HH(lane + 1, c) * y[c - 1];
+        rv[lane] = acc;
     }
-    sm[3] = maxerr;
-    sm[4] = done;
-    *done_host = done;
+    __syncthreads();
+    if (lane == 0) {
+        rv[0] = rv[0] - norm_rhs;
+        double x_c = 0.0;
+        for (int r = 0; r <= iter; ++r) x_c = x_c + rv[r] * rv[r];
+        const double res = sqrt(x_c) / sm[0];
+        sm[2] = res;
+        sm[5] = iter;
+        double maxerr = sm[3];
+        int done = 0;
+        if (res < maxerr) done = 1;
+        else {
+            if (iter == L.maxiter / 2) maxerr = maxerr * 10;
+            if (iter >= L.maxiter - 2) maxerr = maxerr * 10;
+        }
+        sm[3] = maxerr;
+        sm[4] = done;
+        *done_host = done;
+    }
 #undef HH
 #undef RN
 #undef QN
 }
 
 // ------------------------------------------------------------------ driver
+template <typename T>
+static int build_lists(PartBuffers& pb, const PartConst& pc, hipStream_t s)
+{
+    const int n = pc.n;
+    if (!pb.nei_idx) {
+        pb.ell_width = 2 * ELL_STEP;
+        DDNS_HIP(hipMalloc(&pb.nei_idx, (size_t)pb.ell_width * pb.cap * sizeof(int)));
+        DDNS_HIP(hipMalloc(&pb.nei_m, (size_t)6 * pb.ell_width * pb.cap * sizeof(T)));
+    }
+    for (;;) {
+        DDNS_HIP(hipMemsetAsync(pb.flags, 0, 8 * sizeof(int), s));
+        hipLaunchKernelGGL((k_nei<T>), dim3((n + HT - 1) / HT), dim3(HT), 0, s, (const double4*)pb.pk, pb.cell_start,
+                           n, pb.G, pb.hc, pc.pi2, pc.hdi_trunc, pb.ell_width, pb.nei_cnt, pb.nei_idx, (T*)pb.nei_m,
+                           pb.flags);
+        DDNS_HIP(hipGetLastError());
+        int f[8];
+        DDNS_HIP(hipMemcpyAsync(f, pb.flags, sizeof(f), hipMemcpyDeviceToHost, s));
+        DDNS_HIP(hipStreamSynchronize(s));
+        if (f[0] & 1) { set_error("inside NEI_LIST_PRECOMP: overlapping is not allowed"); return DDNS_ERR_OVERLAP; }
+        if (f[0] & 2) { set_error("more than 100 HI neighbours for a droplet"); return DDNS_ERR_NEIGHBOUR_OVERFLOW; }
+        if (f[4] <= pb.ell_width) return 0;
+        // some row has more neighbours than the lists hold: widen them and fill again
+        if (pb.nei_idx) { (void)hipFree(pb.nei_idx); (void)hipFree(pb.nei_m); pb.nei_idx = nullptr; pb.nei_m = nullptr; }
+        pb.ell_width = std::min(DDNS_MAXNEI, (f[4] + ELL_STEP - 1) / ELL_STEP * ELL_STEP);
+        DDNS_HIP(hipMalloc(&pb.nei_idx, (size_t)pb.ell_width * pb.cap * sizeof(int)));
+        DDNS_HIP(hipMalloc(&pb.nei_m, (size_t)6 * pb.ell_width * pb.cap * sizeof(T)));
+    }
+}
+
 template <typename T>
 static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double tol, int* iters_out, double* res_out,
                     hipStream_t s)
@@ -430,51 +554,37 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
     *iters_out = 0;
     *res_out = 0.0;
     if (n == 0) return 0;
-    DDNS_HIP(hipMemsetAsync(pb.flags, 0, 4 * sizeof(int), s));
-    // neighbour lists, two passes
-    hipLaunchKernelGGL((k_nei<T>), dim3((n + HT - 1) / HT), dim3(HT), 0, s, 0, (const double4*)pb.pk, pb.cell_start, pb.sorted,
-                       n, pb.cap, pb.G, pb.hc, pc.pi2, pc.hdi_trunc, pb.nei_cnt, (const int*)nullptr, (int*)nullptr,
-                       (T*)nullptr, pb.flags);
-    int st = exclusive_scan(pb.nei_cnt, pb.nei_ptr, n, (int*)pb.scan_tmp, s);
-    if (st) return st;
-    int nnz = 0, flags = 0;
-    DDNS_HIP(hipMemcpyAsync(&nnz, pb.nei_ptr + n, sizeof(int), hipMemcpyDeviceToHost, s));
-    DDNS_HIP(hipMemcpyAsync(&flags, pb.flags, sizeof(int), hipMemcpyDeviceToHost, s));
-    DDNS_HIP(hipStreamSynchronize(s));
-    if (flags & 1) { set_error("inside NEI_LIST_PRECOMP: overlapping is not allowed"); return DDNS_ERR_OVERLAP; }
-    if (flags & 2) { set_error("more than 100 HI neighbours for a droplet"); return DDNS_ERR_NEIGHBOUR_OVERFLOW; }
-    if (nnz > pb.nnz_cap) {
-        if (pb.nei_idx) { (void)hipFree(pb.nei_idx); (void)hipFree(pb.nei_m); }
-        pb.nnz_cap = (long long)nnz + nnz / 4 + 1024;
-        DDNS_HIP(hipMalloc(&pb.nei_idx, (size_t)pb.nnz_cap * sizeof(int)));
-        DDNS_HIP(hipMalloc(&pb.nei_m, (size_t)pb.nnz_cap * 6 * sizeof(double)));
-    }
-    hipLaunchKernelGGL((k_nei<T>), dim3((n + HT - 1) / HT), dim3(HT), 0, s, 1, (const double4*)pb.pk, pb.cell_start, pb.sorted,
-                       n, pb.cap, pb.G, pb.hc, pc.pi2, pc.hdi_trunc, pb.nei_cnt, (const int*)pb.nei_ptr, pb.nei_idx,
-                       (T*)pb.nei_m, pb.flags);
-    DDNS_HIP(hipGetLastError());
-    // Krylov storage
+    if (maxiter + 1 > LT) { set_error("GMRES maxiter too large (at most 50)"); return DDNS_ERR_ARG; }
     GmSmall L{maxiter};
+    if (L.total() > (size_t)LS_MAX) { set_error("GMRES maxiter too large (at most 50)"); return DDNS_ERR_ARG; }
+    int st = build_lists<T>(pb, pc, s);
+    if (st) return st;
+    // Krylov storage
     if (!pb.gm_small) {
         DDNS_HIP(hipMalloc(&pb.gm_small, 16384 * sizeof(double)));
         DDNS_HIP(hipMalloc(&pb.gm_red, (size_t)(128 + 2) * RED_BLOCKS * sizeof(double)));
         DDNS_HIP(hipMalloc(&pb.gm_x0, 3 * (size_t)pb.cap * sizeof(double)));
         DDNS_HIP(hipMalloc(&pb.gm_b, 3 * (size_t)pb.cap * sizeof(double)));
         DDNS_HIP(hipMalloc(&pb.gm_v, 3 * (size_t)pb.cap * sizeof(double)));
-        DDNS_HIP(hipMalloc(&pb.gm_Q, (size_t)(maxiter + 1) * 3 * pb.cap * sizeof(T)));
+        DDNS_HIP(hipMalloc(&pb.gm_w, 3 * (size_t)pb.cap * sizeof(double)));
+        DDNS_HIP(hipMalloc(&pb.gm_Q, (size_t)(maxiter + 2) * 3 * pb.cap * sizeof(T)));
     }
-    if (L.total() > (size_t)LS_MAX) { set_error("GMRES maxiter too large (at most 50)"); return DDNS_ERR_ARG; }
-    double* normpart = pb.gm_red + (size_t)128 * RED_BLOCKS;
-    double* hbuf = normpart + RED_BLOCKS;
+    double* part1 = pb.gm_red;                              // first-pass projections
+    double* part2 = part1 + (size_t)64 * RED_BLOCKS;        // second-pass projections
+    double* normpart = part2 + (size_t)64 * RED_BLOCKS;
     double* sm = pb.gm_small;
-    double* ax0 = (double*)pb.gm_v;       // reuse as double scratch for A x0
+    double* ax0 = (double*)pb.gm_v;
     T* Q = (T*)pb.gm_Q;
-    T* v = Q + (size_t)maxiter * n3;      // the (maxiter+1)-th column doubles as the work vector
+    T* work[2] = {Q + (size_t)maxiter * n3, Q + (size_t)(maxiter + 1) * n3};
     DDNS_HIP(hipMemsetAsync(sm, 0, L.total() * sizeof(double), s));
     const unsigned nb3 = (unsigned)((n3 + HT - 1) / HT);
-    hipLaunchKernelGGL(k_pert_to_x0, dim3(nb3), dim3(HT), 0, s, pb.pert, pb.gm_x0, n, pb.cap);
-    hipLaunchKernelGGL((k_spmv<T>), dim3((n + HT - 1) / HT), dim3(HT), 0, s, pb.nei_ptr, pb.nei_idx, (const T*)pb.nei_m,
-                       (const T*)nullptr, (T*)nullptr, n, pb.gm_x0, ax0, pb.vel, pb.vp, pb.gm_b, pb.cap);
+    const unsigned nbr = (unsigned)((n + HT - 1) / HT);
+    const unsigned nred = (unsigned)std::min<long long>(RED_BLOCKS, (n3 + RT - 1) / RT);   // reduction grid
+    hipLaunchKernelGGL(k_gather, dim3(nb3), dim3(HT), 0, s, (const int*)pb.sorted, (const double*)pb.pert,
+                       (const double*)pb.vel, (const double*)pb.vp, pb.gm_x0, pb.gm_w, n, pb.cap);
+    hipLaunchKernelGGL((k_spmv<T, 0>), dim3(nbr), dim3(HT), 0, s, (const int*)pb.nei_cnt, (const int*)pb.nei_idx,
+                       (const T*)pb.nei_m, n, (const T*)nullptr, (const double*)nullptr, (T*)nullptr, (T*)nullptr,
+                       (const double*)pb.gm_x0, ax0, (const double*)pb.gm_w, pb.gm_b);
     hipLaunchKernelGGL((k_residual0<T>), dim3(nb3), dim3(HT), 0, s, pb.gm_b, ax0, Q, n3);
     st = dot(pb.gm_b, pb.gm_b, n3, pb.gm_red, sm + 0, 1, s);
     if (st) return st;
@@ -483,7 +593,10 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
     double h2[2];
     DDNS_HIP(hipMemcpyAsync(h2, sm, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     DDNS_HIP(hipStreamSynchronize(s));
-    if (h2[1] < 1e-5) return 0;                                            // 2866-2868
+    if (h2[1] < 1e-5) {                                                     // 2866-2868
+        // pertvel keeps its warm-start value (x0)
+        return 0;
+    }
     {
         double me = tol;
         DDNS_HIP(hipMemcpyAsync(sm + 3, &me, sizeof(double), hipMemcpyHostToDevice, s));
@@ -493,31 +606,45 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
     int iter = 0, done = 0;
     double* Hd = sm + L.H0();
     const int M1 = L.M1();
+    const double* hprev = nullptr;   // H(iter, iter-1): norm of the previous work vector
     for (iter = 1; iter <= maxiter; ++iter) {
-        T* qi = Q + (size_t)(iter - 1) * n3;
-        hipLaunchKernelGGL((k_spmv<T>), dim3((n + HT - 1) / HT), dim3(HT), 0, s, pb.nei_ptr, pb.nei_idx,
-                           (const T*)pb.nei_m, (const T*)qi, v, n, (const double*)nullptr, (double*)nullptr,
-                           (const double*)nullptr, (const double*)nullptr, (double*)nullptr, pb.cap);
+        T* v = work[iter & 1];
+        if (iter == 1)
+            hipLaunchKernelGGL((k_spmv<T, 1>), dim3(nbr), dim3(HT), 0, s, (const int*)pb.nei_cnt, (const int*)pb.nei_idx,
+                               (const T*)pb.nei_m, n, (const T*)Q, (const double*)nullptr, (T*)nullptr, v,
+                               (const double*)nullptr, (double*)nullptr, (const double*)nullptr, (double*)nullptr);
+        else
+            hipLaunchKernelGGL((k_spmv<T, 2>), dim3(nbr), dim3(HT), 0, s, (const int*)pb.nei_cnt, (const int*)pb.nei_idx,
+                               (const T*)pb.nei_m, n, (const T*)work[(iter - 1) & 1], hprev,
+                               Q + (size_t)(iter - 1) * n3, v, (const double*)nullptr, (double*)nullptr,
+                               (const double*)nullptr, (double*)nullptr);
         // orthogonalise v against q_1..q_iter: classical Gram-Schmidt, twice (the
         // reference's modified Gram-Schmidt, 2886-2894, in two block passes)
         double* hcol = Hd + (size_t)M1 * (iter - 1);
-        double* hn = hcol + iter;
-        for (int pass = 0; pass < 2; ++pass) {
-            hipLaunchKernelGGL((k_mdot<T>), dim3(RED_BLOCKS), dim3(HT), 0, s, (const T*)Q, (const T*)v, iter, n3, pb.gm_red);
-            hipLaunchKernelGGL(k_mfinal, dim3(iter), dim3(HT), 0, s, (const double*)pb.gm_red, RED_BLOCKS, hbuf, hcol, pass);
-            hipLaunchKernelGGL((k_maxpy<T>), dim3(RED_BLOCKS), dim3(HT), 0, s, v, (const T*)Q, (const double*)hbuf, iter, n3,
-                               pass ? normpart : (double*)nullptr);
+        hipLaunchKernelGGL((k_mdot<T>), dim3(nred), dim3(RT), 0, s, (const T*)Q, (const T*)v, iter, n3, part1);
+        if (iter <= 8)
+            hipLaunchKernelGGL((k_maxpy_mdot<T, 8>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, iter, n3,
+                               (const double*)part1, hcol, part2);
+        else if (iter <= 12)
+            hipLaunchKernelGGL((k_maxpy_mdot<T, 12>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, iter, n3,
+                               (const double*)part1, hcol, part2);
+        else {
+            hipLaunchKernelGGL((k_maxpy<T>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, iter, n3, (const double*)part1,
+                               hcol, 0, (double*)nullptr);
+            hipLaunchKernelGGL((k_mdot<T>), dim3(nred), dim3(RT), 0, s, (const T*)Q, (const T*)v, iter, n3, part2);
         }
-        hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(HT), 0, s, (const double*)normpart, RED_BLOCKS, hn, 1);
-        if (iter < maxiter) hipLaunchKernelGGL((k_div_dev<T>), dim3(nb3), dim3(HT), 0, s, Q + (size_t)iter * n3, v, hn, n3);
-        hipLaunchKernelGGL(k_ls, dim3(1), dim3(HT), 0, s, sm, L, iter, done_dev);
+        hipLaunchKernelGGL((k_maxpy<T>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, iter, n3, (const double*)part2, hcol,
+                           1, normpart);
+        hipLaunchKernelGGL(k_ls, dim3(1), dim3(LT), 0, s, sm, L, iter, done_dev, (const double*)normpart, (int)nred);
         DDNS_HIP(hipGetLastError());
         DDNS_HIP(hipMemcpyAsync(&done, done_dev, sizeof(int), hipMemcpyDeviceToHost, s));
         DDNS_HIP(hipStreamSynchronize(s));
+        hprev = hcol + iter;
         if (done) break;
     }
     const int used = std::min(iter, maxiter);
-    hipLaunchKernelGGL((k_solution<T>), dim3(nb3), dim3(HT), 0, s, pb.gm_x0, (const T*)Q, sm + L.Y0(), used, pb.pert, n, pb.cap);
+    hipLaunchKernelGGL((k_solution<T>), dim3(nb3), dim3(HT), 0, s, pb.gm_x0, (const T*)Q, sm + L.Y0(), used,
+                       (const int*)pb.sorted, pb.pert, n, pb.cap);
     DDNS_HIP(hipGetLastError());
     double tail[6];
     DDNS_HIP(hipMemcpyAsync(tail, sm, 6 * sizeof(double), hipMemcpyDeviceToHost, s));

@@ -122,7 +122,6 @@ int part_alloc(PartBuffers& pb, int cap, int nset, int G)
     DDNS_HIP(hipMalloc(&pb.mult, (c + 1) * sizeof(int)));
     DDNS_HIP(hipMalloc(&pb.draw_off, (c + 1) * sizeof(int)));
     DDNS_HIP(hipMalloc(&pb.nei_cnt, (c + 1) * sizeof(int)));
-    DDNS_HIP(hipMalloc(&pb.nei_ptr, (c + 1) * sizeof(int)));
     DDNS_HIP(hipMalloc(&pb.flags, 64 * sizeof(int)));
     DDNS_HIP(hipMemset(pb.flags, 0, 64 * sizeof(int)));
     DDNS_HIP(hipMalloc(&pb.tested, 4 * sizeof(long long)));
@@ -147,7 +146,7 @@ void part_free(PartBuffers& pb)
 {
     void* ps[] = {pb.pos, pb.vel, pb.rad, pb.gid, pb.fvp, pb.fdp, pb.dmove, pb.vp, pb.pert,
                   pb.cell_count, pb.cell_start, pb.cell_of, pb.sorted, pb.pk, pb.cursor, pb.scan_tmp,
-                  pb.nei_cnt, pb.nei_ptr, pb.nei_idx, pb.nei_m, pb.gm_Q, pb.gm_v, pb.gm_x0, pb.gm_b,
+                  pb.nei_cnt, pb.nei_idx, pb.nei_m, pb.gm_Q, pb.gm_v, pb.gm_x0, pb.gm_b, pb.gm_w,
                   pb.gm_red, pb.gm_small, pb.mult, pb.flags, pb.tested, pb.nirbin, pb.bins, pb.counts,
                   pb.pairs, pb.draws, pb.draw_off, pb.rng_state, pb.red};
     for (void* p : ps) if (p) (void)hipFree(p);
@@ -188,7 +187,7 @@ __global__ void __launch_bounds__(PT) k_interp(const T* __restrict__ u, int N, d
                                                const double* __restrict__ pos, const int* __restrict__ order,
                                                double* __restrict__ vp, int n, int cap)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int i = order ? order[t] : t;
     const double dx = pi2 / (double)N;
