@@ -179,68 +179,84 @@ __device__ __forceinline__ void load_node(const double* p, double& a, double& b,
     a = v.x; b = v.y; c = p[2];
 }
 
-// One lane per droplet, droplets taken in cell order (sorted[], cells x-fastest like
-// the field) so that neighbouring lanes read overlapping stencil rows.  The 6 x 6 x 6
-// stencil is fully unrolled: weights and node indices stay in registers.
+// Six lanes per droplet, one per x offset of the 6 x 6 x 6 stencil: for each of the
+// 36 (y, z) stencil rows the six lanes read the row's six nodes as one contiguous run,
+// so a wavefront load touches about 15 cache lines instead of one line per lane.  Ten
+// droplets share a wavefront (lanes 60..63 idle), taken in cell order (sorted[]) so
+// that they read overlapping rows.  Each lane sums its 36 nodes with the weights
+// bg(x) bg(y) bg(z) of partlib.f90:509; the six partial sums are added in x order.
+constexpr int IG = 6;              // lanes per droplet
+constexpr int IPW = 64 / IG;       // droplets per wavefront
+
 template <typename T>
 __global__ void __launch_bounds__(PT) k_interp(const T* __restrict__ u, int N, double pi2,
                                                const double* __restrict__ pos, const int* __restrict__ order,
                                                double* __restrict__ vp, int n, int cap)
 {
-    const int t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int i = order ? order[t] : t;
-    const double dx = pi2 / (double)N;
-    double wx[6], wy[6], wz[6];
-    int nx, ny, nz;
-    {
+    const int lane = threadIdx.x & 63;
+    const int wave = (xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
+    const int g = lane / IG, a = lane - g * IG;
+    const int t = wave * IPW + g;
+    const bool active = g < IPW && t < n;
+    const int i = active ? (order ? order[t] : t) : 0;
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+    if (active) {
+        const double dx = pi2 / (double)N;
+        double wx[6], wy[6], wz[6];
         const double y0 = pos[i], y1 = pos[(size_t)cap + i], y2 = pos[2 * (size_t)cap + i];
-        nx = (int)floor(y0 / dx);
-        ny = (int)floor(y1 / dx);
-        nz = (int)floor(y2 / dx);
+        const int nx = (int)floor(y0 / dx), ny = (int)floor(y1 / dx), nz = (int)floor(y2 / dx);
         lagrange6((y0 - (double)nx * dx) / dx, wx);
         lagrange6((y1 - (double)ny * dx) / dx, wy);
         lagrange6((y2 - (double)nz * dx) / dx, wz);
-    }
-    int ix[6];
-    size_t rowy[6], rowz[6];
+        double wxa = wx[0];
 #pragma unroll
-    for (int m = 0; m < 6; ++m) {
-        ix[m] = ((nx - 2 + m) % N + N) % N;
-        rowy[m] = (size_t)(((ny - 2 + m) % N + N) % N) * N;
-        rowz[m] = (size_t)(((nz - 2 + m) % N + N) % N) * N * N;
-    }
-    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+        for (int m = 1; m < 6; ++m) wxa = a == m ? wx[m] : wxa;
+        const size_t ix = (size_t)(((nx - 2 + a) % N + N) % N) * 4;
+        size_t rowy[6], rowz[6];
 #pragma unroll
-    for (int d = 0; d < 6; ++d) {
+        for (int m = 0; m < 6; ++m) {
+            rowy[m] = (size_t)(((ny - 2 + m) % N + N) % N) * N;
+            rowz[m] = (size_t)(((nz - 2 + m) % N + N) % N) * N * N;
+        }
 #pragma unroll
-        for (int b = 0; b < 6; ++b) {
-            const T* row = u + (rowz[d] + rowy[b]) * 4;
+        for (int d = 0; d < 6; ++d) {
 #pragma unroll
-            for (int a = 0; a < 6; ++a) {
+            for (int b2 = 0; b2 < 6; ++b2) {
                 double u0, u1, u2;
-                load_node(row + (size_t)ix[a] * 4, u0, u1, u2);
-                const double w = wx[a] * wy[b] * wz[d];      // bg(x) bg(y) bg(z), partlib.f90:509
+                load_node(u + (rowz[d] + rowy[b2]) * 4 + ix, u0, u1, u2);
+                const double w = wxa * wy[b2] * wz[d];
                 acc0 += u0 * w;
                 acc1 += u1 * w;
                 acc2 += u2 * w;
             }
         }
     }
-    vp[i] = acc0;
-    vp[(size_t)cap + i] = acc1;
-    vp[2 * (size_t)cap + i] = acc2;
+    // lane a == 0 of each group adds the group's six partial sums in x order
+    const int g0 = g * IG;
+    double s0 = acc0, s1 = acc1, s2 = acc2;
+#pragma unroll
+    for (int m = 1; m < IG; ++m) {
+        const int src = g0 + m < 64 ? g0 + m : lane;
+        const double p0 = __shfl(acc0, src, 64), p1 = __shfl(acc1, src, 64), p2 = __shfl(acc2, src, 64);
+        s0 += p0; s1 += p1; s2 += p2;
+    }
+    if (active && a == 0) {
+        vp[i] = s0;
+        vp[(size_t)cap + i] = s1;
+        vp[2 * (size_t)cap + i] = s2;
+    }
 }
 
 int part_interp(const PartBuffers& pb, const PartConst& pc, const void* uphys, hipStream_t s)
 {
     if (pc.n == 0) return 0;
     const int* order = pb.G > 0 ? pb.sorted : nullptr;
+    const unsigned nb = (unsigned)(((long long)pc.n + IPW * (PT / 64) - 1) / (IPW * (PT / 64)));
     if (pc.precision == DDNS_FP64)
-        hipLaunchKernelGGL(k_interp<double>, dim3(nblk(pc.n)), dim3(PT), 0, s, (const double*)uphys, pc.N, pc.pi2,
+        hipLaunchKernelGGL(k_interp<double>, dim3(nb), dim3(PT), 0, s, (const double*)uphys, pc.N, pc.pi2,
                            pb.pos, order, pb.vp, pc.n, pb.cap);
     else
-        hipLaunchKernelGGL(k_interp<float>, dim3(nblk(pc.n)), dim3(PT), 0, s, (const float*)uphys, pc.N, pc.pi2,
+        hipLaunchKernelGGL(k_interp<float>, dim3(nb), dim3(PT), 0, s, (const float*)uphys, pc.N, pc.pi2,
                            pb.pos, order, pb.vp, pc.n, pb.cap);
     DDNS_HIP(hipGetLastError());
     return 0;
@@ -501,37 +517,36 @@ __global__ void k_cell_key(const double* pos, int* cell_of, int* cnt, int n, int
     atomicAdd(cnt + cell, 1);
 }
 
-__global__ void k_cell_fill(const int* cell_of, const int* start, int* cursor, int* sorted, int n)
+// Droplet i (read in slot order, so the reads coalesce) takes the next free place of
+// its cell and writes its index and its packed (x, y, z, radius) record there: the
+// pair searches then read each candidate cell as one contiguous run.
+__global__ void k_cell_fill(const int* __restrict__ cell_of, const int* __restrict__ start, int* cursor,
+                            int* __restrict__ sorted, const double* __restrict__ pos, const double* __restrict__ rad,
+                            double4* __restrict__ pk, int n, int cap)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int c = cell_of[i];
     const int p = start[c] + atomicAdd(cursor + c, 1);
     sorted[p] = i;
+    pk[p] = make_double4(pos[i], pos[(size_t)cap + i], pos[2 * (size_t)cap + i], rad[i]);
 }
 
-__global__ void k_cell_order(const int* start, int* sorted, int ncell)
+// the order inside a cell is by droplet index, so it does not depend on the order in
+// which the fill's atomics were served; the packed records move with their indices
+__global__ void k_cell_order(const int* __restrict__ start, int* __restrict__ sorted, double4* __restrict__ pk, int ncell)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
     const int b = start[c], e = start[c + 1];
     for (int i = b + 1; i < e; ++i) {
         const int v = sorted[i];
+        const double4 r = pk[i];
         int j = i - 1;
-        while (j >= b && sorted[j] > v) { sorted[j + 1] = sorted[j]; --j; }
+        while (j >= b && sorted[j] > v) { sorted[j + 1] = sorted[j]; pk[j + 1] = pk[j]; --j; }
         sorted[j + 1] = v;
+        pk[j + 1] = r;
     }
-}
-
-// positions and radii in cell order, one 32-byte record per droplet, so the pair
-// searches read each candidate cell as one contiguous run
-__global__ void k_pack_sorted(const int* __restrict__ sorted, const double* __restrict__ pos,
-                              const double* __restrict__ rad, double4* __restrict__ pk, int n, int cap)
-{
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const int i = sorted[p];
-    pk[p] = make_double4(pos[i], pos[(size_t)cap + i], pos[2 * (size_t)cap + i], rad[i]);
 }
 
 int part_build_cells(PartBuffers& pb, const PartConst& pc, double hmin, hipStream_t s)
@@ -564,9 +579,9 @@ int part_build_cells(PartBuffers& pb, const PartConst& pc, double hmin, hipStrea
     int st = exclusive_scan(pb.cell_count, pb.cell_start, (int)ncell, (int*)pb.scan_tmp, s);
     if (st) return st;
     if (pc.n > 0) {
-        hipLaunchKernelGGL(k_cell_fill, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.cell_of, pb.cell_start, pb.cursor, pb.sorted, pc.n);
-        hipLaunchKernelGGL(k_cell_order, dim3(nblk(ncell)), dim3(PT), 0, s, pb.cell_start, pb.sorted, (int)ncell);
-        hipLaunchKernelGGL(k_pack_sorted, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.sorted, pb.pos, pb.rad, pb.pk, pc.n, pb.cap);
+        hipLaunchKernelGGL(k_cell_fill, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.cell_of, pb.cell_start, pb.cursor, pb.sorted,
+                           pb.pos, pb.rad, pb.pk, pc.n, pb.cap);
+        hipLaunchKernelGGL(k_cell_order, dim3(nblk(ncell)), dim3(PT), 0, s, pb.cell_start, pb.sorted, pb.pk, (int)ncell);
     }
     DDNS_HIP(hipGetLastError());
     return 0;
