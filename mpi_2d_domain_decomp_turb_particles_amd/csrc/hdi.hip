@@ -272,24 +272,59 @@ __device__ void sum_partials(const double* __restrict__ part, int P, int k, doub
     __syncthreads();
 }
 
+// Krylov columns start every ld elements (ld a multiple of 64), so each lane moves
+// 16 bytes per access: 4 floats or 2 doubles.  Element e of a vector sits at
+// Q + j * ld + e for e < n3; the last, partial vector is done element by element.
+template <typename T> struct kvec;
+template <> struct kvec<float> { using V = float4; static constexpr int W = 4; };
+template <> struct kvec<double> { using V = double2; static constexpr int W = 2; };
+
+template <typename T>
+__device__ __forceinline__ void vld(const T* p, T (&x)[kvec<T>::W])
+{
+    using V = typename kvec<T>::V;
+    *(V*)x = *(const V*)p;
+}
+template <typename T>
+__device__ __forceinline__ void vst(T* p, const T (&x)[kvec<T>::W])
+{
+    using V = typename kvec<T>::V;
+    *(V*)p = *(const V*)x;
+}
+
 // partials of Q_j . v for j < k
 template <typename T>
 __global__ void __launch_bounds__(RT) k_mdot(const T* __restrict__ Q, const T* __restrict__ v, int k, long long n3,
-                                             double* __restrict__ part)
+                                             long long ld, double* __restrict__ part)
 {
+    constexpr int W = kvec<T>::W;
     __shared__ double sh[RW][MD_J];
     __shared__ double red[MD_J];
     const int tid = threadIdx.x;
+    const long long nv = (n3 + W - 1) / W;
     for (int j0 = 0; j0 < k; j0 += MD_J) {
         double acc[MD_J];
 #pragma unroll
         for (int jj = 0; jj < MD_J; ++jj) acc[jj] = 0.0;
-#pragma unroll 2
-        for (long long i = (long long)blockIdx.x * RT + tid; i < n3; i += (long long)gridDim.x * RT) {
-            const double vi = (double)v[i];
+        for (long long iv = (long long)blockIdx.x * RT + tid; iv < nv; iv += (long long)gridDim.x * RT) {
+            const long long i = iv * W;
+            if (i + W <= n3) {
+                T x[W];
+                vld(v + i, x);
 #pragma unroll
-            for (int jj = 0; jj < MD_J; ++jj)
-                if (j0 + jj < k) acc[jj] += (double)Q[(size_t)(j0 + jj) * n3 + i] * vi;
+                for (int jj = 0; jj < MD_J; ++jj)
+                    if (j0 + jj < k) {
+                        T q[W];
+                        vld(Q + (size_t)(j0 + jj) * ld + i, q);
+#pragma unroll
+                        for (int e = 0; e < W; ++e) acc[jj] += (double)q[e] * (double)x[e];
+                    }
+            } else {
+                for (long long e = i; e < n3; ++e)
+#pragma unroll
+                    for (int jj = 0; jj < MD_J; ++jj)
+                        if (j0 + jj < k) acc[jj] += (double)Q[(size_t)(j0 + jj) * ld + e] * (double)v[e];
+            }
         }
         block_sums<MD_J>(acc, sh, red);
         if (tid < MD_J && j0 + tid < k) part[(size_t)(j0 + tid) * gridDim.x + blockIdx.x] = red[tid];
@@ -300,9 +335,10 @@ __global__ void __launch_bounds__(RT) k_mdot(const T* __restrict__ Q, const T* _
 // h = sum of part_in (first pass), v -= sum_j h_j Q_j, partials of Q_j . v into part_out
 template <typename T, int KC>
 __global__ void __launch_bounds__(RT) k_maxpy_mdot(T* __restrict__ v, const T* __restrict__ Q, int k, long long n3,
-                                                   const double* __restrict__ part_in, double* __restrict__ hcol,
-                                                   double* __restrict__ part_out)
+                                                   long long ld, const double* __restrict__ part_in,
+                                                   double* __restrict__ hcol, double* __restrict__ part_out)
 {
+    constexpr int W = kvec<T>::W;
     __shared__ double hs[KC];
     __shared__ double sh[RW][KC];
     __shared__ double red[KC];
@@ -311,18 +347,43 @@ __global__ void __launch_bounds__(RT) k_maxpy_mdot(T* __restrict__ v, const T* _
     double acc[KC];
 #pragma unroll
     for (int jj = 0; jj < KC; ++jj) acc[jj] = 0.0;
-    for (long long i = (long long)blockIdx.x * RT + tid; i < n3; i += (long long)gridDim.x * RT) {
-        T q[KC];
-        double x = (double)v[i];
+    const long long nv = (n3 + W - 1) / W;
+    for (long long iv = (long long)blockIdx.x * RT + tid; iv < nv; iv += (long long)gridDim.x * RT) {
+        const long long i = iv * W;
+        if (i + W <= n3) {
+            T q[KC][W];
+            T xv[W];
+            vld(v + i, xv);
+            double x[W];
 #pragma unroll
-        for (int jj = 0; jj < KC; ++jj)
-            if (jj < k) { q[jj] = Q[(size_t)jj * n3 + i]; x -= hs[jj] * (double)q[jj]; }
-        const T xt = (T)x;
-        v[i] = xt;
-        const double xd = (double)xt;
+            for (int e = 0; e < W; ++e) x[e] = (double)xv[e];
 #pragma unroll
-        for (int jj = 0; jj < KC; ++jj)
-            if (jj < k) acc[jj] += (double)q[jj] * xd;
+            for (int jj = 0; jj < KC; ++jj)
+                if (jj < k) {
+                    vld(Q + (size_t)jj * ld + i, q[jj]);
+#pragma unroll
+                    for (int e = 0; e < W; ++e) x[e] -= hs[jj] * (double)q[jj][e];
+                }
+#pragma unroll
+            for (int e = 0; e < W; ++e) xv[e] = (T)x[e];
+            vst(v + i, xv);
+#pragma unroll
+            for (int jj = 0; jj < KC; ++jj)
+                if (jj < k) {
+#pragma unroll
+                    for (int e = 0; e < W; ++e) acc[jj] += (double)q[jj][e] * (double)xv[e];
+                }
+        } else {
+            for (long long e = i; e < n3; ++e) {
+                double x = (double)v[e];
+                for (int jj = 0; jj < k; ++jj) x -= hs[jj] * (double)Q[(size_t)jj * ld + e];
+                const T xt = (T)x;
+                v[e] = xt;
+#pragma unroll
+                for (int jj = 0; jj < KC; ++jj)
+                    if (jj < k) acc[jj] += (double)Q[(size_t)jj * ld + e] * (double)xt;
+            }
+        }
     }
     block_sums<KC>(acc, sh, red);
     if (tid < k) part_out[(size_t)tid * gridDim.x + blockIdx.x] = red[tid];
@@ -331,23 +392,48 @@ __global__ void __launch_bounds__(RT) k_maxpy_mdot(T* __restrict__ v, const T* _
 // h = sum of part_in, v -= sum_j h[j] Q_j; with normpart, also the partials of |v|^2
 template <typename T>
 __global__ void __launch_bounds__(RT) k_maxpy(T* __restrict__ v, const T* __restrict__ Q, int k, long long n3,
-                                              const double* __restrict__ part_in, double* __restrict__ hcol, int add,
-                                              double* __restrict__ normpart)
+                                              long long ld, const double* __restrict__ part_in,
+                                              double* __restrict__ hcol, int add, double* __restrict__ normpart)
 {
+    constexpr int W = kvec<T>::W;
     __shared__ double hs[64];
     __shared__ double sh[RW][1];
     __shared__ double red[1];
     const int tid = threadIdx.x;
     sum_partials(part_in, gridDim.x, k, hs, hcol, add);
     double nacc[1] = {0.0};
-#pragma unroll 2
-    for (long long i = (long long)blockIdx.x * RT + tid; i < n3; i += (long long)gridDim.x * RT) {
-        double x = (double)v[i];
+    const long long nv = (n3 + W - 1) / W;
+    for (long long iv = (long long)blockIdx.x * RT + tid; iv < nv; iv += (long long)gridDim.x * RT) {
+        const long long i = iv * W;
+        if (i + W <= n3) {
+            T xv[W];
+            vld(v + i, xv);
+            double x[W];
+#pragma unroll
+            for (int e = 0; e < W; ++e) x[e] = (double)xv[e];
 #pragma unroll 4
-        for (int j = 0; j < k; ++j) x -= hs[j] * (double)Q[(size_t)j * n3 + i];
-        const T xt = (T)x;
-        v[i] = xt;
-        nacc[0] += (double)xt * (double)xt;
+            for (int j = 0; j < k; ++j) {
+                T q[W];
+                vld(Q + (size_t)j * ld + i, q);
+                const double hj = hs[j];
+#pragma unroll
+                for (int e = 0; e < W; ++e) x[e] -= hj * (double)q[e];
+            }
+#pragma unroll
+            for (int e = 0; e < W; ++e) {
+                xv[e] = (T)x[e];
+                nacc[0] += (double)xv[e] * (double)xv[e];
+            }
+            vst(v + i, xv);
+        } else {
+            for (long long e = i; e < n3; ++e) {
+                double x = (double)v[e];
+                for (int j = 0; j < k; ++j) x -= hs[j] * (double)Q[(size_t)j * ld + e];
+                const T xt = (T)x;
+                v[e] = xt;
+                nacc[0] += (double)xt * (double)xt;
+            }
+        }
     }
     if (!normpart) return;
     block_sums<1>(nacc, sh, red);
@@ -356,14 +442,14 @@ __global__ void __launch_bounds__(RT) k_maxpy(T* __restrict__ v, const T* __rest
 
 // pertvel = x0 + Q y (2951-2956), scattered back to the reference's droplet order
 template <typename T>
-__global__ void k_solution(const double* x0, const T* Q, const double* y, int iters, const int* sorted,
+__global__ void k_solution(const double* x0, const T* Q, long long ld, const double* y, int iters, const int* sorted,
                            double* pert, int n, int cap)
 {
     const long long i = (long long)blockIdx.x * HT + threadIdx.x;
     const long long n3 = 3LL * n;
     if (i >= n3) return;
     double acc = 0.0;
-    for (int j = 0; j < iters; ++j) acc = acc + (double)Q[(size_t)j * n3 + i] * y[j];
+    for (int j = 0; j < iters; ++j) acc = acc + (double)Q[(size_t)j * ld + i] * y[j];
     const int c = (int)(i / n), t = (int)(i - (long long)c * n);
     pert[(size_t)c * cap + sorted[t]] = x0[i] + acc;
 }
@@ -567,7 +653,7 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
         DDNS_HIP(hipMalloc(&pb.gm_b, 3 * (size_t)pb.cap * sizeof(double)));
         DDNS_HIP(hipMalloc(&pb.gm_v, 3 * (size_t)pb.cap * sizeof(double)));
         DDNS_HIP(hipMalloc(&pb.gm_w, 3 * (size_t)pb.cap * sizeof(double)));
-        DDNS_HIP(hipMalloc(&pb.gm_Q, (size_t)(maxiter + 2) * 3 * pb.cap * sizeof(T)));
+        DDNS_HIP(hipMalloc(&pb.gm_Q, (size_t)(maxiter + 2) * ((3LL * pb.cap + 63) / 64 * 64) * sizeof(T)));
     }
     double* part1 = pb.gm_red;                              // first-pass projections
     double* part2 = part1 + (size_t)64 * RED_BLOCKS;        // second-pass projections
@@ -575,11 +661,12 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
     double* sm = pb.gm_small;
     double* ax0 = (double*)pb.gm_v;
     T* Q = (T*)pb.gm_Q;
-    T* work[2] = {Q + (size_t)maxiter * n3, Q + (size_t)(maxiter + 1) * n3};
+    const long long ld = (n3 + 63) / 64 * 64;     // column stride of the Krylov vectors
+    T* work[2] = {Q + (size_t)maxiter * ld, Q + (size_t)(maxiter + 1) * ld};
     DDNS_HIP(hipMemsetAsync(sm, 0, L.total() * sizeof(double), s));
     const unsigned nb3 = (unsigned)((n3 + HT - 1) / HT);
     const unsigned nbr = (unsigned)((n + HT - 1) / HT);
-    const unsigned nred = (unsigned)std::min<long long>(RED_BLOCKS, (n3 + RT - 1) / RT);   // reduction grid
+    const unsigned nred = (unsigned)std::min<long long>(RED_BLOCKS, (n3 + 4LL * RT - 1) / (4LL * RT));   // reduction grid
     hipLaunchKernelGGL(k_gather, dim3(nb3), dim3(HT), 0, s, (const int*)pb.sorted, (const double*)pb.pert,
                        (const double*)pb.vel, (const double*)pb.vp, pb.gm_x0, pb.gm_w, n, pb.cap);
     hipLaunchKernelGGL((k_spmv<T, 0>), dim3(nbr), dim3(HT), 0, s, (const int*)pb.nei_cnt, (const int*)pb.nei_idx,
@@ -616,25 +703,25 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
         else
             hipLaunchKernelGGL((k_spmv<T, 2>), dim3(nbr), dim3(HT), 0, s, (const int*)pb.nei_cnt, (const int*)pb.nei_idx,
                                (const T*)pb.nei_m, n, (const T*)work[(iter - 1) & 1], hprev,
-                               Q + (size_t)(iter - 1) * n3, v, (const double*)nullptr, (double*)nullptr,
+                               Q + (size_t)(iter - 1) * ld, v, (const double*)nullptr, (double*)nullptr,
                                (const double*)nullptr, (double*)nullptr);
         // orthogonalise v against q_1..q_iter: classical Gram-Schmidt, twice (the
         // reference's modified Gram-Schmidt, 2886-2894, in two block passes)
         double* hcol = Hd + (size_t)M1 * (iter - 1);
-        hipLaunchKernelGGL((k_mdot<T>), dim3(nred), dim3(RT), 0, s, (const T*)Q, (const T*)v, iter, n3, part1);
+        hipLaunchKernelGGL((k_mdot<T>), dim3(nred), dim3(RT), 0, s, (const T*)Q, (const T*)v, iter, n3, ld, part1);
         if (iter <= 8)
-            hipLaunchKernelGGL((k_maxpy_mdot<T, 8>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, iter, n3,
+            hipLaunchKernelGGL((k_maxpy_mdot<T, 8>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, iter, n3, ld,
                                (const double*)part1, hcol, part2);
         else if (iter <= 12)
-            hipLaunchKernelGGL((k_maxpy_mdot<T, 12>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, iter, n3,
+            hipLaunchKernelGGL((k_maxpy_mdot<T, 12>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, iter, n3, ld,
                                (const double*)part1, hcol, part2);
         else {
-            hipLaunchKernelGGL((k_maxpy<T>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, iter, n3, (const double*)part1,
-                               hcol, 0, (double*)nullptr);
-            hipLaunchKernelGGL((k_mdot<T>), dim3(nred), dim3(RT), 0, s, (const T*)Q, (const T*)v, iter, n3, part2);
+            hipLaunchKernelGGL((k_maxpy<T>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, iter, n3, ld,
+                               (const double*)part1, hcol, 0, (double*)nullptr);
+            hipLaunchKernelGGL((k_mdot<T>), dim3(nred), dim3(RT), 0, s, (const T*)Q, (const T*)v, iter, n3, ld, part2);
         }
-        hipLaunchKernelGGL((k_maxpy<T>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, iter, n3, (const double*)part2, hcol,
-                           1, normpart);
+        hipLaunchKernelGGL((k_maxpy<T>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, iter, n3, ld, (const double*)part2,
+                           hcol, 1, normpart);
         hipLaunchKernelGGL(k_ls, dim3(1), dim3(LT), 0, s, sm, L, iter, done_dev, (const double*)normpart, (int)nred);
         DDNS_HIP(hipGetLastError());
         DDNS_HIP(hipMemcpyAsync(&done, done_dev, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -643,7 +730,7 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
         if (done) break;
     }
     const int used = std::min(iter, maxiter);
-    hipLaunchKernelGGL((k_solution<T>), dim3(nb3), dim3(HT), 0, s, pb.gm_x0, (const T*)Q, sm + L.Y0(), used,
+    hipLaunchKernelGGL((k_solution<T>), dim3(nb3), dim3(HT), 0, s, pb.gm_x0, (const T*)Q, ld, sm + L.Y0(), used,
                        (const int*)pb.sorted, pb.pert, n, pb.cap);
     DDNS_HIP(hipGetLastError());
     double tail[6];
