@@ -128,8 +128,10 @@ __global__ void __launch_bounds__(HT) k_spmv(const int* __restrict__ cnt, const 
                                              const T* __restrict__ mm, int n, const T* __restrict__ x,
                                              const double* __restrict__ hx, T* __restrict__ qout, T* __restrict__ y,
                                              const double* __restrict__ x0d, double* __restrict__ y0d,
-                                             const double* __restrict__ w, double* __restrict__ b)
+                                             const double* __restrict__ w, double* __restrict__ b,
+                                             const double* __restrict__ stop)
 {
+    if (stop && *stop != 0.0) return;   // converged in an earlier iteration of this batch
     const int t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const size_t N = (size_t)n;
@@ -295,8 +297,9 @@ __device__ __forceinline__ void vst(T* p, const T (&x)[kvec<T>::W])
 // partials of Q_j . v for j < k
 template <typename T>
 __global__ void __launch_bounds__(RT) k_mdot(const T* __restrict__ Q, const T* __restrict__ v, int k, long long n3,
-                                             long long ld, double* __restrict__ part)
+                                             long long ld, double* __restrict__ part, const double* __restrict__ stop)
 {
+    if (*stop != 0.0) return;
     constexpr int W = kvec<T>::W;
     __shared__ double sh[RW][MD_J];
     __shared__ double red[MD_J];
@@ -336,8 +339,10 @@ __global__ void __launch_bounds__(RT) k_mdot(const T* __restrict__ Q, const T* _
 template <typename T, int KC>
 __global__ void __launch_bounds__(RT) k_maxpy_mdot(T* __restrict__ v, const T* __restrict__ Q, int k, long long n3,
                                                    long long ld, const double* __restrict__ part_in,
-                                                   double* __restrict__ hcol, double* __restrict__ part_out)
+                                                   double* __restrict__ hcol, double* __restrict__ part_out,
+                                                   const double* __restrict__ stop)
 {
+    if (*stop != 0.0) return;
     constexpr int W = kvec<T>::W;
     __shared__ double hs[KC];
     __shared__ double sh[RW][KC];
@@ -393,8 +398,10 @@ __global__ void __launch_bounds__(RT) k_maxpy_mdot(T* __restrict__ v, const T* _
 template <typename T>
 __global__ void __launch_bounds__(RT) k_maxpy(T* __restrict__ v, const T* __restrict__ Q, int k, long long n3,
                                               long long ld, const double* __restrict__ part_in,
-                                              double* __restrict__ hcol, int add, double* __restrict__ normpart)
+                                              double* __restrict__ hcol, int add, double* __restrict__ normpart,
+                                              const double* __restrict__ stop)
 {
+    if (*stop != 0.0) return;
     constexpr int W = kvec<T>::W;
     __shared__ double hs[64];
     __shared__ double sh[RW][1];
@@ -474,16 +481,17 @@ struct GmSmall {
 constexpr int LT = 64;         // k_ls: one wavefront; maxiter + 1 <= LT
 constexpr int LS_MAX = 8192;   // GmSmall::total() for maxiter <= 50, the reference's value
 
-__device__ void ls_solve(double* sm, GmSmall L, int iter, int* done_host);
+__device__ void ls_solve(double* sm, GmSmall L, int iter);
 
 __device__ __forceinline__ void copy_range(double* dst, const double* src, size_t a, size_t b)
 {
     for (size_t i = a + threadIdx.x; i < b; i += LT) dst[i] = src[i];
 }
 
-__global__ void __launch_bounds__(LT) k_ls(double* gsm, GmSmall L, int iter, int* done_host,
-                                          const double* __restrict__ normpart, int P)
+__global__ void __launch_bounds__(LT) k_ls(double* gsm, GmSmall L, int iter, const double* __restrict__ normpart,
+                                          int P)
 {
+    if (gsm[4] != 0.0) return;   // converged in an earlier iteration of this batch
     // the part of the small problem this iteration touches is copied to LDS, solved
     // there, and the changed part copied back
     __shared__ double sm[LS_MAX];
@@ -503,7 +511,7 @@ __global__ void __launch_bounds__(LT) k_ls(double* gsm, GmSmall L, int iter, int
         sm[e] = gsm[e] = sqrt(acc);
     }
     __syncthreads();
-    ls_solve(sm, L, iter, done_host);
+    ls_solve(sm, L, iter);
     __syncthreads();
     copy_range(gsm, sm, 0, 16);
     copy_range(gsm, sm, L.R0(), L.R0() + M1 * iter);
@@ -513,7 +521,7 @@ __global__ void __launch_bounds__(LT) k_ls(double* gsm, GmSmall L, int iter, int
 
 // All LT lanes call it.  Lane r owns row r + 1 of the triangular factor and of the
 // residual; the Givens rotation and the back substitution run over the lanes.
-__device__ void ls_solve(double* sm, GmSmall L, int iter, int* done_host)
+__device__ void ls_solve(double* sm, GmSmall L, int iter)
 {
     const int lane = threadIdx.x;
     const int M1 = L.M1();
@@ -594,7 +602,6 @@ __device__ void ls_solve(double* sm, GmSmall L, int iter, int* done_host)
         }
         sm[3] = maxerr;
         sm[4] = done;
-        *done_host = done;
     }
 #undef HH
 #undef RN
@@ -671,7 +678,7 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
                        (const double*)pb.vel, (const double*)pb.vp, pb.gm_x0, pb.gm_w, n, pb.cap);
     hipLaunchKernelGGL((k_spmv<T, 0>), dim3(nbr), dim3(HT), 0, s, (const int*)pb.nei_cnt, (const int*)pb.nei_idx,
                        (const T*)pb.nei_m, n, (const T*)nullptr, (const double*)nullptr, (T*)nullptr, (T*)nullptr,
-                       (const double*)pb.gm_x0, ax0, (const double*)pb.gm_w, pb.gm_b);
+                       (const double*)pb.gm_x0, ax0, (const double*)pb.gm_w, pb.gm_b, (const double*)nullptr);
     hipLaunchKernelGGL((k_residual0<T>), dim3(nb3), dim3(HT), 0, s, pb.gm_b, ax0, Q, n3);
     st = dot(pb.gm_b, pb.gm_b, n3, pb.gm_red, sm + 0, 1, s);
     if (st) return st;
@@ -689,47 +696,57 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
         DDNS_HIP(hipMemcpyAsync(sm + 3, &me, sizeof(double), hipMemcpyHostToDevice, s));
     }
     hipLaunchKernelGGL((k_scale<T>), dim3(nb3), dim3(HT), 0, s, Q, sm + 1, n3);
-    int* done_dev = pb.flags + 8;
-    int iter = 0, done = 0;
+    // Iterations are launched in batches; every kernel of an iteration returns at
+    // once when the least-squares update of an earlier one has set the converged
+    // flag sm[4], so the host checks it once per batch.
+    constexpr int BATCH = 3;
+    const double* stop = sm + 4;
     double* Hd = sm + L.H0();
     const int M1 = L.M1();
-    const double* hprev = nullptr;   // H(iter, iter-1): norm of the previous work vector
-    for (iter = 1; iter <= maxiter; ++iter) {
-        T* v = work[iter & 1];
-        if (iter == 1)
-            hipLaunchKernelGGL((k_spmv<T, 1>), dim3(nbr), dim3(HT), 0, s, (const int*)pb.nei_cnt, (const int*)pb.nei_idx,
-                               (const T*)pb.nei_m, n, (const T*)Q, (const double*)nullptr, (T*)nullptr, v,
-                               (const double*)nullptr, (double*)nullptr, (const double*)nullptr, (double*)nullptr);
-        else
-            hipLaunchKernelGGL((k_spmv<T, 2>), dim3(nbr), dim3(HT), 0, s, (const int*)pb.nei_cnt, (const int*)pb.nei_idx,
-                               (const T*)pb.nei_m, n, (const T*)work[(iter - 1) & 1], hprev,
-                               Q + (size_t)(iter - 1) * ld, v, (const double*)nullptr, (double*)nullptr,
-                               (const double*)nullptr, (double*)nullptr);
-        // orthogonalise v against q_1..q_iter: classical Gram-Schmidt, twice (the
-        // reference's modified Gram-Schmidt, 2886-2894, in two block passes)
-        double* hcol = Hd + (size_t)M1 * (iter - 1);
-        hipLaunchKernelGGL((k_mdot<T>), dim3(nred), dim3(RT), 0, s, (const T*)Q, (const T*)v, iter, n3, ld, part1);
-        if (iter <= 8)
-            hipLaunchKernelGGL((k_maxpy_mdot<T, 8>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, iter, n3, ld,
-                               (const double*)part1, hcol, part2);
-        else if (iter <= 12)
-            hipLaunchKernelGGL((k_maxpy_mdot<T, 12>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, iter, n3, ld,
-                               (const double*)part1, hcol, part2);
-        else {
-            hipLaunchKernelGGL((k_maxpy<T>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, iter, n3, ld,
-                               (const double*)part1, hcol, 0, (double*)nullptr);
-            hipLaunchKernelGGL((k_mdot<T>), dim3(nred), dim3(RT), 0, s, (const T*)Q, (const T*)v, iter, n3, ld, part2);
+    int iter = 0;
+    double st6[6] = {0, 0, 0, 0, 0, 0};
+    while (iter < maxiter && st6[4] == 0.0) {
+        const int last = std::min(maxiter, iter + BATCH);
+        for (int it = iter + 1; it <= last; ++it) {
+            T* v = work[it & 1];
+            if (it == 1)
+                hipLaunchKernelGGL((k_spmv<T, 1>), dim3(nbr), dim3(HT), 0, s, (const int*)pb.nei_cnt,
+                                   (const int*)pb.nei_idx, (const T*)pb.nei_m, n, (const T*)Q, (const double*)nullptr,
+                                   (T*)nullptr, v, (const double*)nullptr, (double*)nullptr, (const double*)nullptr,
+                                   (double*)nullptr, stop);
+            else    // q_it = v_(it-1) / H(it, it-1), formed inside the product
+                hipLaunchKernelGGL((k_spmv<T, 2>), dim3(nbr), dim3(HT), 0, s, (const int*)pb.nei_cnt,
+                                   (const int*)pb.nei_idx, (const T*)pb.nei_m, n, (const T*)work[(it - 1) & 1],
+                                   (const double*)(Hd + (size_t)M1 * (it - 2) + (it - 1)), Q + (size_t)(it - 1) * ld, v,
+                                   (const double*)nullptr, (double*)nullptr, (const double*)nullptr, (double*)nullptr,
+                                   stop);
+            // orthogonalise v against q_1..q_it: classical Gram-Schmidt, twice (the
+            // reference's modified Gram-Schmidt, 2886-2894, in two block passes)
+            double* hcol = Hd + (size_t)M1 * (it - 1);
+            hipLaunchKernelGGL((k_mdot<T>), dim3(nred), dim3(RT), 0, s, (const T*)Q, (const T*)v, it, n3, ld, part1, stop);
+            if (it <= 8)
+                hipLaunchKernelGGL((k_maxpy_mdot<T, 8>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, it, n3, ld,
+                                   (const double*)part1, hcol, part2, stop);
+            else if (it <= 12)
+                hipLaunchKernelGGL((k_maxpy_mdot<T, 12>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, it, n3, ld,
+                                   (const double*)part1, hcol, part2, stop);
+            else {
+                hipLaunchKernelGGL((k_maxpy<T>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, it, n3, ld,
+                                   (const double*)part1, hcol, 0, (double*)nullptr, stop);
+                hipLaunchKernelGGL((k_mdot<T>), dim3(nred), dim3(RT), 0, s, (const T*)Q, (const T*)v, it, n3, ld, part2,
+                                   stop);
+            }
+            hipLaunchKernelGGL((k_maxpy<T>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, it, n3, ld, (const double*)part2,
+                               hcol, 1, normpart, stop);
+            hipLaunchKernelGGL(k_ls, dim3(1), dim3(LT), 0, s, sm, L, it, (const double*)normpart, (int)nred);
         }
-        hipLaunchKernelGGL((k_maxpy<T>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, iter, n3, ld, (const double*)part2,
-                           hcol, 1, normpart);
-        hipLaunchKernelGGL(k_ls, dim3(1), dim3(LT), 0, s, sm, L, iter, done_dev, (const double*)normpart, (int)nred);
         DDNS_HIP(hipGetLastError());
-        DDNS_HIP(hipMemcpyAsync(&done, done_dev, sizeof(int), hipMemcpyDeviceToHost, s));
+        iter = last;
+        DDNS_HIP(hipMemcpyAsync(st6, sm, sizeof(st6), hipMemcpyDeviceToHost, s));
         DDNS_HIP(hipStreamSynchronize(s));
-        hprev = hcol + iter;
-        if (done) break;
     }
-    const int used = std::min(iter, maxiter);
+    const bool done = st6[4] != 0.0;
+    const int used = (int)st6[5];   // the last iteration that ran
     hipLaunchKernelGGL((k_solution<T>), dim3(nb3), dim3(HT), 0, s, pb.gm_x0, (const T*)Q, ld, sm + L.Y0(), used,
                        (const int*)pb.sorted, pb.pert, n, pb.cap);
     DDNS_HIP(hipGetLastError());
