@@ -72,6 +72,25 @@ __device__ __forceinline__ int xcd_chunk(int b, int nb)
     return x * q + (x < m ? x : m) + r;
 }
 
+// The candidates of the 27 neighbour cells as one sequence per lane: cells in the
+// order k = 0..26, droplets in cell order inside each, read from the ranges that
+// neighbour_ranges left in LDS.  A lane's loop then runs once per candidate instead
+// of the wavefront running, for every cell, as long as its fullest lane.
+struct CandCursor {
+    int kc = -1, p = 0, e = 0;
+};
+__device__ __forceinline__ bool cand_next(CandCursor& c, const int2* rng, int stride)
+{
+    ++c.p;
+    while (c.p >= c.e) {
+        if (++c.kc >= 27) return false;
+        const int2 r = rng[c.kc * stride];
+        c.p = r.x;
+        c.e = r.y;
+    }
+    return true;
+}
+
 // periodic shift (-1, 0, +1) of neighbour k along each axis for a lane in cell cc
 __device__ __forceinline__ void neighbour_shift(int k, int G, const int cc[3], int* wx, int* wy, int* wz)
 {

@@ -43,12 +43,15 @@ __device__ __forceinline__ double wave_sum(double x)
 // ------------------------------------------------------------------ neighbour lists
 // Row t (sorted position): every droplet p != t in the 27 cells around t with
 // |x_t - x_p| < trunc * r_p, in cell order, as partlib.f90:3280-3330 visits them.
+constexpr int NT_NEI = 128;   // k_nei work-group: 27 LDS cell ranges per lane
+
 template <typename T>
-__global__ void __launch_bounds__(HT) k_nei(const double4* __restrict__ pk, const int* __restrict__ cell_start,
-                                            int n, int G, double hc, double pi2, double trunc, int width,
-                                            int* __restrict__ cnt, int* __restrict__ idx, T* __restrict__ mm,
-                                            int* flags)
+__global__ void __launch_bounds__(NT_NEI) k_nei(const double4* __restrict__ pk, const int* __restrict__ cell_start,
+                                                int n, int G, double hc, double pi2, double trunc, int width,
+                                                int* __restrict__ cnt, int* __restrict__ idx, T* __restrict__ mm,
+                                                int* flags)
 {
+    __shared__ int2 rng[27 * NT_NEI];
     const int t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     int k = 0;
     if (t < n) {
@@ -62,40 +65,42 @@ __global__ void __launch_bounds__(HT) k_nei(const double4* __restrict__ pk, cons
             cc[c] = q < 0 ? 0 : (q >= G ? G - 1 : q);
         }
         const size_t ld = (size_t)n;
+        neighbour_ranges(cell_start, G, cc, rng + threadIdx.x, NT_NEI);
+        CandCursor cur;
+        bool more = cand_next(cur, rng + threadIdx.x, NT_NEI);
+        double4 o_next = more ? pk[cur.p] : me;
 #pragma unroll 1
-        for (int kc = 0; kc < 27; ++kc) {
+        while (more) {
+            const int p = cur.p, kc = cur.kc;
+            const double4 o = o_next;
+            more = cand_next(cur, rng + threadIdx.x, NT_NEI);
+            if (more) o_next = pk[cur.p];   // the next candidate's load in flight
+            if (p == t) continue;
             int wx, wy, wz;
             neighbour_shift(kc, G, cc, &wx, &wy, &wz);
-            const int cell = cell_code(cc[0] + kc / 9 - 1 - wx * G, cc[1] + (kc / 3) % 3 - 1 - wy * G,
-                                       cc[2] + kc % 3 - 1 - wz * G);
-            const int e = cell_start[cell + 1];
-            for (int p = cell_start[cell]; p < e; ++p) {
-                if (p == t) continue;
-                const double4 o = pk[p];
-                const double d0 = xi - (o.x + (double)wx * pi2);
-                const double d1 = yi - (o.y + (double)wy * pi2);
-                const double d2 = zi - (o.z + (double)wz * pi2);
-                const double dnij = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-                const double rj = o.w;
-                if (dnij < ri + rj) atomicOr(flags, 1);                         // 3294-3299
-                if (!(dnij < trunc * rj)) continue;                             // 3302-3305
-                if (k < width) {
-                    double fact1 = rj / dnij;                                   // 3312-3324
-                    const double fact2 = fact1 * fact1 * fact1;
-                    fact1 = 0.75 * fact1;
-                    const double gamma = (fact1 - 0.75 * fact2) / (dnij * dnij);
-                    const double beta = fact1 + 0.25 * fact2;
-                    idx[(size_t)k * ld + t] = p;
-                    T* m = mm + (size_t)(6 * k) * ld + t;
-                    m[0] = (T)(gamma * (d0 * d0) + beta);
-                    m[ld] = (T)(gamma * d0 * d1);
-                    m[2 * ld] = (T)(gamma * d0 * d2);
-                    m[3 * ld] = (T)(gamma * (d1 * d1) + beta);
-                    m[4 * ld] = (T)(gamma * d1 * d2);
-                    m[5 * ld] = (T)(gamma * (d2 * d2) + beta);
-                }
-                ++k;
+            const double d0 = xi - (o.x + (double)wx * pi2);
+            const double d1 = yi - (o.y + (double)wy * pi2);
+            const double d2 = zi - (o.z + (double)wz * pi2);
+            const double dnij = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+            const double rj = o.w;
+            if (dnij < ri + rj) atomicOr(flags, 1);                         // 3294-3299
+            if (!(dnij < trunc * rj)) continue;                             // 3302-3305
+            if (k < width) {
+                double fact1 = rj / dnij;                                   // 3312-3324
+                const double fact2 = fact1 * fact1 * fact1;
+                fact1 = 0.75 * fact1;
+                const double gamma = (fact1 - 0.75 * fact2) / (dnij * dnij);
+                const double beta = fact1 + 0.25 * fact2;
+                idx[(size_t)k * ld + t] = p;
+                T* m = mm + (size_t)(6 * k) * ld + t;
+                m[0] = (T)(gamma * (d0 * d0) + beta);
+                m[ld] = (T)(gamma * d0 * d1);
+                m[2 * ld] = (T)(gamma * d0 * d2);
+                m[3 * ld] = (T)(gamma * (d1 * d1) + beta);
+                m[4 * ld] = (T)(gamma * d1 * d2);
+                m[5 * ld] = (T)(gamma * (d2 * d2) + beta);
             }
+            ++k;
         }
         if (k > DDNS_MAXNEI) atomicOr(flags, 2);
         cnt[t] = k;
@@ -620,7 +625,7 @@ static int build_lists(PartBuffers& pb, const PartConst& pc, hipStream_t s)
     }
     for (;;) {
         DDNS_HIP(hipMemsetAsync(pb.flags, 0, 8 * sizeof(int), s));
-        hipLaunchKernelGGL((k_nei<T>), dim3((n + HT - 1) / HT), dim3(HT), 0, s, (const double4*)pb.pk, pb.cell_start,
+        hipLaunchKernelGGL((k_nei<T>), dim3((n + NT_NEI - 1) / NT_NEI), dim3(NT_NEI), 0, s, (const double4*)pb.pk, pb.cell_start,
                            n, pb.G, pb.hc, pc.pi2, pc.hdi_trunc, pb.ell_width, pb.nei_cnt, pb.nei_idx, (T*)pb.nei_m,
                            pb.flags);
         DDNS_HIP(hipGetLastError());
