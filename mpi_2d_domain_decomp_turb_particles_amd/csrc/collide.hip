@@ -369,30 +369,20 @@ __global__ void __launch_bounds__(CT) k_overlap(OverArgs a)
         cc[c] = q < 0 ? 0 : (q >= g.G ? g.G - 1 : q);
     }
     int cnt = 0;
-    // a light kernel: direct loads of the cell ranges keep its occupancy high
-    for (int kc = 0; kc < 27; ++kc) {
-                int wx, wy, wz;
-                neighbour_shift(kc, g.G, cc, &wx, &wy, &wz);
-                int qx = cc[0] + kc / 9 - 1 - wx * g.G, qy = cc[1] + (kc / 3) % 3 - 1 - wy * g.G;
-                int qz = cc[2] + kc % 3 - 1 - wz * g.G;
-                const int cell = cell_code(qx, qy, qz);
-                const int e = a.cell_start[cell + 1];
-                for (int p = a.cell_start[cell]; p < e; ++p) {
-                    if (p == t) continue;
-                    const int ib = a.sorted[p];
-                    const double4 cb = a.pk[p];
-                    double xb, yb, zb;
-                    int di, dj, dk;
-                    if (!ref_visit(g, ia, ib, xa, ya, za, cb.x, cb.y, cb.z, wx, wy, wz,
-                                   &xb, &yb, &zb, &di, &dj, &dk))
-                        continue;
-                    if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) continue;   // 2091
-                    const double d0 = xa - xb, d1 = ya - yb, d2 = za - zb;
-                    const double dnij = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-                    const double rcoll = ra + cb.w;
-                    if (dnij <= rcoll) ++cnt;
-                }
-            }
+    // a light kernel: the row-wise walk needs no LDS, which keeps its occupancy high
+    for_each_candidate(a.cell_start, g.G, cc, a.pk, a.sorted, [&](int p, int kc, const double4& cb, int ib) {
+        if (p == t) return;
+        int wx, wy, wz;
+        neighbour_shift(kc, g.G, cc, &wx, &wy, &wz);
+        double xb, yb, zb;
+        int di, dj, dk;
+        if (!ref_visit(g, ia, ib, xa, ya, za, cb.x, cb.y, cb.z, wx, wy, wz, &xb, &yb, &zb, &di, &dj, &dk)) return;
+        if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) return;   // 2091
+        const double d0 = xa - xb, d1 = ya - yb, d2 = za - zb;
+        const double dnij = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        const double rcoll = ra + cb.w;
+        if (dnij <= rcoll) ++cnt;
+    });
     a.cnt[ia] = cnt;
     if (cnt) atomicAdd(a.total, cnt);
 }

@@ -91,6 +91,59 @@ __device__ __forceinline__ bool cand_next(CandCursor& c, const int2* rng, int st
     return true;
 }
 
+// The candidates of the 27 neighbour cells in 9 rows: for each (y, z) offset the three
+// cells x-1, x, x+1 as one sequence, with the next candidate's packed record loaded
+// while f works on the current one.  The three ranges of a row stay in registers, so
+// no LDS is needed; a wavefront runs, per row, as long as its fullest lane.
+// f(p, kc, rec, ib) gets the sorted position, the cell number k = 9 (dx+1) + 3 (dy+1)
+// + (dz+1) of neighbour_shift, the record pk[p] and, when sorted is given, the
+// droplet index sorted[p].
+template <typename F>
+__device__ __forceinline__ void for_each_candidate(const int* __restrict__ cell_start, int G, const int cc[3],
+                                                   const double4* __restrict__ pk, const int* __restrict__ sorted,
+                                                   F&& f)
+{
+    int qy[3], qz[3], qx[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        int q = cc[0] + m - 1; qx[m] = q < 0 ? q + G : (q >= G ? q - G : q);
+        q = cc[1] + m - 1;     qy[m] = q < 0 ? q + G : (q >= G ? q - G : q);
+        q = cc[2] + m - 1;     qz[m] = q < 0 ? q + G : (q >= G ? q - G : q);
+    }
+#pragma unroll 1
+    for (int r = 0; r < 9; ++r) {
+        const int my = r / 3, mz = r % 3;
+        int s[3], e[3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            const int cell = cell_code(qx[m], qy[my], qz[mz]);
+            s[m] = cell_start[cell];
+            e[m] = cell_start[cell + 1];
+        }
+        // cursor over the three ranges
+        int m = 0, p = s[0], end = e[0];
+        while (p >= end && m < 2) { ++m; p = m == 1 ? s[1] : s[2]; end = m == 1 ? e[1] : e[2]; }
+        if (p >= end) continue;
+        double4 next = pk[p];
+        int inext = sorted ? sorted[p] : 0;
+#pragma unroll 1
+        while (true) {
+            const int pc = p, kc = 9 * m + 3 * my + mz;
+            const double4 rec = next;
+            const int ib = inext;
+            ++p;
+            while (p >= end && m < 2) { ++m; p = m == 1 ? s[1] : s[2]; end = m == 1 ? e[1] : e[2]; }
+            const bool more = p < end;
+            if (more) {
+                next = pk[p];
+                if (sorted) inext = sorted[p];
+            }
+            f(pc, kc, rec, ib);
+            if (!more) break;
+        }
+    }
+}
+
 // periodic shift (-1, 0, +1) of neighbour k along each axis for a lane in cell cc
 __device__ __forceinline__ void neighbour_shift(int k, int G, const int cc[3], int* wx, int* wy, int* wz)
 {

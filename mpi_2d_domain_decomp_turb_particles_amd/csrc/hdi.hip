@@ -43,7 +43,7 @@ __device__ __forceinline__ double wave_sum(double x)
 // ------------------------------------------------------------------ neighbour lists
 // Row t (sorted position): every droplet p != t in the 27 cells around t with
 // |x_t - x_p| < trunc * r_p, in cell order, as partlib.f90:3280-3330 visits them.
-constexpr int NT_NEI = 128;   // k_nei work-group: 27 LDS cell ranges per lane
+constexpr int NT_NEI = HT;
 
 template <typename T>
 __global__ void __launch_bounds__(NT_NEI) k_nei(const double4* __restrict__ pk, const int* __restrict__ cell_start,
@@ -51,7 +51,6 @@ __global__ void __launch_bounds__(NT_NEI) k_nei(const double4* __restrict__ pk, 
                                                 int* __restrict__ cnt, int* __restrict__ idx, T* __restrict__ mm,
                                                 int* flags)
 {
-    __shared__ int2 rng[27 * NT_NEI];
     const int t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     int k = 0;
     if (t < n) {
@@ -65,17 +64,9 @@ __global__ void __launch_bounds__(NT_NEI) k_nei(const double4* __restrict__ pk, 
             cc[c] = q < 0 ? 0 : (q >= G ? G - 1 : q);
         }
         const size_t ld = (size_t)n;
-        neighbour_ranges(cell_start, G, cc, rng + threadIdx.x, NT_NEI);
-        CandCursor cur;
-        bool more = cand_next(cur, rng + threadIdx.x, NT_NEI);
-        double4 o_next = more ? pk[cur.p] : me;
-#pragma unroll 1
-        while (more) {
-            const int p = cur.p, kc = cur.kc;
-            const double4 o = o_next;
-            more = cand_next(cur, rng + threadIdx.x, NT_NEI);
-            if (more) o_next = pk[cur.p];   // the next candidate's load in flight
-            if (p == t) continue;
+        bool overlap = false;
+        for_each_candidate(cell_start, G, cc, pk, (const int*)nullptr, [&](int p, int kc, const double4& o, int) {
+            if (p == t) return;
             int wx, wy, wz;
             neighbour_shift(kc, G, cc, &wx, &wy, &wz);
             const double d0 = xi - (o.x + (double)wx * pi2);
@@ -83,8 +74,8 @@ __global__ void __launch_bounds__(NT_NEI) k_nei(const double4* __restrict__ pk, 
             const double d2 = zi - (o.z + (double)wz * pi2);
             const double dnij = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
             const double rj = o.w;
-            if (dnij < ri + rj) atomicOr(flags, 1);                         // 3294-3299
-            if (!(dnij < trunc * rj)) continue;                             // 3302-3305
+            if (dnij < ri + rj) overlap = true;                             // 3294-3299
+            if (!(dnij < trunc * rj)) return;                               // 3302-3305
             if (k < width) {
                 double fact1 = rj / dnij;                                   // 3312-3324
                 const double fact2 = fact1 * fact1 * fact1;
@@ -101,7 +92,8 @@ __global__ void __launch_bounds__(NT_NEI) k_nei(const double4* __restrict__ pk, 
                 m[5 * ld] = (T)(gamma * (d2 * d2) + beta);
             }
             ++k;
-        }
+        });
+        if (overlap) atomicOr(flags, 1);
         if (k > DDNS_MAXNEI) atomicOr(flags, 2);
         cnt[t] = k;
     }
