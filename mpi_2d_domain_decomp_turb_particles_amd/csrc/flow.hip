@@ -16,6 +16,7 @@
 // The z stage therefore closes step n and opens step n + 1 in one pass over HBM.
 #include "fft_engine.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -33,6 +34,20 @@ template <typename T, int N> constexpr int ky_tc() { return (sizeof(T) == 4) ? (
 #endif
 #ifndef DDNS_KZ_TC_F64
 #define DDNS_KZ_TC_F64 8
+#endif
+// persistent z stage, overridable at build time for tuning sweeps: the update
+// operands of DDNS_KZ_EPTP_BYTES / sizeof(complex) elements per thread are loaded
+// before the forward transforms, the rest of the update loop is unrolled
+// DDNS_KZ_UNROLL times, and with DDNS_KZ_NEXTB the next tile's input is loaded during
+// this tile's inverse transforms (single precision only)
+#ifndef DDNS_KZ_UNROLL
+#define DDNS_KZ_UNROLL 4
+#endif
+#ifndef DDNS_KZ_NEXTB
+#define DDNS_KZ_NEXTB 1
+#endif
+#ifndef DDNS_KZ_EPTP_BYTES
+#define DDNS_KZ_EPTP_BYTES 32
 #endif
 template <typename T, int N> constexpr int kz_tc()
 {
@@ -143,6 +158,8 @@ __device__ __forceinline__ void update_mode(int kx, int ky, int kz, int N, int n
 }
 
 // ------------------------------------------------------------------ z stage
+// One tile per work-group: curl and inverse z transforms of u-hat (KZ_CURL_INV, the
+// first step) or a plain forward / inverse z transform of one field.
 template <typename T, int N, int MODE>
 __global__ void __launch_bounds__(NT) k_z(KzArgs a)
 {
@@ -150,7 +167,7 @@ __global__ void __launch_bounds__(NT) k_z(KzArgs a)
     constexpr int M = N / 2;
     constexpr int TC = kz_tc<T, N>();
     constexpr int LDR = kz_ldr<T, N>();
-    constexpr int NTILE = (MODE == KZ_FULL || MODE == KZ_CURL_INV) ? 4 : 1;
+    constexpr int NTILE = MODE == KZ_CURL_INV ? 4 : 1;
     __shared__ C lds[NTILE][N * LDR];
     __shared__ C tws[N];
     const int tid = threadIdx.x;
@@ -173,35 +190,8 @@ __global__ void __launch_bounds__(NT) k_z(KzArgs a)
         tile_store<C, N, TC, LDR, NT>(dst + base, lds[0], rowstride, tid, (T)a.scale);
         return;
     } else {
-        if constexpr (MODE == KZ_FULL) {
 #pragma unroll
-            for (int q = 0; q < 3; ++q) tile_load<C, N, TC, LDR, NT>(lds[q], (const C*)a.B[q] + base, rowstride, tid);
-            __syncthreads();
-            fft_tile<T, N, false, TC, LDR, NT, 3, N * LDR>(lds[0], tw, tid);     // the three components together
-            const T inv_n = (T)1 / (T)N;
-#pragma unroll 2
-            for (int e = tid; e < TC * N; e += NT) {
-                const int n = e / TC, c = e % TC;
-                const size_t g = base + n * rowstride + c;
-                C nn[3], uu[3], oo[3], vv[3], nd[3];
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    nn[q] = cscale(lds[q][n * LDR + c], inv_n);
-                    uu[q] = ((const C*)a.U[q])[g];
-                    oo[q] = ((const C*)a.O[q])[g];
-                }
-                update_mode<T>(kx0 + c, ky, n, N, a.nek, a.forcing, (T)a.dt, (T)a.rnu, a.low, nn, uu, oo, vv, nd);
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    ((C*)a.U[q])[g] = vv[q];
-                    ((C*)a.O[q])[g] = nd[q];
-                    lds[q][n * LDR + c] = vv[q];
-                }
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < 3; ++q) tile_load<C, N, TC, LDR, NT>(lds[q], (const C*)a.U[q] + base, rowstride, tid);
-        }
+        for (int q = 0; q < 3; ++q) tile_load<C, N, TC, LDR, NT>(lds[q], (const C*)a.U[q] + base, rowstride, tid);
         __syncthreads();
         // vorticity w = i k x u (turb.f90:506-517), one component at a time in tile 3
 #pragma unroll 1
@@ -224,6 +214,141 @@ __global__ void __launch_bounds__(NT) k_z(KzArgs a)
         fft_tile<T, N, true, TC, LDR, NT, 3, N * LDR>(lds[0], tw, tid);
 #pragma unroll
         for (int q = 0; q < 3; ++q) tile_store<C, N, TC, LDR, NT>((C*)a.A[q] + base, lds[q], rowstride, tid);
+    }
+}
+
+// The full z stage as a persistent kernel: one work-group per CU walks the tiles
+// tile = blockIdx.x, blockIdx.x + gridDim.x, ...  The LDS tiles allow only one
+// work-group per CU, so instead of a second work-group hiding HBM latency, a tile's
+// update operands (U-hat, O-hat) are loaded into registers before its forward
+// transforms, and the next tile's input B is loaded into registers during this
+// tile's curl and inverse transforms.  The curl and the inverse transforms are those
+// of k_z<KZ_CURL_INV>.
+#ifndef DDNS_KZ_NT
+#define DDNS_KZ_NT 256
+#endif
+// threads of the persistent z-stage work-group: DDNS_KZ_NT where the tile splits
+// evenly over them (16-byte moves, one update element per thread and pass), else NT
+template <typename T, int N> constexpr int kz_nt()
+{
+    constexpr int tot = N * kz_tc<T, N>() * (int)sizeof(cpx<T>) / 16;
+    return (tot % DDNS_KZ_NT == 0 && (N * kz_tc<T, N>()) % DDNS_KZ_NT == 0) ? DDNS_KZ_NT : NT;
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__((kz_nt<T, N>())) k_z_full(KzArgs a, int ntiles)
+{
+    constexpr int NTZ = kz_nt<T, N>();
+    using C = cpx<T>;
+    constexpr int M = N / 2;
+    constexpr int TC = kz_tc<T, N>();
+    constexpr int LDR = kz_ldr<T, N>();
+    constexpr int NBX = M / TC;
+    constexpr int EPT = TC * N / NTZ;          // update elements per thread
+    static_assert((TC * N) % NTZ == 0, "the update must split evenly over the work-group");
+    // register budget: the three-tile transforms already hold 48 complex values per
+    // lane, so only part of the update operands is loaded ahead, and the next tile's
+    // input only in single precision
+    constexpr int EPTP = DDNS_KZ_EPTP_BYTES > 0 ? std::max(1, std::min(EPT, (int)(DDNS_KZ_EPTP_BYTES / sizeof(C)))) : EPT;
+    constexpr bool NEXT_B = DDNS_KZ_NEXTB && sizeof(T) == 4;
+    __shared__ C lds[4][N * LDR];
+    __shared__ C tws[N];
+    const int tid = threadIdx.x;
+    stage_twiddles<C, N, NTZ>(tws, (const C*)a.tw, tid);
+    const C* tw = tws;
+    const size_t rowstride = (size_t)N * M;
+    using TR = TileRegs<C, N, TC, NTZ>;
+    TR nb[3];
+    int tile = blockIdx.x;
+    if (tile < ntiles) {
+        const size_t base = (size_t)(tile / NBX) * M + (tile % NBX) * TC;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) tile_fetch<C, N, TC, NTZ>(nb[q], (const C*)a.B[q] + base, rowstride, tid);
+    }
+    for (; tile < ntiles; tile += gridDim.x) {
+        const int kx0 = (tile % NBX) * TC, ky = tile / NBX;
+        const size_t base = (size_t)ky * M + kx0;
+        if (!NEXT_B && tile != (int)blockIdx.x) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) tile_fetch<C, N, TC, NTZ>(nb[q], (const C*)a.B[q] + base, rowstride, tid);
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) tile_put<C, N, TC, LDR, NTZ>(lds[q], nb[q], tid);
+        // the first update operands, in flight during the forward transforms
+        C uu[EPTP][3], oo[EPTP][3];
+#pragma unroll
+        for (int k = 0; k < EPTP; ++k) {
+            const int e = tid + k * NTZ, n = e / TC, c = e % TC;
+            const size_t g = base + (size_t)n * rowstride + c;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                uu[k][q] = ((const C*)a.U[q])[g];
+                oo[k][q] = ((const C*)a.O[q])[g];
+            }
+        }
+        __syncthreads();
+        fft_tile<T, N, false, TC, LDR, NTZ, 3, N * LDR>(lds[0], tw, tid);     // the three components together
+        const T inv_n = (T)1 / (T)N;
+        auto update = [&](int k, const C* u3, const C* o3) {
+            const int e = tid + k * NTZ, n = e / TC, c = e % TC;
+            const size_t g = base + (size_t)n * rowstride + c;
+            C nn[3], vv[3], nd[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) nn[q] = cscale(lds[q][n * LDR + c], inv_n);
+            update_mode<T>(kx0 + c, ky, n, N, a.nek, a.forcing, (T)a.dt, (T)a.rnu, a.low, nn, u3, o3, vv, nd);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                ((C*)a.U[q])[g] = vv[q];
+                ((C*)a.O[q])[g] = nd[q];
+                lds[q][n * LDR + c] = vv[q];
+            }
+        };
+#pragma unroll
+        for (int k = 0; k < EPTP; ++k) update(k, uu[k], oo[k]);
+#pragma unroll DDNS_KZ_UNROLL
+        for (int k = EPTP; k < EPT; ++k) {
+            const int e = tid + k * NTZ, n = e / TC, c = e % TC;
+            const size_t g = base + (size_t)n * rowstride + c;
+            C u3[3], o3[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                u3[q] = ((const C*)a.U[q])[g];
+                o3[q] = ((const C*)a.O[q])[g];
+            }
+            update(k, u3, o3);
+        }
+        // the next tile's input, in flight during the curl and the inverse transforms
+        // (the barrier keeps the compiler from hoisting these loads over the update)
+        asm volatile("" ::: "memory");
+        const int nt = tile + (int)gridDim.x;
+        if (NEXT_B && nt < ntiles) {
+            const size_t nbase = (size_t)(nt / NBX) * M + (nt % NBX) * TC;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) tile_fetch<C, N, TC, NTZ>(nb[q], (const C*)a.B[q] + nbase, rowstride, tid);
+        }
+        __syncthreads();
+        // vorticity w = i k x u (turb.f90:506-517), one component at a time in tile 3
+#pragma unroll 1
+        for (int q = 0; q < 3; ++q) {
+            for (int e = tid; e < TC * N; e += NTZ) {
+                const int n = e / TC, c = e % TC;
+                const T fx = (T)(kx0 + c), fy = (T)wave_index(ky, N), fz = (T)wave_index(n, N);
+                const C ux = lds[0][n * LDR + c], uy = lds[1][n * LDR + c], uz = lds[2][n * LDR + c];
+                C w;
+                if (q == 0) w = csub(cscale(uz, fy), cscale(uy, fz));
+                else if (q == 1) w = csub(cscale(ux, fz), cscale(uz, fx));
+                else w = csub(cscale(uy, fx), cscale(ux, fy));
+                lds[3][n * LDR + c] = mk<C>(-w.y, w.x);
+            }
+            __syncthreads();
+            fft_tile<T, N, true, TC, LDR, NTZ>(lds[3], tw, tid);
+            tile_store<C, N, TC, LDR, NTZ>((C*)a.A[3 + q] + base, lds[3], rowstride, tid);
+            __syncthreads();
+        }
+        fft_tile<T, N, true, TC, LDR, NTZ, 3, N * LDR>(lds[0], tw, tid);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) tile_store<C, N, TC, LDR, NTZ>((C*)a.A[q] + base, lds[q], rowstride, tid);
+        __syncthreads();   // the stores have read the tiles before the next tile is put
     }
 }
 
@@ -762,8 +887,17 @@ static int launch_z(int mode, const FlowBuffers& fb, const FlowConst& fc, hipStr
     if (mode == KZ_PLAIN_FWD || mode == KZ_PLAIN_INV) { a.B[0] = src; a.U[0] = dst; }
     constexpr int TC = kz_tc<T, N>();
     dim3 grid((N / 2 / TC) * N);
+    if (mode == KZ_FULL) {
+        // persistent: one work-group per CU (the LDS tiles allow no more)
+        int dev = 0, ncu = 0;
+        DDNS_HIP(hipGetDevice(&dev));
+        DDNS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        const int ntiles = (int)grid.x;
+        hipLaunchKernelGGL((k_z_full<T, N>), dim3(std::max(1, std::min(ncu, ntiles))), dim3(kz_nt<T, N>()), 0, s, a, ntiles);
+        DDNS_HIP(hipGetLastError());
+        return 0;
+    }
     switch (mode) {
-    case KZ_FULL: hipLaunchKernelGGL((k_z<T, N, KZ_FULL>), grid, dim3(NT), 0, s, a); break;
     case KZ_CURL_INV: hipLaunchKernelGGL((k_z<T, N, KZ_CURL_INV>), grid, dim3(NT), 0, s, a); break;
     case KZ_PLAIN_FWD: hipLaunchKernelGGL((k_z<T, N, KZ_PLAIN_FWD>), grid, dim3(NT), 0, s, a); break;
     default: hipLaunchKernelGGL((k_z<T, N, KZ_PLAIN_INV>), grid, dim3(NT), 0, s, a); break;
