@@ -128,7 +128,33 @@ struct CollArgs {
     long long* pairs;      // DDNS_PAIR_REC per pair, see k_collide
     int pair_cap;
     int* mult;
+    const unsigned long long* dmax;   // bits of the largest |dmove| of this step
 };
+
+// the largest displacement length |dmove| of the step, as the bits of a non-negative
+// double (whose order is the numeric order), for the exact pre-filter of the
+// collision time window in k_collide
+__global__ void __launch_bounds__(256) k_dmove_max(const double* __restrict__ dmove, int n, int cap,
+                                                   unsigned long long* out)
+{
+    // grid-stride over a small fixed grid, then one atomic per work-group (one per
+    // wavefront would serialise thousands of atomics on the single address)
+    __shared__ double wm[4];
+    const size_t C = (size_t)cap;
+    double m = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double d0 = dmove[i], d1 = dmove[C + i], d2 = dmove[2 * C + i];
+        m = fmax(m, sqrt(d0 * d0 + d1 * d1 + d2 * d2));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
+        if (m > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    }
+}
 
 __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
 {
@@ -158,6 +184,10 @@ __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
     int ncollip1 = 0;
     long long tested = 0, instat = 0;
     const int nbin = DDNS_NBIN, ns = a.nset;
+    // rcoll + dmove_a + dmove_b <= rcoll + dmove_a + dmax: a pair farther apart than the
+    // right side cannot pass the time-window test below, so its partner's displacement
+    // is not loaded
+    const double dmax = __longlong_as_double((long long)*a.dmax);
     __shared__ int2 rng[27 * CT];
     neighbour_ranges(a.cell_start, g.G, cc, rng + threadIdx.x, CT);
     CandCursor cur;
@@ -250,6 +280,7 @@ __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
             }
         }
         if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) continue;   // 1445
+        if (dnij0 > rcoll + dmove_a + dmax) continue;
         // -------- coarse check and types I/II/III, 1451-1516
         double bx1[3], xbn[3];
 #pragma unroll
@@ -424,6 +455,11 @@ int part_collide(PartBuffers& pb, const PartConst& pc, hipStream_t s)
     a.g = make_geom(pb, pc, dstat, pc.hdi ? pc.d_hdi : pc.d_coll, pc.nx_coll);
     a.nirbin = pb.nirbin; a.bins = pb.bins; a.counts = pb.counts; a.tested = pb.tested;
     a.pairs = pb.pairs; a.pair_cap = pb.pair_cap; a.mult = pb.mult;
+    unsigned long long* dmax = (unsigned long long*)(pb.flags + 32);
+    DDNS_HIP(hipMemsetAsync(dmax, 0, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_dmove_max, dim3(std::min(256, (pc.n + 255) / 256)), dim3(256), 0, s, (const double*)pb.dmove, pc.n,
+                       pb.cap, dmax);
+    a.dmax = dmax;
     hipLaunchKernelGGL(k_collide, dim3((pc.n + CT - 1) / CT), dim3(CT), 0, s, a);
     DDNS_HIP(hipGetLastError());
     return 0;
@@ -641,6 +677,7 @@ __global__ void __launch_bounds__(64) k_coalesce(CoalArgs a)
 
 int part_coalesce(PartBuffers& pb, const PartConst& pc, int* removed, hipStream_t s)
 {
+    invalidate_cells(pb);
     *removed = 0;
     int np = 0;
     DDNS_HIP(hipMemcpyAsync(&np, pb.counts + 256, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -545,6 +545,7 @@ extern "C" int ddns_set_droplets(ddns_handle* h, int n, const double* yp, const 
     DDNS_HIP(hipMemcpy(h->pb.fvp, fv.data(), fv.size() * 8, hipMemcpyHostToDevice));
     DDNS_HIP(hipMemcpy(h->pb.fdp, fd.data(), fd.size() * 8, hipMemcpyHostToDevice));
     DDNS_HIP(hipMemcpy(h->pb.pert, pv.data(), pv.size() * 8, hipMemcpyHostToDevice));
+    invalidate_cells(h->pb);
     h->n = n;
     h->pc.n = n;
     return DDNS_OK;
@@ -690,12 +691,15 @@ static int droplet_block(ddns_handle* h, float* ms)
     pc.new_particle = 0;
     DDNS_HIP(hipEventRecord(h->ev[12], s));
     if (h->p.hdi_included && std::min(256, (int)std::floor(pc.pi2 / coll_cell_width(h))) > h->pb.G) {
-        st = part_build_cells(h->pb, pc, coll_cell_width(h), s);              // finer grid for the collision search
+        // finer grid for the collision search; the grid of the last overlap sweep has
+        // this width and the same positions, and is swapped in when it is still valid
+        st = part_build_cells(h->pb, pc, coll_cell_width(h), s);
         if (st) return st;
     }
     st = part_collide(h->pb, pc, s);                                          // 584
     if (st) return st;
     const bool reloc = h->p.nonoverlap && h->istep < h->p.nstep_grow;         // 590
+    invalidate_cells(h->pb);
     st = part_update_and_wrap(h->pb, pc, reloc, s);                           // 586, 590, 608
     if (st) return st;
     if (h->p.nonoverlap && h->istep >= h->p.nstep_grow) {                     // 591: COALESCE

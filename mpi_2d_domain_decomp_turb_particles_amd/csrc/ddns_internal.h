@@ -161,6 +161,9 @@ inline int cell_bits(int G)
 }
 
 int hip_fail(hipError_t e, const char* what, const char* file, int line);
+struct PartBuffers;
+// droplet positions or slots changed: neither cell grid describes them any more
+void invalidate_cells(PartBuffers& pb);
 void set_error(const std::string& msg);
 
 // ---------------------------------------------------------------- complex helpers
@@ -242,7 +245,17 @@ struct PartBuffers {
     int* cell_of = nullptr;     // cap
     int* sorted = nullptr;      // cap
     double4* pk = nullptr;      // cap: (x, y, z, radius) of sorted[p], refreshed with the cells
-    int* cursor = nullptr;      // G^3
+    int* cell_rank = nullptr;   // cap: place of a droplet among those of its cell (build scratch)
+    bool cells_valid = false;   // the grid above was built on the current positions
+    // a second grid, kept from the last build of another width while it stays valid:
+    // a search at that width then swaps it in instead of rebuilding
+    struct CellSet {
+        int G = 0;
+        double hc = 0;
+        int *cell_count = nullptr, *cell_start = nullptr, *cell_of = nullptr, *sorted = nullptr;
+        double4* pk = nullptr;
+        bool valid = false;
+    } spare;
     void* scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
     // HI, in cell-sorted droplet order: ELL neighbour lists, slot-major

@@ -18,6 +18,7 @@
 namespace ddns {
 
 constexpr int PT = 256;
+constexpr int CELL_GMAX = 256;   // finest pair-search grid per dimension
 
 static inline unsigned nblk(long long n, int t = PT) { return (unsigned)((n + t - 1) / t); }
 
@@ -117,6 +118,7 @@ int part_alloc(PartBuffers& pb, int cap, int nset, int G)
     DDNS_HIP(hipMemset(pb.dmove, 0, 3 * c * 8));
     DDNS_HIP(hipMemset(pb.vp, 0, 3 * c * 8));
     DDNS_HIP(hipMalloc(&pb.cell_of, c * sizeof(int)));
+    DDNS_HIP(hipMalloc(&pb.cell_rank, c * sizeof(int)));
     DDNS_HIP(hipMalloc(&pb.sorted, c * sizeof(int)));
     DDNS_HIP(hipMalloc(&pb.pk, c * sizeof(double4)));
     DDNS_HIP(hipMalloc(&pb.mult, (c + 1) * sizeof(int)));
@@ -145,11 +147,13 @@ int part_alloc(PartBuffers& pb, int cap, int nset, int G)
 void part_free(PartBuffers& pb)
 {
     void* ps[] = {pb.pos, pb.vel, pb.rad, pb.gid, pb.fvp, pb.fdp, pb.dmove, pb.vp, pb.pert,
-                  pb.cell_count, pb.cell_start, pb.cell_of, pb.sorted, pb.pk, pb.cursor, pb.scan_tmp,
+                  pb.cell_count, pb.cell_start, pb.cell_of, pb.cell_rank, pb.sorted, pb.pk, pb.scan_tmp,
                   pb.nei_cnt, pb.nei_idx, pb.nei_m, pb.gm_Q, pb.gm_v, pb.gm_x0, pb.gm_b, pb.gm_w,
                   pb.gm_red, pb.gm_small, pb.mult, pb.flags, pb.tested, pb.nirbin, pb.bins, pb.counts,
                   pb.pairs, pb.draws, pb.draw_off, pb.rng_state, pb.red};
     for (void* p : ps) if (p) (void)hipFree(p);
+    void* sp[] = {pb.spare.cell_count, pb.spare.cell_start, pb.spare.cell_of, pb.spare.sorted, pb.spare.pk};
+    for (void* p : sp) if (p) (void)hipFree(p);
     pb = PartBuffers{};
 }
 
@@ -470,6 +474,7 @@ int relocate_by_mult(PartBuffers& pb, const PartConst& pc, int* total_host, hipS
     DDNS_HIP(hipStreamSynchronize(s));
     *total_host = total;
     if (total == 0) return 0;
+    invalidate_cells(pb);                 // relocated droplets move
     if (3 * total > pb.draw_cap) {
         DDNS_HIP(hipFree(pb.draws));
         pb.draw_cap = 3 * total + 1024;
@@ -501,7 +506,9 @@ int part_update_and_wrap(const PartBuffers& pb, const PartConst& pc, bool reloca
 // A periodic grid of G^3 cells of width hc = 2pi/G >= the largest pair distance the
 // caller needs, so a 27-cell neighbourhood holds every candidate.  Counting sort by
 // cell, then each cell's slice is ordered by slot number (deterministic).
-__global__ void k_cell_key(const double* pos, int* cell_of, int* cnt, int n, int cap, int G, double hc)
+// cell of droplet i and its place among the droplets of that cell (the order in which
+// the atomics were served; k_cell_order makes the final order deterministic)
+__global__ void k_cell_key(const double* pos, int* cell_of, int* rank, int* cnt, int n, int cap, int G, double hc)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -514,20 +521,20 @@ __global__ void k_cell_key(const double* pos, int* cell_of, int* cnt, int n, int
     }
     const int cell = cell_code(c3[0], c3[1], c3[2]);
     cell_of[i] = cell;
-    atomicAdd(cnt + cell, 1);
+    rank[i] = atomicAdd(cnt + cell, 1);
 }
 
 // Droplet i (read in slot order, so the reads coalesce) takes the next free place of
 // its cell and writes its index and its packed (x, y, z, radius) record there: the
 // pair searches then read each candidate cell as one contiguous run.
-__global__ void k_cell_fill(const int* __restrict__ cell_of, const int* __restrict__ start, int* cursor,
+__global__ void k_cell_fill(const int* __restrict__ cell_of, const int* __restrict__ rank, const int* __restrict__ start,
                             int* __restrict__ sorted, const double* __restrict__ pos, const double* __restrict__ rad,
                             double4* __restrict__ pk, int n, int cap)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int c = cell_of[i];
-    const int p = start[c] + atomicAdd(cursor + c, 1);
+    const int p = start[c] + rank[i];
     sorted[p] = i;
     pk[p] = make_double4(pos[i], pos[(size_t)cap + i], pos[2 * (size_t)cap + i], rad[i]);
 }
@@ -549,9 +556,43 @@ __global__ void k_cell_order(const int* __restrict__ start, int* __restrict__ so
     }
 }
 
+void invalidate_cells(PartBuffers& pb)
+{
+    pb.cells_valid = false;
+    pb.spare.valid = false;
+}
+
+static void swap_cells(PartBuffers& pb)
+{
+    PartBuffers::CellSet& o = pb.spare;
+    std::swap(pb.G, o.G);
+    std::swap(pb.hc, o.hc);
+    std::swap(pb.cell_count, o.cell_count);
+    std::swap(pb.cell_start, o.cell_start);
+    std::swap(pb.cell_of, o.cell_of);
+    std::swap(pb.sorted, o.sorted);
+    std::swap(pb.pk, o.pk);
+    std::swap(pb.cells_valid, o.valid);
+}
+
+static int alloc_spare(PartBuffers& pb)
+{
+    PartBuffers::CellSet& o = pb.spare;
+    if (o.cell_count) return 0;
+    const long long nmax = (long long)CELL_GMAX * CELL_GMAX * CELL_GMAX + 1;
+    const size_t c = (size_t)pb.cap;
+    DDNS_HIP(hipMalloc(&o.cell_count, nmax * sizeof(int)));
+    DDNS_HIP(hipMalloc(&o.cell_start, nmax * sizeof(int)));
+    DDNS_HIP(hipMalloc(&o.cell_of, c * sizeof(int)));
+    DDNS_HIP(hipMalloc(&o.sorted, c * sizeof(int)));
+    DDNS_HIP(hipMalloc(&o.pk, c * sizeof(double4)));
+    o.valid = false;
+    return 0;
+}
+
 int part_build_cells(PartBuffers& pb, const PartConst& pc, double hmin, hipStream_t s)
 {
-    constexpr int GMAX = 256;
+    constexpr int GMAX = CELL_GMAX;
     int G = (int)floor(pc.pi2 / hmin);
     if (G > GMAX) G = GMAX;
     if (G < 3) { set_error("domain too small for the pair search grid (need 3 cells of the interaction radius)"); return DDNS_ERR_DOMAIN; }
@@ -562,7 +603,6 @@ int part_build_cells(PartBuffers& pb, const PartConst& pc, double hmin, hipStrea
         const long long nmax = (long long)GMAX * GMAX * GMAX + 1;
         DDNS_HIP(hipMalloc(&pb.cell_count, nmax * sizeof(int)));
         DDNS_HIP(hipMalloc(&pb.cell_start, nmax * sizeof(int)));
-        DDNS_HIP(hipMalloc(&pb.cursor, nmax * sizeof(int)));
         const size_t need = scan_tmp_ints(std::max(nmax, (long long)pb.cap + 1)) * sizeof(int);
         if (need > pb.scan_tmp_bytes) {
             (void)hipFree(pb.scan_tmp);
@@ -570,20 +610,33 @@ int part_build_cells(PartBuffers& pb, const PartConst& pc, double hmin, hipStrea
             pb.scan_tmp_bytes = need;
         }
     }
+    if (pb.cells_valid && pb.G == G) return 0;                     // already built on these positions
+    if (pb.spare.valid && pb.spare.G == G) {                      // kept from an earlier build
+        swap_cells(pb);
+        return 0;
+    }
+    if (pb.cells_valid) {
+        // keep the current grid (another width) for a later search: build into the spare
+        int st = alloc_spare(pb);
+        if (st) return st;
+        swap_cells(pb);
+    }
+    pb.cells_valid = false;
     pb.G = G;
     pb.hc = pc.pi2 / G;
     DDNS_HIP(hipMemsetAsync(pb.cell_count, 0, (ncell + 1) * sizeof(int), s));
-    DDNS_HIP(hipMemsetAsync(pb.cursor, 0, (ncell + 1) * sizeof(int), s));
     if (pc.n > 0)
-        hipLaunchKernelGGL(k_cell_key, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.pos, pb.cell_of, pb.cell_count, pc.n, pb.cap, G, pb.hc);
+        hipLaunchKernelGGL(k_cell_key, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.pos, pb.cell_of, pb.cell_rank, pb.cell_count, pc.n,
+                           pb.cap, G, pb.hc);
     int st = exclusive_scan(pb.cell_count, pb.cell_start, (int)ncell, (int*)pb.scan_tmp, s);
     if (st) return st;
     if (pc.n > 0) {
-        hipLaunchKernelGGL(k_cell_fill, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.cell_of, pb.cell_start, pb.cursor, pb.sorted,
-                           pb.pos, pb.rad, pb.pk, pc.n, pb.cap);
+        hipLaunchKernelGGL(k_cell_fill, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.cell_of, pb.cell_rank, pb.cell_start,
+                           pb.sorted, pb.pos, pb.rad, pb.pk, pc.n, pb.cap);
         hipLaunchKernelGGL(k_cell_order, dim3(nblk(ncell)), dim3(PT), 0, s, pb.cell_start, pb.sorted, pb.pk, (int)ncell);
     }
     DDNS_HIP(hipGetLastError());
+    pb.cells_valid = true;
     return 0;
 }
 
