@@ -531,7 +531,7 @@ extern "C" int ddns_set_droplets(ddns_handle* h, int n, const double* yp, const 
         for (int c = 0; c < 3; ++c) {
             pos[(size_t)c * cap + i] = yp[8 * (size_t)i + c];
             vel[(size_t)c * cap + i] = yp[8 * (size_t)i + 3 + c];
-            if (pertvel) pv[(size_t)c * cap + i] = pertvel[3 * (size_t)i + c];
+            if (pertvel) pv[3 * (size_t)i + c] = pertvel[3 * (size_t)i + c];
             for (int t = 0; t < 3; ++t) if (fvp) fv[(size_t)(3 * t + c) * cap + i] = fvp[9 * (size_t)i + 3 * t + c];
             for (int t = 0; t < 4; ++t) if (fdp) fd[(size_t)(3 * t + c) * cap + i] = fdp[12 * (size_t)i + 3 * t + c];
         }
@@ -572,7 +572,7 @@ extern "C" int ddns_get_droplets(ddns_handle* h, int* n, double* yp, double* fvp
     for (int i = 0; i < m; ++i) {
         for (int c = 0; c < 3; ++c) {
             if (yp) { yp[8 * (size_t)i + c] = pos[(size_t)c * cap + i]; yp[8 * (size_t)i + 3 + c] = vel[(size_t)c * cap + i]; }
-            if (pertvel) pertvel[3 * (size_t)i + c] = pv[(size_t)c * cap + i];
+            if (pertvel) pertvel[3 * (size_t)i + c] = pv[3 * (size_t)i + c];
             if (vp) vp[3 * (size_t)i + c] = vv[(size_t)c * cap + i];
             if (dmove) dmove[3 * (size_t)i + c] = dm[(size_t)c * cap + i];
             for (int t = 0; t < 3; ++t) if (fvp) fvp[9 * (size_t)i + 3 * t + c] = fv[(size_t)(3 * t + c) * cap + i];
@@ -670,9 +670,11 @@ static int droplet_block(ddns_handle* h, float* ms)
     st = part_build_cells(h->pb, pc, cell_width(h), s);                       // HEADLIST (binning)
     if (st) return st;
     DDNS_HIP(hipEventRecord(h->ev[9], s));
-    st = part_interp(h->pb, pc, h->fb.uphys, s);                              // 553-556
-    if (st) return st;
+    // PART_HISTORY_FVP (564) reads no interpolated velocity, so it runs first: the
+    // interpolation then also gathers the HI inputs from the final droplet velocities
     st = part_history_fvp(h->pb, pc, s);                                      // 564
+    if (st) return st;
+    st = part_interp(h->pb, pc, h->fb.uphys, s);                              // 553-556
     if (st) return st;
     DDNS_HIP(hipEventRecord(h->ev[10], s));
     h->cc = ddns_collision_counts{};

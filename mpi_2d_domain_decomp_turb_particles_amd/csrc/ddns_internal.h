@@ -236,7 +236,8 @@ struct PartBuffers {
     double* fdp = nullptr;      // 12*cap  fdp(:,0:3) as [t][c][cap]
     double* dmove = nullptr;    // 3*cap
     double* vp = nullptr;       // 3*cap
-    double* pert = nullptr;     // 3*cap   pertvel
+    double* pert = nullptr;     // 3*cap   pertvel(3, n), droplet-major as the reference stores it, so one
+                                //         droplet's three components share a cache line
     // cell grid
     int G = 0;                  // coarse cells per dimension
     double hc = 0;
@@ -304,7 +305,8 @@ struct PartConst {
 
 int part_alloc(PartBuffers& pb, int cap, int nset, int G);
 void part_free(PartBuffers& pb);
-int part_interp(const PartBuffers& pb, const PartConst& pc, const void* uphys, hipStream_t s);
+// with HI on, also writes the HI solve's x0 and w in cell order (after part_history_fvp)
+int part_interp(PartBuffers& pb, const PartConst& pc, const void* uphys, hipStream_t s);
 int part_history_fvp(const PartBuffers& pb, const PartConst& pc, hipStream_t s);
 int part_build_cells(PartBuffers& pb, const PartConst& pc, double hmin, hipStream_t s);
 int part_hdi(PartBuffers& pb, const PartConst& pc, int maxiter, double tol, int* iters, double* res, hipStream_t s);

@@ -104,18 +104,6 @@ __global__ void __launch_bounds__(NT_NEI) k_nei(const double4* __restrict__ pk, 
     if ((threadIdx.x & 63) == 0 && km > width) atomicMax(flags + 4, km);
 }
 
-// x0 = pertvel and w = V - U, gathered into sorted order
-__global__ void k_gather(const int* __restrict__ sorted, const double* __restrict__ pert, const double* __restrict__ vel,
-                         const double* __restrict__ vp, double* __restrict__ x0, double* __restrict__ w, int n, int cap)
-{
-    const long long i = (long long)blockIdx.x * HT + threadIdx.x;
-    if (i >= 3LL * n) return;
-    const int c = (int)(i / n), t = (int)(i - (long long)c * n);
-    const size_t j = (size_t)c * cap + sorted[t];
-    x0[i] = pert[j];
-    w[i] = vel[j] - vp[j];
-}
-
 // y = x + M x (AFUN_PRECOMP).  MODE 0: x = x0 in double, y0d = A x0 and b = M w.
 // MODE 1: x is a basis vector.  MODE 2: x = v / h for the previous work vector v; the
 // row's own entries of x are also stored as the next basis vector (q = v / h,
@@ -455,7 +443,7 @@ __global__ void k_solution(const double* x0, const T* Q, long long ld, const dou
     double acc = 0.0;
     for (int j = 0; j < iters; ++j) acc = acc + (double)Q[(size_t)j * ld + i] * y[j];
     const int c = (int)(i / n), t = (int)(i - (long long)c * n);
-    pert[(size_t)c * cap + sorted[t]] = x0[i] + acc;
+    pert[3 * (size_t)sorted[t] + c] = x0[i] + acc;
 }
 
 // Small least-squares update of HDI_GMRES (2904-2971), one wavefront.
@@ -653,10 +641,8 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
     if (!pb.gm_small) {
         DDNS_HIP(hipMalloc(&pb.gm_small, 16384 * sizeof(double)));
         DDNS_HIP(hipMalloc(&pb.gm_red, (size_t)(128 + 2) * RED_BLOCKS * sizeof(double)));
-        DDNS_HIP(hipMalloc(&pb.gm_x0, 3 * (size_t)pb.cap * sizeof(double)));
         DDNS_HIP(hipMalloc(&pb.gm_b, 3 * (size_t)pb.cap * sizeof(double)));
         DDNS_HIP(hipMalloc(&pb.gm_v, 3 * (size_t)pb.cap * sizeof(double)));
-        DDNS_HIP(hipMalloc(&pb.gm_w, 3 * (size_t)pb.cap * sizeof(double)));
         DDNS_HIP(hipMalloc(&pb.gm_Q, (size_t)(maxiter + 2) * ((3LL * pb.cap + 63) / 64 * 64) * sizeof(T)));
     }
     double* part1 = pb.gm_red;                              // first-pass projections
@@ -671,8 +657,7 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
     const unsigned nb3 = (unsigned)((n3 + HT - 1) / HT);
     const unsigned nbr = (unsigned)((n + HT - 1) / HT);
     const unsigned nred = (unsigned)std::min<long long>(RED_BLOCKS, (n3 + 4LL * RT - 1) / (4LL * RT));   // reduction grid
-    hipLaunchKernelGGL(k_gather, dim3(nb3), dim3(HT), 0, s, (const int*)pb.sorted, (const double*)pb.pert,
-                       (const double*)pb.vel, (const double*)pb.vp, pb.gm_x0, pb.gm_w, n, pb.cap);
+    // x0 (pertvel) and w = V - U in cell order were written by the interpolation
     hipLaunchKernelGGL((k_spmv<T, 0>), dim3(nbr), dim3(HT), 0, s, (const int*)pb.nei_cnt, (const int*)pb.nei_idx,
                        (const T*)pb.nei_m, n, (const T*)nullptr, (const double*)nullptr, (T*)nullptr, (T*)nullptr,
                        (const double*)pb.gm_x0, ax0, (const double*)pb.gm_w, pb.gm_b, (const double*)nullptr);

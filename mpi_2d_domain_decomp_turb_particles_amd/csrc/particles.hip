@@ -192,10 +192,15 @@ __device__ __forceinline__ void load_node(const double* p, double& a, double& b,
 constexpr int IG = 6;              // lanes per droplet
 constexpr int IPW = 64 / IG;       // droplets per wavefront
 
+// With HI on it also writes, in cell order, the inputs of the HI solve: the warm start
+// x0 = pertvel and the velocity difference w = V - U, so the solve reads them
+// contiguously.
 template <typename T>
 __global__ void __launch_bounds__(PT) k_interp(const T* __restrict__ u, int N, double pi2,
                                                const double* __restrict__ pos, const int* __restrict__ order,
-                                               double* __restrict__ vp, int n, int cap)
+                                               const double4* __restrict__ pk, double* __restrict__ vp, int n, int cap,
+                                               const double* __restrict__ vel, const double* __restrict__ pert,
+                                               double* __restrict__ x0, double* __restrict__ w)
 {
     const int lane = threadIdx.x & 63;
     const int wave = (xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
@@ -207,7 +212,13 @@ __global__ void __launch_bounds__(PT) k_interp(const T* __restrict__ u, int N, d
     if (active) {
         const double dx = pi2 / (double)N;
         double wx[6], wy[6], wz[6];
-        const double y0 = pos[i], y1 = pos[(size_t)cap + i], y2 = pos[2 * (size_t)cap + i];
+        double y0, y1, y2;
+        if (pk) {                         // the packed record of the cell order holds the same values
+            const double4 r = pk[t];
+            y0 = r.x; y1 = r.y; y2 = r.z;
+        } else {
+            y0 = pos[i]; y1 = pos[(size_t)cap + i]; y2 = pos[2 * (size_t)cap + i];
+        }
         const int nx = (int)floor(y0 / dx), ny = (int)floor(y1 / dx), nz = (int)floor(y2 / dx);
         lagrange6((y0 - (double)nx * dx) / dx, wx);
         lagrange6((y1 - (double)ny * dx) / dx, wy);
@@ -245,23 +256,43 @@ __global__ void __launch_bounds__(PT) k_interp(const T* __restrict__ u, int N, d
         s0 += p0; s1 += p1; s2 += p2;
     }
     if (active && a == 0) {
+        const size_t C = (size_t)cap, M = (size_t)n;
         vp[i] = s0;
-        vp[(size_t)cap + i] = s1;
-        vp[2 * (size_t)cap + i] = s2;
+        vp[C + i] = s1;
+        vp[2 * C + i] = s2;
+        if (x0) {
+            const double s3[3] = {s0, s1, s2};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                x0[c * M + t] = pert[3 * (size_t)i + c];
+                w[c * M + t] = vel[c * C + i] - s3[c];
+            }
+        }
     }
 }
 
-int part_interp(const PartBuffers& pb, const PartConst& pc, const void* uphys, hipStream_t s)
+int part_interp(PartBuffers& pb, const PartConst& pc, const void* uphys, hipStream_t s)
 {
     if (pc.n == 0) return 0;
     const int* order = pb.G > 0 ? pb.sorted : nullptr;
+    const double4* pk = order ? pb.pk : nullptr;
+    if (pc.hdi) {
+        // the HI solve works in the cell order of this grid
+        if (!order) { set_error("HI needs the cell grid before the interpolation"); return DDNS_ERR_ARG; }
+        if (!pb.gm_x0) {
+            DDNS_HIP(hipMalloc(&pb.gm_x0, 3 * (size_t)pb.cap * sizeof(double)));
+            DDNS_HIP(hipMalloc(&pb.gm_w, 3 * (size_t)pb.cap * sizeof(double)));
+        }
+    }
+    double* x0 = pc.hdi ? pb.gm_x0 : nullptr;
+    double* w = pc.hdi ? pb.gm_w : nullptr;
     const unsigned nb = (unsigned)(((long long)pc.n + IPW * (PT / 64) - 1) / (IPW * (PT / 64)));
     if (pc.precision == DDNS_FP64)
         hipLaunchKernelGGL(k_interp<double>, dim3(nb), dim3(PT), 0, s, (const double*)uphys, pc.N, pc.pi2,
-                           pb.pos, order, pb.vp, pc.n, pb.cap);
+                           pb.pos, order, pk, pb.vp, pc.n, pb.cap, (const double*)pb.vel, (const double*)pb.pert, x0, w);
     else
         hipLaunchKernelGGL(k_interp<float>, dim3(nb), dim3(PT), 0, s, (const float*)uphys, pc.N, pc.pi2,
-                           pb.pos, order, pb.vp, pc.n, pb.cap);
+                           pb.pos, order, pk, pb.vp, pc.n, pb.cap, (const double*)pb.vel, (const double*)pb.pert, x0, w);
     DDNS_HIP(hipGetLastError());
     return 0;
 }
@@ -309,7 +340,7 @@ __global__ void k_advance(double* vel, const double* rad, double* fvp, double* f
     double f0[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        f0[c] = hdi ? vp[c * C + i] + pert[c * C + i] : vp[c * C + i];
+        f0[c] = hdi ? vp[c * C + i] + pert[3 * (size_t)i + c] : vp[c * C + i];
         fdp[c * C + i] = f0[c];
         if (newp) { fdp[(3 + c) * C + i] = f0[c]; fdp[(6 + c) * C + i] = f0[c]; fdp[(9 + c) * C + i] = f0[c]; }
     }
@@ -319,7 +350,7 @@ __global__ void k_advance(double* vel, const double* rad, double* fvp, double* f
         double d[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c)
-            d[c] = hdi ? vel[c * C + i] - vp[c * C + i] - pert[c * C + i] : vel[c * C + i] - vp[c * C + i];
+            d[c] = hdi ? vel[c * C + i] - vp[c * C + i] - pert[3 * (size_t)i + c] : vel[c * C + i] - vp[c * C + i];
         const double Vrel = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
         FSN = 1. + 0.15 * pow(Vrel * 2. * a / rnu, 0.687);
     }
