@@ -129,6 +129,10 @@ struct CollArgs {
     int pair_cap;
     int* mult;
     const unsigned long long* dmax;   // bits of the largest |dmove| of this step
+    int2* cand;            // pairs handed from the search to k_collide_pairs
+    int cand_cap;
+    int* ncand;
+    int* hit;              // per droplet: ih1 of a collision already counted
 };
 
 // the largest displacement length |dmove| of the step, as the bits of a non-negative
@@ -156,16 +160,177 @@ __global__ void __launch_bounds__(256) k_dmove_max(const double* __restrict__ dm
     }
 }
 
+// The reference's work on one pair (ih1 = droplet ia, ih2 = droplet ib or one of its
+// images, at distance dnij0 with offset xd = xa - xb): the statistics of
+// partlib.f90:1358-1441 and the collision tests and record of 1445-1562.
+__device__ __forceinline__ void collide_pair(const CollArgs& a, int ia, const double (&xa)[3], double ra, double dmove_a,
+                                             int ib, const double4& cb, int wy, int wz, const double (&xb)[3],
+                                             const double (&xd)[3], double dnij0, int di, int dj, int dk,
+                                             long long& instat, int& ncollip1)
+{
+    const size_t C = (size_t)a.cap;
+    const int nbin = DDNS_NBIN, ns = a.nset;
+    const double rb = cb.w;
+    const double rcoll = ra + rb;
+    const double rcolla = a.hdi ? rcoll : rcoll * (1. - a.st / 100.);
+    const double rcollb = rcoll * (1. + a.st / 100.);
+    if (dnij0 > rcolla && dnij0 <= 10 * rcoll) {      // 1358-1441
+        ++instat;
+        int i1 = 1, i2 = 1;
+        if (dnij0 > rcoll) {
+            double va[3], vb[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                va[c] = a.fvp0[c * C + ia];
+                vb[c] = a.fvp0[c * C + ib];
+            }
+            double xh[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) xh[c] = xd[c] / dnij0;
+            const double Vr1 = va[0] * xh[0] + va[1] * xh[1] + va[2] * xh[2];
+            const double Vr2 = vb[0] * xh[0] + vb[1] * xh[1] + vb[2] * xh[2];
+            const double Wr = Vr1 - Vr2;
+            int ir = (int)((dnij0 - rcoll) / ((9. / (double)nbin) * rcoll));
+            if (ir > nbin - 1) ir = nbin - 1;
+            if (ir < 0) ir = 0;
+            findindex(a.radius, ns, ra, rb, &i1, &i2);
+            const int bi = (i1 - 1) + ns * ((i2 - 1) + ns * ir);
+            const size_t nb = (size_t)ns * ns * nbin;
+            atomicAdd(a.nirbin + bi, 1);
+            if (i2 == i1) {
+                atomicAdd(a.bins + 1 * nb + bi, va[0] - vb[0]);
+                atomicAdd(a.bins + 2 * nb + bi, va[1] - vb[1]);
+                atomicAdd(a.bins + 3 * nb + bi, va[2] - vb[2]);
+            }
+            if (ra > rb) {
+                atomicAdd(a.bins + 1 * nb + bi, va[0] - vb[0]);
+                atomicAdd(a.bins + 2 * nb + bi, va[1] - vb[1]);
+                atomicAdd(a.bins + 3 * nb + bi, va[2] - vb[2]);
+            } else if (ra < rb) {
+                atomicAdd(a.bins + 1 * nb + bi, vb[0] - va[0]);
+                atomicAdd(a.bins + 2 * nb + bi, vb[1] - va[1]);
+                atomicAdd(a.bins + 3 * nb + bi, vb[2] - va[2]);
+            }
+            atomicAdd(a.bins + 0 * nb + bi, Wr);
+            const double du0 = a.vp[ia] - a.vp[ib], du1 = a.vp[C + ia] - a.vp[C + ib], du2 = a.vp[2 * C + ia] - a.vp[2 * C + ib];
+            atomicAdd(a.bins + 4 * nb + bi, du0 * xh[0] + du1 * xh[1] + du2 * xh[2]);
+            const double aaa = (va[0] - vb[0]) * xh[0] + (va[1] - vb[1]) * xh[1] + (va[2] - vb[2]) * xh[2];
+            const double bbb = sqrt((va[0] - vb[0]) * (va[0] - vb[0]) + (va[1] - vb[1]) * (va[1] - vb[1]) +
+                                    (va[2] - vb[2]) * (va[2] - vb[2]));
+            if (bbb != 0.) {
+                const double q = fabs(aaa / bbb);
+                const double vangle = acos(q);
+                atomicAdd(a.bins + 5 * nb + bi, vangle);
+                // sin(acos q) = sqrt(1 - q^2) for q in [0, 1]: the same value
+                // as the reference's sin(vangle) (partlib.f90:1400) to rounding,
+                // without the register cost of a general double sine
+                const double sb = bbb * sqrt(fmax(0.0, 1.0 - q * q));
+                atomicAdd(a.bins + 6 * nb + bi, sb * sb / dnij0);
+            }
+        }
+        if (dnij0 <= rcollb) {
+            if (dnij0 <= rcoll) findindex(a.radius, ns, ra, rb, &i1, &i2);
+            atomicAdd(a.counts + 108 + (i1 - 1) + DDNS_MAXSET * (i2 - 1), 1);
+        }
+    }
+    if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) return;   // 1445
+    // -------- coarse check and types I/II/III, 1451-1516
+    double bx1[3], xbn[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        bx1[c] = a.dmove[c * C + ib];
+        xbn[c] = xb[c] + bx1[c];
+    }
+    const double dmove_b = sqrt(bx1[0] * bx1[0] + bx1[1] * bx1[1] + bx1[2] * bx1[2]);
+    const double tmove = rcoll + dmove_a + dmove_b;
+    if (dnij0 > tmove) return;
+    double ax1[3], xan[3], xe[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        ax1[c] = a.dmove[c * C + ia];
+        xan[c] = xa[c] + ax1[c];
+        xe[c] = xan[c] - xbn[c];
+    }
+    const double dnij1 = sqrt(xe[0] * xe[0] + xe[1] * xe[1] + xe[2] * xe[2]);
+    if (dnij0 < rcoll && dnij1 > rcoll) return;
+    int type = 0;
+    if (dnij0 > rcoll && dnij1 <= rcoll) {
+        type = 1;
+    } else {
+        double vbn[3], vb[3], va[3], vna[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            vbn[c] = a.vel[c * C + ib];
+            vb[c] = a.fvp0[c * C + ib];
+            va[c] = a.fvp0[c * C + ia];
+            vna[c] = a.vel[c * C + ia];
+        }
+        const double dt = a.dt;
+        #pragma unroll 1
+        for (int icheck = 1; icheck <= 19 && type == 0; ++icheck) {
+            const double tfac = 0.05 * (double)icheck;
+            const double tfac1 = dt * tfac, tfac2 = tfac * tfac, tfac3 = tfac2 * tfac;
+            double xt[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double xta = xa[c] + va[c] * tfac1 + (3. * ax1[c] - (2. * va[c] + vna[c]) * dt) * tfac2 +
+                                   ((vna[c] + va[c]) * dt - 2. * ax1[c]) * tfac3;
+                const double xtb = xb[c] + vb[c] * tfac1 + (3. * bx1[c] - (2. * vb[c] + vbn[c]) * dt) * tfac2 +
+                                   ((vbn[c] + vb[c]) * dt - 2. * bx1[c]) * tfac3;
+                xt[c] = xta - xtb;
+            }
+            const double dnij = sqrt(xt[0] * xt[0] + xt[1] * xt[1] + xt[2] * xt[2]);
+            if (dnij0 > rcoll && dnij < rcoll) type = 2;
+            else if (dnij1 <= rcoll && dnij > rcoll) type = 3;
+        }
+    }
+    if (type == 0) return;
+    // -------- count and record, 1518-1562
+    ++ncollip1;
+    atomicAdd(a.counts + (type - 1), 1);
+    int i1, i2;
+    findindex(a.radius, ns, ra, rb, &i1, &i2);
+    atomicAdd(a.counts + 8 + (i1 - 1) + DDNS_MAXSET * (i2 - 1), 1);
+    atomicAdd(a.mult + ib, 1);
+    const int slot = atomicAdd(a.counts + 256, 1);
+    if (slot < a.pair_cap) {
+        // the collist record (1555-1562) plus where the reference meets
+        // the pair: ih1, the cell offset (i, j, k) of its loop and the
+        // partner's ypn index (halo images follow the locals, by HEADLIST
+        // direction 5..8, then by droplet index)
+        const long long ga = (long long)a.gid[ia], gb = (long long)a.gid[ib];
+        const long long lo = ga < gb ? ga : gb, hi = ga < gb ? gb : ga;
+        const int dir = (wy == 0 && wz == 0) ? 0 : (wz == 0 ? 5 : (wy == 1 ? 6 : (wy == 0 ? 7 : 8)));
+        const long long ypn = (long long)dir * a.cap + ib;
+        const int off = ((di + 1) * 3 + (dj + 1)) * 3 + (dk + 1);
+        long long* r = a.pairs + DDNS_PAIR_REC * (size_t)slot;
+        r[0] = (lo << 32) | (hi << 2) | type;
+        r[1] = ia;
+        r[2] = ib;
+        r[3] = ((long long)(di + 8) << 16) | ((dj + 8) << 8) | (dk + 8);
+        r[4] = dir;
+        r[5] = __double_as_longlong(ra);
+        r[6] = __double_as_longlong(rb);
+        // visiting order: ih1 ascending, then the (i, j, k) loop, then the
+        // linked list of the cell, which runs from high to low ypn index
+        r[7] = ((long long)ia << 32) | ((long long)off << 27) | (((1LL << 27) - 1) - ypn);
+    }
+}
+
+// Pair search, the first half of COLLISION_DET (partlib.f90:1166-1846): one lane per
+// droplet ih1 walks the candidates, replays the reference's visiting rules and
+// counts the pairs it tests.  Only the few pairs that can reach the statistics or
+// the collision tests are handed on (cell-order position of ih1, of the partner and
+// the neighbour-cell number), so the heavy, rarely taken branches do not stall
+// whole wavefronts here.
 __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
 {
     const int t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
     if (t >= a.n) return;
     const int ia = a.sorted[t];
     const size_t C = (size_t)a.cap;
     const PairGeom& g = a.g;
-    const double h = g.h;
-    // only the position, radius and displacement length stay in registers; the
-    // velocities and the displacement itself are read in the rare branches that use them
     double xa[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) xa[c] = a.pos[c * C + ia];
@@ -181,194 +346,96 @@ __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
         int q = (int)(xa[c] / g.hc);
         cc[c] = q < 0 ? 0 : (q >= g.G ? g.G - 1 : q);
     }
-    int ncollip1 = 0;
-    long long tested = 0, instat = 0;
-    const int nbin = DDNS_NBIN, ns = a.nset;
+    long long tested = 0;
     // rcoll + dmove_a + dmove_b <= rcoll + dmove_a + dmax: a pair farther apart than the
-    // right side cannot pass the time-window test below, so its partner's displacement
-    // is not loaded
+    // right side cannot pass the collision time-window test
     const double dmax = __longlong_as_double((long long)*a.dmax);
-    __shared__ int2 rng[27 * CT];
-    neighbour_ranges(a.cell_start, g.G, cc, rng + threadIdx.x, CT);
-    CandCursor cur;
-    bool more = cand_next(cur, rng + threadIdx.x, CT);
-    int ib_next = 0;
-    double4 cb_next = make_double4(0.0, 0.0, 0.0, 0.0);
-    if (more) { ib_next = a.sorted[cur.p]; cb_next = a.pk[cur.p]; }
-#pragma unroll 1
-    while (more) {
-        const int p = cur.p, kc = cur.kc;
-        const int ib = ib_next;
-        const double4 cb = cb_next;
-        more = cand_next(cur, rng + threadIdx.x, CT);
-        if (more) { ib_next = a.sorted[cur.p]; cb_next = a.pk[cur.p]; }   // the next candidate's loads in flight
-        if (p == t) continue;
+    for_each_candidate(a.cell_start, g.G, cc, a.pk, a.sorted, [&](int p, int kc, const double4& cb, int ib) {
+        if (p == t) return;
         int wx, wy, wz;
         neighbour_shift(kc, g.G, cc, &wx, &wy, &wz);
         double xb[3];
         int di, dj, dk;
         if (!ref_visit(g, ia, ib, xa[0], xa[1], xa[2], cb.x, cb.y, cb.z,
                        wx, wy, wz, &xb[0], &xb[1], &xb[2], &di, &dj, &dk))
-            continue;
-        // -------- partlib.f90:1330-1353
+            return;
         double xd[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) xd[c] = xa[c] - xb[c];
         const double dnij0 = sqrt(xd[0] * xd[0] + xd[1] * xd[1] + xd[2] * xd[2]);
         ++tested;
-        const double rb = cb.w;
-        const double rcoll = ra + rb;
+        const double rcoll = ra + cb.w;
         const double rcolla = a.hdi ? rcoll : rcoll * (1. - a.st / 100.);
-        const double rcollb = rcoll * (1. + a.st / 100.);
-        if (dnij0 > rcolla && dnij0 <= 10 * rcoll) {      // 1358-1441
-            ++instat;
-            int i1 = 1, i2 = 1;
-            if (dnij0 > rcoll) {
-                double va[3], vb[3];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    va[c] = a.fvp0[c * C + ia];
-                    vb[c] = a.fvp0[c * C + ib];
-                }
-                double xh[3];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) xh[c] = xd[c] / dnij0;
-                const double Vr1 = va[0] * xh[0] + va[1] * xh[1] + va[2] * xh[2];
-                const double Vr2 = vb[0] * xh[0] + vb[1] * xh[1] + vb[2] * xh[2];
-                const double Wr = Vr1 - Vr2;
-                int ir = (int)((dnij0 - rcoll) / ((9. / (double)nbin) * rcoll));
-                if (ir > nbin - 1) ir = nbin - 1;
-                if (ir < 0) ir = 0;
-                findindex(a.radius, ns, ra, rb, &i1, &i2);
-                const int bi = (i1 - 1) + ns * ((i2 - 1) + ns * ir);
-                const size_t nb = (size_t)ns * ns * nbin;
-                atomicAdd(a.nirbin + bi, 1);
-                if (i2 == i1) {
-                    atomicAdd(a.bins + 1 * nb + bi, va[0] - vb[0]);
-                    atomicAdd(a.bins + 2 * nb + bi, va[1] - vb[1]);
-                    atomicAdd(a.bins + 3 * nb + bi, va[2] - vb[2]);
-                }
-                if (ra > rb) {
-                    atomicAdd(a.bins + 1 * nb + bi, va[0] - vb[0]);
-                    atomicAdd(a.bins + 2 * nb + bi, va[1] - vb[1]);
-                    atomicAdd(a.bins + 3 * nb + bi, va[2] - vb[2]);
-                } else if (ra < rb) {
-                    atomicAdd(a.bins + 1 * nb + bi, vb[0] - va[0]);
-                    atomicAdd(a.bins + 2 * nb + bi, vb[1] - va[1]);
-                    atomicAdd(a.bins + 3 * nb + bi, vb[2] - va[2]);
-                }
-                atomicAdd(a.bins + 0 * nb + bi, Wr);
-                const double du0 = a.vp[ia] - a.vp[ib], du1 = a.vp[C + ia] - a.vp[C + ib], du2 = a.vp[2 * C + ia] - a.vp[2 * C + ib];
-                atomicAdd(a.bins + 4 * nb + bi, du0 * xh[0] + du1 * xh[1] + du2 * xh[2]);
-                const double aaa = (va[0] - vb[0]) * xh[0] + (va[1] - vb[1]) * xh[1] + (va[2] - vb[2]) * xh[2];
-                const double bbb = sqrt((va[0] - vb[0]) * (va[0] - vb[0]) + (va[1] - vb[1]) * (va[1] - vb[1]) +
-                                        (va[2] - vb[2]) * (va[2] - vb[2]));
-                if (bbb != 0.) {
-                    const double q = fabs(aaa / bbb);
-                    const double vangle = acos(q);
-                    atomicAdd(a.bins + 5 * nb + bi, vangle);
-                    // sin(acos q) = sqrt(1 - q^2) for q in [0, 1]: the same value
-                    // as the reference's sin(vangle) (partlib.f90:1400) to rounding,
-                    // without the register cost of a general double sine
-                    const double sb = bbb * sqrt(fmax(0.0, 1.0 - q * q));
-                    atomicAdd(a.bins + 6 * nb + bi, sb * sb / dnij0);
-                }
-            }
-            if (dnij0 <= rcollb) {
-                if (dnij0 <= rcoll) findindex(a.radius, ns, ra, rb, &i1, &i2);
-                atomicAdd(a.counts + 108 + (i1 - 1) + DDNS_MAXSET * (i2 - 1), 1);
+        const bool stat = dnij0 > rcolla && dnij0 <= 10 * rcoll;                         // 1358
+        const bool near = !(di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) &&   // 1445
+                          !(dnij0 > rcoll + dmove_a + dmax);
+        const bool emit = stat || near;
+        // one atomic per wavefront for the slots of its emitting lanes
+        const unsigned long long m = __ballot(emit);
+        if (m) {
+            const int leader = __ffsll((long long)m) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(a.ncand, __popcll(m));
+            base = __shfl(base, leader, 64);
+            if (emit) {
+                const int slot = base + __popcll(m & ((1ull << lane) - 1));
+                if (slot < a.cand_cap) a.cand[slot] = make_int2(t, (int)((unsigned)p | ((unsigned)kc << 27)));
             }
         }
-        if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) continue;   // 1445
-        if (dnij0 > rcoll + dmove_a + dmax) continue;
-        // -------- coarse check and types I/II/III, 1451-1516
-        double bx1[3], xbn[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            bx1[c] = a.dmove[c * C + ib];
-            xbn[c] = xb[c] + bx1[c];
-        }
-        const double dmove_b = sqrt(bx1[0] * bx1[0] + bx1[1] * bx1[1] + bx1[2] * bx1[2]);
-        const double tmove = rcoll + dmove_a + dmove_b;
-        if (dnij0 > tmove) continue;
-        double ax1[3], xan[3], xe[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            ax1[c] = a.dmove[c * C + ia];
-            xan[c] = xa[c] + ax1[c];
-            xe[c] = xan[c] - xbn[c];
-        }
-        const double dnij1 = sqrt(xe[0] * xe[0] + xe[1] * xe[1] + xe[2] * xe[2]);
-        if (dnij0 < rcoll && dnij1 > rcoll) continue;
-        int type = 0;
-        if (dnij0 > rcoll && dnij1 <= rcoll) {
-            type = 1;
-        } else {
-            double vbn[3], vb[3], va[3], vna[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                vbn[c] = a.vel[c * C + ib];
-                vb[c] = a.fvp0[c * C + ib];
-                va[c] = a.fvp0[c * C + ia];
-                vna[c] = a.vel[c * C + ia];
-            }
-            const double dt = a.dt;
-            #pragma unroll 1
-            for (int icheck = 1; icheck <= 19 && type == 0; ++icheck) {
-                const double tfac = 0.05 * (double)icheck;
-                const double tfac1 = dt * tfac, tfac2 = tfac * tfac, tfac3 = tfac2 * tfac;
-                double xt[3];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const double xta = xa[c] + va[c] * tfac1 + (3. * ax1[c] - (2. * va[c] + vna[c]) * dt) * tfac2 +
-                                       ((vna[c] + va[c]) * dt - 2. * ax1[c]) * tfac3;
-                    const double xtb = xb[c] + vb[c] * tfac1 + (3. * bx1[c] - (2. * vb[c] + vbn[c]) * dt) * tfac2 +
-                                       ((vbn[c] + vb[c]) * dt - 2. * bx1[c]) * tfac3;
-                    xt[c] = xta - xtb;
-                }
-                const double dnij = sqrt(xt[0] * xt[0] + xt[1] * xt[1] + xt[2] * xt[2]);
-                if (dnij0 > rcoll && dnij < rcoll) type = 2;
-                else if (dnij1 <= rcoll && dnij > rcoll) type = 3;
-            }
-        }
-        if (type == 0) continue;
-        // -------- count and record, 1518-1562
-        ++ncollip1;
-        atomicAdd(a.counts + (type - 1), 1);
-        int i1, i2;
-        findindex(a.radius, ns, ra, rb, &i1, &i2);
-        atomicAdd(a.counts + 8 + (i1 - 1) + DDNS_MAXSET * (i2 - 1), 1);
-        atomicAdd(a.mult + ib, 1);
-        const int slot = atomicAdd(a.counts + 256, 1);
-        if (slot < a.pair_cap) {
-            // the collist record (1555-1562) plus where the reference meets
-            // the pair: ih1, the cell offset (i, j, k) of its loop and the
-            // partner's ypn index (halo images follow the locals, by HEADLIST
-            // direction 5..8, then by droplet index)
-            const long long ga = (long long)a.gid[ia], gb = (long long)a.gid[ib];
-            const long long lo = ga < gb ? ga : gb, hi = ga < gb ? gb : ga;
-            const int dir = (wy == 0 && wz == 0) ? 0 : (wz == 0 ? 5 : (wy == 1 ? 6 : (wy == 0 ? 7 : 8)));
-            const long long ypn = (long long)dir * a.cap + ib;
-            const int off = ((di + 1) * 3 + (dj + 1)) * 3 + (dk + 1);
-            long long* r = a.pairs + DDNS_PAIR_REC * (size_t)slot;
-            r[0] = (lo << 32) | (hi << 2) | type;
-            r[1] = ia;
-            r[2] = ib;
-            r[3] = ((long long)(di + 8) << 16) | ((dj + 8) << 8) | (dk + 8);
-            r[4] = dir;
-            r[5] = __double_as_longlong(ra);
-            r[6] = __double_as_longlong(rb);
-            // visiting order: ih1 ascending, then the (i, j, k) loop, then the
-            // linked list of the cell, which runs from high to low ypn index
-            r[7] = ((long long)ia << 32) | ((long long)off << 27) | (((1LL << 27) - 1) - ypn);
-        }
-    }
-    if (ncollip1 != 0) {                                              // 1578-1583
-        atomicAdd(a.mult + ia, 1);
-        atomicAdd(a.counts + 257, 1);
-    }
+    });
     if (tested) atomicAdd((unsigned long long*)a.tested, (unsigned long long)tested);
-    if (instat) atomicAdd((unsigned long long*)(a.tested + 1), (unsigned long long)instat);
+}
+
+// Second half: one thread per handed-on pair redoes the pair's geometry exactly as the
+// search did and runs the statistics and collision tests.
+__global__ void __launch_bounds__(256) k_collide_pairs(CollArgs a, int ncand)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    long long instat = 0;
+    if (r < ncand) {
+        const int2 rec = a.cand[r];
+        const int t = rec.x, p = rec.y & ((1 << 27) - 1), kc = (int)((unsigned)rec.y >> 27);   // kc < 27: 5 bits
+        const int ia = a.sorted[t];
+        const size_t C = (size_t)a.cap;
+        const PairGeom& g = a.g;
+        double xa[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xa[c] = a.pos[c * C + ia];
+        const double ra = a.rad[ia];
+        double dmove_a;
+        {
+            const double d0 = a.dmove[ia], d1 = a.dmove[C + ia], d2 = a.dmove[2 * C + ia];
+            dmove_a = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        }
+        int cc[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            int q = (int)(xa[c] / g.hc);
+            cc[c] = q < 0 ? 0 : (q >= g.G ? g.G - 1 : q);
+        }
+        int wx, wy, wz;
+        neighbour_shift(kc, g.G, cc, &wx, &wy, &wz);
+        const int ib = a.sorted[p];
+        const double4 cb = a.pk[p];
+        double xb[3];
+        int di, dj, dk;
+        if (ref_visit(g, ia, ib, xa[0], xa[1], xa[2], cb.x, cb.y, cb.z, wx, wy, wz, &xb[0], &xb[1], &xb[2], &di, &dj, &dk)) {
+            double xd[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) xd[c] = xa[c] - xb[c];
+            const double dnij0 = sqrt(xd[0] * xd[0] + xd[1] * xd[1] + xd[2] * xd[2]);
+            int ncollip1 = 0;
+            collide_pair(a, ia, xa, ra, dmove_a, ib, cb, wy, wz, xb, xd, dnij0, di, dj, dk, instat, ncollip1);
+            // ih1 counts once among the droplets that collided (1578-1583)
+            if (ncollip1 != 0 && atomicExch(a.hit + ia, 1) == 0) {
+                atomicAdd(a.mult + ia, 1);
+                atomicAdd(a.counts + 257, 1);
+            }
+        }
+    }
+    const unsigned long long m = __ballot(instat != 0);
+    if (m && (threadIdx.x & 63) == __ffsll((long long)m) - 1)
+        atomicAdd((unsigned long long*)(a.tested + 1), (unsigned long long)__popcll(m));
 }
 
 // OVERLAP_DET sweep (2076-2157): count, per droplet as ih1, the overlapping pairs
@@ -460,7 +527,32 @@ int part_collide(PartBuffers& pb, const PartConst& pc, hipStream_t s)
     hipLaunchKernelGGL(k_dmove_max, dim3(std::min(256, (pc.n + 255) / 256)), dim3(256), 0, s, (const double*)pb.dmove, pc.n,
                        pb.cap, dmax);
     a.dmax = dmax;
-    hipLaunchKernelGGL(k_collide, dim3((pc.n + CT - 1) / CT), dim3(CT), 0, s, a);
+    if (!pb.hit) DDNS_HIP(hipMalloc(&pb.hit, (size_t)pb.cap * sizeof(int)));
+    if (!pb.cand) {
+        pb.cand_cap = std::max(1 << 16, 2 * pb.cap);
+        DDNS_HIP(hipMalloc(&pb.cand, (size_t)pb.cand_cap * sizeof(int2)));
+    }
+    DDNS_HIP(hipMemsetAsync(pb.hit, 0, (size_t)pc.n * sizeof(int), s));
+    a.hit = pb.hit;
+    a.ncand = pb.flags + 40;
+    int ncand = 0;
+    for (;;) {
+        a.cand = (int2*)pb.cand;
+        a.cand_cap = pb.cand_cap;
+        DDNS_HIP(hipMemsetAsync(a.ncand, 0, sizeof(int), s));
+        hipLaunchKernelGGL(k_collide, dim3((pc.n + CT - 1) / CT), dim3(CT), 0, s, a);
+        DDNS_HIP(hipGetLastError());
+        DDNS_HIP(hipMemcpyAsync(&ncand, a.ncand, sizeof(int), hipMemcpyDeviceToHost, s));
+        DDNS_HIP(hipStreamSynchronize(s));
+        if (ncand <= pb.cand_cap) break;
+        // more pairs to hand on than the buffer holds: widen it and search again
+        DDNS_HIP(hipFree(pb.cand));
+        pb.cand_cap = ncand + ncand / 2;
+        DDNS_HIP(hipMalloc(&pb.cand, (size_t)pb.cand_cap * sizeof(int2)));
+        DDNS_HIP(hipMemsetAsync(pb.tested, 0, sizeof(long long), s));
+    }
+    if (ncand > 0)
+        hipLaunchKernelGGL(k_collide_pairs, dim3((ncand + 255) / 256), dim3(256), 0, s, a, ncand);
     DDNS_HIP(hipGetLastError());
     return 0;
 }

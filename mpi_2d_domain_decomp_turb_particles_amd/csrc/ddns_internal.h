@@ -273,6 +273,9 @@ struct PartBuffers {
     double* gm_small = nullptr; // Hessenberg etc
     // collisions
     int* mult = nullptr;        // cap: removal-list multiplicity
+    void* cand = nullptr;       // cand_cap int2: pairs handed from the collision search to the pair kernel
+    int cand_cap = 0;
+    int* hit = nullptr;         // cap: droplet already counted as ih1 of a collision
     int* flags = nullptr;       // device flags / counters (64 ints)
     long long* tested = nullptr;
     int* nirbin = nullptr;
