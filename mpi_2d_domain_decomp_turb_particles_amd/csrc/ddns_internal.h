@@ -194,7 +194,7 @@ struct FlowBuffers {
     void* uphys = nullptr;                      // [z][y][x][4]: u, v, w, unused
     void* tw = nullptr;                         // N twiddles exp(-2 pi i m / N)
     void* twh = nullptr;                        // N/2 twiddles for the half-length x FFT
-    double* low = nullptr;                      // pre-kernel scratch: ff[2], energies, forcing table
+    double* low = nullptr;                      // pre-kernel scratch (1024): ff[2], energies, forcing table, DFT sums
     int* sth_state = nullptr;                   // ranff state on the device: iseedf, iyf, ivf[32]
     double* sth_b = nullptr;                    // b1r..b3r (450)
 };

@@ -579,6 +579,8 @@ __global__ void __launch_bounds__(NT) k_x_plain(const void* src, void* dst, cons
 // energies (kx = 0 halved, flowlib.f90:207-217) give ff = sqrt(force/E); low[0..1].
 // STH (forcing = 2): ranff white noise, OU step, projection and Hermitian fix-up
 // (flowlib.f90:399-510) into low[8 .. 8+450).
+constexpr int LOW_DFT = 512;   // offset of the supforDET z-DFT sums in FlowBuffers::low
+
 struct PreArgs {
     const void* B[3];
     const void* U[3];
@@ -623,6 +625,44 @@ __device__ double dev_ranff(int* idum, int* iy, int* iv)
     iv[j - 1] = *idum;
     const double r = AM * (double)(*iy);
     return r < RNMX ? r : RNMX;
+}
+
+// supforDET's 225 low-mode sums (75 modes kx in 0..2, ky, kz in -2..2, times 3
+// components) of the forward z transform of N-hat, by direct DFT: one work-group per
+// sum, lanes over z, a fixed-order reduction; the result, scaled by 1/N like the
+// forward FFT, goes to low[LOW_DFT ...] for k_pre.
+template <typename T, int N>
+__global__ void __launch_bounds__(NT) k_pre_dft(PreArgs a)
+{
+    using C = cpx<T>;
+    constexpr int M = N / 2;
+    const int id = blockIdx.x, comp = id / 75, mi = id % 75;
+    const int kx = mi % 3, kyw = (mi / 3) % 5 - 2, kzw = mi / 15 - 2;
+    const int ky = (kyw + N) % N, kzi = (kzw + N) % N;
+    const C* b = (const C*)a.B[comp];
+    const C* tw = (const C*)a.tw;
+    double sr = 0.0, si = 0.0;
+    for (int z = threadIdx.x; z < N; z += NT) {
+        const C v = b[((size_t)z * N + ky) * M + kx];
+        const C w = tw[(kzi * z) & (N - 1)];
+        sr += (double)v.x * (double)w.x - (double)v.y * (double)w.y;
+        si += (double)v.x * (double)w.y + (double)v.y * (double)w.x;
+    }
+    __shared__ double wr[NT / 64], wi[NT / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        sr += __shfl_xor(sr, o, 64);
+        si += __shfl_xor(si, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) { wr[threadIdx.x >> 6] = sr; wi[threadIdx.x >> 6] = si; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0, i = 0.0;
+#pragma unroll
+        for (int w = 0; w < NT / 64; ++w) { r += wr[w]; i += wi[w]; }
+        a.low[LOW_DFT + id] = r / N;
+        a.low[LOW_DFT + 225 + id] = i / N;
+    }
 }
 
 template <typename T, int N>
@@ -721,22 +761,9 @@ __global__ void __launch_bounds__(NT) k_pre(PreArgs a)
     // 75 low modes: kx in 0..2, ky, kz in -2..2; 225 (mode, component) DFT sums
     __shared__ double nre[225], nim[225];
     __shared__ double emode[75];
-    const C* tw = (const C*)a.tw;
-    if (tid < 225) {
-        const int comp = tid / 75, mi = tid % 75;
-        const int kx = mi % 3, kyw = (mi / 3) % 5 - 2, kzw = mi / 15 - 2;
-        const int ky = (kyw + N) % N;
-        const int kzi = (kzw + N) % N;
-        const C* b = (const C*)a.B[comp];
-        double sr = 0.0, si = 0.0;
-        for (int z = 0; z < N; ++z) {
-            const C v = b[((size_t)z * N + ky) * M + kx];
-            const C w = tw[(kzi * z) & (N - 1)];
-            sr += (double)v.x * (double)w.x - (double)v.y * (double)w.y;
-            si += (double)v.x * (double)w.y + (double)v.y * (double)w.x;
-        }
-        nre[tid] = sr / N;
-        nim[tid] = si / N;
+    if (tid < 225) {                      // the z-DFT sums of k_pre_dft
+        nre[tid] = a.low[LOW_DFT + tid];
+        nim[tid] = a.low[LOW_DFT + 225 + tid];
     }
     __syncthreads();
     if (tid < 75) {
@@ -857,8 +884,8 @@ int flow_alloc(FlowBuffers& fb, int N, int precision)
             DDNS_HIP(hipMemcpy(*dst, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
         }
     }
-    DDNS_HIP(hipMalloc(&fb.low, 512 * sizeof(double)));
-    DDNS_HIP(hipMemset(fb.low, 0, 512 * sizeof(double)));
+    DDNS_HIP(hipMalloc(&fb.low, 1024 * sizeof(double)));
+    DDNS_HIP(hipMemset(fb.low, 0, 1024 * sizeof(double)));
     DDNS_HIP(hipMalloc(&fb.sth_state, 64 * sizeof(int)));
     DDNS_HIP(hipMemset(fb.sth_state, 0, 64 * sizeof(int)));
     DDNS_HIP(hipMalloc(&fb.sth_b, 450 * sizeof(double)));
@@ -944,6 +971,7 @@ static int launch_pre(const FlowBuffers& fb, const FlowConst& fc, hipStream_t s)
     a.tw = fb.tw; a.low = fb.low; a.sth_state = fb.sth_state; a.sth_b = fb.sth_b;
     a.dt = fc.dt; a.rnu = fc.rnu; a.force0 = fc.force0; a.force1 = fc.force1;
     a.ou_tf = fc.ou_tf; a.ou_var = fc.ou_var; a.pi2 = fc.pi2; a.nek = fc.nek; a.forcing = fc.forcing;
+    if (fc.forcing == 1) hipLaunchKernelGGL((k_pre_dft<T, N>), dim3(225), dim3(NT), 0, s, a);
     hipLaunchKernelGGL((k_pre<T, N>), dim3(1), dim3(NT), 0, s, a);
     DDNS_HIP(hipGetLastError());
     return 0;

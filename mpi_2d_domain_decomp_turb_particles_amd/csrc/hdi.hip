@@ -205,6 +205,12 @@ __global__ void k_residual0(const double* b, const double* ax0, T* q, long long 
     if (i < n) q[i] = (T)(b[i] - ax0[i]);
 }
 
+__global__ void k_gm_start(double* sm, double tol)
+{
+    sm[3] = tol;
+    if (sm[1] < 1e-5) sm[4] = -1.0;
+}
+
 template <typename T>
 __global__ void k_scale(T* q, const double* den, long long n)
 {
@@ -666,17 +672,9 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
     if (st) return st;
     st = dot(Q, Q, n3, pb.gm_red, sm + 1, 1, s);
     if (st) return st;
-    double h2[2];
-    DDNS_HIP(hipMemcpyAsync(h2, sm, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-    DDNS_HIP(hipStreamSynchronize(s));
-    if (h2[1] < 1e-5) {                                                     // 2866-2868
-        // pertvel keeps its warm-start value (x0)
-        return 0;
-    }
-    {
-        double me = tol;
-        DDNS_HIP(hipMemcpyAsync(sm + 3, &me, sizeof(double), hipMemcpyHostToDevice, s));
-    }
+    // the tolerance, and the stop flag when the initial residual is below 1e-5
+    // (2866-2868): then no iteration runs and pertvel keeps its warm-start value
+    hipLaunchKernelGGL(k_gm_start, dim3(1), dim3(1), 0, s, sm, tol);
     hipLaunchKernelGGL((k_scale<T>), dim3(nb3), dim3(HT), 0, s, Q, sm + 1, n3);
     // Iterations are launched in batches; every kernel of an iteration returns at
     // once when the least-squares update of an earlier one has set the converged
@@ -727,6 +725,7 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
         DDNS_HIP(hipMemcpyAsync(st6, sm, sizeof(st6), hipMemcpyDeviceToHost, s));
         DDNS_HIP(hipStreamSynchronize(s));
     }
+    if (st6[4] < 0.0) return 0;     // initial residual below 1e-5: no iteration
     const bool done = st6[4] != 0.0;
     const int used = (int)st6[5];   // the last iteration that ran
     hipLaunchKernelGGL((k_solution<T>), dim3(nb3), dim3(HT), 0, s, pb.gm_x0, (const T*)Q, ld, sm + L.Y0(), used,
