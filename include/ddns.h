@@ -103,6 +103,19 @@ typedef struct ddns_params {
     int    istep0;              /* step counter before the first ddns_step call          */
 } ddns_params;
 
+/* Flow statistics of the current u-hat, as POSTPROC computes them
+ * (postprocessing.f90:76-272) for monitora.dat, monitorb.dat and monitorc.dat. */
+typedef struct ddns_monitors {
+    /* monitora.dat (postprocessing.f90:208) */
+    double eta, tmse, xintls, tk, et, xl, uprime;
+    /* monitorb.dat (postprocessing.f90:209); cfl = pmax dt / dx (the reference uses dt/2
+     * at jstep 0, postprocessing.f90:206) */
+    double qq, dissp, ree, resol, cfl, vskew, vflat;
+    double pmax;                             /* max over the grid of |u|+|v|+|w|   */
+    /* monitorc.dat (postprocessing.f90:217-270), filled when divstat is set */
+    double vdiv1, vdiv2, vort1, vort2, divvt;
+} ddns_monitors;
+
 /* Per-step collision outcome (COLLISION_DET, partlib.f90:1166-1846). */
 typedef struct ddns_collision_counts {
     int     icc[3];                          /* type I, II, III collisions          */
@@ -191,6 +204,16 @@ int ddns_get_physical_velocity(ddns_handle* h, double* ux, double* uy, double* u
 
 /* Energy spectrum E(1..nek) of the current u-hat (postprocessing.f90:151-170). */
 int ddns_get_spectrum(ddns_handle* h, double* ek);
+
+/* Energy, dissipation and transfer spectra of the current u-hat, nek shells each (any
+ * pointer may be NULL): the spectrum.dat pairs of POSTPROC (postprocessing.f90:151-170)
+ * and TRANSPOSTPROC's transfer.dat (postprocessing.f90:275-315, with the dealiased
+ * nonlinear term of this u-hat, as the next step forms it, turb.f90:634-655). */
+int ddns_get_spectra(ddns_handle* h, double* ek, double* dk, double* tk);
+
+/* POSTPROC's statistics of the current u-hat (postprocessing.f90:76-272); divstat as
+ * parameter.d's divstat (turb.f90:131) adds the monitorc.dat sums. */
+int ddns_get_monitors(ddns_handle* h, int divstat, ddns_monitors* m);
 
 /* ---- droplets ------------------------------------------------------------------------ */
 

@@ -158,6 +158,18 @@ class Solver:
         self._check(self.lib.ddns_get_spectrum(self.h, _dp(ek)))
         return ek
 
+    def spectra(self):
+        """Energy, dissipation and transfer spectra (spectrum.dat / transfer.dat)."""
+        e, d, t = (np.zeros(self.nek) for _ in range(3))
+        self._check(self.lib.ddns_get_spectra(self.h, _dp(e), _dp(d), _dp(t)))
+        return e, d, t
+
+    def monitors(self, divstat: bool = False) -> "_abi.Monitors":
+        """POSTPROC statistics (monitora.dat, monitorb.dat, monitorc.dat)."""
+        m = _abi.Monitors()
+        self._check(self.lib.ddns_get_monitors(self.h, 1 if divstat else 0, C.byref(m)))
+        return m
+
     # -- droplets
     def set_droplets(self, yp, fvp=None, fdp=None, pertvel=None):
         yp = np.asfortranarray(yp, dtype=np.float64)

@@ -63,6 +63,14 @@ class CollisionCounts(C.Structure):
     ]
 
 
+class Monitors(C.Structure):
+    """ddns_monitors: POSTPROC statistics (postprocessing.f90:76-272)."""
+    _fields_ = [(f, C.c_double) for f in (
+        "eta", "tmse", "xintls", "tk", "et", "xl", "uprime",
+        "qq", "dissp", "ree", "resol", "cfl", "vskew", "vflat", "pmax",
+        "vdiv1", "vdiv2", "vort1", "vort2", "divvt")]
+
+
 class Timing(C.Structure):
     _fields_ = [
         ("flow_ms", C.c_double), ("interp_ms", C.c_double), ("cells_ms", C.c_double),
@@ -153,6 +161,8 @@ SIGNATURES = {
     "ddns_init_flow": (C.c_int, [_P, C.c_int, C.c_int, C.c_double, C.c_double]),
     "ddns_get_physical_velocity": (C.c_int, [_P, _D, _D, _D]),
     "ddns_get_spectrum": (C.c_int, [_P, _D]),
+    "ddns_get_spectra": (C.c_int, [_P, _D, _D, _D]),
+    "ddns_get_monitors": (C.c_int, [_P, C.c_int, C.POINTER(Monitors)]),
     "ddns_set_droplets": (C.c_int, [_P, C.c_int, _D, _D, _D, _D]),
     "ddns_get_droplets": (C.c_int, [_P, _I, _D, _D, _D, _D, _D, _D]),
     "ddns_init_droplets": (C.c_int, [_P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]),

@@ -518,6 +518,60 @@ extern "C" int ddns_get_spectrum(ddns_handle* h, double* ek)
     return keep(h, flow_spectrum(h->fb, h->N, h->nek, h->prec, ek, h->stream));
 }
 
+extern "C" int ddns_get_spectra(ddns_handle* h, double* ek, double* dk, double* tk)
+{
+    if (!h) return DDNS_ERR_ARG;
+    double mom[9];
+    return keep(h, flow_diagnostics(h->fb, h->N, h->nek, h->p.rnu, h->prec, 0, ek, dk, tk, mom, h->stream));
+}
+
+// POSTPROC, postprocessing.f90:97-270: the sums come from the device, the rest is the
+// reference's arithmetic on them
+extern "C" int ddns_get_monitors(ddns_handle* h, int divstat, ddns_monitors* m)
+{
+    if (!h || !m) return DDNS_ERR_ARG;
+    const int nek = h->nek, N = h->N;
+    std::vector<double> ek(nek), dk(nek);
+    double mom[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int st = flow_diagnostics(h->fb, N, nek, h->p.rnu, h->prec, 1 | (divstat ? 2 : 0), ek.data(), dk.data(),
+                                    nullptr, mom, h->stream);
+    if (st) return keep(h, st);
+    const double n3 = (double)N * N * N, rnu = h->p.rnu;
+    *m = ddns_monitors{};
+    const double a2 = mom[1] / 3.0, a3 = mom[2] / 3.0, a4 = mom[3] / 3.0;          // 115-137
+    m->vskew = a3 / n3 / std::pow(a2 / n3, 1.5);
+    m->vflat = a4 / n3 / std::pow(a2 / n3, 2.0);
+    double qq = 0.0, dissp = 0.0, xintls = 0.0;
+    for (int i = 1; i <= nek; ++i) {                                                 // 160-170
+        qq += ek[i - 1];
+        dissp += dk[i - 1];
+        xintls += ek[i - 1] / (double)i;
+    }
+    const double pi2 = 8.0 * std::atan(1.0);
+    m->qq = qq;
+    m->dissp = dissp;
+    m->eta = std::pow(rnu * rnu * rnu / dissp, 0.25);                               // 190-202
+    m->tk = std::sqrt(rnu / dissp);
+    m->uprime = std::sqrt(2.0 * qq / 3.0);
+    m->tmse = std::sqrt(15.0 * rnu * m->uprime * m->uprime / dissp);
+    m->ree = m->uprime * m->tmse / rnu;
+    m->xl = m->uprime * m->uprime * m->uprime / dissp;
+    m->et = m->xl / m->uprime;
+    const double akmax = (double)(N / 2) - 1.5;                                      // turb.f90:303
+    m->resol = akmax * m->eta;
+    m->xintls = xintls * (pi2 / 2.0) / 2.0 / (m->uprime * m->uprime);
+    m->pmax = mom[0];
+    m->cfl = mom[0] * h->p.dt / (pi2 / (double)N);                                   // 204-206
+    if (divstat) {                                                                   // 229-266
+        m->vdiv1 = mom[4] / n3;
+        m->vdiv2 = std::sqrt((mom[5] - m->vdiv1 * m->vdiv1) / n3);
+        m->vort1 = mom[6] / n3;
+        m->vort2 = std::sqrt((mom[7] - m->vort1 * m->vort1) / n3);
+        m->divvt = (mom[8] - m->vdiv1 * m->vort1) / (2.0 * n3);
+    }
+    return DDNS_OK;
+}
+
 // ------------------------------------------------------------------ droplets
 extern "C" int ddns_set_droplets(ddns_handle* h, int n, const double* yp, const double* fvp, const double* fdp,
                                  const double* pertvel)

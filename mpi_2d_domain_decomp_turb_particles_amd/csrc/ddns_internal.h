@@ -197,6 +197,8 @@ struct FlowBuffers {
     double* low = nullptr;                      // pre-kernel scratch (1024): ff[2], energies, forcing table, DFT sums
     int* sth_state = nullptr;                   // ranff state on the device: iseedf, iyf, ivf[32]
     double* sth_b = nullptr;                    // b1r..b3r (450)
+    void* dg[8] = {};                           // diagnostics scratch, 8 fields, allocated on first use
+    double* dg_part = nullptr;                  // diagnostics reduction partials
 };
 
 struct FlowConst {
@@ -218,6 +220,13 @@ int flow_step(const FlowBuffers& fb, const FlowConst& fc, int precision, bool la
 int flow_upload_state(const FlowBuffers& fb, int N, int precision, const double* const src[6], hipStream_t s);
 int flow_download_state(const FlowBuffers& fb, int N, int precision, double* const dst[6], hipStream_t s);
 int flow_download_phys(const FlowBuffers& fb, int N, int precision, double* ux, double* uy, double* uz, hipStream_t s);
+// POSTPROC / TRANSPOSTPROC sums of the current u-hat.  want: 1 = CFL maximum and the
+// velocity-derivative moments, 2 = divergence / vorticity sums.  ek, dk, tk (nek each,
+// any may be null): energy, dissipation and transfer spectra.  mom: [0] max |u|+|v|+|w|,
+// [1..3] sums of d^2, d^3, d^4 over the three longitudinal derivatives, [4..5] sums of
+// div, div^2, [6..8] sums of |w|, |w|^2, |w| div
+int flow_diagnostics(FlowBuffers& fb, int N, int nek, double rnu, int precision, int want, double* ek, double* dk,
+                     double* tk, double* mom, hipStream_t s);
 int flow_spectrum(const FlowBuffers& fb, int N, int nek, int precision, double* ek, hipStream_t s);
 int fft_r2c_host(const FlowBuffers& fb, int N, int precision, double* v, hipStream_t s);
 int fft_c2r_host(const FlowBuffers& fb, int N, int precision, double* v, hipStream_t s);

@@ -899,6 +899,8 @@ void flow_free(FlowBuffers& fb)
     for (int i = 0; i < 6; ++i) (void)hipFree(fb.A[i]);
     (void)hipFree(fb.uphys); (void)hipFree(fb.tw); (void)hipFree(fb.twh);
     (void)hipFree(fb.low); (void)hipFree(fb.sth_state); (void)hipFree(fb.sth_b);
+    for (int q = 0; q < 8; ++q) if (fb.dg[q]) (void)hipFree(fb.dg[q]);
+    if (fb.dg_part) (void)hipFree(fb.dg_part);
     fb = FlowBuffers{};
 }
 
@@ -1221,6 +1223,280 @@ int fft_c2r_host(const FlowBuffers& fb, int N, int precision, double* v, hipStre
     int st = 0;
     if (precision == DDNS_FP64) { DDNS_DISPATCH_N(N, st = (c2r_impl<double, NN>(fb, v, s))) }
     else { DDNS_DISPATCH_N(N, st = (c2r_impl<float, NN>(fb, v, s))) }
+    return st;
+}
+
+// ---------------------------------------------------------------- diagnostics
+// POSTPROC (postprocessing.f90:76-272) and TRANSPOSTPROC (275-315) on the current
+// u-hat.  Spectral fields are formed by the kernels below, taken to physical space by
+// the plain z / y / x transforms into scratch fields, and reduced with fixed grids.
+
+// dst = i k_c src (c = 0, 1, 2: x, y, z), the derivative of component src along c
+// (postprocessing.f90:97-111)
+template <typename T, int N>
+__global__ void k_mul_ik(const void* src, void* dst, int c)
+{
+    using C = cpx<T>;
+    constexpr int M = N / 2;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)N * N * M) return;
+    const int kx = (int)(t % M), ky = (int)((t / M) % N), kz = (int)(t / ((size_t)N * M));
+    const T k = (T)(c == 0 ? kx : (c == 1 ? wave_index(ky, N) : wave_index(kz, N)));
+    const C v = ((const C*)src)[t];
+    ((C*)dst)[t] = mk<C>(-(k * v.y), k * v.x);
+}
+
+// dst = i (k x u)_c for c < 3 (the vorticity, turb.f90:506-517), or i k . u for c = 3
+// (the divergence, postprocessing.f90:219-223)
+template <typename T, int N>
+__global__ void k_curl_div(const void* u0, const void* u1, const void* u2, void* dst, int c)
+{
+    using C = cpx<T>;
+    constexpr int M = N / 2;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)N * N * M) return;
+    const int kx = (int)(t % M), ky = (int)((t / M) % N), kz = (int)(t / ((size_t)N * M));
+    const T fx = (T)kx, fy = (T)wave_index(ky, N), fz = (T)wave_index(kz, N);
+    const C a = ((const C*)u0)[t], b = ((const C*)u1)[t], d = ((const C*)u2)[t];
+    C w;
+    if (c == 0) w = csub(cscale(d, fy), cscale(b, fz));
+    else if (c == 1) w = csub(cscale(a, fz), cscale(d, fx));
+    else if (c == 2) w = csub(cscale(b, fx), cscale(a, fy));
+    else w = cadd(cadd(cscale(a, fx), cscale(b, fy)), cscale(d, fz));
+    ((C*)dst)[t] = mk<C>(-w.y, w.x);
+}
+
+// in place: (w0, w1, w2) <- u x w, the nonlinear term in physical space (turb.f90:634-637)
+template <typename T>
+__global__ void k_cross(const T* __restrict__ u0, const T* __restrict__ u1, const T* __restrict__ u2,
+                        T* w0, T* w1, T* w2, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const T ux = u0[i], uy = u1[i], uz = u2[i], wx = w0[i], wy = w1[i], wz = w2[i];
+    w0[i] = uy * wz - uz * wy;
+    w1[i] = uz * wx - ux * wz;
+    w2[i] = ux * wy - uy * wx;
+}
+
+constexpr int DG_BLOCKS = 256;   // fixed grid of the diagnostic reductions
+
+// per work-group sums over a real field: mode 0: f^2, f^3, f^4 of one field;
+// mode 1: the max of |f0| + |f1| + |f2|; mode 2: sum |g|, |g|^2 and |g| f0 with
+// |g| = sqrt(f1^2 + f2^2 + f3^2) (vorticity magnitude against the divergence f0);
+// mode 3: f, f^2 of one field
+template <typename T>
+__global__ void __launch_bounds__(256) k_dg_reduce(const T* f0, const T* f1, const T* f2, const T* f3, size_t n,
+                                                  int mode, double* part)
+{
+    __shared__ double sh[3][256];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (mode == 0) {
+            const double v = (double)f0[i], v2 = v * v;
+            a0 += v2; a1 += v2 * v; a2 += v2 * v2;
+        } else if (mode == 1) {
+            const double v = fabs((double)f0[i]) + fabs((double)f1[i]) + fabs((double)f2[i]);
+            a0 = fmax(a0, v);
+        } else if (mode == 3) {
+            const double v = (double)f0[i];
+            a0 += v; a1 += v * v;
+        } else {
+            const double w1 = (double)f1[i], w2 = (double)f2[i], w3 = (double)f3[i];
+            const double g = sqrt(w1 * w1 + w2 * w2 + w3 * w3);
+            a0 += g; a1 += g * g; a2 += g * (double)f0[i];
+        }
+    }
+    sh[0][threadIdx.x] = a0; sh[1][threadIdx.x] = a1; sh[2][threadIdx.x] = a2;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            if (mode == 1) sh[0][threadIdx.x] = fmax(sh[0][threadIdx.x], sh[0][threadIdx.x + o]);
+            else {
+                sh[0][threadIdx.x] += sh[0][threadIdx.x + o];
+                sh[1][threadIdx.x] += sh[1][threadIdx.x + o];
+                sh[2][threadIdx.x] += sh[2][threadIdx.x + o];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = sh[0][0];
+        part[gridDim.x + blockIdx.x] = sh[1][0];
+        part[2 * gridDim.x + blockIdx.x] = sh[2][0];
+    }
+}
+
+// shell sums over |k| shells 1..nek (kx = 0 plane halved, postprocessing.f90:143-170,
+// 285-313): e = |u|^2, d = 2 nu k^2 |u|^2, tr = Re(conj(u) . F) with F the nonlinear term
+template <typename T, int N>
+__global__ void k_shells(const void* u0, const void* u1, const void* u2, const void* F0, const void* F1,
+                         const void* F2, int nek, double rnu, double* e, double* d, double* tr)
+{
+    using C = cpx<T>;
+    constexpr int M = N / 2;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)N * N * M) return;
+    const int kx = (int)(t % M), ky = (int)((t / M) % N), kz = (int)(t / ((size_t)N * M));
+    const int kyw = wave_index(ky, N), kzw = wave_index(kz, N);
+    double k2 = (double)(kx * kx + kyw * kyw + kzw * kzw);
+    if (k2 == 0.0) k2 = 1.e-5;
+    const int ik2 = (int)(sqrt(k2) + 0.5);
+    if (ik2 < 1 || ik2 > nek) return;
+    const C a = ((const C*)u0)[t], b = ((const C*)u1)[t], c = ((const C*)u2)[t];
+    const double h = kx == 0 ? 0.5 : 1.0;
+    const double ee = h * ((double)a.x * a.x + (double)a.y * a.y + (double)b.x * b.x + (double)b.y * b.y +
+                           (double)c.x * c.x + (double)c.y * c.y);
+    atomicAdd(e + ik2 - 1, ee);
+    atomicAdd(d + ik2 - 1, 2.0 * rnu * ee * k2);
+    if (F0) {
+        const C p = ((const C*)F0)[t], q = ((const C*)F1)[t], r = ((const C*)F2)[t];
+        const double tt = (double)a.x * p.x + (double)a.y * p.y + (double)b.x * q.x + (double)b.y * q.y +
+                          (double)c.x * r.x + (double)c.y * r.y;
+        atomicAdd(tr + ik2 - 1, h * tt);
+    }
+}
+
+template <typename T, int N>
+static int c2r_dev(const FlowBuffers& fb, const void* src, void* tmp, void* dst, hipStream_t s)
+{
+    FlowConst fc{};
+    fc.N = N;
+    int st = launch_z<T, N>(KZ_PLAIN_INV, fb, fc, s, src, tmp, 1.0);
+    if (st) return st;
+    void* f[1] = {tmp};
+    st = launch_y<T, N>(true, false, f, 1, fb, s);
+    if (st) return st;
+    hipLaunchKernelGGL((k_x_plain<T, N, true>), dim3((unsigned)((size_t)N * N / kx_lines<N>())), dim3(NT), 0, s, tmp, dst,
+                       fb.tw, fb.twh);
+    DDNS_HIP(hipGetLastError());
+    return 0;
+}
+
+template <typename T, int N>
+static int r2c_dev(const FlowBuffers& fb, const void* src, void* tmp, void* dst, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_x_plain<T, N, false>), dim3((unsigned)((size_t)N * N / kx_lines<N>())), dim3(NT), 0, s, src, tmp,
+                       fb.tw, fb.twh);
+    DDNS_HIP(hipGetLastError());
+    void* f[1] = {tmp};
+    int st = launch_y<T, N>(false, false, f, 1, fb, s);
+    if (st) return st;
+    FlowConst fc{};
+    fc.N = N;
+    return launch_z<T, N>(KZ_PLAIN_FWD, fb, fc, s, tmp, dst, 1.0 / N);
+}
+
+// partials of a k_dg_reduce launch summed (or maxed) on the host in block order
+template <typename T>
+static int dg_reduce(const T* f0, const T* f1, const T* f2, const T* f3, size_t n, int mode, double* part, double out[3],
+                     hipStream_t s)
+{
+    hipLaunchKernelGGL((k_dg_reduce<T>), dim3(DG_BLOCKS), dim3(256), 0, s, f0, f1, f2, f3, n, mode, part);
+    DDNS_HIP(hipGetLastError());
+    double h[3 * DG_BLOCKS];
+    DDNS_HIP(hipMemcpyAsync(h, part, sizeof(h), hipMemcpyDeviceToHost, s));
+    DDNS_HIP(hipStreamSynchronize(s));
+    for (int q = 0; q < 3; ++q) {
+        double a = 0.0;
+        for (int b = 0; b < DG_BLOCKS; ++b) a = mode == 1 ? fmax(a, h[q * DG_BLOCKS + b]) : a + h[q * DG_BLOCKS + b];
+        out[q] = a;
+    }
+    return 0;
+}
+
+template <typename T, int N>
+static int diag_impl(FlowBuffers& fb, int nek, double rnu, int want, double* ek, double* dk, double* tk, double* mom,
+                     hipStream_t s)
+{
+    const size_t n3 = (size_t)N * N * N, nc = field_elems(N);
+    const size_t fbytes = n3 * sizeof(T);                    // one real field = one complex field
+    if (!fb.dg[0]) {
+        for (int q = 0; q < 8; ++q) DDNS_HIP(hipMalloc(&fb.dg[q], fbytes));
+        DDNS_HIP(hipMalloc(&fb.dg_part, 3 * DG_BLOCKS * sizeof(double)));
+    }
+    T* R[6];
+    for (int q = 0; q < 6; ++q) R[q] = (T*)fb.dg[q];
+    void* ctmp = fb.dg[6];
+    void* cout = fb.dg[7];
+    const unsigned gc = (unsigned)((nc + 255) / 256), gr = (unsigned)((n3 + 255) / 256);
+    int st;
+    // physical u and w
+    for (int c = 0; c < 3; ++c) {
+        st = c2r_dev<T, N>(fb, fb.U[c], ctmp, R[c], s);
+        if (st) return st;
+        hipLaunchKernelGGL((k_curl_div<T, N>), dim3(gc), dim3(256), 0, s, fb.U[0], fb.U[1], fb.U[2], cout, c);
+        st = c2r_dev<T, N>(fb, cout, ctmp, R[3 + c], s);
+        if (st) return st;
+    }
+    if (want & 1) {
+        // max |u| + |v| + |w| (the CFL number, postprocessing.f90:204-207)
+        double o[3];
+        st = dg_reduce<T>(R[0], R[1], R[2], nullptr, n3, 1, fb.dg_part, o, s);
+        if (st) return st;
+        mom[0] = o[0];
+        // longitudinal velocity derivatives: sums of d^2, d^3, d^4 over the three
+        // components (postprocessing.f90:97-137), into the ctmp / cout scratch
+        double s2 = 0.0, s3 = 0.0, s4 = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            hipLaunchKernelGGL((k_mul_ik<T, N>), dim3(gc), dim3(256), 0, s, fb.U[c], cout, c);
+            // the derivative field replaces nothing needed later: use the second half of R
+            // only after the vorticity moments (below) have been taken
+            st = c2r_dev<T, N>(fb, cout, ctmp, cout, s);
+            if (st) return st;
+            st = dg_reduce<T>((const T*)cout, nullptr, nullptr, nullptr, n3, 0, fb.dg_part, o, s);
+            if (st) return st;
+            s2 += o[0]; s3 += o[1]; s4 += o[2];
+        }
+        mom[1] = s2; mom[2] = s3; mom[3] = s4;
+    }
+    if (want & 2) {
+        // divergence and vorticity statistics (postprocessing.f90:217-270)
+        hipLaunchKernelGGL((k_curl_div<T, N>), dim3(gc), dim3(256), 0, s, fb.U[0], fb.U[1], fb.U[2], cout, 3);
+        st = c2r_dev<T, N>(fb, cout, ctmp, cout, s);
+        if (st) return st;
+        double o[3], g[3];
+        st = dg_reduce<T>((const T*)cout, nullptr, nullptr, nullptr, n3, 3, fb.dg_part, o, s);
+        if (st) return st;
+        st = dg_reduce<T>((const T*)cout, R[3], R[4], R[5], n3, 2, fb.dg_part, g, s);
+        if (st) return st;
+        mom[4] = o[0]; mom[5] = o[1];                  // sum div, sum div^2
+        mom[6] = g[0]; mom[7] = g[1]; mom[8] = g[2];   // sum |w|, sum |w|^2, sum |w| div
+    }
+    // spectra; the transfer needs the nonlinear term F = FFT(u x w) (turb.f90:634-650)
+    double* dd = fb.dg_part;           // 3 nek shell sums (nek <= 3 * DG_BLOCKS / 3)
+    DDNS_HIP(hipMemsetAsync(dd, 0, 3 * (size_t)nek * sizeof(double), s));
+    const void* F[3] = {nullptr, nullptr, nullptr};
+    if (tk) {
+        hipLaunchKernelGGL((k_cross<T>), dim3(gr), dim3(256), 0, s, R[0], R[1], R[2], R[3], R[4], R[5], n3);
+        DDNS_HIP(hipGetLastError());
+        for (int c = 0; c < 3; ++c) {
+            st = r2c_dev<T, N>(fb, R[3 + c], ctmp, R[c], s);        // F_c over u_c, no longer needed
+            if (st) return st;
+            F[c] = R[c];
+        }
+    }
+    hipLaunchKernelGGL((k_shells<T, N>), dim3(gc), dim3(256), 0, s, fb.U[0], fb.U[1], fb.U[2], F[0], F[1], F[2], nek, rnu,
+                       dd, dd + nek, dd + 2 * nek);
+    DDNS_HIP(hipGetLastError());
+    std::vector<double> h(3 * (size_t)nek);
+    DDNS_HIP(hipMemcpyAsync(h.data(), dd, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    DDNS_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < nek; ++i) {
+        if (ek) ek[i] = h[i];
+        if (dk) dk[i] = h[nek + i];
+        if (tk) tk[i] = h[2 * nek + i];
+    }
+    return 0;
+}
+
+int flow_diagnostics(FlowBuffers& fb, int N, int nek, double rnu, int precision, int want, double* ek, double* dk,
+                     double* tk, double* mom, hipStream_t s)
+{
+    if (nek > DG_BLOCKS) { set_error("too many shells for the diagnostics"); return DDNS_ERR_ARG; }
+    int st = 0;
+    if (precision == DDNS_FP64) { DDNS_DISPATCH_N(N, st = (diag_impl<double, NN>(fb, nek, rnu, want, ek, dk, tk, mom, s))) }
+    else { DDNS_DISPATCH_N(N, st = (diag_impl<float, NN>(fb, nek, rnu, want, ek, dk, tk, mom, s))) }
     return st;
 }
 

@@ -75,6 +75,13 @@ module ddns_mod
     real(c_double) :: gmres_residual
   end type ddns_collision_counts
 
+  type, bind(C), public :: ddns_monitors
+    real(c_double) :: eta, tmse, xintls, tk, et, xl, uprime
+    real(c_double) :: qq, dissp, ree, resol, cfl, vskew, vflat
+    real(c_double) :: pmax
+    real(c_double) :: vdiv1, vdiv2, vort1, vort2, divvt
+  end type ddns_monitors
+
   type, bind(C), public :: ddns_timing
     real(c_double) :: flow_ms, interp_ms, cells_ms, hdi_ms, advance_ms, colldet_ms, overlap_ms, total_ms
     integer(c_int) :: nsteps, pad
@@ -85,6 +92,7 @@ module ddns_mod
   public :: ddns_last_error_c, ddns_nek, ddns_set_flow, ddns_get_flow
   public :: ddns_set_forcing_state, ddns_get_forcing_state, ddns_init_flow
   public :: ddns_get_physical_velocity, ddns_get_spectrum, ddns_set_droplets, ddns_get_droplets
+  public :: ddns_get_spectra, ddns_get_monitors
   public :: ddns_init_droplets, ddns_overlap_fix, ddns_step, ddns_get_collision_counts
   public :: ddns_get_collision_bins, ddns_get_collision_pairs, ddns_get_timing
   public :: ddns_fft_r2c, ddns_fft_c2r, ddns_check
@@ -155,6 +163,17 @@ module ddns_mod
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h
       real(c_double), intent(out) :: ek(*)
+    end function
+    integer(c_int) function ddns_get_spectra(h, ek, dk, tk) bind(C, name="ddns_get_spectra")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: ek(*), dk(*), tk(*)
+    end function
+    integer(c_int) function ddns_get_monitors(h, divstat, m) bind(C, name="ddns_get_monitors")
+      import :: c_int, c_ptr, ddns_monitors
+      type(c_ptr), value :: h
+      integer(c_int), value :: divstat
+      type(ddns_monitors), intent(out) :: m
     end function
     integer(c_int) function ddns_set_droplets(h, n, yp, fvp, fdp, pertvel) bind(C, name="ddns_set_droplets")
       import :: c_int, c_ptr, c_double

@@ -30,7 +30,8 @@ program turb_gpu
   double precision :: dt, rnu, time, force(2), rho_water, rho_air, etk_DNS, tauk_DNS, st
   double precision :: ediss_CLOUD, viscosity_CLOUD, gravity, hdi_trunc, radius(10)
   double precision :: t_total
-  double precision, allocatable :: ek(:)
+  double precision, allocatable :: ek(:), dk(:), tk(:)
+  type(ddns_monitors) :: mon
   double precision, allocatable :: vel(:,:,:,:), blk(:,:,:), yp(:,:), fvp(:,:,:), fdp(:,:,:), pert(:,:)
   double precision, allocatable :: vpd(:,:), dmv(:,:)
   double precision :: bforce(450)
@@ -135,7 +136,7 @@ program turb_gpu
   stat = ddns_create(p, h)
   call ddns_check(stat, h, 'ddns_create')
   nek = ddns_nek(h)
-  allocate(ek(nek))
+  allocate(ek(nek), dk(nek), tk(nek))
 
   ! ---- initial flow, turb.f90:438-454
   nprocY_in = 1
@@ -192,10 +193,39 @@ program turb_gpu
   end if
 
   open(40, file='collision.dat', status='unknown')
+  ! the reference's statistics files (turb.f90:356-360), written from the state at the
+  ! start of step jstep as POSTPROC / TRANSPOSTPROC see it (turb.f90:628-656)
   open(41, file='spectrum.dat', status='unknown')
+  open(42, file='transfer.dat', status='unknown')
+  open(43, file='monitora.dat', status='unknown')
+  open(44, file='monitorb.dat', status='unknown')
+  if (divstat) open(45, file='monitorc.dat', status='unknown')
   t_total = 0.d0
   call system_clock(c0, crate)
   do istep = 1, nstep
+    if (ispec > 0 .and. mod(istep, ispec) == 0) then
+      stat = ddns_get_spectra(h, ek, dk, tk)
+      call ddns_check(stat, h, 'ddns_get_spectra')
+    end if
+    if (ieout > 0 .and. mod(istep, ieout) == 0) then                  ! POSTPROC, postprocessing.f90:76-272
+      stat = ddns_get_monitors(h, merge(1_c_int, 0_c_int, divstat), mon)
+      call ddns_check(stat, h, 'ddns_get_monitors')
+      if (ispec > 0 .and. mod(istep, ispec) == 0) then                ! 157-168
+        write(41, *) istep / ispec - 2
+        do il = 1, nek
+          write(41, *) il, ek(il), dk(il)
+        end do
+      end if
+      write(43, 601) time, mon%eta, mon%tmse, mon%xintls, mon%tk, mon%et, mon%xl, mon%uprime
+      write(44, 601) time, mon%qq, mon%dissp, mon%ree, mon%resol, mon%cfl, mon%vskew, mon%vflat
+      if (divstat) write(45, 602) time, mon%vdiv1, mon%vdiv2, mon%vort1, mon%vort2, mon%divvt
+    end if
+    if (ispec > 0 .and. mod(istep, ispec) == 0) then                  ! TRANSPOSTPROC, 294-313
+      write(42, *) istep / ispec - 1
+      do il = 1, nek
+        write(42, *) il, tk(il)
+      end do
+    end if
     stat = ddns_step(h, 1_c_int)
     call ddns_check(stat, h, 'ddns_step')
     stat = ddns_get_timing(h, tm)
@@ -210,16 +240,16 @@ program turb_gpu
       write(*, '(a, i8, a, 1pe14.6, a, 1pe14.6, a, 0pf9.3, a)') ' istep', istep, '  time', time, &
           '  energy', sum(ek), '  step', tm%total_ms, ' ms'
     end if
-    if (ispec > 0 .and. mod(istep, ispec) == 0) then
-      stat = ddns_get_spectrum(h, ek)
-      do il = 1, nek
-        write(41, '(i8, i6, 1pe20.10)') istep, il, ek(il)
-      end do
-    end if
   end do
   call system_clock(c1)
   close(40)
   close(41)
+  close(42)
+  close(43)
+  close(44)
+  if (divstat) close(45)
+601 format ( 2x,8(1pe15.5) )
+602 format ( 2x,7(1pe15.5) )
   write(*,*) 'total time              =', dble(c1 - c0) / dble(crate)
   write(*,*) 'GPU time in ddns_step   =', t_total / 1000.d0
   write(*,*) 'Next run starts at time =', time

@@ -995,6 +995,210 @@ int orc_get_spectrum(orc_sim* s, double* ek)
 }
 
 /* ------------------------------------------------------------------------------- */
+/* Flow statistics: POSTPROC (postprocessing.f90:76-272) and TRANSPOSTPROC         */
+/* (275-315) of the current state, on scratch copies.                               */
+/* Pinned by: single Fourier modes (skewness 0, flatness 3/2, E, D, CFL in closed   */
+/* form), sum_k T(k) = <u . (u x w)> = 0 for a dealiased field, a solenoidal field   */
+/* with zero divergence statistics.                                                  */
+/* ------------------------------------------------------------------------------- */
+
+/* multiply a spectral field by i over the rows 1:lx (postprocessing.f90:100-111) */
+static void mul_i(int N, int L, double* v)
+{
+    int i, j, k;
+    for (k = 0; k < N; ++k)
+        for (j = 0; j < L; j += 2)
+            for (i = 0; i < N; ++i) {
+                size_t o0 = IDX(i, j, k, L), o1 = IDX(i, j + 1, k, L);
+                double t = v[o0];
+                v[o0] = -v[o1];
+                v[o1] = t;
+            }
+}
+
+/* the three wavenumbers of entry (i, j, k) of the spectral layout */
+static void wn3(int i, int j, int k, int N, double* kx_, double* ky_, double* kz_)
+{
+    *kx_ = (i >= N) ? 0.0 : wn_x(j);
+    *ky_ = (i >= N) ? 0.0 : wn_y(k, N);
+    *kz_ = wn_z(i, N);
+}
+
+/* physical velocity and vorticity of the state (turb.f90:506-539) into u[3], w[3] */
+static void phys_u_w(orc_sim* s, double** u, double** w)
+{
+    const int N = s->N, L = s->L;
+    int i, j, k, c;
+    const double* v[3] = {s->vx, s->vy, s->vz};
+    for (c = 0; c < 3; ++c) memcpy(u[c], v[c], s->fsz * 8);
+    for (k = 0; k < N; ++k)
+        for (j = 0; j < L; ++j)
+            for (i = 0; i < L; ++i) {
+                size_t o = IDX(i, j, k, L);
+                double kx_, ky_, kz_;
+                wn3(i, j, k, N, &kx_, &ky_, &kz_);
+                w[0][o] = ky_ * s->vz[o] - kz_ * s->vy[o];
+                w[1][o] = kz_ * s->vx[o] - kx_ * s->vz[o];
+                w[2][o] = kx_ * s->vy[o] - ky_ * s->vx[o];
+            }
+    for (c = 0; c < 3; ++c) {
+        mul_i(N, L, w[c]);
+        orc_fft_c2r(N, w[c]);
+        orc_fft_c2r(N, u[c]);
+    }
+}
+
+int orc_get_spectra(orc_sim* s, double* ek, double* dk, double* tk)
+{
+    const int N = s->N, L = s->L, nek = s->nek;
+    const double rnu = s->p.rnu;
+    double* F[3] = {NULL, NULL, NULL};
+    int sh, i, j, k, c;
+    if (tk) {
+        /* the dealiased nonlinear term of this state (turb.f90:634-650) */
+        double *u[3], *w[3];
+        for (c = 0; c < 3; ++c) { u[c] = (double*)xcalloc(s->fsz, 8); w[c] = (double*)xcalloc(s->fsz, 8); }
+        phys_u_w(s, u, w);
+        for (k = 0; k < N; ++k)
+            for (j = 0; j < L; ++j)
+                for (i = 0; i < L; ++i) {
+                    size_t o = IDX(i, j, k, L);
+                    double ux = u[0][o], uy = u[1][o], uz = u[2][o];
+                    double wx = w[0][o], wy = w[1][o], wz = w[2][o];
+                    w[0][o] = uy * wz - uz * wy;
+                    w[1][o] = uz * wx - ux * wz;
+                    w[2][o] = ux * wy - uy * wx;
+                }
+        for (c = 0; c < 3; ++c) {
+            orc_fft_r2c(N, w[c]);
+            orc_dealias(N, w[c]);
+            F[c] = w[c];
+            free(u[c]);
+        }
+    }
+    for (sh = 1; sh <= nek; ++sh) {                            /* 143-170, 285-313 */
+        double e = 0.0, d = 0.0, t = 0.0;
+        for (k = 0; k < N; ++k)
+            for (j = 0; j < L; ++j)
+                for (i = 0; i < N; ++i) {
+                    size_t o = IDX(i, j, k, L);
+                    double ee;
+                    if (wn_ik2(i, j, k, N) != sh) continue;
+                    ee = s->vx[o] * s->vx[o] + s->vy[o] * s->vy[o] + s->vz[o] * s->vz[o];
+                    if (j <= 1) ee = 0.5 * ee;
+                    e += ee;
+                    d += 2.0 * rnu * ee * wn_k2(i, j, k, N);
+                    if (F[0]) {
+                        double tt = s->vx[o] * F[0][o] + s->vy[o] * F[1][o] + s->vz[o] * F[2][o];
+                        if (j <= 1) tt = 0.5 * tt;
+                        t += tt;
+                    }
+                }
+        if (ek) ek[sh - 1] = e;
+        if (dk) dk[sh - 1] = d;
+        if (tk) tk[sh - 1] = t;
+    }
+    for (c = 0; c < 3; ++c) free(F[c]);
+    return 0;
+}
+
+int orc_get_monitors(orc_sim* s, int divstat, ddns_monitors* m)
+{
+    const int N = s->N, L = s->L, nek = s->nek;
+    const double rnu = s->p.rnu, n3 = (double)N * N * N;
+    const double pi2 = 8.0 * atan(1.0);
+    double *u[3], *w[3], *d;
+    double a2 = 0.0, a3 = 0.0, a4 = 0.0, pmax = 0.0;
+    double qq = 0.0, dissp = 0.0, xintls = 0.0;
+    double* ek = (double*)xcalloc(nek, 8);
+    double* dk = (double*)xcalloc(nek, 8);
+    const double* v[3] = {s->vx, s->vy, s->vz};
+    int i, j, k, c;
+    memset(m, 0, sizeof(*m));
+    for (c = 0; c < 3; ++c) { u[c] = (double*)xcalloc(s->fsz, 8); w[c] = (double*)xcalloc(s->fsz, 8); }
+    d = (double*)xcalloc(s->fsz, 8);
+    /* longitudinal derivatives d_c = d u_c / d x_c (97-111) and their moments (113-137) */
+    for (c = 0; c < 3; ++c) {
+        for (k = 0; k < N; ++k)
+            for (j = 0; j < L; ++j)
+                for (i = 0; i < L; ++i) {
+                    size_t o = IDX(i, j, k, L);
+                    double kk[3];
+                    wn3(i, j, k, N, &kk[0], &kk[1], &kk[2]);
+                    d[o] = kk[c] * v[c][o];
+                }
+        mul_i(N, L, d);
+        orc_fft_c2r(N, d);
+        for (k = 0; k < N; ++k)
+            for (j = 0; j < N; ++j)
+                for (i = 0; i < N; ++i) {
+                    double x = d[IDX(i, j, k, L)], x2 = x * x;
+                    a2 += x2; a3 += x2 * x; a4 += x2 * x2;
+                }
+    }
+    a2 /= 3.0; a3 /= 3.0; a4 /= 3.0;
+    m->vskew = a3 / n3 / pow(a2 / n3, 1.5);
+    m->vflat = a4 / n3 / pow(a2 / n3, 2.0);
+    orc_get_spectra(s, ek, dk, NULL);                          /* 143-170 */
+    for (i = 1; i <= nek; ++i) {
+        qq += ek[i - 1];
+        dissp += dk[i - 1];
+        xintls += ek[i - 1] / (double)i;
+    }
+    m->qq = qq;
+    m->dissp = dissp;
+    m->eta = pow(rnu * rnu * rnu / dissp, 0.25);               /* 190-202 */
+    m->tk = sqrt(rnu / dissp);
+    m->uprime = sqrt(2.0 * qq / 3.0);
+    m->tmse = sqrt(15.0 * rnu * m->uprime * m->uprime / dissp);
+    m->ree = m->uprime * m->tmse / rnu;
+    m->xl = m->uprime * m->uprime * m->uprime / dissp;
+    m->et = m->xl / m->uprime;
+    m->resol = ((double)(N / 2) - 1.5) * m->eta;               /* akmax, turb.f90:303 */
+    m->xintls = xintls * (pi2 / 2.0) / 2.0 / (m->uprime * m->uprime);
+    phys_u_w(s, u, w);
+    for (k = 0; k < N; ++k)                                    /* 204-207 */
+        for (j = 0; j < N; ++j)
+            for (i = 0; i < N; ++i) {
+                size_t o = IDX(i, j, k, L);
+                double t = fabs(u[0][o]) + fabs(u[1][o]) + fabs(u[2][o]);
+                if (t > pmax) pmax = t;
+            }
+    m->pmax = pmax;
+    m->cfl = pmax * s->p.dt / (pi2 / (double)N);
+    if (divstat) {                                             /* 217-266 */
+        double s1 = 0.0, s2 = 0.0, g1 = 0.0, g2 = 0.0, gd = 0.0;
+        for (k = 0; k < N; ++k)
+            for (j = 0; j < L; ++j)
+                for (i = 0; i < L; ++i) {
+                    size_t o = IDX(i, j, k, L);
+                    double kx_, ky_, kz_;
+                    wn3(i, j, k, N, &kx_, &ky_, &kz_);
+                    d[o] = kx_ * s->vx[o] + ky_ * s->vy[o] + kz_ * s->vz[o];
+                }
+        mul_i(N, L, d);
+        orc_fft_c2r(N, d);
+        for (k = 0; k < N; ++k)
+            for (j = 0; j < N; ++j)
+                for (i = 0; i < N; ++i) {
+                    size_t o = IDX(i, j, k, L);
+                    double dv = d[o];
+                    double g = sqrt(w[0][o] * w[0][o] + w[1][o] * w[1][o] + w[2][o] * w[2][o]);
+                    s1 += dv; s2 += dv * dv;
+                    g1 += g; g2 += g * g; gd += g * dv;
+                }
+        m->vdiv1 = s1 / n3;
+        m->vdiv2 = sqrt((s2 - m->vdiv1 * m->vdiv1) / n3);
+        m->vort1 = g1 / n3;
+        m->vort2 = sqrt((g2 - m->vort1 * m->vort1) / n3);
+        m->divvt = (gd - m->vdiv1 * m->vort1) / (2.0 * n3);
+    }
+    for (c = 0; c < 3; ++c) { free(u[c]); free(w[c]); }
+    free(d); free(ek); free(dk);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------- */
 /* Droplets                                                                          */
 /* ------------------------------------------------------------------------------- */
 

@@ -38,6 +38,8 @@ int  orc_set_istep(orc_sim* s, int istep);
 int  orc_step(orc_sim* s, int nsteps);
 int  orc_flow_only(orc_sim* s, int on);          /* skip the droplet block when on     */
 int  orc_get_spectrum(orc_sim* s, double* ek);
+int  orc_get_spectra(orc_sim* s, double* ek, double* dk, double* tk);
+int  orc_get_monitors(orc_sim* s, int divstat, ddns_monitors* m);
 int  orc_get_physical_velocity(orc_sim* s, double* ux, double* uy, double* uz);
 int  orc_get_collision_counts(orc_sim* s, ddns_collision_counts* out);
 int  orc_get_collision_bins(orc_sim* s, int* nirbin, double* wrbin, double* wxbin,

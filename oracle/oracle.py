@@ -13,7 +13,7 @@ from pathlib import Path
 import numpy as np
 
 from mpi_2d_domain_decomp_turb_particles_amd import _abi
-from mpi_2d_domain_decomp_turb_particles_amd._abi import CollisionCounts, NBIN, Params
+from mpi_2d_domain_decomp_turb_particles_amd._abi import CollisionCounts, Monitors, NBIN, Params
 
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "liboracle.so"
@@ -48,6 +48,8 @@ def load() -> C.CDLL:
         "orc_step": (C.c_int, [_P, C.c_int]),
         "orc_flow_only": (C.c_int, [_P, C.c_int]),
         "orc_get_spectrum": (C.c_int, [_P, _D]),
+        "orc_get_spectra": (C.c_int, [_P, _D, _D, _D]),
+        "orc_get_monitors": (C.c_int, [_P, C.c_int, C.POINTER(Monitors)]),
         "orc_get_physical_velocity": (C.c_int, [_P, _D, _D, _D]),
         "orc_get_collision_counts": (C.c_int, [_P, C.POINTER(CollisionCounts)]),
         "orc_get_collision_bins": (C.c_int, [_P, _I] + [_D] * 7),
@@ -192,6 +194,16 @@ class Oracle:
         ek = np.zeros(self.nek)
         self._chk(self.lib.orc_get_spectrum(self.h, dp(ek)))
         return ek
+
+    def spectra(self):
+        e, d, t = (np.zeros(self.nek) for _ in range(3))
+        self._chk(self.lib.orc_get_spectra(self.h, dp(e), dp(d), dp(t)))
+        return e, d, t
+
+    def monitors(self, divstat=False):
+        m = Monitors()
+        self._chk(self.lib.orc_get_monitors(self.h, 1 if divstat else 0, C.byref(m)))
+        return m
 
     def physical_velocity(self):
         n3 = self.N ** 3

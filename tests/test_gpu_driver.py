@@ -68,3 +68,35 @@ def test_restart_through_reference_files(tmp_path):
     yb = np.frombuffer(pb[1], dtype=np.float64)[: 8 * n].reshape(n, 8)
     assert np.array_equal(ya[:, 7], yb[:, 7])
     assert np.abs(ya - yb).max() < 1e-10
+
+
+@pytest.mark.skipif(not DRIVER.exists(), reason="Fortran driver not built")
+def test_statistics_files(tmp_path):
+    """The driver writes POSTPROC's monitora/monitorb/monitorc.dat and the spectrum and
+    transfer files of TRANSPOSTPROC in the reference's layout (postprocessing.f90:157-315):
+    8, 8 and 6 columns, and a shell table per output; monitorb's total energy and
+    dissipation are the sums of the spectrum.dat columns of the same step, and the
+    transfer sums to zero (energy conservation of the dealiased nonlinear term)."""
+    lines = BASE.read_text().split("\n")
+    lines[4] = "4"             # nstep
+    lines[24] = "2"            # ieout
+    lines[26] = "2"            # ispec
+    lines[28] = ".TRUE."       # divstat
+    (tmp_path / "p").write_text("\n".join(lines))
+    _run(tmp_path, tmp_path / "p")
+    ma = np.loadtxt(tmp_path / "monitora.dat", ndmin=2)
+    mb = np.loadtxt(tmp_path / "monitorb.dat", ndmin=2)
+    mc = np.loadtxt(tmp_path / "monitorc.dat", ndmin=2)
+    assert ma.shape == (2, 8) and mb.shape == (2, 8) and mc.shape == (2, 6)
+    assert np.all(np.isfinite(ma)) and np.all(np.isfinite(mb)) and np.all(np.isfinite(mc))
+    assert np.all(mb[:, 7] > 1.0)                         # flatness of a Gaussian-like field
+    nek = int(64 / 2 - 1.5)                              # turb.f90:303-304
+    spec = (tmp_path / "spectrum.dat").read_text().split()
+    tran = (tmp_path / "transfer.dat").read_text().split()
+    for out in range(2):
+        s = np.array(spec[out * (1 + 3 * nek) + 1: (out + 1) * (1 + 3 * nek)], dtype=float).reshape(nek, 3)
+        t = np.array(tran[out * (1 + 2 * nek) + 1: (out + 1) * (1 + 2 * nek)], dtype=float).reshape(nek, 2)
+        assert np.array_equal(s[:, 0], np.arange(1, nek + 1))
+        assert abs(s[:, 1].sum() - mb[out, 1]) <= 1e-4 * mb[out, 1]      # list-directed and 1pe15.5 rounding
+        assert abs(s[:, 2].sum() - mb[out, 2]) <= 1e-4 * mb[out, 2]
+        assert abs(t[:, 1].sum()) <= 1e-9 * np.abs(t[:, 1]).sum()

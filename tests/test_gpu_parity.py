@@ -69,6 +69,35 @@ def test_flow_ten_steps_match_oracle(ddns, oracle_mod, forcing, prec):
             assert np.abs(bg - bo).max() < 1e-12 * max(1.0, np.abs(bo).max())
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_flow_statistics_match_oracle(ddns, oracle_mod, prec):
+    """POSTPROC and TRANSPOSTPROC statistics (postprocessing.f90:76-315) of one u-hat:
+    the energy, dissipation and transfer spectra and every monitora/b/c field.  FP64:
+    summation orders differ, 1e-10 relative; FP32-mixed: the physical fields are
+    float, 1e-4 relative (skewness 2e-3 absolute, the divergence of a solenoidal
+    field only to its float rounding)."""
+    N = 32
+    v = random_spectral(N, seed=3, kmax=7)          # a field whose triads carry transfer
+    kw = dict(N=N, forcing=0, dt=0.003, rnu=0.02)
+    with ddns.Solver(_params(ddns, precision=prec, **kw)) as s, oracle_mod.Oracle(_params(ddns, **kw)) as o:
+        s.set_flow(v, v)
+        o.set_flow(v, v)
+        gs, os_ = s.spectra(), o.spectra()
+        gm, om = s.monitors(divstat=True), o.monitors(divstat=True)
+    rt = 1e-10 if prec == 0 else 1e-4
+    assert np.abs(os_[2]).max() > 0.1
+    for g, r in zip(gs, os_):
+        assert np.abs(g - r).max() <= rt * np.abs(r).max(), (g, r)
+    assert abs(os_[2].sum()) < 1e-10 * np.abs(os_[2]).sum()
+    for f in ("eta", "tmse", "xintls", "tk", "et", "xl", "uprime", "qq", "dissp", "ree", "resol", "cfl",
+              "vflat", "pmax", "vort1", "vort2"):
+        a, b = getattr(gm, f), getattr(om, f)
+        assert abs(a - b) <= rt * abs(b), (f, a, b)
+    assert abs(gm.vskew - om.vskew) <= (1e-10 if prec == 0 else 2e-3), (gm.vskew, om.vskew)
+    div_tol = 1e-9 if prec == 0 else 1e-3
+    assert abs(gm.vdiv1) < div_tol and gm.vdiv2 < div_tol and abs(gm.divvt) < div_tol
+
+
 def test_physical_velocity_matches_oracle(ddns, oracle_mod):
     N = 32
     v = random_spectral(N, seed=11, kmax=6)
