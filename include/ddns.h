@@ -256,6 +256,13 @@ int ddns_get_collision_bins(ddns_handle* h, int* nirbin, double* wrbin, double* 
  * at most cap triples are written. */
 int ddns_get_collision_pairs(ddns_handle* h, int64_t* triples, int cap, int* npairs);
 
+/* Records of the last step's droplet pairs in contact (rcolla < d <= rcollb), the
+ * buffer(1:12, ij) that COLLISION_DET writes to Vp<i1><i2>.dat (partlib.f90:1431-1437,
+ * 1775-1797): 12 doubles each, index1, index2, va(3), vb(3), unit offset xd(3), Wr, in
+ * the one-rank visiting order (ih1, cell loop, linked list).  *n gets the full count;
+ * at most cap records are written (rec may be NULL). */
+int ddns_get_contact_records(ddns_handle* h, double* rec, int cap, int* n);
+
 /* HIP-event timings of the last ddns_step call. */
 int ddns_get_timing(ddns_handle* h, ddns_timing* out);
 

@@ -228,6 +228,16 @@ class Solver:
         self._check(self.lib.ddns_get_collision_pairs(self.h, t.ctypes.data_as(C.POINTER(C.c_int64)), n.value, C.byref(n)))
         return t
 
+    def contact_records(self):
+        """(m, 12) records of the last step's pairs in contact, buffer(1:12, ij) of
+        COLLISION_DET (partlib.f90:1431-1437), in the one-rank visiting order."""
+        n = C.c_int()
+        self._check(self.lib.ddns_get_contact_records(self.h, None, 0, C.byref(n)))
+        r = np.zeros((12, n.value), order="F")        # record-major in memory
+        if n.value:
+            self._check(self.lib.ddns_get_contact_records(self.h, _dp(r), n.value, C.byref(n)))
+        return r.T
+
     def timing(self) -> Timing:
         t = Timing()
         self._check(self.lib.ddns_get_timing(self.h, C.byref(t)))

@@ -170,6 +170,7 @@ SIGNATURES = {
     "ddns_step": (C.c_int, [_P, C.c_int]),
     "ddns_get_collision_counts": (C.c_int, [_P, C.POINTER(CollisionCounts)]),
     "ddns_get_collision_bins": (C.c_int, [_P, _I, _D, _D, _D, _D, _D, _D, _D]),
+    "ddns_get_contact_records": (C.c_int, [_P, _D, C.c_int, _I]),
     "ddns_get_collision_pairs": (C.c_int, [_P, _I64, C.c_int, _I]),
     "ddns_get_timing": (C.c_int, [_P, C.POINTER(Timing)]),
     "ddns_fft_r2c": (C.c_int, [_P, _D]),

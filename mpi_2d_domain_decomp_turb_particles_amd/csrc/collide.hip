@@ -18,6 +18,9 @@
 //   - the cell indices, windows and clamps are the reference's (1288-1306, 2407-2429).
 // The droplet that plays ih1 is therefore the one that sees its partner through an
 // upper image, or the lower slot number for a pair without y/z image.
+#include <algorithm>
+#include <cstring>
+
 #include "ddns_internal.h"
 
 namespace ddns {
@@ -133,6 +136,9 @@ struct CollArgs {
     int cand_cap;
     int* ncand;
     int* hit;              // per droplet: ih1 of a collision already counted
+    double* contact;       // DDNS_CONTACT_REC per pair in contact, count at counts[258]
+    int contact_cap;
+    int* ncontact;         // pairs in contact found by the search (sizes contact)
 };
 
 // the largest displacement length |dmove| of the step, as the bits of a non-negative
@@ -228,9 +234,37 @@ __device__ __forceinline__ void collide_pair(const CollArgs& a, int ia, const do
                 atomicAdd(a.bins + 6 * nb + bi, sb * sb / dnij0);
             }
         }
-        if (dnij0 <= rcollb) {
+        if (dnij0 <= rcollb) {                        // 1411-1437
             if (dnij0 <= rcoll) findindex(a.radius, ns, ra, rb, &i1, &i2);
             atomicAdd(a.counts + 108 + (i1 - 1) + DDNS_MAXSET * (i2 - 1), 1);
+            const int slot = atomicAdd(a.counts + 258, 1);
+            if (slot < a.contact_cap) {
+                // the record buffer(1:12, ij), with the visiting order of the pair in front
+                double va[3], vb[3], xh[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    va[c] = a.fvp0[c * C + ia];
+                    vb[c] = a.fvp0[c * C + ib];
+                    xh[c] = xd[c] / dnij0;
+                }
+                const double Vr1 = va[0] * xh[0] + va[1] * xh[1] + va[2] * xh[2];
+                const double Vr2 = vb[0] * xh[0] + vb[1] * xh[1] + vb[2] * xh[2];
+                const int dir = (wy == 0 && wz == 0) ? 0 : (wz == 0 ? 5 : (wy == 1 ? 6 : (wy == 0 ? 7 : 8)));
+                const long long ypn = (long long)dir * a.cap + ib;
+                const long long cell = ((long long)(di + 128) << 16) | ((dj + 128) << 8) | (dk + 128);
+                double* r = a.contact + (size_t)DDNS_CONTACT_REC * slot;
+                r[0] = __longlong_as_double((long long)ia);
+                r[1] = __longlong_as_double((cell << 40) | (((1LL << 40) - 1) - ypn));
+                r[2] = (double)i1;
+                r[3] = (double)i2;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    r[4 + c] = va[c];
+                    r[7 + c] = vb[c];
+                    r[10 + c] = xh[c];
+                }
+                r[13] = Vr1 - Vr2;
+            }
         }
     }
     if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) return;   // 1445
@@ -347,6 +381,7 @@ __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
         cc[c] = q < 0 ? 0 : (q >= g.G ? g.G - 1 : q);
     }
     long long tested = 0;
+    int touching = 0;
     // rcoll + dmove_a + dmove_b <= rcoll + dmove_a + dmax: a pair farther apart than the
     // right side cannot pass the collision time-window test
     const double dmax = __longlong_as_double((long long)*a.dmax);
@@ -367,6 +402,7 @@ __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
         const double rcoll = ra + cb.w;
         const double rcolla = a.hdi ? rcoll : rcoll * (1. - a.st / 100.);
         const bool stat = dnij0 > rcolla && dnij0 <= 10 * rcoll;                         // 1358
+        touching += (stat && dnij0 <= rcoll * (1. + a.st / 100.)) ? 1 : 0;               // 1411
         const bool near = !(di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) &&   // 1445
                           !(dnij0 > rcoll + dmove_a + dmax);
         const bool emit = stat || near;
@@ -384,6 +420,7 @@ __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
         }
     });
     if (tested) atomicAdd((unsigned long long*)a.tested, (unsigned long long)tested);
+    if (touching) atomicAdd(a.ncontact, touching);
 }
 
 // Second half: one thread per handed-on pair redoes the pair's geometry exactly as the
@@ -535,14 +572,16 @@ int part_collide(PartBuffers& pb, const PartConst& pc, hipStream_t s)
     DDNS_HIP(hipMemsetAsync(pb.hit, 0, (size_t)pc.n * sizeof(int), s));
     a.hit = pb.hit;
     a.ncand = pb.flags + 40;
-    int ncand = 0;
+    a.ncontact = pb.flags + 41;
+    int found[2] = {0, 0};       // handed-on pairs, pairs in contact
+    int& ncand = found[0];
     for (;;) {
         a.cand = (int2*)pb.cand;
         a.cand_cap = pb.cand_cap;
-        DDNS_HIP(hipMemsetAsync(a.ncand, 0, sizeof(int), s));
+        DDNS_HIP(hipMemsetAsync(a.ncand, 0, 2 * sizeof(int), s));
         hipLaunchKernelGGL(k_collide, dim3((pc.n + CT - 1) / CT), dim3(CT), 0, s, a);
         DDNS_HIP(hipGetLastError());
-        DDNS_HIP(hipMemcpyAsync(&ncand, a.ncand, sizeof(int), hipMemcpyDeviceToHost, s));
+        DDNS_HIP(hipMemcpyAsync(found, a.ncand, sizeof(found), hipMemcpyDeviceToHost, s));
         DDNS_HIP(hipStreamSynchronize(s));
         if (ncand <= pb.cand_cap) break;
         // more pairs to hand on than the buffer holds: widen it and search again
@@ -551,9 +590,44 @@ int part_collide(PartBuffers& pb, const PartConst& pc, hipStream_t s)
         DDNS_HIP(hipMalloc(&pb.cand, (size_t)pb.cand_cap * sizeof(int2)));
         DDNS_HIP(hipMemsetAsync(pb.tested, 0, sizeof(long long), s));
     }
+    // the contact records of the pair kernel: the search has counted them
+    if (found[1] > pb.contact_cap) {
+        if (pb.contact) DDNS_HIP(hipFree(pb.contact));
+        pb.contact_cap = std::max(4096, found[1] + found[1] / 2);
+        DDNS_HIP(hipMalloc(&pb.contact, (size_t)pb.contact_cap * DDNS_CONTACT_REC * sizeof(double)));
+    }
+    a.contact = pb.contact;
+    a.contact_cap = pb.contact_cap;
     if (ncand > 0)
         hipLaunchKernelGGL(k_collide_pairs, dim3((ncand + 255) / 256), dim3(256), 0, s, a, ncand);
     DDNS_HIP(hipGetLastError());
+    return 0;
+}
+
+// The last detection's contact records, buffer(1:12, ij) of partlib.f90:1431-1437, in the
+// reference's visiting order: ih1 ascending, then the (i, j, k) cell loop, then the linked
+// list of the cell from high to low ypn index.
+int part_read_contacts(const PartBuffers& pb, double* rec, int cap, int* n, hipStream_t s)
+{
+    int cnt = 0;
+    DDNS_HIP(hipMemcpyAsync(&cnt, pb.counts + 258, sizeof(int), hipMemcpyDeviceToHost, s));
+    DDNS_HIP(hipStreamSynchronize(s));
+    *n = cnt;
+    if (cnt > pb.contact_cap) { set_error("contact record overflow"); return DDNS_ERR_CAPACITY; }
+    if (!rec || cap <= 0 || cnt == 0) return 0;
+    std::vector<double> raw((size_t)cnt * DDNS_CONTACT_REC);
+    DDNS_HIP(hipMemcpyAsync(raw.data(), pb.contact, raw.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    DDNS_HIP(hipStreamSynchronize(s));
+    std::vector<int> order(cnt);
+    for (int i = 0; i < cnt; ++i) order[i] = i;
+    auto key = [&](int i, int w) { long long k; std::memcpy(&k, &raw[(size_t)i * DDNS_CONTACT_REC + w], 8); return k; };
+    std::sort(order.begin(), order.end(), [&](int x, int y) {
+        const long long x0 = key(x, 0), y0 = key(y, 0);
+        return x0 != y0 ? x0 < y0 : key(x, 1) < key(y, 1);
+    });
+    const int m = std::min(cnt, cap);
+    for (int i = 0; i < m; ++i)
+        std::memcpy(rec + (size_t)12 * i, &raw[(size_t)order[i] * DDNS_CONTACT_REC + 2], 12 * sizeof(double));
     return 0;
 }
 

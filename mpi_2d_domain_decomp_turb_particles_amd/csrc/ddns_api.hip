@@ -884,6 +884,12 @@ extern "C" int ddns_get_collision_pairs(ddns_handle* h, int64_t* triples, int ca
     return keep(h, part_read_pairs(h->pb, h->pc, triples, cap, npairs, h->stream));
 }
 
+extern "C" int ddns_get_contact_records(ddns_handle* h, double* rec, int cap, int* n)
+{
+    if (!h || !n || cap < 0) return DDNS_ERR_ARG;
+    return keep(h, part_read_contacts(h->pb, rec, cap, n, h->stream));
+}
+
 extern "C" int ddns_get_timing(ddns_handle* h, ddns_timing* out)
 {
     if (!h || !out) return DDNS_ERR_ARG;

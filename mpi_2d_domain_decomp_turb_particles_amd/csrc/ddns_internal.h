@@ -26,6 +26,11 @@ namespace ddns {
 //   [7] the reference's visiting order of the pair (ih1, cell loop, linked list)
 constexpr int DDNS_PAIR_REC = 8;
 
+// One pair in contact (rcolla < d <= rcollb), in doubles: [0] ih1 slot and [1] the
+// cell-loop / linked-list order of ih2 (bit patterns of 64-bit integers), then the
+// reference's buffer(1:12, ij): index1, index2, va(3), vb(3), unit offset (3), Wr
+constexpr int DDNS_CONTACT_REC = 14;
+
 // Coarse-cell number of cell (x, y, z), 0 <= x, y, z < 256: the Morton (Z-curve)
 // interleave of the three indices, x in the lowest bit.  Droplets sorted by it form
 // compact 3-D blocks, so the lanes of a wavefront touch overlapping field stencils and
@@ -292,6 +297,8 @@ struct PartBuffers {
     int* counts = nullptr;      // icc[3], colltype[100], nir[100]
     long long* pairs = nullptr; // pair_cap * DDNS_PAIR_REC: one record per detected pair
     int pair_cap = 0;
+    double* contact = nullptr;  // contact_cap * DDNS_CONTACT_REC: one record per pair in contact
+    int contact_cap = 0;
     double* draws = nullptr;    // relocation draws
     int draw_cap = 0;
     int* draw_off = nullptr;    // cap + 1
@@ -329,6 +336,7 @@ int part_overlap(PartBuffers& pb, const PartConst& pc, int* sweeps, int* relocat
 int part_read_counts(const PartBuffers& pb, const PartConst& pc, ddns_collision_counts* cc, hipStream_t s);
 int part_read_bins(const PartBuffers& pb, const PartConst& pc, int* nirbin, double* const bins[7], hipStream_t s);
 int part_read_pairs(PartBuffers& pb, const PartConst& pc, int64_t* triples, int cap, int* npairs, hipStream_t s);
+int part_read_contacts(const PartBuffers& pb, double* rec, int cap, int* n, hipStream_t s);
 int part_coalesce(PartBuffers& pb, const PartConst& pc, int* removed, hipStream_t s);
 int part_set_rng(const PartBuffers& pb, int iseedr, hipStream_t s);
 

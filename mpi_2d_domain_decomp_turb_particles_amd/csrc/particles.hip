@@ -150,7 +150,7 @@ void part_free(PartBuffers& pb)
                   pb.cell_count, pb.cell_start, pb.cell_of, pb.cell_rank, pb.sorted, pb.pk, pb.scan_tmp,
                   pb.nei_cnt, pb.nei_idx, pb.nei_m, pb.gm_Q, pb.gm_v, pb.gm_x0, pb.gm_b, pb.gm_w,
                   pb.gm_red, pb.gm_small, pb.mult, pb.cand, pb.hit, pb.flags, pb.tested, pb.nirbin, pb.bins, pb.counts,
-                  pb.pairs, pb.draws, pb.draw_off, pb.rng_state, pb.red};
+                  pb.pairs, pb.contact, pb.draws, pb.draw_off, pb.rng_state, pb.red};
     for (void* p : ps) if (p) (void)hipFree(p);
     void* sp[] = {pb.spare.cell_count, pb.spare.cell_start, pb.spare.cell_of, pb.spare.sorted, pb.spare.pk};
     for (void* p : sp) if (p) (void)hipFree(p);

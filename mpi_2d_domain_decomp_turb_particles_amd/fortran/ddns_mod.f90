@@ -94,7 +94,7 @@ module ddns_mod
   public :: ddns_get_physical_velocity, ddns_get_spectrum, ddns_set_droplets, ddns_get_droplets
   public :: ddns_get_spectra, ddns_get_monitors
   public :: ddns_init_droplets, ddns_overlap_fix, ddns_step, ddns_get_collision_counts
-  public :: ddns_get_collision_bins, ddns_get_collision_pairs, ddns_get_timing
+  public :: ddns_get_collision_bins, ddns_get_collision_pairs, ddns_get_contact_records, ddns_get_timing
   public :: ddns_fft_r2c, ddns_fft_c2r, ddns_check
 
   interface
@@ -221,6 +221,13 @@ module ddns_mod
       integer(c_int64_t), intent(out) :: triples(*)
       integer(c_int), value :: cap
       integer(c_int), intent(out) :: npairs
+    end function
+    integer(c_int) function ddns_get_contact_records(h, rec, cap, n) bind(C, name="ddns_get_contact_records")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: rec(*)
+      integer(c_int), value :: cap
+      integer(c_int), intent(out) :: n
     end function
     integer(c_int) function ddns_get_timing(h, t) bind(C, name="ddns_get_timing")
       import :: c_int, c_ptr, ddns_timing

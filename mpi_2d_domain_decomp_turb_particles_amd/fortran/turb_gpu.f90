@@ -36,6 +36,14 @@ program turb_gpu
   double precision, allocatable :: vpd(:,:), dmv(:,:)
   double precision :: bforce(450)
   integer :: ivf(32), iyf, isf, nin, nprocY_in, nps_r, nps_tot, ir, c
+  ! COLLISION_DET's statistics files (partlib.f90:1641-1797)
+  integer, allocatable :: nirb(:,:,:), nirb0(:,:,:), ubin(:,:,:), uvp(:,:)
+  double precision, allocatable :: wb(:,:,:,:), wb0(:,:,:,:), crec(:,:)
+  integer :: i1, i2, k, itot
+  integer(c_int) :: ncrec
+  character(len=32) :: fname, fmt125, fmt126
+  character(len=8), parameter :: binname(8) = [character(len=8) :: 'nirbin', 'wrbin', 'wxbin', 'wybin', &
+                                                'wzbin', 'urbin', 'thetabin', 'wrtanbin']
   integer(c_int) :: stat, prec, np_c
   integer(8) :: c0, c1, crate
 
@@ -193,6 +201,26 @@ program turb_gpu
   end if
 
   open(40, file='collision.dat', status='unknown')
+  if (npart > 0) then
+    ! turb.f90:351-395: the pair statistics of every set pair i1 <= i2
+    open(51, file='detectpairs.dat', status='unknown')
+    allocate(nirb(nset, nset, DDNS_NBIN), nirb0(nset, nset, DDNS_NBIN), ubin(8, nset, nset), uvp(nset, nset))
+    allocate(wb(nset, nset, DDNS_NBIN, 7), wb0(nset, nset, DDNS_NBIN, 7), crec(12, 1024))
+    nirb = 0
+    wb = 0.d0
+    do i1 = 1, nset
+      do i2 = i1, nset
+        do k = 1, 8
+          write(fname, '(a, i0, i0, a)') trim(binname(k)), i1, i2, '.dat'
+          open(newunit=ubin(k, i1, i2), file=trim(fname), status='unknown')
+        end do
+        write(fname, '(a, i0, i0, a)') 'Vp', i1, i2, '.dat'
+        open(newunit=uvp(i1, i2), file=trim(fname), position='append', status='unknown')
+      end do
+    end do
+    write(fmt125, '(a, i0, a)') '(', DDNS_NBIN, '(i6))'
+    write(fmt126, '(a, i0, a)') '(', DDNS_NBIN, '(1pe12.4))'
+  end if
   ! the reference's statistics files (turb.f90:356-360), written from the state at the
   ! start of step jstep as POSTPROC / TRANSPOSTPROC see it (turb.f90:628-656)
   open(41, file='spectrum.dat', status='unknown')
@@ -230,11 +258,8 @@ program turb_gpu
     call ddns_check(stat, h, 'ddns_step')
     stat = ddns_get_timing(h, tm)
     t_total = t_total + tm%total_ms
-    time = time + dt
-    if (npart > 0) then
-      stat = ddns_get_collision_counts(h, cc)
-      write(40, '(i8, 1pe16.8, 5i10)') istep, time, cc%icc, cc%ncoll, cc%npairs
-    end if
+    if (npart > 0) call collision_files()
+    time = time + dt                                                  ! turb.f90:693
     if (ieout > 0 .and. mod(istep, ieout) == 0) then
       stat = ddns_get_spectrum(h, ek)
       write(*, '(a, i8, a, 1pe14.6, a, 1pe14.6, a, 0pf9.3, a)') ' istep', istep, '  time', time, &
@@ -243,6 +268,17 @@ program turb_gpu
   end do
   call system_clock(c1)
   close(40)
+  if (npart > 0) then
+    close(51)
+    do i1 = 1, nset
+      do i2 = i1, nset
+        do k = 1, 8
+          close(ubin(k, i1, i2))
+        end do
+        close(uvp(i1, i2))
+      end do
+    end do
+  end if
   close(41)
   close(42)
   close(43)
@@ -286,6 +322,51 @@ program turb_gpu
   write(*,*) 'Successfully completed'
 
 contains
+
+  ! COLLISION_DET's output after the step, at the step's start time (partlib.f90:1641-1797):
+  ! the bin increments are summed and written every fifth step (mod(istep+1, 5) = 0), with
+  ! thetabin as the mean angle per pair; collision.dat and detectpairs.dat get a line when
+  ! they have a count; each pair in contact adds a line to Vp<index1><index2>.dat.
+  subroutine collision_files()
+    stat = ddns_get_collision_counts(h, cc)
+    call ddns_check(stat, h, 'ddns_get_collision_counts')
+    stat = ddns_get_collision_bins(h, nirb0, wb0(:,:,:,1), wb0(:,:,:,2), wb0(:,:,:,3), wb0(:,:,:,4), &
+                                   wb0(:,:,:,5), wb0(:,:,:,6), wb0(:,:,:,7))
+    call ddns_check(stat, h, 'ddns_get_collision_bins')
+    nirb = nirb + nirb0                                                ! 1649-1656
+    wb = wb + wb0
+    if (mod(istep + 1, 5) == 0) then                                   ! 1658-1712
+      where (nirb /= 0) wb(:,:,:,6) = wb(:,:,:,6) / real(nirb)
+      do i1 = 1, nset
+        do i2 = i1, nset
+          write(ubin(1, i1, i2), fmt125) nirb(i1, i2, :)
+          do k = 1, 7
+            write(ubin(k + 1, i1, i2), fmt126) wb(i1, i2, :, k)
+          end do
+        end do
+      end do
+      nirb = 0
+      wb = 0.d0
+    end if
+    itot = cc%icc(1) + cc%icc(2) + cc%icc(3)                           ! 1714-1727
+    if (itot > 0) write(40, '(1pe15.5, 7i10)') time, cc%icc, cc%colltype(1, 1), cc%colltype(1, 2), &
+                                                cc%colltype(2, 2)
+    itot = sum(cc%nir(1:nset, 1:nset))                                 ! 1729-1739
+    if (itot > 0) write(51, '(1pe15.5, 3i10)') time, cc%nir(1, 1), cc%nir(1, 2), cc%nir(2, 2)
+    stat = ddns_get_contact_records(h, crec, int(size(crec, 2), c_int), ncrec)
+    call ddns_check(stat, h, 'ddns_get_contact_records')
+    if (ncrec > size(crec, 2)) then
+      deallocate(crec)
+      allocate(crec(12, ncrec))
+      stat = ddns_get_contact_records(h, crec, ncrec, ncrec)
+      call ddns_check(stat, h, 'ddns_get_contact_records')
+    end if
+    do k = 1, ncrec                                                    ! 1775-1797
+      i1 = int(crec(1, k))
+      i2 = int(crec(2, k))
+      write(uvp(i1, i2), '(f10.6, 10(1x, f11.5))') time, crec(3:12, k)
+    end do
+  end subroutine collision_files
 
   ! 'part.0007', load_save.f90:107-116
   function rank_name(stem, id) result(f)

@@ -498,6 +498,8 @@ struct orc_sim {
     double *wrbin0, *wxbin0, *wybin0, *wzbin0, *urbin0, *thetabin0, *wrtan0;
     int64_t* pairs;              /* 3 per pair */
     int npairs, pairs_cap;
+    double* contact;             /* 12 per pair in contact: buffer(1:12,ij) of partlib.f90:1431-1437 */
+    int ncontact, contact_cap;
     int* removepart;
     int ncoll, remove_cap;
     double* collist;             /* 20 per record, partlib.f90:1557-1562 */
@@ -607,6 +609,8 @@ int orc_create(const ddns_params* p, orc_sim** out)
     s->pairs_cap = 1024;
     s->pairs = (int64_t*)xcalloc((size_t)3 * s->pairs_cap, sizeof(int64_t));
     s->collist = (double*)xcalloc((size_t)20 * s->pairs_cap, 8);
+    s->contact_cap = 1024;
+    s->contact = (double*)xcalloc((size_t)12 * s->contact_cap, 8);
     s->remove_cap = 1024;
     s->removepart = (int*)xcalloc(s->remove_cap, sizeof(int));
     /* the head array spans ny_max halo cells on each side (partlib.f90:159) */
@@ -627,7 +631,7 @@ void orc_destroy(orc_sim* s)
     free(s->nei_num); free(s->nei_data);
     free(s->nirbin0); free(s->wrbin0); free(s->wxbin0); free(s->wybin0); free(s->wzbin0);
     free(s->urbin0); free(s->thetabin0); free(s->wrtan0);
-    free(s->pairs); free(s->collist); free(s->removepart);
+    free(s->pairs); free(s->collist); free(s->removepart); free(s->contact);
     free(s);
 }
 
@@ -1540,6 +1544,7 @@ static void collision_det(orc_sim* s)
     memset(s->wrtan0, 0, 8 * s->nbins);
     s->npairs = 0;
     s->ncoll = 0;
+    s->ncontact = 0;
     nps4 = nps + s->ih_p[0] + s->ih_p[1] + s->ih_p[2] + s->ih_p[3];   /* 1256-1260 */
     nps5 = nps4 + s->ih_p[4];
     nps6 = nps5 + s->ih_p[5];
@@ -1628,9 +1633,26 @@ static void collision_det(orc_sim* s)
                                 }
                             }
                             if (dnij0 <= rcollb) {                                       /* 1411-1439 */
-                                if (dnij0 <= rcoll)
+                                double* r;
+                                if (dnij0 <= rcoll) {
+                                    /* the offset is normalised here, once, as in the branch above */
+                                    double Vr1, Vr2;
+                                    for (c = 0; c < 3; ++c) xd[c] = xd[c] / dnij0;
+                                    Vr1 = va[0] * xd[0] + va[1] * xd[1] + va[2] * xd[2];
+                                    Vr2 = vb[0] * xd[0] + vb[1] * xd[1] + vb[2] * xd[2];
+                                    Wr = Vr1 - Vr2;
                                     findindex(s, YP(7, ih1), YPN(7, ih2), &index1, &index2);
+                                }
                                 s->cc.nir[(index1 - 1) + DDNS_MAXSET * (index2 - 1)] += 1;
+                                if (s->ncontact >= s->contact_cap) {
+                                    s->contact_cap *= 2;
+                                    s->contact = (double*)realloc(s->contact, sizeof(double) * 12 * s->contact_cap);
+                                }
+                                r = s->contact + 12 * (size_t)s->ncontact++;             /* 1431-1437 */
+                                r[0] = (double)index1;
+                                r[1] = (double)index2;
+                                for (c = 0; c < 3; ++c) { r[2 + c] = va[c]; r[5 + c] = vb[c]; r[8 + c] = xd[c]; }
+                                r[11] = Wr;
                             }
                         }
                         /* detection over adjacent cells only, 1445 */
@@ -2254,6 +2276,14 @@ int orc_get_collision_pairs(orc_sim* s, int64_t* triples, int cap, int* npairs)
     int m = s->npairs < cap ? s->npairs : cap;
     *npairs = s->npairs;
     if (triples && m > 0) memcpy(triples, s->pairs, sizeof(int64_t) * 3 * (size_t)m);
+    return 0;
+}
+
+int orc_get_contact_records(orc_sim* s, double* rec, int cap, int* n)
+{
+    int m = s->ncontact < cap ? s->ncontact : cap;
+    *n = s->ncontact;
+    if (rec && m > 0) memcpy(rec, s->contact, sizeof(double) * 12 * (size_t)m);
     return 0;
 }
 

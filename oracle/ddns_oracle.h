@@ -46,6 +46,9 @@ int  orc_get_collision_bins(orc_sim* s, int* nirbin, double* wrbin, double* wxbi
                             double* wybin, double* wzbin, double* urbin,
                             double* thetabin, double* wrtanbin);
 int  orc_get_collision_pairs(orc_sim* s, int64_t* triples, int cap, int* npairs);
+/* The last detection's contact records in visiting order, 12 doubles each: index1,
+ * index2, va(3), vb(3), unit offset xd(3), Wr (buffer(1:12,ij), partlib.f90:1431-1437). */
+int  orc_get_contact_records(orc_sim* s, double* rec, int cap, int* n);
 int  orc_get_removal_list(orc_sim* s, int* list, int cap, int* n);
 int  orc_collision_detect(orc_sim* s, int n, const double* yp, const double* dmove,
                           const double* fvp0, const double* vnew, const double* vp);

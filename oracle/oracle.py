@@ -55,6 +55,7 @@ def load() -> C.CDLL:
         "orc_get_collision_bins": (C.c_int, [_P, _I] + [_D] * 7),
         "orc_get_collision_pairs": (C.c_int, [_P, C.POINTER(C.c_int64), C.c_int, _I]),
         "orc_get_removal_list": (C.c_int, [_P, _I, C.c_int, _I]),
+        "orc_get_contact_records": (C.c_int, [_P, _D, C.c_int, _I]),
         "orc_collision_detect": (C.c_int, [_P, C.c_int] + [_D] * 5),
         "orc_overlap_fix": (C.c_int, [_P]),
         "orc_get_relocate_state": (C.c_int, [_P, _I, _I, _I, _I]),
@@ -233,6 +234,14 @@ class Oracle:
         t = np.zeros((max(n.value, 1), 3), dtype=np.int64)
         self.lib.orc_get_collision_pairs(self.h, t.ctypes.data_as(C.POINTER(C.c_int64)), n.value, C.byref(n))
         return t[: n.value]
+
+    def contact_records(self):
+        """(m, 12) records of the pairs in contact, in visiting order (partlib.f90:1431-1437)."""
+        n = C.c_int()
+        self.lib.orc_get_contact_records(self.h, None, 0, C.byref(n))
+        r = np.zeros((12, max(n.value, 1)), order="F")      # record-major in memory
+        self.lib.orc_get_contact_records(self.h, dp(r), n.value, C.byref(n))
+        return r.T[: n.value]
 
     def removal_list(self):
         n = C.c_int()

@@ -100,3 +100,45 @@ def test_statistics_files(tmp_path):
         assert abs(s[:, 1].sum() - mb[out, 1]) <= 1e-4 * mb[out, 1]      # list-directed and 1pe15.5 rounding
         assert abs(s[:, 2].sum() - mb[out, 2]) <= 1e-4 * mb[out, 2]
         assert abs(t[:, 1].sum()) <= 1e-9 * np.abs(t[:, 1]).sum()
+
+
+@pytest.mark.skipif(not DRIVER.exists(), reason="Fortran driver not built")
+def test_collision_statistics_files(tmp_path):
+    """COLLISION_DET's files in the reference's formats (partlib.f90:1641-1797): the
+    summed bins every fifth step, 180 per line, thetabin a mean angle in [0, pi/2];
+    collision.dat lines whose type counts and size-class counts add up to the same
+    total; one Vp<i1><i2>.dat line per contact that detectpairs.dat counts, carrying
+    a unit offset and the relative velocity along it."""
+    lines = BASE.read_text().split("\n")
+    lines[4] = "9"             # nstep: bins written after steps 4 and 9
+    lines[48] = ".FALSE."      # no HI
+    lines[52] = "100000"
+    (tmp_path / "p").write_text("\n".join(lines))
+    _run(tmp_path, tmp_path / "p")
+    total = 0
+    for pair in ("11", "12", "22"):
+        nir = np.loadtxt(tmp_path / f"nirbin{pair}.dat", ndmin=2)
+        assert nir.shape == (2, 180) and np.all(nir >= 0)
+        total += nir.sum()
+        for name in ("wrbin", "wxbin", "wybin", "wzbin", "urbin", "thetabin", "wrtanbin"):
+            b = np.loadtxt(tmp_path / f"{name}{pair}.dat", ndmin=2)
+            assert b.shape == (2, 180) and np.all(np.isfinite(b)), name
+        th = np.loadtxt(tmp_path / f"thetabin{pair}.dat", ndmin=2)
+        assert np.all(th >= 0) and np.all(th <= np.pi / 2 + 1e-4)
+        assert np.all(th[nir == 0] == 0)
+    assert total > 1000
+    coll = np.loadtxt(tmp_path / "collision.dat", ndmin=2)
+    if coll.size:
+        assert coll.shape[1] == 7
+        assert np.array_equal(coll[:, 1:4].sum(1), coll[:, 4:7].sum(1))
+    det = np.loadtxt(tmp_path / "detectpairs.dat", ndmin=2)
+    assert det.shape[1] == 4 and np.all(det[:, 1:].sum(1) > 0)
+    for k, pair in enumerate(("11", "12", "22")):
+        vp = np.loadtxt(tmp_path / f"Vp{pair}.dat", ndmin=2)
+        assert len(vp) == det[:, 1 + k].sum()
+        if len(vp):
+            assert vp.shape[1] == 11
+            assert np.abs(np.linalg.norm(vp[:, 7:10], axis=1) - 1).max() < 5e-5
+            wr = ((vp[:, 1:4] - vp[:, 4:7]) * vp[:, 7:10]).sum(1)
+            assert np.abs(wr - vp[:, 10]).max() < 5e-5 * (1 + np.abs(vp[:, 1:7]).max())
+    assert det[:, 1:].sum() > 0

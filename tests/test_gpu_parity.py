@@ -146,6 +146,13 @@ def test_collision_pairs_bit_identical(ddns, oracle_mod, hdi):
         assert np.array_equal(bo["nirbin"], bg["nirbin"])
         for k in ("wrbin", "wxbin", "wybin", "wzbin", "urbin", "thetabin", "wrtanbin"):
             assert np.allclose(bo[k], bg[k], rtol=1e-9, atol=1e-9), k
+        # the Vp<i1><i2>.dat records, in the reference's visiting order
+        ro, rg = o.contact_records(), s.contact_records()
+        assert len(ro) == sum(co.nir) and len(ro) > 3
+        assert rg.shape == ro.shape
+        assert np.array_equal(ro[:, :2], rg[:, :2])
+        assert np.array_equal(ro[:, 2:8], rg[:, 2:8])          # velocities are copied
+        assert np.allclose(ro[:, 8:], rg[:, 8:], rtol=0, atol=1e-13)
 
 
 def test_droplet_steps_match_oracle(ddns, oracle_mod):

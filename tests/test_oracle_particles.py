@@ -227,6 +227,36 @@ def test_collision_types_one_two_three(oracle_mod):
     assert _two_body(oracle_mod, [0, 0, 0], [2.5, 0, 0], [-1, 0, 0], [-1, 0, 0], [0, 0, 0], [0, 0, 0]) == []
 
 
+@pytest.mark.parametrize("frac", [0.995, 1.005, 1.025, 0.97])
+def test_contact_record_of_two_droplets(oracle_mod, frac):
+    """One pair at distance frac * (ra + rb), st = 2 %: the contact shell is
+    (0.98, 1.02] * rcoll without HI.  Inside it the record is buffer(1:12) of
+    partlib.f90:1431-1437 written out by hand: the size classes, both velocities
+    fvp(:, 0), the unit vector from the partner to ih1 and the relative velocity along
+    it; outside it there is none.  The larger droplet is ih2, so a transposed pair
+    or a swapped size class fails."""
+    p = _params(N=16, npart=2, nset=2, radius=[150.0, 250.0], dt=0.01)
+    with oracle_mod.Oracle(p) as o:
+        ra, rb = o.scaling()["radius"][:2]
+        u = np.array([2.0, -1.0, 2.0]) / 3.0
+        xb = np.array([3.0, 3.1, 2.9])
+        xa = xb + frac * (ra + rb) * u
+        v0 = np.asfortranarray(np.array([[0.3, -0.2, 0.5], [-0.1, 0.4, 0.2]]).T)
+        yp = _yp(np.asfortranarray(np.array([xa, xb]).T), np.array([ra, rb]))
+        z = np.zeros((3, 2), order="F")
+        o.collision_detect(yp, z, v0, z, z)
+        rec = o.contact_records()
+        nir = np.array(o.collision_counts().nir).reshape(10, 10, order="F")
+    if not 0.98 < frac <= 1.02:
+        assert rec.shape == (0, 12) and nir.sum() == 0
+        return
+    # FINDINDEX orders the two size classes, index1 <= index2 (partlib.f90:1878-1882)
+    assert rec.shape == (1, 12) and nir[0, 1] == 1 and nir.sum() == 1
+    va, vb = v0[:, 0], v0[:, 1]
+    expect = np.concatenate([[1.0, 2.0], va, vb, u, [np.dot(va - vb, u)]])
+    assert np.allclose(rec[0], expect, rtol=0, atol=1e-14)
+
+
 def test_rdf_of_uniform_droplets_is_one(oracle_mod):
     """Pair counts in the 180 bins of width R/20 over (R, 10R] match the ideal-gas
     expectation n(n-1)/2 * shell volume / box volume."""
