@@ -131,7 +131,7 @@ struct CollArgs {
     long long* pairs;      // DDNS_PAIR_REC per pair, see k_collide
     int pair_cap;
     int* mult;
-    const unsigned long long* dmax;   // bits of the largest |dmove| of this step
+    const unsigned long long* dmax;   // bits of the largest |dmove| of this step, and of the largest radius
     int2* cand;            // pairs handed from the search to k_collide_pairs
     int cand_cap;
     int* ncand;
@@ -141,28 +141,41 @@ struct CollArgs {
     int* ncontact;         // pairs in contact found by the search (sizes contact)
 };
 
-// the largest displacement length |dmove| of the step, as the bits of a non-negative
-// double (whose order is the numeric order), for the exact pre-filter of the
-// collision time window in k_collide
-__global__ void __launch_bounds__(256) k_dmove_max(const double* __restrict__ dmove, int n, int cap,
-                                                   unsigned long long* out)
+// The largest displacement length |dmove| of the step (out[0], when dmove is given) and
+// the largest droplet radius (out[1]), as the bits of non-negative doubles (whose order
+// is the numeric order): the exact pre-filter of the collision time window and the
+// reach of the candidate walks.  The radius is the droplets' own, which coalescence
+// can grow beyond the largest size class.
+__global__ void __launch_bounds__(256) k_extent_max(const double* __restrict__ dmove, const double* __restrict__ rad,
+                                                    int n, int cap, unsigned long long* out)
 {
     // grid-stride over a small fixed grid, then one atomic per work-group (one per
     // wavefront would serialise thousands of atomics on the single address)
-    __shared__ double wm[4];
+    __shared__ double wm[2][4];
     const size_t C = (size_t)cap;
-    double m = 0.0;
+    double m = 0.0, mr = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const double d0 = dmove[i], d1 = dmove[C + i], d2 = dmove[2 * C + i];
-        m = fmax(m, sqrt(d0 * d0 + d1 * d1 + d2 * d2));
+        if (dmove) {
+            const double d0 = dmove[i], d1 = dmove[C + i], d2 = dmove[2 * C + i];
+            m = fmax(m, sqrt(d0 * d0 + d1 * d1 + d2 * d2));
+        }
+        mr = fmax(mr, rad[i]);
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
-    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    for (int o = 32; o > 0; o >>= 1) {
+        m = fmax(m, __shfl_xor(m, o, 64));
+        mr = fmax(mr, __shfl_xor(mr, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        wm[0][threadIdx.x >> 6] = m;
+        wm[1][threadIdx.x >> 6] = mr;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        m = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
+        m = fmax(fmax(wm[0][0], wm[0][1]), fmax(wm[0][2], wm[0][3]));
+        mr = fmax(fmax(wm[1][0], wm[1][1]), fmax(wm[1][2], wm[1][3]));
         if (m > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+        if (mr > 0.0) atomicMax(out + 1, (unsigned long long)__double_as_longlong(mr));
     }
 }
 
@@ -357,18 +370,16 @@ __device__ __forceinline__ void collide_pair(const CollArgs& a, int ia, const do
 // the collision tests are handed on (cell-order position of ih1, of the partner and
 // the neighbour-cell number), so the heavy, rarely taken branches do not stall
 // whole wavefronts here.
-__global__ void __launch_bounds__(CT) k_collide(CollArgs a)
+__device__ __forceinline__ void collide_search(const CollArgs& a, int t, int lane, int& tested, int& touching)
 {
-    const int t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    if (t >= a.n) return;
     const int ia = a.sorted[t];
     const size_t C = (size_t)a.cap;
     const PairGeom& g = a.g;
-    double xa[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) xa[c] = a.pos[c * C + ia];
-    const double ra = a.rad[ia];
+    // position and radius from the droplet's own packed record (the same doubles as
+    // pos and rad, read contiguously in cell order instead of gathered)
+    const double4 me = a.pk[t];
+    const double xa[3] = {me.x, me.y, me.z};
+    const double ra = me.w;
     double dmove_a;
     {
         const double d0 = a.dmove[ia], d1 = a.dmove[C + ia], d2 = a.dmove[2 * C + ia];
@@ -380,11 +391,14 @@ __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
         int q = (int)(xa[c] / g.hc);
         cc[c] = q < 0 ? 0 : (q >= g.G ? g.G - 1 : q);
     }
-    long long tested = 0;
-    int touching = 0;
     // rcoll + dmove_a + dmove_b <= rcoll + dmove_a + dmax: a pair farther apart than the
     // right side cannot pass the collision time-window test
-    const double dmax = __longlong_as_double((long long)*a.dmax);
+    const double dmax = __longlong_as_double((long long)a.dmax[0]);
+    // a partner farther than 10 (ra + rbig) is outside the statistics and farther than
+    // ra + rbig + dmove_a + dmax outside the collision window, rbig the largest radius:
+    // only the cells within the larger of the two are walked
+    const double rbig = __longlong_as_double((long long)a.dmax[1]);
+    const double reach = fmax(10. * (ra + rbig), ra + rbig + dmove_a + dmax);
     for_each_candidate(a.cell_start, g.G, cc, a.pk, a.sorted, [&](int p, int kc, const double4& cb, int ib) {
         if (p == t) return;
         int wx, wy, wz;
@@ -418,9 +432,26 @@ __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
                 if (slot < a.cand_cap) a.cand[slot] = make_int2(t, (int)((unsigned)p | ((unsigned)kc << 27)));
             }
         }
-    });
-    if (tested) atomicAdd((unsigned long long*)a.tested, (unsigned long long)tested);
-    if (touching) atomicAdd(a.ncontact, touching);
+    }, reach_span(xa, cc, g.hc, reach));
+}
+
+__global__ void __launch_bounds__(CT) k_collide(CollArgs a)
+{
+    const int t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    int tested = 0, touching = 0;
+    if (t < a.n) collide_search(a, t, lane, tested, touching);
+    // one atomic per wavefront for the counters (one per lane would serialise a
+    // million atomics on two addresses)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        tested += __shfl_xor(tested, o, 64);
+        touching += __shfl_xor(touching, o, 64);
+    }
+    if (lane == 0) {
+        if (tested) atomicAdd((unsigned long long*)a.tested, (unsigned long long)tested);
+        if (touching) atomicAdd(a.ncontact, touching);
+    }
 }
 
 // Second half: one thread per handed-on pair redoes the pair's geometry exactly as the
@@ -483,6 +514,7 @@ struct OverArgs {
     const int *cell_start, *sorted;
     int n, cap;
     PairGeom g;
+    const unsigned long long* rbig;   // bits of the largest radius: a partner closer than ra + rbig can overlap
     int* cnt;
     int* total;
 };
@@ -504,7 +536,8 @@ __global__ void __launch_bounds__(CT) k_overlap(OverArgs a)
         cc[c] = q < 0 ? 0 : (q >= g.G ? g.G - 1 : q);
     }
     int cnt = 0;
-    // a light kernel: the row-wise walk needs no LDS, which keeps its occupancy high
+    // a light kernel: the row-wise walk needs no LDS, which keeps its occupancy high;
+    // only the cells within ra + rbig of the droplet can hold an overlapping partner
     for_each_candidate(a.cell_start, g.G, cc, a.pk, a.sorted, [&](int p, int kc, const double4& cb, int ib) {
         if (p == t) return;
         int wx, wy, wz;
@@ -517,7 +550,7 @@ __global__ void __launch_bounds__(CT) k_overlap(OverArgs a)
         const double dnij = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
         const double rcoll = ra + cb.w;
         if (dnij <= rcoll) ++cnt;
-    });
+    }, reach_span(xs, cc, g.hc, ra + __longlong_as_double((long long)*a.rbig)));
     a.cnt[ia] = cnt;
     if (cnt) atomicAdd(a.total, cnt);
 }
@@ -560,9 +593,9 @@ int part_collide(PartBuffers& pb, const PartConst& pc, hipStream_t s)
     a.nirbin = pb.nirbin; a.bins = pb.bins; a.counts = pb.counts; a.tested = pb.tested;
     a.pairs = pb.pairs; a.pair_cap = pb.pair_cap; a.mult = pb.mult;
     unsigned long long* dmax = (unsigned long long*)(pb.flags + 32);
-    DDNS_HIP(hipMemsetAsync(dmax, 0, sizeof(unsigned long long), s));
-    hipLaunchKernelGGL(k_dmove_max, dim3(std::min(256, (pc.n + 255) / 256)), dim3(256), 0, s, (const double*)pb.dmove, pc.n,
-                       pb.cap, dmax);
+    DDNS_HIP(hipMemsetAsync(dmax, 0, 2 * sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_extent_max, dim3(std::min(256, (pc.n + 255) / 256)), dim3(256), 0, s, (const double*)pb.dmove,
+                       (const double*)pb.rad, pc.n, pb.cap, dmax);
     a.dmax = dmax;
     if (!pb.hit) DDNS_HIP(hipMalloc(&pb.hit, (size_t)pb.cap * sizeof(int)));
     if (!pb.cand) {
@@ -637,6 +670,11 @@ int part_overlap(PartBuffers& pb, const PartConst& pc, int* sweeps, int* relocat
     *relocated = 0;
     if (pc.n == 0) return 0;
     const double d_overl = 2.0 * pc.radmax;                                // 2063
+    // the largest radius ([1]; relocation moves droplets, it does not resize them)
+    unsigned long long* rbig = (unsigned long long*)(pb.flags + 36);
+    DDNS_HIP(hipMemsetAsync(rbig, 0, 2 * sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_extent_max, dim3(std::min(256, (pc.n + 255) / 256)), dim3(256), 0, s, (const double*)nullptr,
+                       (const double*)pb.rad, pc.n, pb.cap, rbig);
     for (int it = 0; it < 10000; ++it) {
         // two droplets in adjacent reference cells (the window of partlib.f90:2091) are
         // less than 2 dx apart per component, so cells of width >= 2 dx keep every such
@@ -648,6 +686,7 @@ int part_overlap(PartBuffers& pb, const PartConst& pc, int* sweeps, int* relocat
         a.n = pc.n; a.cap = pb.cap;
         const int jmax = 1 + (int)(d_overl * (double)pc.N / pc.pi2);
         a.g = make_geom(pb, pc, d_overl, d_overl, jmax);
+        a.rbig = rbig + 1;
         a.cnt = pb.mult;
         a.total = pb.flags + 1;
         DDNS_HIP(hipMemsetAsync(pb.flags + 1, 0, sizeof(int), s));

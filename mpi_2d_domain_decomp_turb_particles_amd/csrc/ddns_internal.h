@@ -96,17 +96,40 @@ __device__ __forceinline__ bool cand_next(CandCursor& c, const int2* rng, int st
     return true;
 }
 
-// The candidates of the 27 neighbour cells in 9 rows: for each (y, z) offset the three
-// cells x-1, x, x+1 as one sequence, with the next candidate's packed record loaded
-// while f works on the current one.  The three ranges of a row stay in registers, so
-// no LDS is needed; a wavefront runs, per row, as long as its fullest lane.
-// f(p, kc, rec, ib) gets the sorted position, the cell number k = 9 (dx+1) + 3 (dy+1)
-// + (dz+1) of neighbour_shift, the record pk[p] and, when sorted is given, the
-// droplet index sorted[p].
+// Offsets (-1 or 0 up to 0 or +1) of the neighbour cells to visit along each axis.
+struct CellSpan {
+    int lo[3] = {-1, -1, -1};
+    int hi[3] = {1, 1, 1};
+};
+
+// The neighbour cells that can hold a point within distance reach of x, which lies in
+// cell cc of width hc: along each axis the lower (upper) neighbour is needed only when
+// x is closer than reach to the lower (upper) face.  The margin of 1e-9 hc covers the
+// rounding of the cell index; clamped and periodic cells stay covered because the
+// test only ever adds cells.
+__device__ __forceinline__ CellSpan reach_span(const double x[3], const int cc[3], double hc, double reach)
+{
+    const double m = reach + 1e-9 * hc;
+    CellSpan s;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        s.lo[c] = (x[c] - (double)cc[c] * hc < m) ? -1 : 0;
+        s.hi[c] = ((double)(cc[c] + 1) * hc - x[c] < m) ? 1 : 0;
+    }
+    return s;
+}
+
+// The candidates of the 27 neighbour cells (or of the part of them that span selects)
+// in 9 rows: for each (y, z) offset the cells x-1, x, x+1 as one sequence, with the
+// next candidate's packed record loaded while f works on the current one.  The three
+// ranges of a row stay in registers, so no LDS is needed; a wavefront runs, per row,
+// as long as its fullest lane.  f(p, kc, rec, ib) gets the sorted position, the cell
+// number k = 9 (dx+1) + 3 (dy+1) + (dz+1) of neighbour_shift, the record pk[p] and,
+// when sorted is given, the droplet index sorted[p].
 template <typename F>
 __device__ __forceinline__ void for_each_candidate(const int* __restrict__ cell_start, int G, const int cc[3],
                                                    const double4* __restrict__ pk, const int* __restrict__ sorted,
-                                                   F&& f)
+                                                   F&& f, const CellSpan& span = CellSpan())
 {
     int qy[3], qz[3], qx[3];
 #pragma unroll
@@ -118,12 +141,16 @@ __device__ __forceinline__ void for_each_candidate(const int* __restrict__ cell_
 #pragma unroll 1
     for (int r = 0; r < 9; ++r) {
         const int my = r / 3, mz = r % 3;
+        if (my - 1 < span.lo[1] || my - 1 > span.hi[1] || mz - 1 < span.lo[2] || mz - 1 > span.hi[2]) continue;
         int s[3], e[3];
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
-            const int cell = cell_code(qx[m], qy[my], qz[mz]);
-            s[m] = cell_start[cell];
-            e[m] = cell_start[cell + 1];
+            s[m] = e[m] = 0;
+            if (m - 1 >= span.lo[0] && m - 1 <= span.hi[0]) {
+                const int cell = cell_code(qx[m], qy[my], qz[mz]);
+                s[m] = cell_start[cell];
+                e[m] = cell_start[cell + 1];
+            }
         }
         // cursor over the three ranges
         int m = 0, p = s[0], end = e[0];

@@ -51,7 +51,7 @@ template <typename T, int N> constexpr int ky_tc() { return (sizeof(T) == 4) ? (
 #endif
 template <typename T, int N> constexpr int kz_tc()
 {
-    if constexpr (sizeof(T) == 4) return N <= 256 ? DDNS_KZ_TC_F32 : (N == 512 ? 8 : 4);
+    if constexpr (sizeof(T) == 4) return N < 256 ? 16 : (N == 256 ? DDNS_KZ_TC_F32 : (N == 512 ? 8 : 4));
     else return N <= 256 ? DDNS_KZ_TC_F64 : (N == 512 ? 4 : 2);
 }
 
@@ -391,6 +391,12 @@ __global__ void __launch_bounds__(NT) k_y(KyArgs a)
 // ------------------------------------------------------------------ x stage
 // Half-length packing of a real line of N points as N/2 complex points.
 template <int N> constexpr int kx_lines() { return 2048 / N; }   // 4 complex points per thread
+#ifndef DDNS_KXF_PTS
+#define DDNS_KXF_PTS 2048
+#endif
+// lines of one k_x_fused work-group: DDNS_KXF_PTS / N (QP = DDNS_KXF_PTS / 512 complex
+// points per thread and field)
+template <int N> constexpr int kxf_lines() { return DDNS_KXF_PTS / N >= 1 ? DDNS_KXF_PTS / N : 1; }
 
 // C2R pre-processing of the tile (in place, pairs k and M-k of every line).
 template <typename T, int N, int TL, int LDR>
@@ -473,10 +479,11 @@ __global__ void __launch_bounds__(NT) k_x_fused(KxArgs a)
 {
     using C = cpx<T>;
     constexpr int M = N / 2;
-    constexpr int TL = kx_lines<N>();
+    constexpr int TL = kxf_lines<N>();
     constexpr int LDR = TL + 1;
     constexpr int TS = M * LDR;
     constexpr int QP = TL * M / NT;
+    static_assert((TL * M) % NT == 0, "the lines must split evenly over the work-group");
     __shared__ C lds[3 * TS];
     __shared__ C tws[N], twhs[M];
     const int tid = threadIdx.x;
@@ -958,7 +965,7 @@ static int launch_x(const FlowBuffers& fb, hipStream_t s)
     for (int i = 0; i < 6; ++i) a.A[i] = fb.A[i];
     for (int i = 0; i < 3; ++i) a.B[i] = fb.B[i];
     a.uphys = fb.uphys; a.tw = fb.tw; a.twh = fb.twh;
-    dim3 grid((unsigned)((size_t)N * N / kx_lines<N>()));
+    dim3 grid((unsigned)((size_t)N * N / kxf_lines<N>()));
     hipLaunchKernelGGL((k_x_fused<T, N>), grid, dim3(NT), 0, s, a);
     DDNS_HIP(hipGetLastError());
     return 0;

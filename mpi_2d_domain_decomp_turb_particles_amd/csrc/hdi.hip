@@ -479,23 +479,62 @@ __device__ __forceinline__ void copy_range(double* dst, const double* src, size_
     for (size_t i = a + threadIdx.x; i < b; i += LT) dst[i] = src[i];
 }
 
+// The five ranges of the small problem that an iteration reads, copied to LDS with
+// the loads of up to 8 * LT values issued together: one memory round trip for the
+// usual iteration counts instead of one per range and pass.
+__device__ __forceinline__ void copy_in(double* sm, const double* __restrict__ gsm, GmSmall L, int iter)
+{
+    constexpr int KB = 8;
+    const size_t M1 = L.M1();
+    const size_t n0 = 16, n1 = M1 * iter, n2 = M1 * iter, n3 = M1 * (iter + 1), n4 = L.total() - L.B0();
+    const size_t tot = n0 + n1 + n2 + n3 + n4;
+    for (size_t b = 0; b < tot; b += (size_t)KB * LT) {
+        double r[KB];
+        size_t at[KB];
+#pragma unroll
+        for (int k = 0; k < KB; ++k) {
+            size_t j = b + (size_t)k * LT + threadIdx.x;
+            at[k] = ~(size_t)0;
+            if (j < tot) {
+                size_t a;
+                if (j < n0) a = j;
+                else if ((j -= n0) < n1) a = L.H0() + j;
+                else if ((j -= n1) < n2) a = L.R0() + j;
+                else if ((j -= n2) < n3) a = L.Q0() + j;
+                else a = L.B0() + (j - n3);
+                at[k] = a;
+                r[k] = gsm[a];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < KB; ++k)
+            if (at[k] != ~(size_t)0) sm[at[k]] = r[k];
+    }
+}
+
 __global__ void __launch_bounds__(LT) k_ls(double* gsm, GmSmall L, int iter, const double* __restrict__ normpart,
                                           int P)
 {
-    if (gsm[4] != 0.0) return;   // converged in an earlier iteration of this batch
     // the part of the small problem this iteration touches is copied to LDS, solved
     // there, and the changed part copied back
     __shared__ double sm[LS_MAX];
     const size_t M1 = L.M1();
-    // H(iter+1, iter) = |v| from the partials of the last projection kernel
+    // H(iter+1, iter) = |v| from the partials of the last projection kernel; their
+    // loads are issued together with those of the LDS copy
+    static_assert(RED_BLOCKS % LT == 0, "partials per lane");
+    double np[RED_BLOCKS / LT];
+#pragma unroll
+    for (int k = 0; k < RED_BLOCKS / LT; ++k) {
+        const int b = threadIdx.x + k * LT;
+        np[k] = b < P ? normpart[b] : 0.0;
+    }
+    const double done = gsm[4];
+    copy_in(sm, gsm, L, iter);
+    if (done != 0.0) return;     // converged in an earlier iteration of this batch
     double acc = 0.0;
-    for (int b = threadIdx.x; b < P; b += LT) acc += normpart[b];
+#pragma unroll
+    for (int k = 0; k < RED_BLOCKS / LT; ++k) acc += np[k];
     acc = wave_sum(acc);
-    copy_range(sm, gsm, 0, 16);
-    copy_range(sm, gsm, L.H0(), L.H0() + M1 * iter);
-    copy_range(sm, gsm, L.R0(), L.R0() + M1 * iter);
-    copy_range(sm, gsm, L.Q0(), L.Q0() + M1 * (iter + 1));
-    copy_range(sm, gsm, L.B0(), L.total());
     __syncthreads();
     if (threadIdx.x == 0) {
         const size_t e = L.H0() + iter + M1 * (iter - 1);
