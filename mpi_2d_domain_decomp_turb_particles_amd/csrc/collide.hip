@@ -399,10 +399,20 @@ __device__ __forceinline__ void collide_search(const CollArgs& a, int t, int lan
     // only the cells within the larger of the two are walked
     const double rbig = __longlong_as_double((long long)a.dmax[1]);
     const double reach = fmax(10. * (ra + rbig), ra + rbig + dmove_a + dmax);
+    const double reach2 = reach * reach * (1. + 1e-9);
     for_each_candidate(a.cell_start, g.G, cc, a.pk, a.sorted, [&](int p, int kc, const double4& cb, int ib) {
         if (p == t) return;
         int wx, wy, wz;
         neighbour_shift(kc, g.G, cc, &wx, &wy, &wz);
+        ++tested;
+        // the distance to the image the visiting rules would take (shift w * 2 pi): a
+        // partner beyond the reach can neither be counted nor collide, so the rules are
+        // replayed for the near ones only
+        {
+            const double e0 = xa[0] - (cb.x + (double)wx * g.h), e1 = xa[1] - (cb.y + (double)wy * g.h),
+                         e2 = xa[2] - (cb.z + (double)wz * g.h);
+            if (e0 * e0 + e1 * e1 + e2 * e2 > reach2) return;
+        }
         double xb[3];
         int di, dj, dk;
         if (!ref_visit(g, ia, ib, xa[0], xa[1], xa[2], cb.x, cb.y, cb.z,
@@ -412,7 +422,6 @@ __device__ __forceinline__ void collide_search(const CollArgs& a, int t, int lan
 #pragma unroll
         for (int c = 0; c < 3; ++c) xd[c] = xa[c] - xb[c];
         const double dnij0 = sqrt(xd[0] * xd[0] + xd[1] * xd[1] + xd[2] * xd[2]);
-        ++tested;
         const double rcoll = ra + cb.w;
         const double rcolla = a.hdi ? rcoll : rcoll * (1. - a.st / 100.);
         const bool stat = dnij0 > rcolla && dnij0 <= 10 * rcoll;                         // 1358
@@ -537,11 +546,19 @@ __global__ void __launch_bounds__(CT) k_overlap(OverArgs a)
     }
     int cnt = 0;
     // a light kernel: the row-wise walk needs no LDS, which keeps its occupancy high;
-    // only the cells within ra + rbig of the droplet can hold an overlapping partner
+    // only the cells within ra + rbig of the droplet can hold an overlapping partner,
+    // and the visiting rules are replayed for the partners within that distance only
+    const double reach = ra + __longlong_as_double((long long)*a.rbig);
+    const double reach2 = reach * reach * (1. + 1e-9);
     for_each_candidate(a.cell_start, g.G, cc, a.pk, a.sorted, [&](int p, int kc, const double4& cb, int ib) {
         if (p == t) return;
         int wx, wy, wz;
         neighbour_shift(kc, g.G, cc, &wx, &wy, &wz);
+        {
+            const double e0 = xa - (cb.x + (double)wx * g.h), e1 = ya - (cb.y + (double)wy * g.h),
+                         e2 = za - (cb.z + (double)wz * g.h);
+            if (e0 * e0 + e1 * e1 + e2 * e2 > reach2) return;
+        }
         double xb, yb, zb;
         int di, dj, dk;
         if (!ref_visit(g, ia, ib, xa, ya, za, cb.x, cb.y, cb.z, wx, wy, wz, &xb, &yb, &zb, &di, &dj, &dk)) return;
@@ -550,7 +567,7 @@ __global__ void __launch_bounds__(CT) k_overlap(OverArgs a)
         const double dnij = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
         const double rcoll = ra + cb.w;
         if (dnij <= rcoll) ++cnt;
-    }, reach_span(xs, cc, g.hc, ra + __longlong_as_double((long long)*a.rbig)));
+    }, reach_span(xs, cc, g.hc, reach));
     a.cnt[ia] = cnt;
     if (cnt) atomicAdd(a.total, cnt);
 }
