@@ -191,8 +191,9 @@ int ddns_get_flow(ddns_handle* h, double* vx, double* vy, double* vz,
 int ddns_set_forcing_state(ddns_handle* h, const double* b, int iseedf, int iyf, const int* ivf);
 int ddns_get_forcing_state(ddns_handle* h, double* b, int* iseedf, int* iyf, int* ivf);
 
-/* Generate the initial spectral field on the host and upload it (initial_flow,
- * flowlib.f90:53-144, with a portable seeded generator in place of IFPORT rand).
+/* Generate the initial spectral field on the device (initial_flow, flowlib.f90:53-144,
+ * with a counter-based SplitMix64 generator keyed by seed in place of IFPORT rand):
+ * Gaussian modes, Hermitian kx = 0 plane, projection, shell-by-shell scaling.
  * kind: DDNS_SPEC_*; k0 and urms are used by DDNS_SPEC_K4_GAUSS.  The history is set
  * to the velocity (turb.f90:452-454).  For forcing=2 the reference starts from rest
  * (turb.f90:441-450): pass kind < 0 to get that. */

@@ -398,111 +398,31 @@ extern "C" int ddns_get_forcing_state(ddns_handle* h, double* b, int* iseedf, in
     return DDNS_OK;
 }
 
-// Host restatement of the setup part of initial_flow (flowlib.f90:53-144) on the
-// single-domain layout; the generator is ranff seeded with -seed.
+// initial_flow (flowlib.f90:53-144) on the device: random solenoidal modes from a
+// counter-based generator keyed by seed, scaled shell by shell; kind < 0 is the flow at
+// rest of the stochastic forcing (turb.f90:441-449).  The history starts equal to the
+// velocity (turb.f90:452-454).
 extern "C" int ddns_init_flow(ddns_handle* h, int kind, int seed, double k0, double urms)
 {
     if (!h) return DDNS_ERR_ARG;
-    const int N = h->N, L = N + 2, nek = h->nek;
-    const size_t fs = (size_t)L * L * N;
-    std::vector<double> u[3] = {std::vector<double>(fs, 0.0), std::vector<double>(fs, 0.0), std::vector<double>(fs, 0.0)};
-    if (kind >= 0) {
-        const double pi = 4. * std::atan(1.);
-        int idum = -seed, iy = 0, iv[32] = {0};
-        auto I = [&](int i, int j, int k) { return (size_t)i + (size_t)L * ((size_t)j + (size_t)L * k); };
-        for (int c = 0; c < 3; ++c)                                           // gaussian, 158-172
-            for (int i = 0; i < N; ++i)
-                for (int j = 0; j < L; j += 2)
-                    for (int k = 0; k < N; ++k) {
-                        double t1 = host_ranff(&idum, &iy, iv), t2 = host_ranff(&idum, &iy, iv);
-                        if (t1 <= 1.e-10) t1 = 1.e-10;
-                        if (t2 <= 1.e-10) t2 = 1.e-10;
-                        t2 = 2 * pi * t2;
-                        u[c][I(i, j, k)] = std::sqrt(-2.0 * std::log(t1)) * std::cos(t2);
-                        u[c][I(i, j + 1, k)] = std::sqrt(-2.0 * std::log(t1)) * std::sin(t2);
-                    }
-        auto kxv = [&](int j) { return (double)(j / 2); };
-        auto kyv = [&](int k) { return (double)(k <= N / 2 ? k : k - N); };
-        auto kzv = [&](int i) { return (double)(i <= N / 2 ? i : i - N); };
-        auto k2v = [&](int i, int j, int k) { return (i == 0 && j <= 1 && k == 0) ? 1.e-5 : kxv(j) * kxv(j) + kyv(k) * kyv(k) + kzv(i) * kzv(i); };
-        auto ik2v = [&](int i, int j, int k) { return (int)(std::sqrt(k2v(i, j, k)) + 0.5); };
-        // symmetrize (intended form), 94-96
+    if (kind >= 0 && kind != DDNS_SPEC_K4_GAUSS && kind != DDNS_SPEC_KOLMOGOROV)
+        return fail(h, DDNS_ERR_ARG, "unknown initial spectrum");
+    if (kind == DDNS_SPEC_K4_GAUSS && !(k0 > 0.0)) return fail(h, DDNS_ERR_ARG, "k0 must be positive");
+    int st;
+    if (kind < 0) {
+        const size_t bytes = (size_t)h->N * h->N * (h->N / 2) * (h->prec == DDNS_FP64 ? sizeof(double2) : sizeof(float2));
         for (int c = 0; c < 3; ++c) {
-            std::vector<double>& v = u[c];
-            v[I(0, 0, 0)] = 0.0; v[I(0, 1, 0)] = 0.0;
-            for (int k = 0; k < N; ++k) for (int i = 0; i < L; ++i) { v[I(i, N, k)] = 0.0; v[I(i, N + 1, k)] = 0.0; }
-            for (int j = 0; j < L; ++j) for (int i = 0; i < L; ++i) v[I(i, j, N / 2)] = 0.0;
-            for (int k = 0; k < N; ++k) for (int j = 0; j < L; ++j) v[I(N / 2, j, k)] = 0.0;
-            std::vector<double> re(N * N), im(N * N);
-            for (int k = 0; k < N; ++k) for (int i = 0; i < N; ++i) { re[i + N * k] = v[I(i, 0, k)]; im[i + N * k] = v[I(i, 1, k)]; }
-            for (int k = 0; k < N; ++k)
-                for (int i = 0; i < N; ++i) {
-                    const int kk = (N - k) % N, ii = (N - i) % N;
-                    v[I(i, 0, k)] = 0.5 * (re[i + N * k] + re[ii + N * kk]);
-                    v[I(i, 1, k)] = 0.5 * (im[i + N * k] - im[ii + N * kk]);
-                }
-            v[I(0, 0, 0)] = 0.0; v[I(0, 1, 0)] = 0.0;
+            DDNS_HIP(hipMemsetAsync(h->fb.U[c], 0, bytes, h->stream));
+            DDNS_HIP(hipMemsetAsync(h->fb.O[c], 0, bytes, h->stream));
         }
-        // projection, 98-101
-        for (int k = 0; k < N; ++k)
-            for (int j = 0; j < L; ++j)
-                for (int i = 0; i < N; ++i) {
-                    const size_t o = I(i, j, k);
-                    const double kx = kxv(j), ky = kyv(k), kz = kzv(i), k2 = k2v(i, j, k);
-                    const double t = (kx * u[0][o] + ky * u[1][o] + kz * u[2][o]) / k2;
-                    u[0][o] -= kx * t; u[1][o] -= ky * t; u[2][o] -= kz * t;
-                }
-        // shell energies and scaling, 107-137
-        std::vector<double> e_t(nek + 1, 0.0), enumv(nek + 1, 0.0);
-        for (int k = 0; k < N; ++k)
-            for (int j = 0; j < L; ++j)
-                for (int i = 0; i < N; ++i) {
-                    const int s = ik2v(i, j, k);
-                    if (s < 1 || s > nek) continue;
-                    const size_t o = I(i, j, k);
-                    double t = u[0][o] * u[0][o] + u[1][o] * u[1][o] + u[2][o] * u[2][o];
-                    if (j <= 1) t *= 0.5;
-                    e_t[s] += t;
-                    enumv[s] += (kxv(j) > 0.1) ? 1.0 : 0.5;                   // turb.f90:408-415
-                }
-        std::vector<double> target(nek + 1, 0.0);
-        if (kind == DDNS_SPEC_K4_GAUSS) {
-            double tot = 0.0;
-            for (int s = 1; s <= nek; ++s) {
-                const double q = (double)s / k0;
-                target[s] = q * q * q * q * std::exp(-2.0 * q * q);
-                tot += target[s];
-            }
-            for (int s = 1; s <= nek; ++s) target[s] *= 1.5 * urms * urms / tot;
-        }
-        std::vector<double> fac(nek + 2, 1.0);
-        for (int s = 1; s <= nek; ++s) {
-            if (kind == DDNS_SPEC_K4_GAUSS) {
-                fac[s] = e_t[s] > 0 ? std::sqrt(target[s] / e_t[s]) : 0.0;
-            } else {
-                const double vol = 4. / 3. * pi * ((s + 0.5) * (s + 0.5) * (s + 0.5) - (s - 0.5) * (s - 0.5) * (s - 0.5));
-                const double ccc = h->p.force[0] * 4. / 3. * pi * (1.5 * 1.5 * 1.5 - 0.5 * 0.5 * 0.5) / enumv[1];
-                fac[s] = std::sqrt(ccc / e_t[s] * enumv[s] / vol) / std::pow((double)s, 0.833333);
-            }
-        }
-        // every mode belongs to exactly one shell: one pass applies all the shell factors
-        for (int k = 0; k < N; ++k)
-            for (int j = 0; j < L; ++j)
-                for (int i = 0; i < N; ++i) {
-                    const int s = ik2v(i, j, k);
-                    if (s < 1 || s > nek) continue;
-                    const size_t o = I(i, j, k);
-                    u[0][o] *= fac[s]; u[1][o] *= fac[s]; u[2][o] *= fac[s];
-                }
-        for (int c = 0; c < 3; ++c)                                           // dealiasing, 139-141
-            for (int k = 0; k < N; ++k)
-                for (int j = 0; j < L; ++j)
-                    for (int i = 0; i < L; ++i)
-                        if (i >= N || ik2v(i, j, k) > nek) u[c][I(i, j, k)] = 0.0;
+        DDNS_HIP(hipStreamSynchronize(h->stream));
+        st = 0;
+    } else {
+        st = flow_init(h->fb, h->N, h->nek, h->prec, kind, seed, k0, urms, h->p.force[0], h->stream);
     }
-    // the history starts equal to the velocity (turb.f90:452-454)
-    const int st = ddns_set_flow(h, u[0].data(), u[1].data(), u[2].data(), u[0].data(), u[1].data(), u[2].data());
-    return st;
+    h->a_valid = false;
+    h->phys_valid = false;
+    return keep(h, st);
 }
 
 extern "C" int ddns_get_physical_velocity(ddns_handle* h, double* ux, double* uy, double* uz)

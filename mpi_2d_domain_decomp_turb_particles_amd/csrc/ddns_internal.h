@@ -283,6 +283,8 @@ int flow_download_phys(const FlowBuffers& fb, int N, int precision, double* ux, 
 // any may be null): energy, dissipation and transfer spectra.  mom: [0] max |u|+|v|+|w|,
 // [1..3] sums of d^2, d^3, d^4 over the three longitudinal derivatives, [4..5] sums of
 // div, div^2, [6..8] sums of |w|, |w|^2, |w| div
+int flow_init(FlowBuffers& fb, int N, int nek, int precision, int kind, int seed, double k0, double urms, double force0,
+              hipStream_t s);
 int flow_diagnostics(FlowBuffers& fb, int N, int nek, double rnu, int precision, int want, double* ek, double* dk,
                      double* tk, double* mom, hipStream_t s);
 int flow_spectrum(const FlowBuffers& fb, int N, int nek, int precision, double* ek, hipStream_t s);

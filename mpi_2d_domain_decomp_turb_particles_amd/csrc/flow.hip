@@ -854,6 +854,102 @@ __global__ void k_spectrum(const void* u0, const void* u1, const void* u2, int n
     atomicAdd(ek + ik2 - 1, e);
 }
 
+// ------------------------------------------------------------------ initial field
+// The random start of initial_flow (flowlib.f90:53-144) on the device.  The reference
+// draws from Intel's IFPORT rand, which is not available; here every mode draws its own
+// pair of uniforms from a counter-based generator, SplitMix64 (Steele, Lea and Flood,
+// "Fast splittable pseudorandom number generators", OOPSLA 2014): output i of the
+// stream with key s is the SplitMix64 finaliser of s + (i + 1) * 0x9E3779B97F4A7C15.
+__device__ __forceinline__ double init_uniform(unsigned long long key, unsigned long long i)
+{
+    unsigned long long z = key + (i + 1ull) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * 0x1.0p-53;                      // [0, 1)
+}
+
+// Box-Muller pair of component c at mode (kz, ky, kx), as flowlib.f90:158-172 forms it
+// from two uniforms (each held above 1e-10)
+template <int N>
+__device__ __forceinline__ double2 init_gauss(unsigned long long key, int c, int kz, int ky, int kx)
+{
+    const unsigned long long m = (((unsigned long long)c * N + kz) * N + ky) * (N / 2) + kx;
+    double t1 = init_uniform(key, 2 * m), t2 = init_uniform(key, 2 * m + 1);
+    if (t1 <= 1.e-10) t1 = 1.e-10;
+    if (t2 <= 1.e-10) t2 = 1.e-10;
+    const double a = sqrt(-2.0 * log(t1)), ph = 2.0 * 3.14159265358979323846 * t2;
+    return make_double2(a * cos(ph), a * sin(ph));
+}
+
+// One thread per mode: the three Gaussian components, the Hermitian kx = 0 plane
+// ((g(k) + conj g(-k)) / 2, computed from both draws so the two halves agree), the
+// Nyquist planes and the mean set to zero, the projection onto k.u = 0 (flowlib.f90:
+// 94-101); then the shell energy (kx = 0 modes at half weight) and the shell's mode
+// count, 1 per mode with kx > 0 and 1/2 on the kx = 0 plane (flowlib.f90:107-118).
+template <typename T, int N>
+__global__ void k_init_modes(void* u0, void* u1, void* u2, unsigned long long key, int nek, double* ek, double* cnt)
+{
+    using C = cpx<T>;
+    constexpr int M = N / 2;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)N * N * M) return;
+    const int kx = (int)(t % M), ky = (int)((t / M) % N), kz = (int)(t / ((size_t)N * M));
+    const int kyw = wave_index(ky, N), kzw = wave_index(kz, N);
+    const bool zero = ky == N / 2 || kz == N / 2 || (kx == 0 && ky == 0 && kz == 0);
+    double2 g[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        g[c] = init_gauss<N>(key, c, kz, ky, kx);
+        if (kx == 0) {
+            const double2 h = init_gauss<N>(key, c, (N - kz) & (N - 1), (N - ky) & (N - 1), 0);
+            g[c] = make_double2(0.5 * (g[c].x + h.x), 0.5 * (g[c].y - h.y));
+        }
+        if (zero) g[c] = make_double2(0.0, 0.0);
+    }
+    const double k[3] = {(double)kx, (double)kyw, (double)kzw};
+    double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+    if (kx == 0 && ky == 0 && kz == 0) k2 = 1.e-5;
+    const double pr = (k[0] * g[0].x + k[1] * g[1].x + k[2] * g[2].x) / k2;
+    const double pi = (k[0] * g[0].y + k[1] * g[1].y + k[2] * g[2].y) / k2;
+    void* u[3] = {u0, u1, u2};
+    double e = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double x = g[c].x - k[c] * pr, y = g[c].y - k[c] * pi;
+        const C v = mk<C>((T)x, (T)y);
+        ((C*)u[c])[t] = v;
+        e += (double)v.x * v.x + (double)v.y * v.y;
+    }
+    const int ik2 = (int)(sqrt(k2) + 0.5);
+    if (ik2 < 1 || ik2 > nek) return;
+    atomicAdd(ek + ik2 - 1, kx == 0 ? 0.5 * e : e);
+    atomicAdd(cnt + ik2 - 1, kx == 0 ? 0.5 : 1.0);
+}
+
+// the shell factors (flowlib.f90:119-137), and zero above nek (139-141)
+template <typename T, int N>
+__global__ void k_init_scale(void* u0, void* u1, void* u2, int nek, const double* fac)
+{
+    using C = cpx<T>;
+    constexpr int M = N / 2;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)N * N * M) return;
+    const int kx = (int)(t % M), kyw = wave_index((int)((t / M) % N), N), kzw = wave_index((int)(t / ((size_t)N * M)), N);
+    double k2 = (double)(kx * kx + kyw * kyw + kzw * kzw);
+    if (k2 == 0.0) k2 = 1.e-5;
+    const int ik2 = (int)(sqrt(k2) + 0.5);
+    if (ik2 < 1) return;
+    const double f = ik2 <= nek ? fac[ik2 - 1] : 0.0;
+    void* u[3] = {u0, u1, u2};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        C v = ((C*)u[c])[t];
+        v = mk<C>((T)((double)v.x * f), (T)((double)v.y * f));
+        ((C*)u[c])[t] = v;
+    }
+}
+
 // ================================================================== host side
 static size_t field_elems(int N) { return (size_t)N * N * (N / 2); }
 
@@ -1162,6 +1258,68 @@ int flow_spectrum(const FlowBuffers& fb, int N, int nek, int precision, double* 
     return st;
 }
 
+// ---------------------------------------------------------------- initial field
+template <typename T, int N>
+static int init_impl(FlowBuffers& fb, int nek, int kind, int seed, double k0, double urms, double force0,
+                     hipStream_t s)
+{
+    const size_t tot = field_elems(N);
+    const unsigned g = (unsigned)((tot + 255) / 256);
+    double* d = nullptr;                       // shell energies, counts, factors (3 nek)
+    DDNS_HIP(hipMalloc(&d, 3 * (size_t)nek * sizeof(double)));
+    DDNS_HIP(hipMemsetAsync(d, 0, 2 * (size_t)nek * sizeof(double), s));
+    const unsigned long long key = (unsigned long long)(long long)seed;
+    hipLaunchKernelGGL((k_init_modes<T, N>), dim3(g), dim3(256), 0, s, fb.U[0], fb.U[1], fb.U[2], key, nek, d, d + nek);
+    DDNS_HIP(hipGetLastError());
+    std::vector<double> h(2 * (size_t)nek), fac(nek, 1.0);
+    DDNS_HIP(hipMemcpyAsync(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    DDNS_HIP(hipStreamSynchronize(s));
+    const double* e_t = h.data();
+    const double* enumv = h.data() + nek;
+    const double pi = 4. * std::atan(1.);
+    if (kind == DDNS_SPEC_K4_GAUSS) {
+        // E(k) proportional to (k/k0)^4 exp(-2 (k/k0)^2), total 3/2 urms^2
+        std::vector<double> target(nek);
+        double sum = 0.0;
+        for (int i = 1; i <= nek; ++i) {
+            const double q = (double)i / k0;
+            target[i - 1] = q * q * q * q * std::exp(-2.0 * q * q);
+            sum += target[i - 1];
+        }
+        for (int i = 1; i <= nek; ++i) {
+            const double tg = target[i - 1] * 1.5 * urms * urms / sum;
+            fac[i - 1] = e_t[i - 1] > 0 ? std::sqrt(tg / e_t[i - 1]) : 0.0;
+        }
+    } else {
+        // initial_flow's own scaling: shell 1 at force(1) per shell volume, k^(-5/3) above
+        const double ccc = force0 * 4. / 3. * pi * (1.5 * 1.5 * 1.5 - 0.5 * 0.5 * 0.5) / enumv[0];
+        for (int i = 1; i <= nek; ++i) {
+            const double vol = 4. / 3. * pi * ((i + 0.5) * (i + 0.5) * (i + 0.5) - (i - 0.5) * (i - 0.5) * (i - 0.5));
+            fac[i - 1] = e_t[i - 1] > 0 ? std::sqrt(ccc / e_t[i - 1] * enumv[i - 1] / vol) / std::pow((double)i, 0.833333)
+                                        : 0.0;
+        }
+    }
+    DDNS_HIP(hipMemcpyAsync(d + 2 * nek, fac.data(), nek * sizeof(double), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL((k_init_scale<T, N>), dim3(g), dim3(256), 0, s, fb.U[0], fb.U[1], fb.U[2], nek,
+                       (const double*)(d + 2 * nek));
+    DDNS_HIP(hipGetLastError());
+    // the history starts equal to the velocity (turb.f90:452-454)
+    for (int c = 0; c < 3; ++c)
+        DDNS_HIP(hipMemcpyAsync(fb.O[c], fb.U[c], tot * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s));
+    DDNS_HIP(hipStreamSynchronize(s));
+    DDNS_HIP(hipFree(d));
+    return 0;
+}
+
+int flow_init(FlowBuffers& fb, int N, int nek, int precision, int kind, int seed, double k0, double urms, double force0,
+              hipStream_t s)
+{
+    int st = 0;
+    if (precision == DDNS_FP64) { DDNS_DISPATCH_N(N, st = (init_impl<double, NN>(fb, nek, kind, seed, k0, urms, force0, s))) }
+    else { DDNS_DISPATCH_N(N, st = (init_impl<float, NN>(fb, nek, kind, seed, k0, urms, force0, s))) }
+    return st;
+}
+
 // ---------------------------------------------------------------- stand-alone FFTs
 template <typename T, int N>
 static int r2c_impl(const FlowBuffers& fb, double* v, hipStream_t s)
@@ -1420,7 +1578,8 @@ static int diag_impl(FlowBuffers& fb, int nek, double rnu, int want, double* ek,
     const size_t fbytes = n3 * sizeof(T);                    // one real field = one complex field
     if (!fb.dg[0]) {
         for (int q = 0; q < 8; ++q) DDNS_HIP(hipMalloc(&fb.dg[q], fbytes));
-        DDNS_HIP(hipMalloc(&fb.dg_part, 3 * DG_BLOCKS * sizeof(double)));
+        // reduction partials (3 DG_BLOCKS) or the 3 nek shell sums, nek < N / 2
+        DDNS_HIP(hipMalloc(&fb.dg_part, 3 * (size_t)std::max(DG_BLOCKS, N / 2) * sizeof(double)));
     }
     T* R[6];
     for (int q = 0; q < 6; ++q) R[q] = (T*)fb.dg[q];
@@ -1471,7 +1630,7 @@ static int diag_impl(FlowBuffers& fb, int nek, double rnu, int want, double* ek,
         mom[6] = g[0]; mom[7] = g[1]; mom[8] = g[2];   // sum |w|, sum |w|^2, sum |w| div
     }
     // spectra; the transfer needs the nonlinear term F = FFT(u x w) (turb.f90:634-650)
-    double* dd = fb.dg_part;           // 3 nek shell sums (nek <= 3 * DG_BLOCKS / 3)
+    double* dd = fb.dg_part;           // 3 nek shell sums
     DDNS_HIP(hipMemsetAsync(dd, 0, 3 * (size_t)nek * sizeof(double), s));
     const void* F[3] = {nullptr, nullptr, nullptr};
     if (tk) {
@@ -1500,7 +1659,7 @@ static int diag_impl(FlowBuffers& fb, int nek, double rnu, int want, double* ek,
 int flow_diagnostics(FlowBuffers& fb, int N, int nek, double rnu, int precision, int want, double* ek, double* dk,
                      double* tk, double* mom, hipStream_t s)
 {
-    if (nek > DG_BLOCKS) { set_error("too many shells for the diagnostics"); return DDNS_ERR_ARG; }
+    if (nek > N / 2) { set_error("too many shells for the diagnostics"); return DDNS_ERR_ARG; }
     int st = 0;
     if (precision == DDNS_FP64) { DDNS_DISPATCH_N(N, st = (diag_impl<double, NN>(fb, nek, rnu, want, ek, dk, tk, mom, s))) }
     else { DDNS_DISPATCH_N(N, st = (diag_impl<float, NN>(fb, nek, rnu, want, ek, dk, tk, mom, s))) }

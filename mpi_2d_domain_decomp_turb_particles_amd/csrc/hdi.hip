@@ -443,13 +443,20 @@ template <typename T>
 __global__ void k_solution(const double* x0, const T* Q, long long ld, const double* y, int iters, const int* sorted,
                            double* pert, int n, int cap)
 {
-    const long long i = (long long)blockIdx.x * HT + threadIdx.x;
-    const long long n3 = 3LL * n;
-    if (i >= n3) return;
-    double acc = 0.0;
-    for (int j = 0; j < iters; ++j) acc = acc + (double)Q[(size_t)j * ld + i] * y[j];
-    const int c = (int)(i / n), t = (int)(i - (long long)c * n);
-    pert[3 * (size_t)sorted[t] + c] = x0[i] + acc;
+    // one droplet per thread: its three components go out as one 24-byte record
+    const int t = blockIdx.x * HT + threadIdx.x;
+    if (t >= n) return;
+    double out[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const size_t i = (size_t)c * n + t;
+        double acc = 0.0;
+        for (int j = 0; j < iters; ++j) acc = acc + (double)Q[(size_t)j * ld + i] * y[j];
+        out[c] = x0[i] + acc;
+    }
+    double* dst = pert + 3 * (size_t)sorted[t];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dst[c] = out[c];
 }
 
 // Small least-squares update of HDI_GMRES (2904-2971), one wavefront.
@@ -767,7 +774,7 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
     if (st6[4] < 0.0) return 0;     // initial residual below 1e-5: no iteration
     const bool done = st6[4] != 0.0;
     const int used = (int)st6[5];   // the last iteration that ran
-    hipLaunchKernelGGL((k_solution<T>), dim3(nb3), dim3(HT), 0, s, pb.gm_x0, (const T*)Q, ld, sm + L.Y0(), used,
+    hipLaunchKernelGGL((k_solution<T>), dim3(nbr), dim3(HT), 0, s, pb.gm_x0, (const T*)Q, ld, sm + L.Y0(), used,
                        (const int*)pb.sorted, pb.pert, n, pb.cap);
     DDNS_HIP(hipGetLastError());
     double tail[6];

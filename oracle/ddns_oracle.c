@@ -658,6 +658,124 @@ int orc_set_flow(orc_sim* s, const double* vx, const double* vy, const double* v
     return 0;
 }
 
+/* SplitMix64 (Steele, Lea and Flood, OOPSLA 2014) as a counter-based generator: output
+ * i of the stream with key s is the finaliser of s + (i + 1) * 0x9E3779B97F4A7C15.  With
+ * key 0 the outputs are the published stream 0xe220a8397b1dcdaf, 0x6e789e6aa1b965f4,
+ * 0x06c45d188009454f, ...  Replaces IFPORT rand in initial_flow (flowlib.f90:158-172). */
+uint64_t orc_splitmix64(uint64_t key, uint64_t i)
+{
+    uint64_t z = key + (i + 1u) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static void init_gauss(uint64_t key, int N, int c, int kz, int ky, int kx, double* re, double* im)
+{
+    const uint64_t m = (((uint64_t)c * N + kz) * N + ky) * (uint64_t)(N / 2) + kx;
+    double t1 = (double)(orc_splitmix64(key, 2 * m) >> 11) * (1.0 / 9007199254740992.0);
+    double t2 = (double)(orc_splitmix64(key, 2 * m + 1) >> 11) * (1.0 / 9007199254740992.0);
+    double a, ph;
+    if (t1 <= 1.e-10) t1 = 1.e-10;
+    if (t2 <= 1.e-10) t2 = 1.e-10;
+    a = sqrt(-2.0 * log(t1));
+    ph = 2.0 * 3.14159265358979323846 * t2;
+    *re = a * cos(ph);
+    *im = a * sin(ph);
+}
+
+/* initial_flow (flowlib.f90:53-144) with the counter-based generator: Gaussian modes
+ * (one pair of uniforms per component and mode), the kx = 0 plane made Hermitian as
+ * (g(k) + conj g(-k)) / 2, the Nyquist planes and the mean zero, the projection (94-101),
+ * the shell factors (107-137: the k^4 exp(-2 (k/k0)^2) spectrum at total energy
+ * 3/2 urms^2, or initial_flow's own force(1) / k^(-5/3) scaling) and the dealiasing
+ * (139-141); the history starts equal to the velocity (turb.f90:452-454).  kind < 0
+ * gives the flow at rest. */
+int orc_init_flow(orc_sim* s, int kind, int seed, double k0, double urms)
+{
+    const int N = s->N, L = s->L, M = N / 2, nek = s->nek;
+    const uint64_t key = (uint64_t)(int64_t)seed;
+    double* v[3] = {s->vx, s->vy, s->vz};
+    double *e_t = (double*)xcalloc((size_t)nek + 1, 8), *cnt = (double*)xcalloc((size_t)nek + 1, 8);
+    double *fac = (double*)xcalloc((size_t)nek + 1, 8);
+    const double pi = 4. * atan(1.);
+    int c, kz, ky, kx, i;
+    for (c = 0; c < 3; ++c) memset(v[c], 0, s->fsz * 8);
+    if (kind >= 0) {
+        for (kz = 0; kz < N; ++kz)
+            for (ky = 0; ky < N; ++ky)
+                for (kx = 0; kx < M; ++kx) {
+                    const int zero = ky == N / 2 || kz == N / 2 || (kx == 0 && ky == 0 && kz == 0);
+                    const double k[3] = {(double)kx, (double)(ky <= N / 2 ? ky : ky - N), (double)(kz <= N / 2 ? kz : kz - N)};
+                    double gr[3], gi[3], k2, pr, pim, e = 0.0;
+                    int ik2;
+                    for (c = 0; c < 3; ++c) {
+                        init_gauss(key, N, c, kz, ky, kx, &gr[c], &gi[c]);
+                        if (kx == 0) {
+                            double hr, hi;
+                            init_gauss(key, N, c, (N - kz) % N, (N - ky) % N, 0, &hr, &hi);
+                            gr[c] = 0.5 * (gr[c] + hr);
+                            gi[c] = 0.5 * (gi[c] - hi);
+                        }
+                        if (zero) gr[c] = gi[c] = 0.0;
+                    }
+                    k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+                    if (kx == 0 && ky == 0 && kz == 0) k2 = 1.e-5;
+                    pr = (k[0] * gr[0] + k[1] * gr[1] + k[2] * gr[2]) / k2;
+                    pim = (k[0] * gi[0] + k[1] * gi[1] + k[2] * gi[2]) / k2;
+                    for (c = 0; c < 3; ++c) {
+                        const double x = gr[c] - k[c] * pr, y = gi[c] - k[c] * pim;
+                        v[c][IDX(kz, 2 * kx, ky, L)] = x;
+                        v[c][IDX(kz, 2 * kx + 1, ky, L)] = y;
+                        e += x * x + y * y;
+                    }
+                    ik2 = (int)(sqrt(k2) + 0.5);
+                    if (ik2 < 1 || ik2 > nek) continue;
+                    e_t[ik2] += kx == 0 ? 0.5 * e : e;
+                    cnt[ik2] += kx == 0 ? 0.5 : 1.0;
+                }
+        if (kind == DDNS_SPEC_K4_GAUSS) {
+            double sum = 0.0;
+            for (i = 1; i <= nek; ++i) {
+                const double q = (double)i / k0;
+                fac[i] = q * q * q * q * exp(-2.0 * q * q);
+                sum += fac[i];
+            }
+            for (i = 1; i <= nek; ++i) {
+                const double tg = fac[i] * 1.5 * urms * urms / sum;
+                fac[i] = e_t[i] > 0 ? sqrt(tg / e_t[i]) : 0.0;
+            }
+        } else {
+            const double ccc = s->p.force[0] * 4. / 3. * pi * (1.5 * 1.5 * 1.5 - 0.5 * 0.5 * 0.5) / cnt[1];
+            for (i = 1; i <= nek; ++i) {
+                const double vol = 4. / 3. * pi * ((i + 0.5) * (i + 0.5) * (i + 0.5) - (i - 0.5) * (i - 0.5) * (i - 0.5));
+                fac[i] = e_t[i] > 0 ? sqrt(ccc / e_t[i] * cnt[i] / vol) / pow((double)i, 0.833333) : 0.0;
+            }
+        }
+        for (kz = 0; kz < N; ++kz)
+            for (ky = 0; ky < N; ++ky)
+                for (kx = 0; kx < M; ++kx) {
+                    const double k1 = kx, k2y = ky <= N / 2 ? ky : ky - N, k3 = kz <= N / 2 ? kz : kz - N;
+                    double k2 = k1 * k1 + k2y * k2y + k3 * k3, f;
+                    int ik2;
+                    if (k2 == 0.0) k2 = 1.e-5;
+                    ik2 = (int)(sqrt(k2) + 0.5);
+                    if (ik2 < 1) continue;
+                    f = ik2 <= nek ? fac[ik2] : 0.0;
+                    for (c = 0; c < 3; ++c) {
+                        v[c][IDX(kz, 2 * kx, ky, L)] *= f;
+                        v[c][IDX(kz, 2 * kx + 1, ky, L)] *= f;
+                    }
+                }
+    }
+    memcpy(s->ox, s->vx, s->fsz * 8);
+    memcpy(s->oy, s->vy, s->fsz * 8);
+    memcpy(s->oz, s->vz, s->fsz * 8);
+    free(e_t); free(cnt); free(fac);
+    s->have_phys = 0;
+    return 0;
+}
+
 int orc_get_flow(orc_sim* s, double* vx, double* vy, double* vz, double* ox, double* oy, double* oz)
 {
     double* dst[6] = {vx, vy, vz, ox, oy, oz};

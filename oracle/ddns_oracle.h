@@ -22,6 +22,8 @@ int  orc_create(const ddns_params* p, orc_sim** out);
 void orc_destroy(orc_sim* s);
 int  orc_nek(const orc_sim* s);
 
+uint64_t orc_splitmix64(uint64_t key, uint64_t i);
+int  orc_init_flow(orc_sim* s, int kind, int seed, double k0, double urms);
 int  orc_set_flow(orc_sim* s, const double* vx, const double* vy, const double* vz,
                   const double* ox, const double* oy, const double* oz);
 int  orc_get_flow(orc_sim* s, double* vx, double* vy, double* vz,

@@ -38,6 +38,8 @@ def load() -> C.CDLL:
         "orc_destroy": (None, [_P]),
         "orc_nek": (C.c_int, [_P]),
         "orc_set_flow": (C.c_int, [_P] + [_D] * 6),
+        "orc_init_flow": (C.c_int, [_P, C.c_int, C.c_int, C.c_double, C.c_double]),
+        "orc_splitmix64": (C.c_uint64, [C.c_uint64, C.c_uint64]),
         "orc_get_flow": (C.c_int, [_P] + [_D] * 6),
         "orc_set_forcing_state": (C.c_int, [_P, _D, C.c_int, C.c_int, _I]),
         "orc_get_forcing_state": (C.c_int, [_P, _D, _I, _I, _I]),
@@ -144,6 +146,9 @@ class Oracle:
         if st < 0:
             raise RuntimeError(f"oracle error {st}")
         return st
+
+    def init_flow(self, kind, seed, k0=4.0, urms=1.0):
+        self._chk(self.lib.orc_init_flow(self.h, kind, seed, k0, urms))
 
     def set_flow(self, v, o=None):
         v = [np.asfortranarray(x, dtype=np.float64) for x in v]

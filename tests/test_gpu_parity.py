@@ -69,6 +69,24 @@ def test_flow_ten_steps_match_oracle(ddns, oracle_mod, forcing, prec):
             assert np.abs(bg - bo).max() < 1e-12 * max(1.0, np.abs(bo).max())
 
 
+@pytest.mark.parametrize("kind", [1, 0])          # SPEC_K4_GAUSS, SPEC_KOLMOGOROV
+@pytest.mark.parametrize("prec", [0, 1])
+def test_initial_field_matches_oracle(ddns, oracle_mod, kind, prec):
+    """initial_flow on the device against the oracle's restatement (same counter-based
+    draws; the transcendental functions and the shell sums round differently)."""
+    N = 64
+    with ddns.Solver(_params(ddns, N=N, precision=prec)) as s, oracle_mod.Oracle(_params(ddns, N=N)) as o:
+        s.init_flow(kind, seed=77, k0=4.0, urms=1.0)
+        o.init_flow(kind, 77, 4.0, 1.0)
+        gv, go = s.get_flow()
+        ov, oo = o.get_flow()
+        tol = 1e-12 if prec == 0 else 1e-6
+        assert rel_l2(gv, ov) < tol and rel_l2(go, oo) < tol
+        s.init_flow(-1, seed=77, k0=4.0, urms=1.0)         # at rest
+        gv, go = s.get_flow()
+        assert all(np.all(x == 0) for x in gv + go)
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_flow_statistics_match_oracle(ddns, oracle_mod, prec):
     """POSTPROC and TRANSPOSTPROC statistics (postprocessing.f90:76-315) of one u-hat:
