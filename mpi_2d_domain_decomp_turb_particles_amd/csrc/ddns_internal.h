@@ -340,6 +340,7 @@ struct PartBuffers {
     double* gm_w = nullptr;     // 3 cap
     double* gm_red = nullptr;   // reduction partials
     double* gm_small = nullptr; // Hessenberg etc
+    int gm_last_iters = 0;      // GMRES iterations of the last solve: the size of the next first batch
     // collisions
     int* mult = nullptr;        // cap: removal-list multiplicity
     void* cand = nullptr;       // cand_cap int2: pairs handed from the collision search to the pair kernel

@@ -724,7 +724,9 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
     hipLaunchKernelGGL((k_scale<T>), dim3(nb3), dim3(HT), 0, s, Q, sm + 1, n3);
     // Iterations are launched in batches; every kernel of an iteration returns at
     // once when the least-squares update of an earlier one has set the converged
-    // flag sm[4], so the host checks it once per batch.
+    // flag sm[4], so the host checks it once per batch.  The first batch runs as many
+    // iterations as the last solve needed (consecutive steps converge alike), the
+    // later ones three at a time.
     constexpr int BATCH = 3;
     const double* stop = sm + 4;
     double* Hd = sm + L.H0();
@@ -732,7 +734,7 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
     int iter = 0;
     double st6[6] = {0, 0, 0, 0, 0, 0};
     while (iter < maxiter && st6[4] == 0.0) {
-        const int last = std::min(maxiter, iter + BATCH);
+        const int last = std::min(maxiter, iter + (iter == 0 ? std::max(BATCH, pb.gm_last_iters) : BATCH));
         for (int it = iter + 1; it <= last; ++it) {
             T* v = work[it & 1];
             if (it == 1)
@@ -774,6 +776,7 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
     if (st6[4] < 0.0) return 0;     // initial residual below 1e-5: no iteration
     const bool done = st6[4] != 0.0;
     const int used = (int)st6[5];   // the last iteration that ran
+    pb.gm_last_iters = used;
     hipLaunchKernelGGL((k_solution<T>), dim3(nbr), dim3(HT), 0, s, pb.gm_x0, (const T*)Q, ld, sm + L.Y0(), used,
                        (const int*)pb.sorted, pb.pert, n, pb.cap);
     DDNS_HIP(hipGetLastError());
