@@ -40,6 +40,11 @@ struct ddns_handle {
     ddns_params p;
     int N = 0, nek = 0, prec = 0;
     hipStream_t stream = nullptr;
+    // the droplet block of a step runs here, beside the second half of the flow step on
+    // stream (the droplets read only the physical velocity of the first half)
+    hipStream_t dstream = nullptr;
+    hipEvent_t drop_done = nullptr;
+    bool flow_rest_queued = false;   // the flow's second half of this step is on stream
     FlowBuffers fb;
     FlowConst fc{};
     bool a_valid = false;        // inverse pipeline holds the start of the next step
@@ -294,6 +299,11 @@ extern "C" int ddns_create(const ddns_params* pin, ddns_handle** out)
     h->new_particle = pin->new_particle;
     int st = 0;
     if (hipStreamCreate(&h->stream) != hipSuccess) { delete h; return fail(nullptr, DDNS_ERR_HIP, "stream"); }
+    if (hipStreamCreate(&h->dstream) != hipSuccess) { ddns_destroy(h); return fail(nullptr, DDNS_ERR_HIP, "stream"); }
+    if (hipEventCreateWithFlags(&h->drop_done, hipEventDisableTiming) != hipSuccess) {
+        ddns_destroy(h);
+        return fail(nullptr, DDNS_ERR_HIP, "event");
+    }
     st = flow_alloc(h->fb, N, h->prec);
     if (st) { ddns_destroy(h); return st; }
     h->fc.N = N; h->fc.nek = h->nek; h->fc.forcing = h->p.forcing; h->fc.dt = h->p.dt; h->fc.rnu = h->p.rnu;
@@ -338,10 +348,13 @@ extern "C" int ddns_destroy(ddns_handle* h)
 {
     if (!h) return DDNS_OK;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->dstream) (void)hipStreamSynchronize(h->dstream);
     flow_free(h->fb);
     part_free(h->pb);
     for (int i = 0; i < 16; ++i) if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
     for (int i = 0; i < 8; ++i) if (h->fev[i]) (void)hipEventDestroy(h->fev[i]);
+    if (h->drop_done) (void)hipEventDestroy(h->drop_done);
+    if (h->dstream) (void)hipStreamDestroy(h->dstream);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return DDNS_OK;
@@ -633,12 +646,29 @@ static double cell_width(const ddns_handle* h)
     return w;
 }
 
+// The forward transforms and the update of the step (turb.f90:644-691) on the flow
+// stream.  The droplet block reads only the physical velocity of the first half and the
+// flow reads nothing of the droplets, so this may run beside any part of the block; it
+// is queued to start when the block's gathers are done (event after), where the block
+// turns from bandwidth-heavy gathers to latency-bound work.
+static int queue_flow_rest(ddns_handle* h, hipEvent_t after)
+{
+    if (h->flow_rest_queued) return 0;
+    if (after) DDNS_HIP(hipStreamWaitEvent(h->stream, after, 0));
+    DDNS_HIP(hipEventRecord(h->ev[2], h->stream));
+    const int st = flow_step(h->fb, h->fc, h->prec, true, h->stream, h->fev);
+    if (st) return st;
+    DDNS_HIP(hipEventRecord(h->ev[3], h->stream));
+    h->flow_rest_queued = true;
+    return 0;
+}
+
 static int droplet_block(ddns_handle* h, float* ms)
 {
     PartConst& pc = h->pc;
     pc.n = h->n;
     pc.new_particle = h->new_particle;
-    hipStream_t s = h->stream;
+    hipStream_t s = h->dstream;
     int status = 0, st;
     DDNS_HIP(hipEventRecord(h->ev[8], s));
     st = part_build_cells(h->pb, pc, cell_width(h), s);                       // HEADLIST (binning)
@@ -652,6 +682,8 @@ static int droplet_block(ddns_handle* h, float* ms)
     if (st) return st;
     DDNS_HIP(hipEventRecord(h->ev[10], s));
     h->cc = ddns_collision_counts{};
+    st = queue_flow_rest(h, h->ev[10]);                                       // after the gather
+    if (st) return st;
     if (h->p.hdi_included) {                                                  // 571-572
         int it = 0; double res = 0;
         st = part_hdi(h->pb, pc, h->p.gmres_maxiter, h->p.gmres_tol, &it, &res, s);
@@ -740,16 +772,21 @@ extern "C" int ddns_step(ddns_handle* h, int nsteps)
         if (st) return keep(h, st);
         DDNS_HIP(hipEventRecord(h->ev[1], s));
         h->phys_valid = true;
+        // the droplet block (546-623) on its own stream; it queues the forward transforms
+        // and the update (644-691) on this one when it reaches its latency-bound part
+        h->flow_rest_queued = false;
         if (h->p.npart > 0 && h->n > 0) {
+            DDNS_HIP(hipStreamWaitEvent(h->dstream, h->ev[1], 0));
             st = droplet_block(h, pms);
             if (st < 0) return keep(h, st);
             if (st > 0 && status == 0) status = st;
             if (h->n < h->p.npart / 2) return fail(h, DDNS_ERR_DROPLET_LOSS, "npart_t < npart/2 (turb.f90:600-604)");
+            // the next step's x stage overwrites the physical velocity
+            DDNS_HIP(hipEventRecord(h->drop_done, h->dstream));
+            DDNS_HIP(hipStreamWaitEvent(s, h->drop_done, 0));
         }
-        DDNS_HIP(hipEventRecord(h->ev[2], s));
-        st = flow_step(h->fb, h->fc, h->prec, true, s, h->fev);               // forward, update, 644-691
+        st = queue_flow_rest(h, nullptr);
         if (st) return keep(h, st);
-        DDNS_HIP(hipEventRecord(h->ev[3], s));
         DDNS_HIP(hipEventSynchronize(h->ev[3]));
         {
             const int pairs[6][2] = {{0, 1}, {1, 2}, {3, 4}, {4, 5}, {5, 6}, {6, 7}};
