@@ -678,6 +678,9 @@ static int droplet_block(ddns_handle* h, float* ms)
     // interpolation then also gathers the HI inputs from the final droplet velocities
     st = part_history_fvp(h->pb, pc, s);                                      // 564
     if (st) return st;
+    // the binning and the history ran beside the flow's first half; the gather needs
+    // the physical velocity it writes
+    DDNS_HIP(hipStreamWaitEvent(s, h->ev[1], 0));
     st = part_interp(h->pb, pc, h->fb.uphys, s);                              // 553-556
     if (st) return st;
     DDNS_HIP(hipEventRecord(h->ev[10], s));
@@ -776,7 +779,6 @@ extern "C" int ddns_step(ddns_handle* h, int nsteps)
         // and the update (644-691) on this one when it reaches its latency-bound part
         h->flow_rest_queued = false;
         if (h->p.npart > 0 && h->n > 0) {
-            DDNS_HIP(hipStreamWaitEvent(h->dstream, h->ev[1], 0));
             st = droplet_block(h, pms);
             if (st < 0) return keep(h, st);
             if (st > 0 && status == 0) status = st;

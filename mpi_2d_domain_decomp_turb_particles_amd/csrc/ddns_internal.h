@@ -47,26 +47,6 @@ __host__ __device__ __forceinline__ int cell_code(int x, int y, int z)
 {
     return (int)(cell_spread((unsigned)x) | (cell_spread((unsigned)y) << 1) | (cell_spread((unsigned)z) << 2));
 }
-// The [start, end) ranges of the 27 cells around coarse cell cc (periodic), written to
-// rng[k * stride] for k = (ox+1)*9 + (oy+1)*3 + (oz+1).  The loads go out in three
-// batches of 18, so a lane pays the cell-table latency three times instead of 27
-// times without holding all 54 results in registers.
-__device__ __forceinline__ void neighbour_ranges(const int* __restrict__ cell_start, int G, const int cc[3],
-                                                 int2* rng, int stride)
-{
-#pragma unroll 1
-    for (int k0 = 0; k0 < 27; k0 += 9)
-#pragma unroll
-    for (int k = k0; k < k0 + 9; ++k) {
-        int qx = cc[0] + k / 9 - 1, qy = cc[1] + (k / 3) % 3 - 1, qz = cc[2] + k % 3 - 1;
-        qx += qx < 0 ? G : (qx >= G ? -G : 0);
-        qy += qy < 0 ? G : (qy >= G ? -G : 0);
-        qz += qz < 0 ? G : (qz >= G ? -G : 0);
-        const int cell = cell_code(qx, qy, qz);
-        rng[k * stride] = make_int2(cell_start[cell], cell_start[cell + 1]);
-    }
-}
-
 // Work-group b of a launch runs on XCD b % 8, and each of the 8 XCDs has its own L2.
 // This gives the work-groups of one XCD a contiguous range of the nb logical blocks,
 // so droplets that are close in cell order, which read the same field nodes and
@@ -75,25 +55,6 @@ __device__ __forceinline__ int xcd_chunk(int b, int nb)
 {
     const int x = b & 7, r = b >> 3, q = nb >> 3, m = nb & 7;
     return x * q + (x < m ? x : m) + r;
-}
-
-// The candidates of the 27 neighbour cells as one sequence per lane: cells in the
-// order k = 0..26, droplets in cell order inside each, read from the ranges that
-// neighbour_ranges left in LDS.  A lane's loop then runs once per candidate instead
-// of the wavefront running, for every cell, as long as its fullest lane.
-struct CandCursor {
-    int kc = -1, p = 0, e = 0;
-};
-__device__ __forceinline__ bool cand_next(CandCursor& c, const int2* rng, int stride)
-{
-    ++c.p;
-    while (c.p >= c.e) {
-        if (++c.kc >= 27) return false;
-        const int2 r = rng[c.kc * stride];
-        c.p = r.x;
-        c.e = r.y;
-    }
-    return true;
 }
 
 // Offsets (-1 or 0 up to 0 or +1) of the neighbour cells to visit along each axis.
