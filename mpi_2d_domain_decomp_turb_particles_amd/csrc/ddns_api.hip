@@ -43,7 +43,6 @@ struct ddns_handle {
     // the droplet block of a step runs here, beside the second half of the flow step on
     // stream (the droplets read only the physical velocity of the first half)
     hipStream_t dstream = nullptr;
-    hipEvent_t drop_done = nullptr;
     bool flow_rest_queued = false;   // the flow's second half of this step is on stream
     FlowBuffers fb;
     FlowConst fc{};
@@ -59,7 +58,14 @@ struct ddns_handle {
     ddns_collision_counts cc{};
     ddns_timing timing{};
     hipEvent_t ev[16] = {};
-    hipEvent_t fev[8] = {};      // flow kernel boundaries
+    // flow stage boundaries (0 first half start, 1 end, 2 second half start, 3 end) and
+    // kernel boundaries, one set per parity of the step: the next step's first half may
+    // be queued before this step's times are read
+    hipEvent_t flev[2][4] = {};
+    hipEvent_t fev[2][8] = {};
+    int par = 0;                 // parity of the step in progress
+    bool want_next = false;      // a further step follows in this call
+    bool next_queued = false;    // its first half is already on the flow stream
     std::string err;
 };
 
@@ -300,10 +306,6 @@ extern "C" int ddns_create(const ddns_params* pin, ddns_handle** out)
     int st = 0;
     if (hipStreamCreate(&h->stream) != hipSuccess) { delete h; return fail(nullptr, DDNS_ERR_HIP, "stream"); }
     if (hipStreamCreate(&h->dstream) != hipSuccess) { ddns_destroy(h); return fail(nullptr, DDNS_ERR_HIP, "stream"); }
-    if (hipEventCreateWithFlags(&h->drop_done, hipEventDisableTiming) != hipSuccess) {
-        ddns_destroy(h);
-        return fail(nullptr, DDNS_ERR_HIP, "event");
-    }
     st = flow_alloc(h->fb, N, h->prec);
     if (st) { ddns_destroy(h); return st; }
     h->fc.N = N; h->fc.nek = h->nek; h->fc.forcing = h->p.forcing; h->fc.dt = h->p.dt; h->fc.rnu = h->p.rnu;
@@ -337,8 +339,12 @@ extern "C" int ddns_create(const ddns_params* pin, ddns_handle** out)
     if (st) { ddns_destroy(h); return st; }
     for (int i = 0; i < 16; ++i)
         if (hipEventCreate(&h->ev[i]) != hipSuccess) { ddns_destroy(h); return fail(nullptr, DDNS_ERR_HIP, "event"); }
-    for (int i = 0; i < 8; ++i)
-        if (hipEventCreate(&h->fev[i]) != hipSuccess) { ddns_destroy(h); return fail(nullptr, DDNS_ERR_HIP, "event"); }
+    for (int q = 0; q < 2; ++q) {
+        for (int i = 0; i < 8; ++i)
+            if (hipEventCreate(&h->fev[q][i]) != hipSuccess) { ddns_destroy(h); return fail(nullptr, DDNS_ERR_HIP, "event"); }
+        for (int i = 0; i < 4; ++i)
+            if (hipEventCreate(&h->flev[q][i]) != hipSuccess) { ddns_destroy(h); return fail(nullptr, DDNS_ERR_HIP, "event"); }
+    }
     if (hipStreamSynchronize(h->stream) != hipSuccess) { ddns_destroy(h); return fail(nullptr, DDNS_ERR_HIP, "sync"); }
     *out = h;
     return DDNS_OK;
@@ -352,8 +358,10 @@ extern "C" int ddns_destroy(ddns_handle* h)
     flow_free(h->fb);
     part_free(h->pb);
     for (int i = 0; i < 16; ++i) if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
-    for (int i = 0; i < 8; ++i) if (h->fev[i]) (void)hipEventDestroy(h->fev[i]);
-    if (h->drop_done) (void)hipEventDestroy(h->drop_done);
+    for (int q = 0; q < 2; ++q) {
+        for (int i = 0; i < 8; ++i) if (h->fev[q][i]) (void)hipEventDestroy(h->fev[q][i]);
+        for (int i = 0; i < 4; ++i) if (h->flev[q][i]) (void)hipEventDestroy(h->flev[q][i]);
+    }
     if (h->dstream) (void)hipStreamDestroy(h->dstream);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -651,15 +659,43 @@ static double cell_width(const ddns_handle* h)
 // flow reads nothing of the droplets, so this may run beside any part of the block; it
 // is queued to start when the block's gathers are done (event after), where the block
 // turns from bandwidth-heavy gathers to latency-bound work.
+// The first half of a step's flow (the curl and inverse z transforms when the state
+// is new, turb.f90:506-526; the inverse y and x stages and the physical velocity,
+// 527-539) on the flow stream, with the event set of parity q.
+static int queue_flow_first(ddns_handle* h, int q)
+{
+    hipStream_t s = h->stream;
+    DDNS_HIP(hipEventRecord(h->flev[q][0], s));
+    if (!h->a_valid) {
+        const int st = flow_prologue(h->fb, h->fc, h->prec, s);
+        if (st) return st;
+        h->a_valid = true;
+    }
+    const int st = flow_step(h->fb, h->fc, h->prec, false, s, h->fev[q]);
+    if (st) return st;
+    DDNS_HIP(hipEventRecord(h->flev[q][1], s));
+    return 0;
+}
+
 static int queue_flow_rest(ddns_handle* h, hipEvent_t after)
 {
     if (h->flow_rest_queued) return 0;
-    if (after) DDNS_HIP(hipStreamWaitEvent(h->stream, after, 0));
-    DDNS_HIP(hipEventRecord(h->ev[2], h->stream));
-    const int st = flow_step(h->fb, h->fc, h->prec, true, h->stream, h->fev);
+    hipStream_t s = h->stream;
+    const int q = h->par;
+    if (after) DDNS_HIP(hipStreamWaitEvent(s, after, 0));
+    DDNS_HIP(hipEventRecord(h->flev[q][2], s));
+    int st = flow_step(h->fb, h->fc, h->prec, true, s, h->fev[q]);
     if (st) return st;
-    DDNS_HIP(hipEventRecord(h->ev[3], h->stream));
+    DDNS_HIP(hipEventRecord(h->flev[q][3], s));
     h->flow_rest_queued = true;
+    // The next step's first half can follow at once: it overwrites the physical velocity,
+    // and the gather that reads it (event after) is done before this queue moves on.  It
+    // then runs beside the rest of this droplet block.
+    if (h->want_next && after) {
+        st = queue_flow_first(h, 1 - q);
+        if (st) return st;
+        h->next_queued = true;
+    }
     return 0;
 }
 
@@ -680,7 +716,7 @@ static int droplet_block(ddns_handle* h, float* ms)
     if (st) return st;
     // the binning and the history ran beside the flow's first half; the gather needs
     // the physical velocity it writes
-    DDNS_HIP(hipStreamWaitEvent(s, h->ev[1], 0));
+    DDNS_HIP(hipStreamWaitEvent(s, h->flev[h->par][1], 0));
     st = part_interp(h->pb, pc, h->fb.uphys, s);                              // 553-556
     if (st) return st;
     DDNS_HIP(hipEventRecord(h->ev[10], s));
@@ -763,45 +799,44 @@ extern "C" int ddns_step(ddns_handle* h, int nsteps)
     double flow_ms = 0;
     double kms[6] = {0, 0, 0, 0, 0, 0};
     DDNS_HIP(hipEventRecord(h->ev[15], s));
+    h->next_queued = false;
     for (int k = 0; k < nsteps; ++k) {
         h->istep += 1;                                                        // turb.f90:492
-        DDNS_HIP(hipEventRecord(h->ev[0], s));
-        if (!h->a_valid) {
-            st = flow_prologue(h->fb, h->fc, h->prec, s);                     // curl + inverse z, 506-526
+        // inverse y, x stage (the first half may have been queued during the last step)
+        if (!h->next_queued) {
+            st = queue_flow_first(h, h->par);
             if (st) return keep(h, st);
-            h->a_valid = true;
         }
-        st = flow_step(h->fb, h->fc, h->prec, false, s, h->fev);              // inverse y, x stage
-        if (st) return keep(h, st);
-        DDNS_HIP(hipEventRecord(h->ev[1], s));
+        h->next_queued = false;
         h->phys_valid = true;
         // the droplet block (546-623) on its own stream; it queues the forward transforms
-        // and the update (644-691) on this one when it reaches its latency-bound part
+        // and the update (644-691) on this one when it reaches its latency-bound part, and
+        // then the first half of the next step
         h->flow_rest_queued = false;
+        h->want_next = k + 1 < nsteps;
         if (h->p.npart > 0 && h->n > 0) {
             st = droplet_block(h, pms);
             if (st < 0) return keep(h, st);
             if (st > 0 && status == 0) status = st;
             if (h->n < h->p.npart / 2) return fail(h, DDNS_ERR_DROPLET_LOSS, "npart_t < npart/2 (turb.f90:600-604)");
-            // the next step's x stage overwrites the physical velocity
-            DDNS_HIP(hipEventRecord(h->drop_done, h->dstream));
-            DDNS_HIP(hipStreamWaitEvent(s, h->drop_done, 0));
         }
         st = queue_flow_rest(h, nullptr);
         if (st) return keep(h, st);
-        DDNS_HIP(hipEventSynchronize(h->ev[3]));
+        const int q = h->par;
+        DDNS_HIP(hipEventSynchronize(h->flev[q][3]));
         {
             const int pairs[6][2] = {{0, 1}, {1, 2}, {3, 4}, {4, 5}, {5, 6}, {6, 7}};
-            for (int q = 0; q < 6; ++q) {
+            for (int i = 0; i < 6; ++i) {
                 float t;
-                DDNS_HIP(hipEventElapsedTime(&t, h->fev[pairs[q][0]], h->fev[pairs[q][1]]));
-                kms[q] += t;
+                DDNS_HIP(hipEventElapsedTime(&t, h->fev[q][pairs[i][0]], h->fev[q][pairs[i][1]]));
+                kms[i] += t;
             }
         }
         float t1, t2;
-        DDNS_HIP(hipEventElapsedTime(&t1, h->ev[0], h->ev[1]));
-        DDNS_HIP(hipEventElapsedTime(&t2, h->ev[2], h->ev[3]));
+        DDNS_HIP(hipEventElapsedTime(&t1, h->flev[q][0], h->flev[q][1]));
+        DDNS_HIP(hipEventElapsedTime(&t2, h->flev[q][2], h->flev[q][3]));
         flow_ms += t1 + t2;
+        h->par = 1 - q;
         h->time += h->p.dt;                                                   // 693
     }
     DDNS_HIP(hipEventRecord(h->ev[7], s));
