@@ -590,14 +590,40 @@ static PairGeom make_geom(const PartBuffers& pb, const PartConst& pc, double dst
     return g;
 }
 
+// Several device spans (4-byte multiples) cleared by one launch instead of one fill
+// per span: the collision statistics start from zero every step.
+struct ZeroSpans {
+    unsigned* p[8];
+    size_t words[8];
+    int n;
+};
+
+__global__ void __launch_bounds__(256) k_zero_spans(ZeroSpans z)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (int k = 0; k < z.n; ++k)
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < z.words[k]; i += stride) z.p[k][i] = 0u;
+}
+
 int part_collide(PartBuffers& pb, const PartConst& pc, hipStream_t s)
 {
     const size_t nb = (size_t)pc.nset * pc.nset * DDNS_NBIN;
-    DDNS_HIP(hipMemsetAsync(pb.nirbin, 0, nb * sizeof(int), s));
-    DDNS_HIP(hipMemsetAsync(pb.bins, 0, 7 * nb * sizeof(double), s));
-    DDNS_HIP(hipMemsetAsync(pb.counts, 0, 512 * sizeof(int), s));
-    DDNS_HIP(hipMemsetAsync(pb.tested, 0, 4 * sizeof(long long), s));
-    DDNS_HIP(hipMemsetAsync(pb.mult, 0, (size_t)(pc.n + 1) * sizeof(int), s));
+    if (!pb.hit) DDNS_HIP(hipMalloc(&pb.hit, (size_t)pb.cap * sizeof(int)));
+    {
+        // bins, counters, the removal multiplicities, the per-droplet hit marks and the
+        // search's flags ([32..35] extents, [40..41] pair counts)
+        ZeroSpans z{};
+        auto add = [&](void* p, size_t bytes) { z.p[z.n] = (unsigned*)p; z.words[z.n] = bytes / 4; ++z.n; };
+        add(pb.nirbin, nb * sizeof(int));
+        add(pb.bins, 7 * nb * sizeof(double));
+        add(pb.counts, 512 * sizeof(int));
+        add(pb.tested, 4 * sizeof(long long));
+        add(pb.mult, (size_t)(pc.n + 1) * sizeof(int));
+        add(pb.hit, (size_t)pc.n * sizeof(int));
+        add(pb.flags + 32, 10 * sizeof(int));
+        hipLaunchKernelGGL(k_zero_spans, dim3(512), dim3(256), 0, s, z);
+        DDNS_HIP(hipGetLastError());
+    }
     if (pc.n == 0) return 0;
     CollArgs a{};
     a.pos = pb.pos; a.vel = pb.vel; a.rad = pb.rad; a.gid = pb.gid; a.fvp0 = pb.fvp; a.dmove = pb.dmove; a.vp = pb.vp;
@@ -610,25 +636,22 @@ int part_collide(PartBuffers& pb, const PartConst& pc, hipStream_t s)
     a.nirbin = pb.nirbin; a.bins = pb.bins; a.counts = pb.counts; a.tested = pb.tested;
     a.pairs = pb.pairs; a.pair_cap = pb.pair_cap; a.mult = pb.mult;
     unsigned long long* dmax = (unsigned long long*)(pb.flags + 32);
-    DDNS_HIP(hipMemsetAsync(dmax, 0, 2 * sizeof(unsigned long long), s));
     hipLaunchKernelGGL(k_extent_max, dim3(std::min(256, (pc.n + 255) / 256)), dim3(256), 0, s, (const double*)pb.dmove,
                        (const double*)pb.rad, pc.n, pb.cap, dmax);
     a.dmax = dmax;
-    if (!pb.hit) DDNS_HIP(hipMalloc(&pb.hit, (size_t)pb.cap * sizeof(int)));
     if (!pb.cand) {
         pb.cand_cap = std::max(1 << 16, 2 * pb.cap);
         DDNS_HIP(hipMalloc(&pb.cand, (size_t)pb.cand_cap * sizeof(int2)));
     }
-    DDNS_HIP(hipMemsetAsync(pb.hit, 0, (size_t)pc.n * sizeof(int), s));
     a.hit = pb.hit;
     a.ncand = pb.flags + 40;
     a.ncontact = pb.flags + 41;
     int found[2] = {0, 0};       // handed-on pairs, pairs in contact
     int& ncand = found[0];
-    for (;;) {
+    for (bool again = false;; again = true) {
         a.cand = (int2*)pb.cand;
         a.cand_cap = pb.cand_cap;
-        DDNS_HIP(hipMemsetAsync(a.ncand, 0, 2 * sizeof(int), s));
+        if (again) DDNS_HIP(hipMemsetAsync(a.ncand, 0, 2 * sizeof(int), s));
         hipLaunchKernelGGL(k_collide, dim3((pc.n + CT - 1) / CT), dim3(CT), 0, s, a);
         DDNS_HIP(hipGetLastError());
         DDNS_HIP(hipMemcpyAsync(found, a.ncand, sizeof(found), hipMemcpyDeviceToHost, s));
