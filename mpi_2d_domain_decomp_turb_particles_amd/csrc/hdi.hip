@@ -154,61 +154,67 @@ __global__ void __launch_bounds__(HT) k_spmv(const int* __restrict__ cnt, const 
     }
 }
 
-// ------------------------------------------------------------------ reductions
-template <typename TA, typename TB>
-__global__ void __launch_bounds__(HT) k_dot_part(const TA* a, const TB* b, long long n, double* part)
-{
-    __shared__ double s[HT];
-    double acc = 0.0;
-#pragma unroll 4
-    for (long long i = (long long)blockIdx.x * HT + threadIdx.x; i < n; i += (long long)gridDim.x * HT)
-        acc += (double)a[i] * (double)b[i];
-    s[threadIdx.x] = acc;
-    __syncthreads();
-    for (int o = HT / 2; o > 0; o >>= 1) {
-        if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
-}
-
-// out = sum(part) (mode 0), sqrt(sum) (mode 1)
-__global__ void __launch_bounds__(HT) k_dot_final(const double* part, int np, double* out, int mode)
-{
-    __shared__ double s[HT];
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < np; i += HT) acc += part[i];
-    s[threadIdx.x] = acc;
-    __syncthreads();
-    for (int o = HT / 2; o > 0; o >>= 1) {
-        if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *out = mode ? sqrt(s[0]) : s[0];
-}
-
-template <typename TA, typename TB>
-static int dot(const TA* a, const TB* b, long long n, double* part, double* out, int mode, hipStream_t s)
-{
-    const int nb = (int)std::min<long long>(RED_BLOCKS, (n + HT - 1) / HT);
-    hipLaunchKernelGGL((k_dot_part<TA, TB>), dim3(nb > 0 ? nb : 1), dim3(HT), 0, s, a, b, n, part);
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(HT), 0, s, part, nb > 0 ? nb : 1, out, mode);
-    DDNS_HIP(hipGetLastError());
-    return 0;
-}
-
 // ------------------------------------------------------------------ vector kernels
+// r0 = b - A x0 into the first Krylov column, with the work-group partial sums of b.b
+// and r0.r0 (part[blk], part[RED_BLOCKS + blk]) on a fixed grid
 template <typename T>
-__global__ void k_residual0(const double* b, const double* ax0, T* q, long long n)
+__global__ void __launch_bounds__(HT) k_residual0(const double* __restrict__ b, const double* __restrict__ ax0,
+                                                  T* __restrict__ q, long long n, double* __restrict__ part)
 {
-    const long long i = (long long)blockIdx.x * HT + threadIdx.x;
-    if (i < n) q[i] = (T)(b[i] - ax0[i]);
+    __shared__ double sb[HT], sq[HT];
+    double ab = 0.0, aq = 0.0;
+    for (long long i = (long long)blockIdx.x * HT + threadIdx.x; i < n; i += (long long)gridDim.x * HT) {
+        const double bi = b[i];
+        const T r = (T)(bi - ax0[i]);
+        q[i] = r;
+        ab += bi * bi;
+        aq += (double)r * (double)r;
+    }
+    sb[threadIdx.x] = ab;
+    sq[threadIdx.x] = aq;
+    __syncthreads();
+    for (int o = HT / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            sb[threadIdx.x] += sb[threadIdx.x + o];
+            sq[threadIdx.x] += sq[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = sb[0];
+        part[RED_BLOCKS + blockIdx.x] = sq[0];
+    }
 }
 
-__global__ void k_gm_start(double* sm, double tol)
+// The small problem cleared (ntot values), then norm_b = |b| and norm_rhs = |r0| from the
+// np partials of k_residual0 (fixed order), the tolerance, and the stop flag when the
+// initial residual is below 1e-5 (2866-2868)
+__global__ void __launch_bounds__(HT) k_gm_start(double* sm, int ntot, double tol, const double* __restrict__ part,
+                                                 int np)
 {
-    sm[3] = tol;
-    if (sm[1] < 1e-5) sm[4] = -1.0;
+    __shared__ double sb[HT], sq[HT];
+    for (int i = threadIdx.x; i < ntot; i += HT) sm[i] = 0.0;
+    double ab = 0.0, aq = 0.0;
+    for (int i = threadIdx.x; i < np; i += HT) {
+        ab += part[i];
+        aq += part[RED_BLOCKS + i];
+    }
+    sb[threadIdx.x] = ab;
+    sq[threadIdx.x] = aq;
+    __syncthreads();
+    for (int o = HT / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            sb[threadIdx.x] += sb[threadIdx.x + o];
+            sq[threadIdx.x] += sq[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {                // after the barriers of the tree: the clearing is done
+        sm[0] = sqrt(sb[0]);
+        sm[1] = sqrt(sq[0]);
+        sm[3] = tol;
+        if (sm[1] < 1e-5) sm[4] = -1.0;
+    }
 }
 
 template <typename T>
@@ -705,7 +711,6 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
     T* Q = (T*)pb.gm_Q;
     const long long ld = (n3 + 63) / 64 * 64;     // column stride of the Krylov vectors
     T* work[2] = {Q + (size_t)maxiter * ld, Q + (size_t)(maxiter + 1) * ld};
-    DDNS_HIP(hipMemsetAsync(sm, 0, L.total() * sizeof(double), s));
     const unsigned nb3 = (unsigned)((n3 + HT - 1) / HT);
     const unsigned nbr = (unsigned)((n + HT - 1) / HT);
     const unsigned nred = (unsigned)std::min<long long>(RED_BLOCKS, (n3 + 4LL * RT - 1) / (4LL * RT));   // reduction grid
@@ -713,14 +718,11 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
     hipLaunchKernelGGL((k_spmv<T, 0>), dim3(nbr), dim3(HT), 0, s, (const int*)pb.nei_cnt, (const int*)pb.nei_idx,
                        (const T*)pb.nei_m, n, (const T*)nullptr, (const double*)nullptr, (T*)nullptr, (T*)nullptr,
                        (const double*)pb.gm_x0, ax0, (const double*)pb.gm_w, pb.gm_b, (const double*)nullptr);
-    hipLaunchKernelGGL((k_residual0<T>), dim3(nb3), dim3(HT), 0, s, pb.gm_b, ax0, Q, n3);
-    st = dot(pb.gm_b, pb.gm_b, n3, pb.gm_red, sm + 0, 1, s);
-    if (st) return st;
-    st = dot(Q, Q, n3, pb.gm_red, sm + 1, 1, s);
-    if (st) return st;
+    const int nr0 = (int)std::min<long long>(RED_BLOCKS, (n3 + HT - 1) / HT);
+    hipLaunchKernelGGL((k_residual0<T>), dim3(nr0), dim3(HT), 0, s, pb.gm_b, ax0, Q, n3, pb.gm_red);
     // the tolerance, and the stop flag when the initial residual is below 1e-5
     // (2866-2868): then no iteration runs and pertvel keeps its warm-start value
-    hipLaunchKernelGGL(k_gm_start, dim3(1), dim3(1), 0, s, sm, tol);
+    hipLaunchKernelGGL(k_gm_start, dim3(1), dim3(HT), 0, s, sm, (int)L.total(), tol, (const double*)pb.gm_red, nr0);
     hipLaunchKernelGGL((k_scale<T>), dim3(nb3), dim3(HT), 0, s, Q, sm + 1, n3);
     // Iterations are launched in batches; every kernel of an iteration returns at
     // once when the least-squares update of an earlier one has set the converged
