@@ -657,8 +657,9 @@ static double cell_width(const ddns_handle* h)
 // The forward transforms and the update of the step (turb.f90:644-691) on the flow
 // stream.  The droplet block reads only the physical velocity of the first half and the
 // flow reads nothing of the droplets, so this may run beside any part of the block; it
-// is queued to start when the block's gathers are done (event after), where the block
-// turns from bandwidth-heavy gathers to latency-bound work.
+// is queued to start when the HI solve is done (event after): the collision search,
+// the pair work and the overlap sweep that follow are latency-bound and leave HBM to it,
+// while the GMRES kernels before stream the Krylov basis themselves.
 // The first half of a step's flow (the curl and inverse z transforms when the state
 // is new, turb.f90:506-526; the inverse y and x stages and the physical velocity,
 // 527-539) on the flow stream, with the event set of parity q.
@@ -721,8 +722,6 @@ static int droplet_block(ddns_handle* h, float* ms)
     if (st) return st;
     DDNS_HIP(hipEventRecord(h->ev[10], s));
     h->cc = ddns_collision_counts{};
-    st = queue_flow_rest(h, h->ev[10]);                                       // after the gather
-    if (st) return st;
     if (h->p.hdi_included) {                                                  // 571-572
         int it = 0; double res = 0;
         st = part_hdi(h->pb, pc, h->p.gmres_maxiter, h->p.gmres_tol, &it, &res, s);
@@ -732,6 +731,8 @@ static int droplet_block(ddns_handle* h, float* ms)
         h->cc.gmres_residual = res;
     }
     DDNS_HIP(hipEventRecord(h->ev[11], s));
+    st = queue_flow_rest(h, h->ev[11]);                                       // after the HI solve
+    if (st) return st;
     st = part_advance(h->pb, pc, s);                                          // 574-576
     if (st) return st;
     h->new_particle = 0;
