@@ -654,12 +654,6 @@ static double cell_width(const ddns_handle* h)
     return w;
 }
 
-// The forward transforms and the update of the step (turb.f90:644-691) on the flow
-// stream.  The droplet block reads only the physical velocity of the first half and the
-// flow reads nothing of the droplets, so this may run beside any part of the block; it
-// is queued to start when the HI solve is done (event after): the collision search,
-// the pair work and the overlap sweep that follow are latency-bound and leave HBM to it,
-// while the GMRES kernels before stream the Krylov basis themselves.
 // The first half of a step's flow (the curl and inverse z transforms when the state
 // is new, turb.f90:506-526; the inverse y and x stages and the physical velocity,
 // 527-539) on the flow stream, with the event set of parity q.
@@ -678,6 +672,12 @@ static int queue_flow_first(ddns_handle* h, int q)
     return 0;
 }
 
+// The forward transforms and the update of the step (turb.f90:644-691) on the flow
+// stream.  The droplet block reads only the physical velocity of the first half and the
+// flow reads nothing of the droplets, so this may run beside any part of the block; it
+// is queued to start when the HI solve is done (event after): the collision search,
+// the pair work and the overlap sweep that follow are latency-bound and leave HBM to it,
+// while the GMRES kernels before stream the Krylov basis themselves.
 static int queue_flow_rest(ddns_handle* h, hipEvent_t after)
 {
     if (h->flow_rest_queued) return 0;
@@ -690,8 +690,8 @@ static int queue_flow_rest(ddns_handle* h, hipEvent_t after)
     DDNS_HIP(hipEventRecord(h->flev[q][3], s));
     h->flow_rest_queued = true;
     // The next step's first half can follow at once: it overwrites the physical velocity,
-    // and the gather that reads it (event after) is done before this queue moves on.  It
-    // then runs beside the rest of this droplet block.
+    // and the gather that reads it comes before event after, which this queue has waited
+    // for.  It then runs beside the rest of this droplet block.
     if (h->want_next && after) {
         st = queue_flow_first(h, 1 - q);
         if (st) return st;
