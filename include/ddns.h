@@ -239,7 +239,9 @@ int ddns_overlap_fix(ddns_handle* h);
 /* ---- the step ------------------------------------------------------------------------ */
 
 /* Advance nsteps time steps: the flow block turb.f90:506-691 and, when npart > 0, the
- * droplet block turb.f90:546-623, in the reference order.  Returns the first nonzero
+ * droplet block turb.f90:546-623, with the reference's data flow (the droplets see the
+ * physical velocity of the step's start; the flow does not see the droplets), the two
+ * blocks overlapped on two streams.  Returns, with all work done, the first nonzero
  * status (a warning does not stop the loop). */
 int ddns_step(ddns_handle* h, int nsteps);
 
