@@ -17,6 +17,7 @@
 #include "fft_engine.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 
@@ -70,6 +71,18 @@ __device__ __forceinline__ int xcd_tile(int L)
     }
 }
 template <int TC> constexpr int ldr_of() { return (TC >= 8) ? TC + 1 : TC; }
+// One-tile (plain) z transforms for N >= 512 take rows of 128 bytes: one padded tile of
+// N rows fits in LDS at that width even where the four tiles of the fused stage do not
+template <typename T, int N> constexpr int kz_tc_plain()
+{
+    if constexpr (N < 512) return kz_tc<T, N>();
+    else return sizeof(T) == 4 ? 16 : 8;
+}
+// the fused z stage keeps rows of at least 64 bytes; below that (N = 1024) the stage runs
+// as separate kernels: three plain forward transforms, the pointwise update, the curl,
+// six plain inverse transforms (DDNS_Z_SPLIT=1 in the environment selects this path at
+// any N, to check it against the fused one)
+template <typename T, int N> constexpr bool kz_split_default() { return kz_tc<T, N>() * (int)sizeof(cpx<T>) < 64; }
 template <typename T, int N> constexpr int kz_ldr()
 {
     constexpr int TC = kz_tc<T, N>();
@@ -165,8 +178,9 @@ __global__ void __launch_bounds__(NT) k_z(KzArgs a)
 {
     using C = cpx<T>;
     constexpr int M = N / 2;
-    constexpr int TC = kz_tc<T, N>();
-    constexpr int LDR = kz_ldr<T, N>();
+    constexpr bool PLAIN = MODE == KZ_PLAIN_FWD || MODE == KZ_PLAIN_INV;
+    constexpr int TC = PLAIN ? kz_tc_plain<T, N>() : kz_tc<T, N>();
+    constexpr int LDR = PLAIN ? ldr_of<TC>() : kz_ldr<T, N>();
     constexpr int NTILE = MODE == KZ_CURL_INV ? 4 : 1;
     __shared__ C lds[NTILE][N * LDR];
     __shared__ C tws[N];
@@ -215,6 +229,50 @@ __global__ void __launch_bounds__(NT) k_z(KzArgs a)
 #pragma unroll
         for (int q = 0; q < 3; ++q) tile_store<C, N, TC, LDR, NT>((C*)a.A[q] + base, lds[q], rowstride, tid);
     }
+}
+
+// The split z stage's pointwise parts.  k_z_update: the update of k_z_full (dealias,
+// AB2/CN, forcing, projection, CN divide) on the z-transformed N-hat already scaled by
+// 1/N.  k_z_curl: w-hat = i k x u-hat (turb.f90:506-517) into A[3..5].
+template <typename T, int N>
+__global__ void __launch_bounds__(256) k_z_update(KzArgs a)
+{
+    using C = cpx<T>;
+    constexpr int M = N / 2;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)N * N * M) return;
+    const int kx = (int)(t % M), ky = (int)((t / M) % N), kz = (int)(t / ((size_t)N * M));
+    C nn[3], u3[3], o3[3], vv[3], nd[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        nn[q] = ((const C*)a.B[q])[t];
+        u3[q] = ((const C*)a.U[q])[t];
+        o3[q] = ((const C*)a.O[q])[t];
+    }
+    update_mode<T>(kx, ky, kz, N, a.nek, a.forcing, (T)a.dt, (T)a.rnu, a.low, nn, u3, o3, vv, nd);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        ((C*)a.U[q])[t] = vv[q];
+        ((C*)a.O[q])[t] = nd[q];
+    }
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__(256) k_z_curl(KzArgs a)
+{
+    using C = cpx<T>;
+    constexpr int M = N / 2;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)N * N * M) return;
+    const int kx = (int)(t % M), ky = (int)((t / M) % N), kz = (int)(t / ((size_t)N * M));
+    const T fx = (T)kx, fy = (T)wave_index(ky, N), fz = (T)wave_index(kz, N);
+    const C ux = ((const C*)a.U[0])[t], uy = ((const C*)a.U[1])[t], uz = ((const C*)a.U[2])[t];
+    const C w0 = csub(cscale(uz, fy), cscale(uy, fz));
+    const C w1 = csub(cscale(ux, fz), cscale(uz, fx));
+    const C w2 = csub(cscale(uy, fx), cscale(ux, fy));
+    ((C*)a.A[3])[t] = mk<C>(-w0.y, w0.x);
+    ((C*)a.A[4])[t] = mk<C>(-w1.y, w1.x);
+    ((C*)a.A[5])[t] = mk<C>(-w2.y, w2.x);
 }
 
 // The full z stage as a persistent kernel: one work-group per CU walks the tiles
@@ -1019,6 +1077,33 @@ static int launch_z(int mode, const FlowBuffers& fb, const FlowConst& fc, hipStr
     if (mode == KZ_PLAIN_FWD || mode == KZ_PLAIN_INV) { a.B[0] = src; a.U[0] = dst; }
     constexpr int TC = kz_tc<T, N>();
     dim3 grid((N / 2 / TC) * N);
+    const char* env = (mode == KZ_FULL || mode == KZ_CURL_INV) ? std::getenv("DDNS_Z_SPLIT") : nullptr;
+    const bool split = kz_split_default<T, N>() || (env && env[0] == '1');
+    if (split && (mode == KZ_FULL || mode == KZ_CURL_INV)) {
+        const unsigned gp = (unsigned)(N / 2 / kz_tc_plain<T, N>() * N);
+        const unsigned ge = (unsigned)(((size_t)N * N * (N / 2) + 255) / 256);
+        KzArgs p = a;
+        int st;
+        if (mode == KZ_FULL) {
+            p.scale = 1.0 / N;
+            for (int q = 0; q < 3; ++q) {                  // forward z of N-hat, in place, x 1/N
+                p.B[0] = a.B[q]; p.U[0] = (void*)a.B[q];
+                hipLaunchKernelGGL((k_z<T, N, KZ_PLAIN_FWD>), dim3(gp), dim3(NT), 0, s, p);
+            }
+            hipLaunchKernelGGL((k_z_update<T, N>), dim3(ge), dim3(256), 0, s, a);
+        }
+        hipLaunchKernelGGL((k_z_curl<T, N>), dim3(ge), dim3(256), 0, s, a);
+        p.scale = 1.0;
+        for (int q = 0; q < 6; ++q) {                      // inverse z of u-hat and w-hat
+            p.B[0] = q < 3 ? (const void*)a.U[q] : (const void*)a.A[q];
+            p.U[0] = a.A[q];
+            hipLaunchKernelGGL((k_z<T, N, KZ_PLAIN_INV>), dim3(gp), dim3(NT), 0, s, p);
+        }
+        st = 0;
+        DDNS_HIP(hipGetLastError());
+        return st;
+    }
+    if (mode == KZ_PLAIN_FWD || mode == KZ_PLAIN_INV) grid = dim3((N / 2 / kz_tc_plain<T, N>()) * N);
     if (mode == KZ_FULL) {
         // persistent: one work-group per CU (the LDS tiles allow no more)
         int dev = 0, ncu = 0;

@@ -333,3 +333,24 @@ def test_config_128_stochastic_hi_matches_oracle(ddns, oracle_mod):
         dg, do = s.get_droplets(), o.get_droplets()
         assert np.allclose(dg["yp"][:6], do["yp"][:6], rtol=1e-9, atol=1e-10)
         assert np.abs(dg["pertvel"] - do["pertvel"]).max() < 1e-8 * max(1e-30, np.abs(do["pertvel"]).max())
+
+
+@pytest.mark.parametrize("forcing", [1, 2])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_split_z_stage_matches_oracle(ddns, oracle_mod, monkeypatch, forcing, prec):
+    """The z stage as separate kernels (plain transforms, pointwise update and curl), the
+    path the largest grids take, selected here at N = 32 through DDNS_Z_SPLIT."""
+    monkeypatch.setenv("DDNS_Z_SPLIT", "1")
+    N = 32
+    p = _params(ddns, N=N, forcing=forcing, precision=prec, dt=0.002, rnu=0.02)
+    v = random_spectral(N, seed=7 + forcing, kmax=6)
+    with ddns.Solver(p) as s, oracle_mod.Oracle(_params(ddns, N=N, forcing=forcing, dt=0.002, rnu=0.02)) as o:
+        s.set_flow(v, v)
+        o.set_flow(v, v)
+        o.flow_only(True)
+        s.step(10)
+        o.step(10)
+        gv, go = s.get_flow()
+        ov, oo = o.get_flow()
+        assert rel_l2(gv, ov) < (1e-11 if prec == 0 else 1e-5)
+        assert rel_l2(go, oo) < (1e-10 if prec == 0 else 1e-4)

@@ -50,3 +50,16 @@ def test_flow_512_fp32_mixed_against_fp64(ddns):
         assert np.all(np.isfinite(ek)) and ek.sum() > 0
         assert abs(ek.sum() - m.qq) <= 1e-9 * m.qq          # total energy of monitorb.dat
         assert m.vdiv2 <= tol * m.vort2                     # solenoidal: rms div u against rms |w|
+
+
+def test_split_z_stage_matches_fused_512(ddns, monkeypatch):
+    """At 512^3 both z-stage paths fit: the separate kernels give the fused result."""
+    kw = dict(N=N, forcing=1, dt=0.001, rnu=0.002, precision=0)
+    out = []
+    for split in ("0", "1"):
+        monkeypatch.setenv("DDNS_Z_SPLIT", split)
+        with ddns.Solver(ddns.default_params(**kw)) as s:
+            s.init_flow(ddns.SPEC_K4_GAUSS, seed=5, k0=4.0, urms=1.0)
+            s.step(3)
+            out.append(s.get_flow()[0])
+    assert rel_l2(out[1], out[0]) < 1e-13
