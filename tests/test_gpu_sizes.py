@@ -2,7 +2,8 @@
 that do not depend on the size: the 3-D transform pair is an identity, the
 FP32-mixed velocity stays within 1e-5 relative L2 of the FP64 one after 10 steps,
 the field stays solenoidal, and the spectrum sums to the monitors' energy.  The
-512-point plans (z tiles of 8 columns in single precision) run nowhere else."""
+512-point plans and the z stage as separate kernels (the path from N = 512 up) run
+nowhere else at their own size."""
 import numpy as np
 import pytest
 
@@ -53,7 +54,8 @@ def test_flow_512_fp32_mixed_against_fp64(ddns):
 
 
 def test_split_z_stage_matches_fused_512(ddns, monkeypatch):
-    """At 512^3 both z-stage paths fit: the separate kernels give the fused result."""
+    """At 512^3 both z-stage paths fit (the separate kernels are the default there): they
+    give the same field."""
     kw = dict(N=N, forcing=1, dt=0.001, rnu=0.002, precision=0)
     out = []
     for split in ("0", "1"):
