@@ -81,7 +81,8 @@ template <typename T, int N> constexpr int kz_tc_plain()
 // Where the fused z stage's tiles hold rows shorter than 128 bytes (N >= 512) the stage
 // runs as separate kernels instead: three plain forward transforms, the pointwise update,
 // the curl, six plain inverse transforms.  They move 39 fields instead of 21 but at the
-// rate of full rows: 9 % (FP32) and 15 % (FP64) faster at 512^3, 38 % at 1024^3.
+// rate of full rows (36 with the curl formed in the update): faster by 9 % (FP32) and
+// 15 % (FP64) at 512^3, 38 % at 1024^3.
 // DDNS_Z_SPLIT=1 or 0 in the environment selects one path at any N, to check them
 // against each other.
 template <typename T, int N> constexpr bool kz_split_default() { return kz_tc<T, N>() * (int)sizeof(cpx<T>) < 128; }
@@ -235,7 +236,20 @@ __global__ void __launch_bounds__(NT) k_z(KzArgs a)
 
 // The split z stage's pointwise parts.  k_z_update: the update of k_z_full (dealias,
 // AB2/CN, forcing, projection, CN divide) on the z-transformed N-hat already scaled by
-// 1/N.  k_z_curl: w-hat = i k x u-hat (turb.f90:506-517) into A[3..5].
+// 1/N, and the curl of the new u-hat.  k_z_curl: the curl alone, for a new state.
+// The curl is w-hat = i k x u-hat (turb.f90:506-517), into A[3..5].
+template <typename T>
+__device__ __forceinline__ void curl_to(const KzArgs& a, size_t t, int kx, int ky, int kz, int N, const cpx<T>* u)
+{
+    using C = cpx<T>;
+    const T fx = (T)kx, fy = (T)wave_index(ky, N), fz = (T)wave_index(kz, N);
+    const C w0 = csub(cscale(u[2], fy), cscale(u[1], fz));
+    const C w1 = csub(cscale(u[0], fz), cscale(u[2], fx));
+    const C w2 = csub(cscale(u[1], fx), cscale(u[0], fy));
+    ((C*)a.A[3])[t] = mk<C>(-w0.y, w0.x);
+    ((C*)a.A[4])[t] = mk<C>(-w1.y, w1.x);
+    ((C*)a.A[5])[t] = mk<C>(-w2.y, w2.x);
+}
 template <typename T, int N>
 __global__ void __launch_bounds__(256) k_z_update(KzArgs a)
 {
@@ -257,6 +271,7 @@ __global__ void __launch_bounds__(256) k_z_update(KzArgs a)
         ((C*)a.U[q])[t] = vv[q];
         ((C*)a.O[q])[t] = nd[q];
     }
+    curl_to<T>(a, t, kx, ky, kz, N, vv);
 }
 
 template <typename T, int N>
@@ -267,14 +282,8 @@ __global__ void __launch_bounds__(256) k_z_curl(KzArgs a)
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)N * N * M) return;
     const int kx = (int)(t % M), ky = (int)((t / M) % N), kz = (int)(t / ((size_t)N * M));
-    const T fx = (T)kx, fy = (T)wave_index(ky, N), fz = (T)wave_index(kz, N);
-    const C ux = ((const C*)a.U[0])[t], uy = ((const C*)a.U[1])[t], uz = ((const C*)a.U[2])[t];
-    const C w0 = csub(cscale(uz, fy), cscale(uy, fz));
-    const C w1 = csub(cscale(ux, fz), cscale(uz, fx));
-    const C w2 = csub(cscale(uy, fx), cscale(ux, fy));
-    ((C*)a.A[3])[t] = mk<C>(-w0.y, w0.x);
-    ((C*)a.A[4])[t] = mk<C>(-w1.y, w1.x);
-    ((C*)a.A[5])[t] = mk<C>(-w2.y, w2.x);
+    const C u[3] = {((const C*)a.U[0])[t], ((const C*)a.U[1])[t], ((const C*)a.U[2])[t]};
+    curl_to<T>(a, t, kx, ky, kz, N, u);
 }
 
 // The full z stage as a persistent kernel: one work-group per CU walks the tiles
@@ -1092,9 +1101,10 @@ static int launch_z(int mode, const FlowBuffers& fb, const FlowConst& fc, hipStr
                 p.B[0] = a.B[q]; p.U[0] = (void*)a.B[q];
                 hipLaunchKernelGGL((k_z<T, N, KZ_PLAIN_FWD>), dim3(gp), dim3(NT), 0, s, p);
             }
-            hipLaunchKernelGGL((k_z_update<T, N>), dim3(ge), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((k_z_update<T, N>), dim3(ge), dim3(256), 0, s, a);      // and the curl
+        } else {
+            hipLaunchKernelGGL((k_z_curl<T, N>), dim3(ge), dim3(256), 0, s, a);
         }
-        hipLaunchKernelGGL((k_z_curl<T, N>), dim3(ge), dim3(256), 0, s, a);
         p.scale = 1.0;
         for (int q = 0; q < 6; ++q) {                      // inverse z of u-hat and w-hat
             p.B[0] = q < 3 ? (const void*)a.U[q] : (const void*)a.A[q];
