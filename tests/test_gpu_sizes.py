@@ -65,3 +65,23 @@ def test_split_z_stage_matches_fused_512(ddns, monkeypatch):
             s.step(3)
             out.append(s.get_flow()[0])
     assert rel_l2(out[1], out[0]) < 1e-13
+
+
+def test_flow_1024_fp32_mixed():
+    """The largest grid (1024^3, the 5M-droplet configuration's flow) through the device
+    initial field and two steps: the z stage as separate kernels, 1024-point plans.  The
+    start holds 3/2 u_rms^2 exactly; after a step the deterministic forcing has set the
+    energies of shells 1 and 2 to force(1:2) (supforDET, flowlib.f90:195-229), and the
+    field stays solenoidal."""
+    import mpi_2d_domain_decomp_turb_particles_amd as dd
+    p = dd.default_params(N=1024, precision=dd.FP32_MIXED, dt=0.00025, rnu=0.0008, forcing=1)
+    with dd.Solver(p) as s:
+        s.init_flow(dd.SPEC_K4_GAUSS, seed=123, k0=4.0, urms=1.0)
+        m0 = s.monitors(divstat=True)
+        assert abs(m0.qq - 1.5) < 1e-5
+        s.step(2)
+        m = s.monitors(divstat=True)
+        ek = s.spectrum()
+    assert np.all(np.isfinite(ek)) and abs(ek.sum() - m.qq) <= 1e-6 * m.qq
+    assert abs(ek[0] - p.force[0]) <= 1e-5 * p.force[0] and abs(ek[1] - p.force[1]) <= 1e-5 * p.force[1]
+    assert m.vdiv2 <= 1e-5 * m.vort2
