@@ -85,6 +85,13 @@ template <typename T, int N> constexpr int kz_tc_plain()
 // 15 % (FP64) at 512^3, 38 % at 1024^3.
 // DDNS_Z_SPLIT=1 or 0 in the environment selects one path at any N, to check them
 // against each other.
+// threads of a z work-group: the single-tile transforms of N = 1024 take 512, so that a CU,
+// which holds one such tile in LDS, has twice the loads in flight (8 % faster there; at
+// N = 512 two 256-thread tiles fit in LDS and do better)
+template <typename T, int N, int MODE> constexpr int kz_threads()
+{
+    return ((MODE == KZ_PLAIN_FWD || MODE == KZ_PLAIN_INV) && N >= 1024) ? 512 : NT;
+}
 template <typename T, int N> constexpr bool kz_split_default() { return kz_tc<T, N>() * (int)sizeof(cpx<T>) < 128; }
 template <typename T, int N> constexpr int kz_ldr()
 {
@@ -177,8 +184,9 @@ __device__ __forceinline__ void update_mode(int kx, int ky, int kz, int N, int n
 // One tile per work-group: curl and inverse z transforms of u-hat (KZ_CURL_INV, the
 // first step) or a plain forward / inverse z transform of one field.
 template <typename T, int N, int MODE>
-__global__ void __launch_bounds__(NT) k_z(KzArgs a)
+__global__ void __launch_bounds__((kz_threads<T, N, MODE>())) k_z(KzArgs a)
 {
+    constexpr int NTK = kz_threads<T, N, MODE>();
     using C = cpx<T>;
     constexpr int M = N / 2;
     constexpr bool PLAIN = MODE == KZ_PLAIN_FWD || MODE == KZ_PLAIN_INV;
@@ -193,7 +201,7 @@ __global__ void __launch_bounds__(NT) k_z(KzArgs a)
     const int tile = xcd_tile<(NBX % P == 0) ? P : 1>(blockIdx.x);
     const int kx0 = (tile % NBX) * TC;
     const int ky = tile / NBX;
-    stage_twiddles<C, N, NT>(tws, (const C*)a.tw, tid);
+    stage_twiddles<C, N, NTK>(tws, (const C*)a.tw, tid);
     const C* tw = tws;
     const size_t rowstride = (size_t)N * M;
     const size_t base = (size_t)ky * M + kx0;
@@ -201,19 +209,19 @@ __global__ void __launch_bounds__(NT) k_z(KzArgs a)
     if constexpr (MODE == KZ_PLAIN_FWD || MODE == KZ_PLAIN_INV) {
         const C* src = (const C*)a.B[0];
         C* dst = (C*)a.U[0];
-        tile_load<C, N, TC, LDR, NT>(lds[0], src + base, rowstride, tid);
+        tile_load<C, N, TC, LDR, NTK>(lds[0], src + base, rowstride, tid);
         __syncthreads();
-        fft_tile<T, N, MODE == KZ_PLAIN_INV, TC, LDR, NT>(lds[0], tw, tid);
-        tile_store<C, N, TC, LDR, NT>(dst + base, lds[0], rowstride, tid, (T)a.scale);
+        fft_tile<T, N, MODE == KZ_PLAIN_INV, TC, LDR, NTK>(lds[0], tw, tid);
+        tile_store<C, N, TC, LDR, NTK>(dst + base, lds[0], rowstride, tid, (T)a.scale);
         return;
     } else {
 #pragma unroll
-        for (int q = 0; q < 3; ++q) tile_load<C, N, TC, LDR, NT>(lds[q], (const C*)a.U[q] + base, rowstride, tid);
+        for (int q = 0; q < 3; ++q) tile_load<C, N, TC, LDR, NTK>(lds[q], (const C*)a.U[q] + base, rowstride, tid);
         __syncthreads();
         // vorticity w = i k x u (turb.f90:506-517), one component at a time in tile 3
 #pragma unroll 1
         for (int q = 0; q < 3; ++q) {
-            for (int e = tid; e < TC * N; e += NT) {
+            for (int e = tid; e < TC * N; e += NTK) {
                 const int n = e / TC, c = e % TC;
                 const T fx = (T)(kx0 + c), fy = (T)wave_index(ky, N), fz = (T)wave_index(n, N);
                 const C ux = lds[0][n * LDR + c], uy = lds[1][n * LDR + c], uz = lds[2][n * LDR + c];
@@ -224,13 +232,13 @@ __global__ void __launch_bounds__(NT) k_z(KzArgs a)
                 lds[3][n * LDR + c] = mk<C>(-w.y, w.x);
             }
             __syncthreads();
-            fft_tile<T, N, true, TC, LDR, NT>(lds[3], tw, tid);
-            tile_store<C, N, TC, LDR, NT>((C*)a.A[3 + q] + base, lds[3], rowstride, tid);
+            fft_tile<T, N, true, TC, LDR, NTK>(lds[3], tw, tid);
+            tile_store<C, N, TC, LDR, NTK>((C*)a.A[3 + q] + base, lds[3], rowstride, tid);
             __syncthreads();
         }
-        fft_tile<T, N, true, TC, LDR, NT, 3, N * LDR>(lds[0], tw, tid);
+        fft_tile<T, N, true, TC, LDR, NTK, 3, N * LDR>(lds[0], tw, tid);
 #pragma unroll
-        for (int q = 0; q < 3; ++q) tile_store<C, N, TC, LDR, NT>((C*)a.A[q] + base, lds[q], rowstride, tid);
+        for (int q = 0; q < 3; ++q) tile_store<C, N, TC, LDR, NTK>((C*)a.A[q] + base, lds[q], rowstride, tid);
     }
 }
 
@@ -1099,7 +1107,7 @@ static int launch_z(int mode, const FlowBuffers& fb, const FlowConst& fc, hipStr
             p.scale = 1.0 / N;
             for (int q = 0; q < 3; ++q) {                  // forward z of N-hat, in place, x 1/N
                 p.B[0] = a.B[q]; p.U[0] = (void*)a.B[q];
-                hipLaunchKernelGGL((k_z<T, N, KZ_PLAIN_FWD>), dim3(gp), dim3(NT), 0, s, p);
+                hipLaunchKernelGGL((k_z<T, N, KZ_PLAIN_FWD>), dim3(gp), dim3(kz_threads<T, N, KZ_PLAIN_FWD>()), 0, s, p);
             }
             hipLaunchKernelGGL((k_z_update<T, N>), dim3(ge), dim3(256), 0, s, a);      // and the curl
         } else {
@@ -1109,7 +1117,7 @@ static int launch_z(int mode, const FlowBuffers& fb, const FlowConst& fc, hipStr
         for (int q = 0; q < 6; ++q) {                      // inverse z of u-hat and w-hat
             p.B[0] = q < 3 ? (const void*)a.U[q] : (const void*)a.A[q];
             p.U[0] = a.A[q];
-            hipLaunchKernelGGL((k_z<T, N, KZ_PLAIN_INV>), dim3(gp), dim3(NT), 0, s, p);
+            hipLaunchKernelGGL((k_z<T, N, KZ_PLAIN_INV>), dim3(gp), dim3(kz_threads<T, N, KZ_PLAIN_INV>()), 0, s, p);
         }
         st = 0;
         DDNS_HIP(hipGetLastError());
@@ -1128,8 +1136,8 @@ static int launch_z(int mode, const FlowBuffers& fb, const FlowConst& fc, hipStr
     }
     switch (mode) {
     case KZ_CURL_INV: hipLaunchKernelGGL((k_z<T, N, KZ_CURL_INV>), grid, dim3(NT), 0, s, a); break;
-    case KZ_PLAIN_FWD: hipLaunchKernelGGL((k_z<T, N, KZ_PLAIN_FWD>), grid, dim3(NT), 0, s, a); break;
-    default: hipLaunchKernelGGL((k_z<T, N, KZ_PLAIN_INV>), grid, dim3(NT), 0, s, a); break;
+    case KZ_PLAIN_FWD: hipLaunchKernelGGL((k_z<T, N, KZ_PLAIN_FWD>), grid, dim3(kz_threads<T, N, KZ_PLAIN_FWD>()), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_z<T, N, KZ_PLAIN_INV>), grid, dim3(kz_threads<T, N, KZ_PLAIN_INV>()), 0, s, a); break;
     }
     DDNS_HIP(hipGetLastError());
     return 0;
