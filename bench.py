@@ -5,18 +5,21 @@ deterministic forcing, 1,000,000 droplets with log-normal radii (median 25 um,
 sigma_ln 0.4, clipped to 5-60 um), hydrodynamic interactions with a cutoff of 50
 radii, collision detection and statistics.  A "step" is one full time step of the
 flow and of every droplet.  Synthetic inputs: seeded initial flow and droplets,
-generated on the host before the timed region; all state is resident in HBM.
+generated before the timed region; all state is resident in HBM.
 
-Prints ONE JSON line.  value = FP32-mixed steps/s; the FP64 path is timed on the
-same workload for the speed-up, and the two are compared after 10 steps for the
-velocity error.  The CPU baseline runs the FP64 oracle (the project's CPU
-restatement of the reference algorithm) on a bounded sample of the same workload.
+Prints ONE JSON line.  value = FP32-mixed steps/s; the FP64 path is timed on the same
+workload for the speed-up.  Accuracy is stated against the FP64 oracle (the project's
+CPU restatement of the reference algorithm): the velocity relative L2 and the per-shell
+E(k) error after 10 steps, and the collision kernel and contact RDF of the FP32-mixed
+path next to the oracle's over the steps of the CPU baseline, with Poisson error bars.
+The CPU baseline times the oracle on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import statistics
 import sys
 import time
 
@@ -29,6 +32,7 @@ import mpi_2d_domain_decomp_turb_particles_amd as dd  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 WORKLOAD = "256^3 DNS (deterministic forcing) + 1M log-normal droplets, HI 50 radii, collisions"
+FFT_KERNELS = ("y_inv", "x_fused", "y_fwd", "z_stage")
 
 
 def log(msg):
@@ -52,6 +56,14 @@ def make_solver(precision, N, npart, seed_flow=123, seed_drop=290710):
     return s
 
 
+def solver_from(state, precision, N, npart):
+    s = dd.Solver(workload_params(precision, N, npart))
+    s.set_flow(*state["flow"])
+    d = state["drop"]
+    s.set_droplets(d["yp"], d["fvp"], d["fdp"], d["pertvel"])
+    return s
+
+
 def kernel_bytes(N: int, prec: int):
     """Compulsory HBM bytes per launch of each flow kernel (every input read once,
     every output written once).  C = one complex field N*N*(N/2), R = one real field."""
@@ -60,11 +72,13 @@ def kernel_bytes(N: int, prec: int):
     C = N * N * (N // 2) * cs
     R = N * N * N * rs
     return {
-        "y_inv": 12 * C,            # 6 fields read + written
-        "x_fused": 6 * C + 3 * C + 3 * R,   # read u,w; write u x w and physical u
+        "y_inv": 12 * C,                    # 6 fields read + written
+        "x_fused": 6 * C + 3 * C + 3 * R,   # read u, w; write u x w and the physical u
         "y_fwd": 6 * C,
         "pre": 0,
-        "z_fused": 21 * C,          # read N, u, o (9C); write u, o, 6 inverse outputs (12C)
+        # below N = 512 one fused kernel reads N, u, o (9C) and writes u, o and the six
+        # inverse outputs (12C); from 512 the separate kernels move 36C
+        "z_stage": (21 if N < 512 else 36) * C,
         "sym": 0,
     }
 
@@ -92,9 +106,55 @@ def timed_run(s, steps, warmup):
     return (t1 - t0), s.timing(), st
 
 
-def cpu_baseline(N, npart, state, steps=2):
-    """FP64 oracle on the same workload: one untimed step (droplet history start),
-    then `steps` timed steps.  Returns steps/s and the thread count used."""
+def rel_l2(a, b):
+    num = sum(float(np.sum((x - y) ** 2)) for x, y in zip(a, b))
+    den = sum(float(np.sum(y ** 2)) for y in b)
+    return (num / den) ** 0.5
+
+
+def accuracy_vs_oracle(N, npart, state, steps=10):
+    """The FP32-mixed and FP64 GPU paths (with their droplets) and the oracle's flow
+    (which does not see the droplets) run `steps` steps from the same state."""
+    from oracle import oracle as O
+    out = {}
+    vel = {}
+    spec = {}
+    for prec, name in ((dd.FP32_MIXED, "fp32_mixed"), (dd.FP64, "fp64")):
+        with solver_from(state, prec, N, npart) as s:
+            s.step(steps)
+            vel[name] = s.get_flow()[0]
+            spec[name] = s.spectrum()
+    with O.Oracle(workload_params(dd.FP64, N, 0)) as o:
+        o.set_flow(*state["flow"])
+        o.flow_only(True)
+        o.step(steps)
+        vo = o.get_flow()[0]
+        eo = o.spectrum()
+    live = eo > 1e-12 * eo.max()
+    for name in ("fp32_mixed", "fp64"):
+        out[f"vel_rel_l2_{steps}_steps_{name}_vs_oracle"] = rel_l2(vel[name], vo)
+        out[f"ek_shell_max_rel_err_{name}_vs_oracle"] = float(np.max(np.abs(spec[name][live] / eo[live] - 1)))
+    out["vel_rel_l2_10_steps_fp32_vs_fp64"] = rel_l2(vel["fp32_mixed"], vel["fp64"])
+    out["ek_shells_compared"] = int(live.sum())
+    return out
+
+
+def contact_pairs_uniform(radii, st_percent, volume):
+    """Expected number of droplet pairs with R < d <= R (1 + st/100), R = a_i + a_j,
+    for uniformly random positions: sum over i < j of the shell volume over the box
+    volume.  With S_k = sum a^k, sum_{i<j} (a_i + a_j)^3 = n S3 + 3 S1 S2 - 4 S3."""
+    a = np.asarray(radii, dtype=np.float64)
+    n = a.size
+    s1, s2, s3 = a.sum(), (a ** 2).sum(), (a ** 3).sum()
+    pair_r3 = n * s3 + 3 * s1 * s2 - 4 * s3
+    return 4.0 * np.pi / 3.0 * ((1 + st_percent / 100.0) ** 3 - 1) * pair_r3 / volume
+
+
+def cpu_baseline_and_statistics(N, npart, state, steps):
+    """The FP64 oracle on the same workload: one untimed step (the droplet history
+    start), then `steps` timed steps; the FP32-mixed GPU path runs the same steps from
+    the same state.  Returns the CPU rate and the collision kernel / contact RDF of
+    both, summed over all the steps, with Poisson (counting) error bars."""
     from oracle import oracle as O
     p = workload_params(dd.FP64, N, npart)
     o = O.Oracle(p)
@@ -102,12 +162,50 @@ def cpu_baseline(N, npart, state, steps=2):
     d = state["drop"]
     o.set_droplets(d["yp"], d["fvp"], d["fdp"], d["pertvel"])
     o.set_new_particle(1)
-    o.step(1)
-    t0 = time.perf_counter()
-    o.step(steps)
-    dt = time.perf_counter() - t0
+    g = solver_from(state, dd.FP32_MIXED, N, npart)
+    tot = {"oracle": np.zeros(3), "gpu": np.zeros(3)}          # collisions, contacts, pairs within 10 R
+    secs = []
+    for k in range(1 + steps):
+        t0 = time.perf_counter()
+        o.step(1)
+        dt = time.perf_counter() - t0
+        if k > 0:
+            secs.append(dt)
+        g.step(1)
+        for name, x in (("oracle", o), ("gpu", g)):
+            c = x.collision_counts()
+            tot[name] += (sum(c.icc), sum(c.nir), c.pairs_in_dstat)
+        log(f"CPU baseline step {k + 1}/{1 + steps}: {dt:.1f} s")
+    threads = O.lib().orc_num_threads()
     o.close()
-    return steps / dt, dt
+    g.close()
+    nsteps = 1 + steps
+    dtime = p.dt
+    volume = (2 * np.pi) ** 3
+    expect = contact_pairs_uniform(state["drop"]["yp"][6], p.st, volume)
+
+    def stat(name, k):
+        v = float(tot[name][k])
+        return {"count": v, "sigma": v ** 0.5}
+
+    stats = {
+        "steps": nsteps,
+        "collision_kernel": {
+            "unit": "collisions per unit time per unit volume (DNS units)",
+            **{name: {"value": tot[name][0] / (nsteps * dtime * volume),
+                      "sigma": tot[name][0] ** 0.5 / (nsteps * dtime * volume),
+                      "collisions": int(tot[name][0])} for name in ("gpu", "oracle")},
+        },
+        "rdf_at_contact": {
+            "unit": "contact pairs / expected for uniform positions (R < d <= R(1+st%))",
+            **{name: {"value": tot[name][1] / (nsteps * expect),
+                      "sigma": tot[name][1] ** 0.5 / (nsteps * expect),
+                      "contacts": int(tot[name][1])} for name in ("gpu", "oracle")},
+        },
+        "pairs_within_10R": {name: stat(name, 2) for name in ("gpu", "oracle")},
+        "gpu": "fp32-mixed", "oracle": "fp64",
+    }
+    return 1.0 / statistics.median(secs), statistics.median(secs), threads, stats
 
 
 def main(argv=None):
@@ -118,7 +216,8 @@ def main(argv=None):
     ap.add_argument("--N", type=int, default=256)
     ap.add_argument("--npart", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--only-fp32", action="store_true",
                     help="time the FP32-mixed path only (for profiler runs)")
     a = ap.parse_args(argv)
@@ -136,62 +235,56 @@ def main(argv=None):
     cc = s32.collision_counts()
     steps_per_s = a.steps / el32
     kms = list(t32.flow_kernel_ms)
-    names = ["y_inv", "x_fused", "y_fwd", "pre", "z_fused", "sym"]
+    names = ["y_inv", "x_fused", "y_fwd", "pre", "z_stage", "sym"]
     per_launch = {n: kms[i] / a.steps for i, n in enumerate(names)}
     kb = kernel_bytes(N, dd.FP32_MIXED)
-    dom = max(("y_inv", "x_fused", "y_fwd", "z_fused"), key=lambda n: per_launch[n])
-    achieved = kb[dom] / (per_launch[dom] * 1e-3) / 1e9
-    flow_bytes = sum(kb.values())
     flow_ms = t32.flow_ms / a.steps
+    # the flow step alone (no droplet block beside it), after the timed run
+    alone = s32.profile_flow(5)
     copy_ms, copy_bytes = s32.profile_copy(20)
     s32.close()
+    dom = max(FFT_KERNELS, key=lambda n: per_launch[n])
+    achieved = kb[dom] / (per_launch[dom] * 1e-3) / 1e9
+    achieved_alone = kb[dom] / (alone[dom] * 1e-3) / 1e9
+    flow_kernels = {
+        n: {"bytes": kb[n], "ms_in_step": per_launch[n], "ms_alone": alone[n],
+            "GBps_in_step": kb[n] / (per_launch[n] * 1e-3) / 1e9, "GBps_alone": kb[n] / (alone[n] * 1e-3) / 1e9}
+        for n in FFT_KERNELS
+    }
+    phases = {"flow": flow_ms, "cells": t32.cells_ms / a.steps, "interp": t32.interp_ms / a.steps,
+              "hdi": t32.hdi_ms / a.steps, "advance": t32.advance_ms / a.steps,
+              "colldet": t32.colldet_ms / a.steps, "overlap": t32.overlap_ms / a.steps}
     if a.only_fp32:
         print(json.dumps({"metric": "fp32-mixed steps/s (profiling run)", "value": steps_per_s,
-                          "flow_kernel_ms_per_launch": per_launch, "phase_ms_per_step": {
-                              "flow": flow_ms, "cells": t32.cells_ms / a.steps, "interp": t32.interp_ms / a.steps,
-                              "hdi": t32.hdi_ms / a.steps, "advance": t32.advance_ms / a.steps,
-                              "colldet": t32.colldet_ms / a.steps, "overlap": t32.overlap_ms / a.steps}}))
+                          "ms_per_step": 1e3 * el32 / a.steps, "flow_kernels": flow_kernels,
+                          "phase_ms_per_step": phases, "gmres_iters_last_step": cc.gmres_iters}))
         return 0
 
     # ---- FP64 plain path, same initial state
     log(f"FP32-mixed {steps_per_s:.3f} steps/s; FP64 timed run")
-    s64 = dd.Solver(workload_params(dd.FP64, N, npart))
-    s64.set_flow(*init_state["flow"])
-    d = init_state["drop"]
-    s64.set_droplets(d["yp"], d["fvp"], d["fdp"], d["pertvel"])
+    s64 = solver_from(init_state, dd.FP64, N, npart)
     el64, t64, _ = timed_run(s64, a.steps, a.warmup)
     fp64_sps = a.steps / el64
     s64.close()
 
-    # ---- accuracy: 10 steps FP32-mixed vs FP64 from the same state
-    log(f"FP64 {fp64_sps:.3f} steps/s; accuracy run")
-    vel_err = None
-    try:
-        ra = dd.Solver(workload_params(dd.FP32_MIXED, N, npart))
-        rb = dd.Solver(workload_params(dd.FP64, N, npart))
-        for r in (ra, rb):
-            r.set_flow(*init_state["flow"])
-            r.set_droplets(d["yp"], d["fvp"], d["fdp"], d["pertvel"])
-            r.step(10)
-        va, _ = ra.get_flow()
-        vb, _ = rb.get_flow()
-        num = sum(float(np.sum((x - y) ** 2)) for x, y in zip(va, vb))
-        den = sum(float(np.sum(y ** 2)) for y in vb)
-        vel_err = (num / den) ** 0.5
-        ra.close()
-        rb.close()
-    except dd.DDNSError as e:
-        print(f"accuracy run failed: {e}", file=sys.stderr)
+    # ---- accuracy against the FP64 oracle after 10 steps
+    acc = {}
+    if not a.no_accuracy:
+        log(f"FP64 {fp64_sps:.3f} steps/s; accuracy run (10 steps, GPU and oracle)")
+        try:
+            acc = accuracy_vs_oracle(N, npart, init_state)
+        except Exception as e:  # reported, never required
+            print(f"accuracy run failed: {e}", file=sys.stderr)
 
-    # ---- CPU baseline (oracle, FP64) on a bounded sample
-    cpu = None
+    # ---- CPU baseline (oracle, FP64) on a bounded sample, with collision statistics
+    cpu, stats = None, None
     if not a.no_cpu:
         log("CPU baseline")
         try:
-            sps, secs = cpu_baseline(N, npart, init_state, a.cpu_steps)
-            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            sps, med, threads, stats = cpu_baseline_and_statistics(N, npart, init_state, a.cpu_steps)
             cpu = {"value": sps, "unit": "steps/s", "cores": threads, "kind": "port",
-                   "sample": f"{a.cpu_steps} full steps of the same workload after one untimed step ({secs:.1f} s), FP64 oracle"}
+                   "sample": f"median of {a.cpu_steps} full steps of the same workload ({med:.1f} s each) after one "
+                             f"untimed step, FP64 oracle, {threads} OpenMP threads"}
         except Exception as e:  # the baseline is reported, never required
             print(f"cpu baseline failed: {e}", file=sys.stderr)
 
@@ -211,22 +304,23 @@ def main(argv=None):
         "config": {"workload": WORKLOAD, "N": N, "droplets": npart, "parallelism": "single GPU"},
         "fp64_steps_per_s": fp64_sps,
         "speedup_fp32_mixed_vs_fp64": steps_per_s / fp64_sps,
-        "vel_rel_l2_10_steps_fp32_vs_fp64": vel_err,
+        **acc,
         "droplet_updates_per_s": steps_per_s * npart,
         "pairs_tested_per_s": cc.pairs_tested / (t32.colldet_ms / a.steps * 1e-3) if t32.colldet_ms > 0 else None,
         "gmres_iters_last_step": cc.gmres_iters,
         "flow_ms_per_step": flow_ms,
-        "flow_algorithmic_GBps": flow_bytes / (flow_ms * 1e-3) / 1e9,
-        "phase_ms_per_step": {
-            "flow": flow_ms, "cells": t32.cells_ms / a.steps, "interp": t32.interp_ms / a.steps,
-            "hdi": t32.hdi_ms / a.steps, "advance": t32.advance_ms / a.steps,
-            "colldet": t32.colldet_ms / a.steps, "overlap": t32.overlap_ms / a.steps,
-        },
-        "flow_kernel_ms_per_launch": per_launch,
+        "flow_ms_per_step_alone": alone["step"],
+        "flow_algorithmic_GBps": sum(kb.values()) / (flow_ms * 1e-3) / 1e9,
+        "phase_ms_per_step": phases,
+        "flow_kernels": flow_kernels,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, N, dd.FP32_MIXED),
-                     "algorithmic_bytes": kb[dom]},
+                     "algorithmic_bytes": kb[dom],
+                     "timing": "frac: mean launch time in the timed steps (HIP events on the flow stream, "
+                               "with the droplet block running beside it); frac_alone: the flow step alone",
+                     "achieved_alone": achieved_alone, "frac_alone": achieved_alone / HBM_PEAK_GBS},
         "hbm_copy_GBps": copy_bytes / (copy_ms * 1e-3) / 1e9,
+        "collision_statistics": stats,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -146,7 +146,8 @@ typedef struct ddns_timing {
     int    nsteps;
     int    pad;
     /* flow kernels, summed over the call: inverse y stage, fused x stage, forward y
-       stage, low-mode pre-kernel, fused z stage, kx = 0 symmetrisation */
+       stage, low-mode pre-kernel, z stage (see ddns_profile_flow), kx = 0
+       symmetrisation */
     double flow_kernel_ms[6];
 } ddns_timing;
 
@@ -286,14 +287,18 @@ int ddns_collision_detect(ddns_handle* h, int n, const double* yp, const double*
 
 /* Kernel timing helper (no counterpart in the reference): run the flow step alone k
  * times and return the mean time (ms) of, in order: inverse y stage, fused x stage,
- * forward y stage, low-mode pre-kernel, fused z stage, kx = 0 symmetrisation, whole
- * flow step.  nkernels <= 7 values are written. */
+ * forward y stage, low-mode pre-kernel, z stage (one fused kernel below N = 512; from
+ * N = 512 the three forward transforms, the update and the six inverse transforms),
+ * kx = 0 symmetrisation, whole flow step.  nkernels <= 7 values are written.
+ * The flow state (u-hat, history) advances by k steps; time, the step counter and the
+ * droplets do not. */
 int ddns_profile_flow(ddns_handle* h, int k, double* ms_per_kernel, int nkernels);
 
 /* HBM calibration helper (no counterpart in the reference): a plain streaming copy
  * between two of the handle's spectral work buffers, run reps times.  Returns the mean
  * time per launch (ms) and the bytes the copy moves per launch (read + write), so
- * that profiler byte counters can be scaled against a kernel of known traffic. */
+ * that profiler byte counters can be scaled against a kernel of known traffic.  The
+ * flow state is kept: the next step rebuilds its work buffers from u-hat. */
 int ddns_profile_copy(ddns_handle* h, int reps, double* ms_per_launch, double* bytes_per_launch);
 
 #ifdef __cplusplus

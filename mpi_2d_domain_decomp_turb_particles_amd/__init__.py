@@ -261,7 +261,7 @@ class Solver:
     def profile_flow(self, k: int = 5):
         ms = np.zeros(7)
         self._check(self.lib.ddns_profile_flow(self.h, k, _dp(ms), 7))
-        return dict(zip(("y_inv", "x_fused", "y_fwd", "pre", "z_fused", "sym", "step"), ms))
+        return dict(zip(("y_inv", "x_fused", "y_fwd", "pre", "z_stage", "sym", "step"), ms))
 
     def profile_copy(self, reps: int = 20):
         """Mean time (ms) and bytes moved per launch of a plain HBM copy kernel."""

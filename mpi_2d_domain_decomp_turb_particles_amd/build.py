@@ -36,6 +36,7 @@ SOURCES = {
     "particles.hip": [],
     "hdi.hip": [],
     "collide.hip": ["-ffp-contract=off"],
+    "spectrum.hip": [],
     "ddns_api.hip": [],
 }
 

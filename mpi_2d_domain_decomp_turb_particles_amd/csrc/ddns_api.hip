@@ -791,6 +791,21 @@ static int droplet_block(ddns_handle* h, float* ms)
     return status;
 }
 
+// A step that stops part-way leaves the inverse pipeline A partly transformed in place
+// (the inverse y stage overwrites its input) and may leave the next step's first half
+// queued.  Wait for both streams and have the next call rebuild A from u-hat.  The flow
+// state is then at the end of the failed step when its update had been queued, else at
+// its start; time and istep count the failed step either way.
+static int step_fail(ddns_handle* h, int st)
+{
+    (void)hipStreamSynchronize(h->dstream);
+    (void)hipStreamSynchronize(h->stream);
+    h->a_valid = false;
+    h->next_queued = false;
+    h->flow_rest_queued = false;
+    return keep(h, st);
+}
+
 extern "C" int ddns_step(ddns_handle* h, int nsteps)
 {
     if (!h || nsteps < 0) return DDNS_ERR_ARG;
@@ -806,7 +821,7 @@ extern "C" int ddns_step(ddns_handle* h, int nsteps)
         // inverse y, x stage (the first half may have been queued during the last step)
         if (!h->next_queued) {
             st = queue_flow_first(h, h->par);
-            if (st) return keep(h, st);
+            if (st) return step_fail(h, st);
         }
         h->next_queued = false;
         h->phys_valid = true;
@@ -817,12 +832,13 @@ extern "C" int ddns_step(ddns_handle* h, int nsteps)
         h->want_next = k + 1 < nsteps;
         if (h->p.npart > 0 && h->n > 0) {
             st = droplet_block(h, pms);
-            if (st < 0) return keep(h, st);
+            if (st < 0) return step_fail(h, st);
             if (st > 0 && status == 0) status = st;
-            if (h->n < h->p.npart / 2) return fail(h, DDNS_ERR_DROPLET_LOSS, "npart_t < npart/2 (turb.f90:600-604)");
+            if (h->n < h->p.npart / 2)
+                return step_fail(h, fail(h, DDNS_ERR_DROPLET_LOSS, "npart_t < npart/2 (turb.f90:600-604)"));
         }
         st = queue_flow_rest(h, nullptr);
-        if (st) return keep(h, st);
+        if (st) return step_fail(h, st);
         const int q = h->par;
         DDNS_HIP(hipEventSynchronize(h->flev[q][3]));
         {
@@ -953,5 +969,7 @@ extern "C" int ddns_profile_flow(ddns_handle* h, int k, double* ms_per_kernel, i
 extern "C" int ddns_profile_copy(ddns_handle* h, int reps, double* ms_per_launch, double* bytes_per_launch)
 {
     if (!h || reps <= 0 || !ms_per_launch || !bytes_per_launch) return DDNS_ERR_ARG;
+    // the copy overwrites the inverse pipeline, which the next step would start from
+    h->a_valid = false;
     return keep(h, flow_profile_copy(h->fb, h->N, h->prec, reps, ms_per_launch, bytes_per_launch, h->stream));
 }

@@ -218,7 +218,13 @@ struct FlowBuffers {
     double* sth_b = nullptr;                    // b1r..b3r (450)
     void* dg[8] = {};                           // diagnostics scratch, 8 fields, allocated on first use
     double* dg_part = nullptr;                  // diagnostics reduction partials
+    double* spec_part = nullptr;                // E(k): SPEC_BLOCKS partial rows of SPEC_NEK_MAX shells, then the result
 };
+
+// E(k) shell sums (spectrum.hip): work-groups of the partial-sum kernel and the most
+// shells any grid has (nek = 510 at N = 1024)
+constexpr int SPEC_BLOCKS = 256;
+constexpr int SPEC_NEK_MAX = 512;
 
 struct FlowConst {
     int N;

@@ -910,27 +910,6 @@ __global__ void k_sym(void* u0, void* u1, void* u2)
     }
 }
 
-// ------------------------------------------------------------------ energy spectrum
-template <typename T, int N>
-__global__ void k_spectrum(const void* u0, const void* u1, const void* u2, int nek, double* ek)
-{
-    using C = cpx<T>;
-    constexpr int M = N / 2;
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)N * N * M) return;
-    const int kx = (int)(t % M), ky = (int)((t / M) % N), kz = (int)(t / ((size_t)N * M));
-    const int kyw = wave_index(ky, N), kzw = wave_index(kz, N);
-    double k2 = (double)(kx * kx + kyw * kyw + kzw * kzw);
-    if (k2 == 0.0) k2 = 1.e-5;
-    const int ik2 = (int)(sqrt(k2) + 0.5);
-    if (ik2 < 1 || ik2 > nek) return;
-    const C a = ((const C*)u0)[t], b = ((const C*)u1)[t], c = ((const C*)u2)[t];
-    double e = (double)a.x * a.x + (double)a.y * a.y + (double)b.x * b.x + (double)b.y * b.y +
-               (double)c.x * c.x + (double)c.y * c.y;
-    if (kx == 0) e *= 0.5;
-    atomicAdd(ek + ik2 - 1, e);
-}
-
 // ------------------------------------------------------------------ initial field
 // The random start of initial_flow (flowlib.f90:53-144) on the device.  The reference
 // draws from Intel's IFPORT rand, which is not available; here every mode draws its own
@@ -1070,6 +1049,7 @@ int flow_alloc(FlowBuffers& fb, int N, int precision)
     DDNS_HIP(hipMemset(fb.sth_state, 0, 64 * sizeof(int)));
     DDNS_HIP(hipMalloc(&fb.sth_b, 450 * sizeof(double)));
     DDNS_HIP(hipMemset(fb.sth_b, 0, 450 * sizeof(double)));
+    DDNS_HIP(hipMalloc(&fb.spec_part, ((size_t)SPEC_BLOCKS + 1) * SPEC_NEK_MAX * sizeof(double)));
     return 0;
 }
 
@@ -1081,6 +1061,7 @@ void flow_free(FlowBuffers& fb)
     (void)hipFree(fb.low); (void)hipFree(fb.sth_state); (void)hipFree(fb.sth_b);
     for (int q = 0; q < 8; ++q) if (fb.dg[q]) (void)hipFree(fb.dg[q]);
     if (fb.dg_part) (void)hipFree(fb.dg_part);
+    if (fb.spec_part) (void)hipFree(fb.spec_part);
     fb = FlowBuffers{};
 }
 
@@ -1337,30 +1318,6 @@ int flow_download_phys(const FlowBuffers& fb, int N, int precision, double* ux, 
 {
     return precision == DDNS_FP64 ? download_phys_t<double>(fb.uphys, N, ux, uy, uz, s)
                                   : download_phys_t<float>(fb.uphys, N, ux, uy, uz, s);
-}
-
-template <typename T, int N>
-static int spectrum_impl(const FlowBuffers& fb, int nek, double* ek, hipStream_t s)
-{
-    double* d = nullptr;
-    DDNS_HIP(hipMalloc(&d, sizeof(double) * nek));
-    DDNS_HIP(hipMemsetAsync(d, 0, sizeof(double) * nek, s));
-    const size_t tot = field_elems(N);
-    hipLaunchKernelGGL((k_spectrum<T, N>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
-                       fb.U[0], fb.U[1], fb.U[2], nek, d);
-    DDNS_HIP(hipGetLastError());
-    DDNS_HIP(hipMemcpyAsync(ek, d, sizeof(double) * nek, hipMemcpyDeviceToHost, s));
-    DDNS_HIP(hipStreamSynchronize(s));
-    DDNS_HIP(hipFree(d));
-    return 0;
-}
-
-int flow_spectrum(const FlowBuffers& fb, int N, int nek, int precision, double* ek, hipStream_t s)
-{
-    int st = 0;
-    if (precision == DDNS_FP64) { DDNS_DISPATCH_N(N, st = (spectrum_impl<double, NN>(fb, nek, ek, s))) }
-    else { DDNS_DISPATCH_N(N, st = (spectrum_impl<float, NN>(fb, nek, ek, s))) }
-    return st;
 }
 
 // ---------------------------------------------------------------- initial field

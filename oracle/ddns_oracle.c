@@ -835,6 +835,16 @@ int orc_get_droplets(orc_sim* s, int* n, double* yp, double* fvp, double* fdp,
 }
 
 int orc_set_new_particle(orc_sim* s, int flag) { s->new_particle = flag; return 0; }
+
+/* OpenMP threads the oracle's parallel loops use (1 without OpenMP). */
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 int orc_set_istep(orc_sim* s, int istep) { s->istep = istep; return 0; }
 int orc_flow_only(orc_sim* s, int on) { s->flow_only = on; return 0; }
 

@@ -35,6 +35,7 @@ int  orc_set_droplets(orc_sim* s, int n, const double* yp, const double* fvp,
 int  orc_get_droplets(orc_sim* s, int* n, double* yp, double* fvp, double* fdp,
                       double* pertvel, double* vp, double* dmove);
 int  orc_set_new_particle(orc_sim* s, int flag);
+int  orc_num_threads(void);
 int  orc_set_istep(orc_sim* s, int istep);
 
 int  orc_step(orc_sim* s, int nsteps);

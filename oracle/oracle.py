@@ -46,6 +46,7 @@ def load() -> C.CDLL:
         "orc_set_droplets": (C.c_int, [_P, C.c_int, _D, _D, _D, _D]),
         "orc_get_droplets": (C.c_int, [_P, _I, _D, _D, _D, _D, _D, _D]),
         "orc_set_new_particle": (C.c_int, [_P, C.c_int]),
+        "orc_num_threads": (C.c_int, []),
         "orc_set_istep": (C.c_int, [_P, C.c_int]),
         "orc_step": (C.c_int, [_P, C.c_int]),
         "orc_flow_only": (C.c_int, [_P, C.c_int]),
@@ -171,6 +172,21 @@ class Oracle:
         iv = np.zeros(32, dtype=np.int32)
         self._chk(self.lib.orc_get_forcing_state(self.h, dp(b), C.byref(s), C.byref(y), ip(iv)))
         return b, s.value, y.value, iv
+
+    def supfor_sth(self, v=None):
+        """One supforSTH call (flowlib.f90:378-562) on the handle's forcing state.
+        Returns the forcing table a1r..a3r as an array (3, 6, 5, 5): component, kx
+        re/im entry, ky slot, kz slot (slots 1..5 hold k = 0, 1, 2, -2, -1).  When v
+        (a spectral triple in the reference layout) is given, dt * a is added to it in
+        place, as the reference adds it to the velocity."""
+        a = np.zeros(450)
+        if v is None:
+            self._chk(self.lib.orc_supfor_sth(self.h, None, None, None, dp(a)))
+        else:
+            for x in v:
+                assert x.flags.f_contiguous and x.dtype == np.float64
+            self._chk(self.lib.orc_supfor_sth(self.h, *[dp(x) for x in v], dp(a)))
+        return np.stack([a[150 * c:150 * (c + 1)].reshape((6, 5, 5), order="F") for c in range(3)])
 
     def set_droplets(self, yp, fvp=None, fdp=None, pertvel=None):
         yp = np.asfortranarray(yp, dtype=np.float64)

@@ -196,6 +196,28 @@ def test_droplet_steps_match_oracle(ddns, oracle_mod):
         assert rel_l2(gv, ov) < 1e-11
 
 
+@pytest.mark.parametrize("hdi", [0, 1])
+def test_nonlinear_drag_steps_match_oracle(ddns, oracle_mod, hdi):
+    """PART_ADVANCE with the Schiller-Naumann factor (ndrag = 1, partlib.f90:645-651):
+    three FP64 steps against the oracle; with HI the relative velocity includes the
+    disturbance velocity."""
+    N = 32
+    kw = dict(N=N, npart=3000, nset=2, radius=[20.0, 40.0], forcing=1, dt=0.002, rnu=0.02, ndrag=1,
+              hdi_included=hdi, hdi_trunc=10.0, nonoverlap=hdi)
+    v = random_spectral(N, seed=21, kmax=6, amp=3.0)
+    with ddns.Solver(_params(ddns, **kw)) as s, oracle_mod.Oracle(_params(ddns, **kw)) as o:
+        for x in (s, o):
+            x.set_flow(v, v)
+            x.init_droplets(seed=2024)
+        assert np.array_equal(s.get_droplets()["yp"], o.get_droplets()["yp"])
+        s.step(3)
+        o.step(3)
+        dg, do = s.get_droplets(), o.get_droplets()
+        tol = 1e-12 if hdi == 0 else 1e-9
+        assert np.allclose(dg["yp"], do["yp"], rtol=tol, atol=tol)
+        assert np.allclose(dg["fvp"], do["fvp"], rtol=tol, atol=tol)
+
+
 def test_overlap_relocation_matches_oracle(ddns, oracle_mod):
     """OVERLAP_DET sweeps that really relocate droplets (partlib.f90:2047-2167,
     RELOCATE 1984-2012): the relocation draws come from one ranff2 stream, so the
