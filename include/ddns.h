@@ -207,6 +207,17 @@ int ddns_get_physical_velocity(ddns_handle* h, double* ux, double* uy, double* u
 /* Energy spectrum E(1..nek) of the current u-hat (postprocessing.f90:151-170). */
 int ddns_get_spectrum(ddns_handle* h, double* ek);
 
+/* Allocation-free E(k) (no counterpart in the reference): the values of
+ * ddns_get_spectrum written to ek_dev, a device array of nek doubles, by kernels
+ * queued on stream (a hipStream_t; NULL means the handle's own stream, which orders
+ * the call after the last step).  It allocates nothing, copies nothing and does not
+ * wait, so it is for callers that cannot allocate device memory: code under HIP
+ * graph capture and loops that forbid hipMalloc.  Its work buffer is a fixed 1 MiB
+ * device array per GPU (512 shells x 256 partial rows), held until the process ends;
+ * a grid with more than 512 shells is refused with DDNS_ERR_ARG.  Calls that overlap
+ * on one GPU share that array and race. */
+int ddns_get_spectrum_device(ddns_handle* h, double* ek_dev, void* stream);
+
 /* Energy, dissipation and transfer spectra of the current u-hat, nek shells each (any
  * pointer may be NULL): the spectrum.dat pairs of POSTPROC (postprocessing.f90:151-170)
  * and TRANSPOSTPROC's transfer.dat (postprocessing.f90:275-315, with the dealiased

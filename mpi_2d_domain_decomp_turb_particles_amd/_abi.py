@@ -161,6 +161,7 @@ SIGNATURES = {
     "ddns_init_flow": (C.c_int, [_P, C.c_int, C.c_int, C.c_double, C.c_double]),
     "ddns_get_physical_velocity": (C.c_int, [_P, _D, _D, _D]),
     "ddns_get_spectrum": (C.c_int, [_P, _D]),
+    "ddns_get_spectrum_device": (C.c_int, [_P, _P, _P]),
     "ddns_get_spectra": (C.c_int, [_P, _D, _D, _D]),
     "ddns_get_monitors": (C.c_int, [_P, C.c_int, C.POINTER(Monitors)]),
     "ddns_set_droplets": (C.c_int, [_P, C.c_int, _D, _D, _D, _D]),

@@ -39,6 +39,7 @@ using namespace ddns;
 struct ddns_handle {
     ddns_params p;
     int N = 0, nek = 0, prec = 0;
+    int dev = 0;                 // the GPU the handle's buffers live on (current at ddns_create)
     hipStream_t stream = nullptr;
     // the droplet block of a step runs here, beside the second half of the flow step on
     // stream (the droplets read only the physical velocity of the first half)
@@ -304,6 +305,10 @@ extern "C" int ddns_create(const ddns_params* pin, ddns_handle** out)
     h->time = pin->time;
     h->new_particle = pin->new_particle;
     int st = 0;
+    if (hipGetDevice(&h->dev) != hipSuccess) { delete h; return fail(nullptr, DDNS_ERR_HIP, "hipGetDevice"); }
+    // the device address of the allocation-free spectrum's scratch, once per device
+    st = spectrum_scratch_init();
+    if (st) { delete h; return st; }
     if (hipStreamCreate(&h->stream) != hipSuccess) { delete h; return fail(nullptr, DDNS_ERR_HIP, "stream"); }
     if (hipStreamCreate(&h->dstream) != hipSuccess) { ddns_destroy(h); return fail(nullptr, DDNS_ERR_HIP, "stream"); }
     st = flow_alloc(h->fb, N, h->prec);
@@ -457,6 +462,13 @@ extern "C" int ddns_get_spectrum(ddns_handle* h, double* ek)
 {
     if (!h || !ek) return DDNS_ERR_ARG;
     return keep(h, flow_spectrum(h->fb, h->N, h->nek, h->prec, ek, h->stream));
+}
+
+extern "C" int ddns_get_spectrum_device(ddns_handle* h, double* ek_dev, void* stream)
+{
+    if (!h || !ek_dev) return DDNS_ERR_ARG;
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    return keep(h, flow_spectrum_device(h->fb, h->N, h->nek, h->prec, h->dev, ek_dev, s));
 }
 
 extern "C" int ddns_get_spectra(ddns_handle* h, double* ek, double* dk, double* tk)

@@ -255,6 +255,10 @@ int flow_init(FlowBuffers& fb, int N, int nek, int precision, int kind, int seed
 int flow_diagnostics(FlowBuffers& fb, int N, int nek, double rnu, int precision, int want, double* ek, double* dk,
                      double* tk, double* mom, hipStream_t s);
 int flow_spectrum(const FlowBuffers& fb, int N, int nek, int precision, double* ek, hipStream_t s);
+// allocation-free E(k) (spectrum.hip): the partial rows in a file-scope device array whose
+// address is resolved once per device by spectrum_scratch_init
+int spectrum_scratch_init();
+int flow_spectrum_device(const FlowBuffers& fb, int N, int nek, int precision, int dev, double* ek_dev, hipStream_t s);
 int fft_r2c_host(const FlowBuffers& fb, int N, int precision, double* v, hipStream_t s);
 int fft_c2r_host(const FlowBuffers& fb, int N, int precision, double* v, hipStream_t s);
 int flow_profile(const FlowBuffers& fb, const FlowConst& fc, int precision, int k, double* ms, int nk, hipStream_t s);

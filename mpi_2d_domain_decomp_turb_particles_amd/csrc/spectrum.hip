@@ -11,8 +11,11 @@
 // partial row per work-group, and a second kernel adds the SPEC_BLOCKS partial rows of
 // each shell in order.  Products and sums are in double in both precision modes.
 //
-// ddns_get_spectrum runs them with the partial rows in a buffer of the handle.
+// ddns_get_spectrum runs them with the partial rows in a buffer of the handle;
+// ddns_get_spectrum_device with the partial rows in a file-scope device array.
 #include "ddns_internal.h"
+
+#include <mutex>
 
 namespace ddns {
 
@@ -119,6 +122,68 @@ int flow_spectrum(const FlowBuffers& fb, int N, int nek, int precision, double* 
     DDNS_HIP(hipMemcpyAsync(ek, res, sizeof(double) * nek, hipMemcpyDeviceToHost, s));
     DDNS_HIP(hipStreamSynchronize(s));
     return 0;
+}
+
+// ---------------------------------------------------------------- allocation-free variant
+// The partial rows of ddns_get_spectrum_device live in this array: SPEC_BLOCKS rows of
+// SPEC_NEK_MAX shells, 1 MiB, enough for every grid a handle accepts (nek <= 510 at
+// N = 1024).  Each GPU holds its own copy from the first use until the process ends.
+// In host code the name is only a stand-in; kernels get the device address that
+// hipGetSymbolAddress returns for the current device (g_scratch below).  Every call on
+// one GPU shares the array, so calls that overlap on one GPU race.  Nothing needs
+// clearing between calls: the partial kernel's SPEC_BLOCKS work-groups write every
+// entry (shell s < nek, work-group b) that the final kernel then reads.
+constexpr int SCRATCH_DEVICES = 64;
+static __device__ __align__(256) double scratch[SPEC_BLOCKS * SPEC_NEK_MAX];
+static double* g_scratch[SCRATCH_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
+// Resolves the device address of this device's copy of scratch, once per device, with
+// the device current.  ddns_create calls it for the device the handle lives on.
+int spectrum_scratch_init()
+{
+    int dev = 0;
+    DDNS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= SCRATCH_DEVICES) {
+        set_error("spectrum scratch: device index beyond 63");
+        return DDNS_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[dev]) return 0;
+    void* p = nullptr;
+    DDNS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+        set_error("spectrum scratch: hipGetSymbolAddress gave a null address");
+        return DDNS_ERR_HIP;
+    }
+    g_scratch[dev] = (double*)p;
+    return 0;
+}
+
+// ddns_get_spectrum_device: the same two kernels as flow_spectrum on stream s, the
+// partial rows in scratch and the nek sums straight into the caller's device array
+// ek_dev.  No allocation, no copy, no wait.
+int flow_spectrum_device(const FlowBuffers& fb, int N, int nek, int precision, int dev, double* ek_dev, hipStream_t s)
+{
+    if (nek < 1 || nek > SPEC_NEK_MAX) {
+        set_error("ddns_get_spectrum_device: nek = " + std::to_string(nek) + " exceeds the static scratch of " +
+                  std::to_string(SPEC_NEK_MAX) + " shells");
+        return DDNS_ERR_ARG;
+    }
+    if (dev < 0 || dev >= SCRATCH_DEVICES) {
+        set_error("ddns_get_spectrum_device: device index beyond 63");
+        return DDNS_ERR_ARG;
+    }
+    double* part = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        part = g_scratch[dev];
+    }
+    if (!part) {
+        set_error("ddns_get_spectrum_device: the spectrum scratch of this device was never resolved");
+        return DDNS_ERR_HIP;
+    }
+    return launch_shells(fb, N, nek, precision, part, ek_dev, s);
 }
 
 }  // namespace ddns
