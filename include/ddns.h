@@ -254,7 +254,13 @@ int ddns_overlap_fix(ddns_handle* h);
  * droplet block turb.f90:546-623, with the reference's data flow (the droplets see the
  * physical velocity of the step's start; the flow does not see the droplets), the two
  * blocks overlapped on two streams.  Returns, with all work done, the first nonzero
- * status (a warning does not stop the loop). */
+ * status (a warning does not stop the loop).
+ * The two halves of the flow step are captured into HIP graphs by the first step and
+ * replayed from then on (DDNS_GRAPH=0 in the environment at ddns_create launches the
+ * kernels one by one; the results are the same bits).  Without droplets the call then
+ * waits for the device once, at its end, and ddns_timing holds the whole call's time as
+ * flow_ms with no per-kernel split; with droplets the droplet block waits each step on
+ * device results (GMRES convergence, buffer sizes, relocations, overlap sweeps). */
 int ddns_step(ddns_handle* h, int nsteps);
 
 /* Outcome of the last step's COLLISION_DET (and HI solve). */
@@ -277,6 +283,20 @@ int ddns_get_collision_pairs(ddns_handle* h, int64_t* triples, int cap, int* npa
  * the one-rank visiting order (ih1, cell loop, linked list).  *n gets the full count;
  * at most cap records are written (rec may be NULL). */
 int ddns_get_contact_records(ddns_handle* h, double* rec, int cap, int* n);
+
+/* Tile shapes chosen at ddns_create by timing the candidates (no counterpart in the
+ * reference; the role its FFTW_MEASURE plans play, mpifft.f90:36-56):
+ *   shapes[0] = kx columns per y-stage tile, shapes[1] = its default;
+ *   shapes[2] = threads of the fused z-stage work-group, shapes[3] = its default;
+ *   ms[0..2] = time of one inverse (6 fields) plus forward (3 fields) y stage at half,
+ *   once and twice the default width; ms[3..5] = time of one fused z stage at once
+ *   and twice the default threads (ms[5] is kept for a third size and is 0), 0 where a
+ *   shape does not fit the tile constraints or was not timed (the z stage where the
+ *   step runs it as separate kernels, from N = 512).
+ * A shape changes how work is grouped, never a column's or a mode's arithmetic, so
+ * results are the same bits.  Either pointer may be NULL; shapes takes 4 ints, ms 6
+ * doubles. */
+int ddns_get_tuning(ddns_handle* h, int* shapes, double* ms);
 
 /* HIP-event timings of the last ddns_step call. */
 int ddns_get_timing(ddns_handle* h, ddns_timing* out);

@@ -238,6 +238,16 @@ class Solver:
             self._check(self.lib.ddns_get_contact_records(self.h, _dp(r), n.value, C.byref(n)))
         return r.T
 
+    def tuning(self):
+        """Tile shapes chosen at set-up: dict with the y-stage tile width and its default,
+        the fused z stage's threads and their default, and the timings of the candidates
+        (ms; 0 where a candidate was not timed)."""
+        sh = np.zeros(4, dtype=np.int32)
+        ms = np.zeros(6)
+        self._check(self.lib.ddns_get_tuning(self.h, sh.ctypes.data_as(C.POINTER(C.c_int)), _dp(ms)))
+        return {"ky_tc": int(sh[0]), "ky_tc_default": int(sh[1]), "kz_nt": int(sh[2]),
+                "kz_nt_default": int(sh[3]), "ky_ms": ms[:3].copy(), "kz_ms": ms[3:].copy()}
+
     def timing(self) -> Timing:
         t = Timing()
         self._check(self.lib.ddns_get_timing(self.h, C.byref(t)))

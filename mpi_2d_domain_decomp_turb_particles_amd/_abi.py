@@ -174,6 +174,7 @@ SIGNATURES = {
     "ddns_get_contact_records": (C.c_int, [_P, _D, C.c_int, _I]),
     "ddns_get_collision_pairs": (C.c_int, [_P, _I64, C.c_int, _I]),
     "ddns_get_timing": (C.c_int, [_P, C.POINTER(Timing)]),
+    "ddns_get_tuning": (C.c_int, [_P, _I, _D]),
     "ddns_fft_r2c": (C.c_int, [_P, _D]),
     "ddns_fft_c2r": (C.c_int, [_P, _D]),
     "ddns_collision_detect": (C.c_int, [_P, C.c_int, _D, _D, _D, _D, _D]),

@@ -64,6 +64,11 @@ struct ddns_handle {
     // be queued before this step's times are read
     hipEvent_t flev[2][4] = {};
     hipEvent_t fev[2][8] = {};
+    // the two halves of the flow step as instantiated graphs, one per half and event set
+    // (step parity), captured by the first step that runs them; graph_mode 0 launches
+    // the kernels one by one (DDNS_GRAPH=0 in the environment at ddns_create)
+    hipGraphExec_t gflow[2][2] = {};
+    int graph_mode = 1;
     int par = 0;                 // parity of the step in progress
     bool want_next = false;      // a further step follows in this call
     bool next_queued = false;    // its first half is already on the flow stream
@@ -306,6 +311,7 @@ extern "C" int ddns_create(const ddns_params* pin, ddns_handle** out)
     h->new_particle = pin->new_particle;
     int st = 0;
     if (hipGetDevice(&h->dev) != hipSuccess) { delete h; return fail(nullptr, DDNS_ERR_HIP, "hipGetDevice"); }
+    if (const char* g = std::getenv("DDNS_GRAPH")) h->graph_mode = g[0] != '0';
     // the device address of the allocation-free spectrum's scratch, once per device
     st = spectrum_scratch_init();
     if (st) { delete h; return st; }
@@ -316,6 +322,8 @@ extern "C" int ddns_create(const ddns_params* pin, ddns_handle** out)
     h->fc.N = N; h->fc.nek = h->nek; h->fc.forcing = h->p.forcing; h->fc.dt = h->p.dt; h->fc.rnu = h->p.rnu;
     h->fc.force0 = h->p.force[0]; h->fc.force1 = h->p.force[1];
     h->fc.ou_tf = h->p.ou_tf; h->fc.ou_var = h->p.ou_var; h->fc.pi2 = pi2_ref();
+    st = flow_tune(h->fb, h->fc, h->prec, h->stream);      // tile shapes, as FFTW_MEASURE plans (mpifft.f90:36-56)
+    if (st) { ddns_destroy(h); return st; }
     // stochastic forcing starts from rest with iseedf = -iseedf (turb.f90:441-447)
     {
         int stt[34] = {0};
@@ -366,6 +374,7 @@ extern "C" int ddns_destroy(ddns_handle* h)
     for (int q = 0; q < 2; ++q) {
         for (int i = 0; i < 8; ++i) if (h->fev[q][i]) (void)hipEventDestroy(h->fev[q][i]);
         for (int i = 0; i < 4; ++i) if (h->flev[q][i]) (void)hipEventDestroy(h->flev[q][i]);
+        for (int a = 0; a < 2; ++a) if (h->gflow[a][q]) (void)hipGraphExecDestroy(h->gflow[a][q]);
     }
     if (h->dstream) (void)hipStreamDestroy(h->dstream);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -666,6 +675,46 @@ static double cell_width(const ddns_handle* h)
     return w;
 }
 
+// Half `rest` of the flow step (0: inverse stages, 1: forward stages and update) with
+// the timing events of parity q.  In graph mode the half is captured once into a graph
+// (its kernels and their events) and replayed with one launch: the flow step takes no
+// decision on the host and its buffers never move, so every step replays the same
+// graph.  A failed capture turns graph mode off for the handle and the kernels run one
+// by one.
+static int run_flow_half(ddns_handle* h, int rest, int q)
+{
+    hipStream_t s = h->stream;
+    if (h->graph_mode) {
+        hipGraphExec_t& gx = h->gflow[rest][q];
+        if (!gx) {
+            hipGraph_t g = nullptr;
+            int st = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess ? 0 : DDNS_ERR_HIP;
+            if (!st) {
+                st = flow_step(h->fb, h->fc, h->prec, rest != 0, s, h->fev[q], true);
+                const hipError_t e = hipStreamEndCapture(s, &g);
+                if (!st && e != hipSuccess) st = DDNS_ERR_HIP;
+            }
+            if (!st && hipGraphInstantiate(&gx, g, nullptr, nullptr, 0) != hipSuccess) {
+                gx = nullptr;
+                st = DDNS_ERR_HIP;
+            }
+            if (g) (void)hipGraphDestroy(g);
+            if (st) {
+                (void)hipGetLastError();
+                h->graph_mode = 0;
+                for (int a = 0; a < 2; ++a)
+                    for (int b = 0; b < 2; ++b)
+                        if (h->gflow[a][b]) { (void)hipGraphExecDestroy(h->gflow[a][b]); h->gflow[a][b] = nullptr; }
+            }
+        }
+        if (gx) {
+            DDNS_HIP(hipGraphLaunch(gx, s));
+            return 0;
+        }
+    }
+    return flow_step(h->fb, h->fc, h->prec, rest != 0, s, h->fev[q]);
+}
+
 // The first half of a step's flow (the curl and inverse z transforms when the state
 // is new, turb.f90:506-526; the inverse y and x stages and the physical velocity,
 // 527-539) on the flow stream, with the event set of parity q.
@@ -678,7 +727,7 @@ static int queue_flow_first(ddns_handle* h, int q)
         if (st) return st;
         h->a_valid = true;
     }
-    const int st = flow_step(h->fb, h->fc, h->prec, false, s, h->fev[q]);
+    const int st = run_flow_half(h, 0, q);
     if (st) return st;
     DDNS_HIP(hipEventRecord(h->flev[q][1], s));
     return 0;
@@ -697,7 +746,7 @@ static int queue_flow_rest(ddns_handle* h, hipEvent_t after)
     const int q = h->par;
     if (after) DDNS_HIP(hipStreamWaitEvent(s, after, 0));
     DDNS_HIP(hipEventRecord(h->flev[q][2], s));
-    int st = flow_step(h->fb, h->fc, h->prec, true, s, h->fev[q]);
+    int st = run_flow_half(h, 1, q);
     if (st) return st;
     DDNS_HIP(hipEventRecord(h->flev[q][3], s));
     h->flow_rest_queued = true;
@@ -828,6 +877,7 @@ extern "C" int ddns_step(ddns_handle* h, int nsteps)
     double kms[6] = {0, 0, 0, 0, 0, 0};
     DDNS_HIP(hipEventRecord(h->ev[15], s));
     h->next_queued = false;
+    const bool flow_only_graph = h->graph_mode && !(h->p.npart > 0 && h->n > 0);
     for (int k = 0; k < nsteps; ++k) {
         h->istep += 1;                                                        // turb.f90:492
         // inverse y, x stage (the first half may have been queued during the last step)
@@ -852,19 +902,21 @@ extern "C" int ddns_step(ddns_handle* h, int nsteps)
         st = queue_flow_rest(h, nullptr);
         if (st) return step_fail(h, st);
         const int q = h->par;
-        DDNS_HIP(hipEventSynchronize(h->flev[q][3]));
-        {
+        // A flow-only step replayed from graphs is not waited for: the call waits once, at
+        // its end, and reports its whole time as flow time, without the per-kernel split.
+        if (!flow_only_graph) {
+            DDNS_HIP(hipEventSynchronize(h->flev[q][3]));
             const int pairs[6][2] = {{0, 1}, {1, 2}, {3, 4}, {4, 5}, {5, 6}, {6, 7}};
             for (int i = 0; i < 6; ++i) {
                 float t;
                 DDNS_HIP(hipEventElapsedTime(&t, h->fev[q][pairs[i][0]], h->fev[q][pairs[i][1]]));
                 kms[i] += t;
             }
+            float t1, t2;
+            DDNS_HIP(hipEventElapsedTime(&t1, h->flev[q][0], h->flev[q][1]));
+            DDNS_HIP(hipEventElapsedTime(&t2, h->flev[q][2], h->flev[q][3]));
+            flow_ms += t1 + t2;
         }
-        float t1, t2;
-        DDNS_HIP(hipEventElapsedTime(&t1, h->flev[q][0], h->flev[q][1]));
-        DDNS_HIP(hipEventElapsedTime(&t2, h->flev[q][2], h->flev[q][3]));
-        flow_ms += t1 + t2;
         h->par = 1 - q;
         h->time += h->p.dt;                                                   // 693
     }
@@ -873,7 +925,7 @@ extern "C" int ddns_step(ddns_handle* h, int nsteps)
     float tot = 0;
     DDNS_HIP(hipEventElapsedTime(&tot, h->ev[15], h->ev[7]));
     h->timing = ddns_timing{};
-    h->timing.flow_ms = flow_ms;
+    h->timing.flow_ms = flow_only_graph ? tot : flow_ms;
     h->timing.cells_ms = pms[0];
     h->timing.interp_ms = pms[1];
     h->timing.hdi_ms = pms[2];
@@ -911,6 +963,23 @@ extern "C" int ddns_get_contact_records(ddns_handle* h, double* rec, int cap, in
 {
     if (!h || !n || cap < 0) return DDNS_ERR_ARG;
     return keep(h, part_read_contacts(h->pb, rec, cap, n, h->stream));
+}
+
+extern "C" int ddns_get_tuning(ddns_handle* h, int* shapes, double* ms)
+{
+    if (!h) return DDNS_ERR_ARG;
+    if (shapes) {
+        shapes[0] = h->fb.ky_tc;
+        shapes[1] = flow_default_ky_tc(h->N, h->prec);
+        shapes[2] = h->fb.kz_nt;
+        shapes[3] = flow_default_kz_nt(h->N, h->prec);
+    }
+    if (ms)
+        for (int k = 0; k < 3; ++k) {
+            ms[k] = h->fb.ky_ms[k];
+            ms[3 + k] = h->fb.kz_ms[k];
+        }
+    return DDNS_OK;
 }
 
 extern "C" int ddns_get_timing(ddns_handle* h, ddns_timing* out)

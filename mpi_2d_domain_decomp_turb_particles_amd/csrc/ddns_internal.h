@@ -219,6 +219,10 @@ struct FlowBuffers {
     void* dg[8] = {};                           // diagnostics scratch, 8 fields, allocated on first use
     double* dg_part = nullptr;                  // diagnostics reduction partials
     double* spec_part = nullptr;                // E(k): SPEC_BLOCKS partial rows of SPEC_NEK_MAX shells, then the result
+    int ky_tc = 0;                              // y-stage tile width chosen by flow_tune (0: the default)
+    float ky_ms[3] = {0, 0, 0};                 // the tuner's time per y stage pair of each candidate (0: not valid)
+    int kz_nt = 0;                              // threads of the fused z stage chosen by flow_tune (0: the default)
+    float kz_ms[3] = {0, 0, 0};                 // the tuner's time per fused z stage of each candidate (0: not valid)
 };
 
 // E(k) shell sums (spectrum.hip): work-groups of the partial-sum kernel and the most
@@ -240,8 +244,9 @@ struct FlowConst {
 int flow_alloc(FlowBuffers& fb, int N, int precision);
 void flow_free(FlowBuffers& fb);
 int flow_prologue(const FlowBuffers& fb, const FlowConst& fc, int precision, hipStream_t s);
+// capturing: the stream is being captured into a graph (the events become graph nodes)
 int flow_step(const FlowBuffers& fb, const FlowConst& fc, int precision, bool last, hipStream_t s,
-              hipEvent_t* ev /* 7 events or nullptr */);
+              hipEvent_t* ev /* 8 events or nullptr */, bool capturing = false);
 int flow_upload_state(const FlowBuffers& fb, int N, int precision, const double* const src[6], hipStream_t s);
 int flow_download_state(const FlowBuffers& fb, int N, int precision, double* const dst[6], hipStream_t s);
 int flow_download_phys(const FlowBuffers& fb, int N, int precision, double* ux, double* uy, double* uz, hipStream_t s);
@@ -262,6 +267,12 @@ int flow_spectrum_device(const FlowBuffers& fb, int N, int nek, int precision, i
 int fft_r2c_host(const FlowBuffers& fb, int N, int precision, double* v, hipStream_t s);
 int fft_c2r_host(const FlowBuffers& fb, int N, int precision, double* v, hipStream_t s);
 int flow_profile(const FlowBuffers& fb, const FlowConst& fc, int precision, int k, double* ms, int nk, hipStream_t s);
+// times the candidate y-stage tile widths and fused z-stage work-group sizes on fb's
+// buffers (before any state is loaded) and keeps the fastest in fb.ky_tc, fb.kz_nt
+// (cached per grid and precision); the defaults of a grid
+int flow_tune(FlowBuffers& fb, const FlowConst& fc, int precision, hipStream_t s);
+int flow_default_ky_tc(int N, int precision);
+int flow_default_kz_nt(int N, int precision);
 int flow_profile_copy(const FlowBuffers& fb, int N, int precision, int reps, double* ms, double* bytes, hipStream_t s);
 
 // ---------------------------------------------------------------- droplets

@@ -20,6 +20,8 @@
 #include <cstdlib>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 namespace ddns {
 
@@ -29,6 +31,18 @@ enum { KZ_FULL = 0, KZ_CURL_INV = 1, KZ_PLAIN_FWD = 2, KZ_PLAIN_INV = 3 };
 
 // tile widths (kx columns per work-group) for the y and z stages
 template <typename T, int N> constexpr int ky_tc() { return (sizeof(T) == 4) ? (N >= 1024 ? 8 : 16) : (N >= 1024 ? 4 : 8); }
+// The y stage runs at one of three tile widths, chosen per grid and precision by timing
+// (flow_tune): half, once and twice the default, where the tile splits evenly over the
+// work-group's 16-byte accesses and fits LDS with the twiddles.  The width changes only
+// which columns a work-group transforms together, never the arithmetic of a column, so
+// every choice gives the same bits.
+template <typename T, int N, int TC> constexpr bool ky_fits()
+{
+    constexpr int VW = sizeof(T) == 4 ? 2 : 1;
+    return TC >= VW && TC <= N / 2 && TC % VW == 0 && (N * (TC / VW)) % 256 == 0 &&
+           (N * (TC >= 8 ? TC + 1 : TC) + N) * (int)sizeof(cpx<T>) <= 160 * 1024;
+}
+template <typename T, int N> constexpr int ky_cand(int k) { return k == 0 ? ky_tc<T, N>() / 2 : (k == 1 ? ky_tc<T, N>() : 2 * ky_tc<T, N>()); }
 // z-stage tile widths for N <= 256, overridable at build time for tuning sweeps
 #ifndef DDNS_KZ_TC_F32
 #define DDNS_KZ_TC_F32 16
@@ -312,10 +326,23 @@ template <typename T, int N> constexpr int kz_nt()
     return (tot % DDNS_KZ_NT == 0 && (N * kz_tc<T, N>()) % DDNS_KZ_NT == 0) ? DDNS_KZ_NT : NT;
 }
 
-template <typename T, int N>
-__global__ void __launch_bounds__((kz_nt<T, N>())) k_z_full(KzArgs a, int ntiles)
+// The persistent z stage runs with kz_nt threads or twice as many, chosen per grid and
+// precision by timing (flow_tune), where the tile splits evenly over the threads.  More
+// threads keep more of a CU's loads in flight (a CU holds one work-group: its four
+// tiles fill LDS) but leave each thread half the registers; the elements a thread
+// updates and the columns it transforms change, their arithmetic does not.  (Four times
+// as many threads leave 128 registers per thread, and the kernel spills heavily there:
+// that size is not built.)
+template <typename T, int N, int NTZ> constexpr bool kz_nt_fits()
 {
-    constexpr int NTZ = kz_nt<T, N>();
+    constexpr int tot = N * kz_tc<T, N>() * (int)sizeof(cpx<T>) / 16;
+    return NTZ <= 512 && tot % NTZ == 0 && (N * kz_tc<T, N>()) % NTZ == 0;
+}
+template <typename T, int N> constexpr int kz_nt_cand(int k) { return kz_nt<T, N>() << k; }
+
+template <typename T, int N, int NTZ>
+__global__ void __launch_bounds__(NTZ) k_z_full(KzArgs a, int ntiles)
+{
     using C = cpx<T>;
     constexpr int M = N / 2;
     constexpr int TC = kz_tc<T, N>();
@@ -430,12 +457,11 @@ __global__ void __launch_bounds__((kz_nt<T, N>())) k_z_full(KzArgs a, int ntiles
 }
 
 // ------------------------------------------------------------------ y stage
-template <typename T, int N, bool INV, bool SYM>
+template <typename T, int N, bool INV, bool SYM, int TC>
 __global__ void __launch_bounds__(NT) k_y(KyArgs a)
 {
     using C = cpx<T>;
     constexpr int M = N / 2;
-    constexpr int TC = ky_tc<T, N>();
     constexpr int LDR = ldr_of<TC>();
     __shared__ C lds[N * LDR];
     __shared__ C tws[N];
@@ -1106,12 +1132,29 @@ static int launch_z(int mode, const FlowBuffers& fb, const FlowConst& fc, hipStr
     }
     if (mode == KZ_PLAIN_FWD || mode == KZ_PLAIN_INV) grid = dim3((N / 2 / kz_tc_plain<T, N>()) * N);
     if (mode == KZ_FULL) {
-        // persistent: one work-group per CU (the LDS tiles allow no more)
+        // persistent: one work-group per CU (the LDS tiles allow no more), of fb.kz_nt
+        // threads (0: the default)
         int dev = 0, ncu = 0;
         DDNS_HIP(hipGetDevice(&dev));
         DDNS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
         const int ntiles = (int)grid.x;
-        hipLaunchKernelGGL((k_z_full<T, N>), dim3(std::max(1, std::min(ncu, ntiles))), dim3(kz_nt<T, N>()), 0, s, a, ntiles);
+        const dim3 g(std::max(1, std::min(ncu, ntiles)));
+        constexpr int N0 = kz_nt_cand<T, N>(0), N1 = kz_nt_cand<T, N>(1), N2 = kz_nt_cand<T, N>(2);
+        if constexpr (kz_nt_fits<T, N, N1>()) {
+            if (fb.kz_nt == N1) {
+                hipLaunchKernelGGL((k_z_full<T, N, N1>), g, dim3(N1), 0, s, a, ntiles);
+                DDNS_HIP(hipGetLastError());
+                return 0;
+            }
+        }
+        if constexpr (kz_nt_fits<T, N, N2>()) {
+            if (fb.kz_nt == N2) {
+                hipLaunchKernelGGL((k_z_full<T, N, N2>), g, dim3(N2), 0, s, a, ntiles);
+                DDNS_HIP(hipGetLastError());
+                return 0;
+            }
+        }
+        hipLaunchKernelGGL((k_z_full<T, N, N0>), g, dim3(N0), 0, s, a, ntiles);
         DDNS_HIP(hipGetLastError());
         return 0;
     }
@@ -1124,6 +1167,29 @@ static int launch_z(int mode, const FlowBuffers& fb, const FlowConst& fc, hipStr
     return 0;
 }
 
+template <typename T, int N, int TC>
+static void launch_y_tc(bool inv, bool sym, const KyArgs& a, int nf, hipStream_t s)
+{
+    dim3 grid(N / 2 / TC, N, nf);
+    if (inv && sym) hipLaunchKernelGGL((k_y<T, N, true, true, TC>), grid, dim3(NT), 0, s, a);
+    else if (inv) hipLaunchKernelGGL((k_y<T, N, true, false, TC>), grid, dim3(NT), 0, s, a);
+    else hipLaunchKernelGGL((k_y<T, N, false, false, TC>), grid, dim3(NT), 0, s, a);
+}
+
+// candidate k of the y-stage tile widths (0 = half, 1 = default, 2 = twice the default)
+template <typename T, int N, int K>
+static bool launch_y_cand(bool inv, bool sym, const KyArgs& a, int nf, hipStream_t s)
+{
+    constexpr int TC = ky_cand<T, N>(K);
+    if constexpr (ky_fits<T, N, TC>()) {
+        launch_y_tc<T, N, TC>(inv, sym, a, nf, s);
+        return true;
+    } else {
+        return false;
+    }
+}
+
+// the y stage at the tile width fb.ky_tc (0: the default)
 template <typename T, int N>
 static int launch_y(bool inv, bool sym, void* const* f, int nf, const FlowBuffers& fb, hipStream_t s)
 {
@@ -1131,11 +1197,11 @@ static int launch_y(bool inv, bool sym, void* const* f, int nf, const FlowBuffer
     for (int i = 0; i < nf; ++i) a.F[i] = f[i];
     a.tw = fb.tw;
     a.scale = 1.0 / N;
-    constexpr int TC = ky_tc<T, N>();
-    dim3 grid(N / 2 / TC, N, nf);
-    if (inv && sym) hipLaunchKernelGGL((k_y<T, N, true, true>), grid, dim3(NT), 0, s, a);
-    else if (inv) hipLaunchKernelGGL((k_y<T, N, true, false>), grid, dim3(NT), 0, s, a);
-    else hipLaunchKernelGGL((k_y<T, N, false, false>), grid, dim3(NT), 0, s, a);
+    const int tc = fb.ky_tc;
+    bool done = false;
+    if (tc == ky_cand<T, N>(0)) done = launch_y_cand<T, N, 0>(inv, sym, a, nf, s);
+    else if (tc == ky_cand<T, N>(2)) done = launch_y_cand<T, N, 2>(inv, sym, a, nf, s);
+    if (!done) launch_y_tc<T, N, ky_tc<T, N>()>(inv, sym, a, nf, s);
     DDNS_HIP(hipGetLastError());
     return 0;
 }
@@ -1178,34 +1244,37 @@ static int launch_sym(const FlowBuffers& fb, hipStream_t s)
 }
 
 template <typename T, int N>
-static int step_impl(const FlowBuffers& fb, const FlowConst& fc, hipStream_t s, hipEvent_t* ev, int part)
+static int step_impl(const FlowBuffers& fb, const FlowConst& fc, hipStream_t s, hipEvent_t* ev, int part,
+                     unsigned evflags)
 {
     // part 0: inverse y + fused x (produces the physical velocity);
-    // part 1: forward y + pre + z + sym (closes the step, opens the next)
+    // part 1: forward y + pre + z + sym (closes the step, opens the next).
+    // evflags = hipEventRecordExternal while the stream is being captured into a graph:
+    // the timing events then become event-record nodes of the graph
     if (part == 0) {
-        if (ev) DDNS_HIP(hipEventRecord(ev[0], s));
+        if (ev) DDNS_HIP(hipEventRecordWithFlags(ev[0], s, evflags));
         int st = launch_y<T, N>(true, true, fb.A, 6, fb, s);
         if (st) return st;
-        if (ev) DDNS_HIP(hipEventRecord(ev[1], s));
+        if (ev) DDNS_HIP(hipEventRecordWithFlags(ev[1], s, evflags));
         st = launch_x<T, N>(fb, s);
         if (st) return st;
-        if (ev) DDNS_HIP(hipEventRecord(ev[2], s));
+        if (ev) DDNS_HIP(hipEventRecordWithFlags(ev[2], s, evflags));
         return 0;
     }
-    if (ev) DDNS_HIP(hipEventRecord(ev[3], s));
+    if (ev) DDNS_HIP(hipEventRecordWithFlags(ev[3], s, evflags));
     void* bb[3] = {fb.B[0], fb.B[1], fb.B[2]};
     int st = launch_y<T, N>(false, false, bb, 3, fb, s);
     if (st) return st;
-    if (ev) DDNS_HIP(hipEventRecord(ev[4], s));
+    if (ev) DDNS_HIP(hipEventRecordWithFlags(ev[4], s, evflags));
     st = launch_pre<T, N>(fb, fc, s);
     if (st) return st;
-    if (ev) DDNS_HIP(hipEventRecord(ev[5], s));
+    if (ev) DDNS_HIP(hipEventRecordWithFlags(ev[5], s, evflags));
     st = launch_z<T, N>(KZ_FULL, fb, fc, s);
     if (st) return st;
-    if (ev) DDNS_HIP(hipEventRecord(ev[6], s));
+    if (ev) DDNS_HIP(hipEventRecordWithFlags(ev[6], s, evflags));
     st = launch_sym<T, N>(fb, s);
     if (st) return st;
-    if (ev) DDNS_HIP(hipEventRecord(ev[7], s));
+    if (ev) DDNS_HIP(hipEventRecordWithFlags(ev[7], s, evflags));
     return 0;
 }
 
@@ -1228,13 +1297,15 @@ int flow_prologue(const FlowBuffers& fb, const FlowConst& fc, int precision, hip
     return st;
 }
 
-int flow_step(const FlowBuffers& fb, const FlowConst& fc, int precision, bool last, hipStream_t s, hipEvent_t* ev)
+int flow_step(const FlowBuffers& fb, const FlowConst& fc, int precision, bool last, hipStream_t s, hipEvent_t* ev,
+              bool capturing)
 {
     // 'last' selects the part: false = first half (inverse), true = second half (forward + update)
     const int part = last ? 1 : 0;
+    const unsigned evflags = capturing ? hipEventRecordExternal : hipEventRecordDefault;
     int st = 0;
-    if (precision == DDNS_FP64) { DDNS_DISPATCH_N(fc.N, st = (step_impl<double, NN>(fb, fc, s, ev, part))) }
-    else { DDNS_DISPATCH_N(fc.N, st = (step_impl<float, NN>(fb, fc, s, ev, part))) }
+    if (precision == DDNS_FP64) { DDNS_DISPATCH_N(fc.N, st = (step_impl<double, NN>(fb, fc, s, ev, part, evflags))) }
+    else { DDNS_DISPATCH_N(fc.N, st = (step_impl<float, NN>(fb, fc, s, ev, part, evflags))) }
     return st;
 }
 
@@ -1736,9 +1807,9 @@ static int profile_impl(const FlowBuffers& fb, const FlowConst& fc, int k, doubl
     for (int i = 0; i < 8; ++i) DDNS_HIP(hipEventCreate(&ev[i]));
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int it = 0; it < k; ++it) {
-        int st = step_impl<T, N>(fb, fc, s, ev, 0);
+        int st = step_impl<T, N>(fb, fc, s, ev, 0, hipEventRecordDefault);
         if (st) return st;
-        st = step_impl<T, N>(fb, fc, s, ev, 1);
+        st = step_impl<T, N>(fb, fc, s, ev, 1, hipEventRecordDefault);
         if (st) return st;
         DDNS_HIP(hipEventSynchronize(ev[7]));
         float t;
@@ -1746,7 +1817,7 @@ static int profile_impl(const FlowBuffers& fb, const FlowConst& fc, int k, doubl
         DDNS_HIP(hipEventElapsedTime(&t, ev[1], ev[2])); acc[1] += t;   // x fused
         DDNS_HIP(hipEventElapsedTime(&t, ev[3], ev[4])); acc[2] += t;   // y forward
         DDNS_HIP(hipEventElapsedTime(&t, ev[4], ev[5])); acc[3] += t;   // pre
-        DDNS_HIP(hipEventElapsedTime(&t, ev[5], ev[6])); acc[4] += t;   // z fused
+        DDNS_HIP(hipEventElapsedTime(&t, ev[5], ev[6])); acc[4] += t;   // z stage
         DDNS_HIP(hipEventElapsedTime(&t, ev[6], ev[7])); acc[5] += t;   // sym
         DDNS_HIP(hipEventElapsedTime(&t, ev[0], ev[7])); acc[6] += t;   // whole
     }
@@ -1784,6 +1855,192 @@ int flow_profile_copy(const FlowBuffers& fb, int N, int precision, int reps, dou
     *ms = (double)t / reps;
     *bytes = 2.0 * (double)n4 * sizeof(float4);
     return 0;
+}
+
+// ---------------------------------------------------------------- tile-shape tuner
+// The role the FFTW_MEASURE plans play in the reference (mpifft.f90:36-56): at set-up,
+// before any state is loaded, each candidate shape is timed on the handle's own buffers
+// (cleared first), one untimed run and then the mean of three, and the fastest is kept:
+//   - the y-stage tile width: the inverse stage over six fields and the forward stage
+//     over three, as a step runs them;
+//   - the fused z stage's work-group size (where the step uses the fused z stage).
+// The choices are cached per grid and precision for the life of the process.
+// DDNS_KY_TC / DDNS_KZ_NT in the environment force a shape (one that is not a candidate,
+// or does not fit, means the default) and skip the timing; DDNS_TUNE=0 keeps the defaults.
+// No choice changes the arithmetic of a column or of a mode, so every one gives the
+// same bits.
+template <typename T, int N>
+static bool ky_cand_fits(int k)
+{
+    switch (k) {
+    case 0: return ky_fits<T, N, ky_cand<T, N>(0)>();
+    case 1: return ky_fits<T, N, ky_cand<T, N>(1)>();
+    default: return ky_fits<T, N, ky_cand<T, N>(2)>();
+    }
+}
+
+template <typename T, int N>
+static bool kz_nt_cand_fits(int k)
+{
+    switch (k) {
+    case 0: return kz_nt_fits<T, N, kz_nt_cand<T, N>(0)>();
+    case 1: return kz_nt_fits<T, N, kz_nt_cand<T, N>(1)>();
+    default: return kz_nt_fits<T, N, kz_nt_cand<T, N>(2)>();
+    }
+}
+
+struct TuneRecord {
+    int ky_tc, kz_nt;
+    float ky_ms[3], kz_ms[3];
+};
+
+// the fused z stage is the one a step runs (not the separate kernels of the split path)
+template <typename T, int N>
+static bool z_fused_in_use()
+{
+    const char* env = std::getenv("DDNS_Z_SPLIT");
+    return env && (env[0] == '0' || env[0] == '1') ? env[0] == '0' : !kz_split_default<T, N>();
+}
+
+// times `reps` runs of launch after one untimed run (ms per run), 0 on failure
+template <typename F>
+static float time_runs(F&& launch, int reps, hipStream_t s, hipEvent_t e0, hipEvent_t e1, int* st)
+{
+    for (int rep = 0; rep <= reps && !*st; ++rep) {
+        if (rep == 1 && hipEventRecord(e0, s) != hipSuccess) *st = DDNS_ERR_HIP;
+        if (!*st) *st = launch();
+    }
+    float ms = 0.f;
+    if (!*st && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                 hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        *st = DDNS_ERR_HIP;
+    return *st ? 0.f : ms / (float)reps;
+}
+
+template <typename T, int N>
+static int tune_impl(FlowBuffers& fb, const FlowConst& fc, hipStream_t s)
+{
+    static std::mutex mu;
+    static std::map<int, TuneRecord> cache;          // key: N, per precision (one map per T)
+    fb.ky_tc = ky_tc<T, N>();
+    fb.kz_nt = kz_nt<T, N>();
+    for (int k = 0; k < 3; ++k) fb.ky_ms[k] = fb.kz_ms[k] = 0.f;
+    bool forced = false;
+    if (const char* f = std::getenv("DDNS_KY_TC")) {
+        const int want = std::atoi(f);
+        for (int k = 0; k < 3; ++k)
+            if (want == ky_cand<T, N>(k) && ky_cand_fits<T, N>(k)) fb.ky_tc = want;
+        forced = true;
+    }
+    if (const char* f = std::getenv("DDNS_KZ_NT")) {
+        const int want = std::atoi(f);
+        for (int k = 0; k < 3; ++k)
+            if (want == kz_nt_cand<T, N>(k) && kz_nt_cand_fits<T, N>(k)) fb.kz_nt = want;
+        forced = true;
+    }
+    if (forced) return 0;
+    if (const char* t = std::getenv("DDNS_TUNE"))
+        if (t[0] == '0') return 0;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = cache.find(N);
+        if (it != cache.end()) {
+            fb.ky_tc = it->second.ky_tc;
+            fb.kz_nt = it->second.kz_nt;
+            for (int k = 0; k < 3; ++k) {
+                fb.ky_ms[k] = it->second.ky_ms[k];
+                fb.kz_ms[k] = it->second.kz_ms[k];
+            }
+            return 0;
+        }
+    }
+    const size_t bytes = field_elems(N) * sizeof(cpx<T>);
+    for (int q = 0; q < 6; ++q) DDNS_HIP(hipMemsetAsync(fb.A[q], 0, bytes, s));
+    for (int q = 0; q < 3; ++q) DDNS_HIP(hipMemsetAsync(fb.B[q], 0, bytes, s));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    DDNS_HIP(hipEventCreate(&e0));
+    DDNS_HIP(hipEventCreate(&e1));
+    for (int q = 0; q < 3; ++q) {
+        DDNS_HIP(hipMemsetAsync(fb.U[q], 0, bytes, s));
+        DDNS_HIP(hipMemsetAsync(fb.O[q], 0, bytes, s));
+    }
+    DDNS_HIP(hipMemsetAsync(fb.low, 0, 1024 * sizeof(double), s));
+    float best = 0.f;
+    int best_tc = ky_tc<T, N>(), best_nt = kz_nt<T, N>();
+    int st = 0;
+    for (int k = 0; k < 3 && !st; ++k) {
+        if (!ky_cand_fits<T, N>(k)) continue;
+        fb.ky_tc = ky_cand<T, N>(k);
+        fb.ky_ms[k] = time_runs([&]() {
+            int r = launch_y<T, N>(true, true, fb.A, 6, fb, s);
+            return r ? r : launch_y<T, N>(false, false, fb.B, 3, fb, s);
+        }, 3, s, e0, e1, &st);
+        if (!st && (best == 0.f || fb.ky_ms[k] < best)) {
+            best = fb.ky_ms[k];
+            best_tc = fb.ky_tc;
+        }
+    }
+    if (z_fused_in_use<T, N>()) {
+        best = 0.f;
+        for (int k = 0; k < 3 && !st; ++k) {
+            if (!kz_nt_cand_fits<T, N>(k)) continue;
+            fb.kz_nt = kz_nt_cand<T, N>(k);
+            fb.kz_ms[k] = time_runs([&]() { return launch_z<T, N>(KZ_FULL, fb, fc, s); }, 3, s, e0, e1, &st);
+            if (!st && (best == 0.f || fb.kz_ms[k] < best)) {
+                best = fb.kz_ms[k];
+                best_nt = fb.kz_nt;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st) {
+        set_error("tile-width tuner: a HIP call failed");
+        return st;
+    }
+    // the state the timing runs wrote is cleared again: a new handle starts at rest
+    for (int q = 0; q < 3; ++q) {
+        DDNS_HIP(hipMemsetAsync(fb.U[q], 0, bytes, s));
+        DDNS_HIP(hipMemsetAsync(fb.O[q], 0, bytes, s));
+    }
+    DDNS_HIP(hipMemsetAsync(fb.low, 0, 1024 * sizeof(double), s));
+    DDNS_HIP(hipStreamSynchronize(s));
+    fb.ky_tc = best_tc;
+    fb.kz_nt = best_nt;
+    std::lock_guard<std::mutex> lock(mu);
+    TuneRecord r{};
+    r.ky_tc = best_tc;
+    r.kz_nt = best_nt;
+    for (int k = 0; k < 3; ++k) {
+        r.ky_ms[k] = fb.ky_ms[k];
+        r.kz_ms[k] = fb.kz_ms[k];
+    }
+    cache[N] = r;
+    return 0;
+}
+
+int flow_tune(FlowBuffers& fb, const FlowConst& fc, int precision, hipStream_t s)
+{
+    int st = 0;
+    if (precision == DDNS_FP64) { DDNS_DISPATCH_N(fc.N, st = (tune_impl<double, NN>(fb, fc, s))) }
+    else { DDNS_DISPATCH_N(fc.N, st = (tune_impl<float, NN>(fb, fc, s))) }
+    return st;
+}
+
+int flow_default_kz_nt(int N, int precision)
+{
+    int nt = 0;
+    if (precision == DDNS_FP64) { DDNS_DISPATCH_N(N, nt = (kz_nt<double, NN>())) }
+    else { DDNS_DISPATCH_N(N, nt = (kz_nt<float, NN>())) }
+    return nt;
+}
+
+int flow_default_ky_tc(int N, int precision)
+{
+    int tc = 0;
+    if (precision == DDNS_FP64) { DDNS_DISPATCH_N(N, tc = (ky_tc<double, NN>())) }
+    else { DDNS_DISPATCH_N(N, tc = (ky_tc<float, NN>())) }
+    return tc;
 }
 
 int flow_profile(const FlowBuffers& fb, const FlowConst& fc, int precision, int k, double* ms, int nk, hipStream_t s)

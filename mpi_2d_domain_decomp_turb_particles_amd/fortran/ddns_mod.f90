@@ -92,7 +92,7 @@ module ddns_mod
   public :: ddns_last_error_c, ddns_nek, ddns_set_flow, ddns_get_flow
   public :: ddns_set_forcing_state, ddns_get_forcing_state, ddns_init_flow
   public :: ddns_get_physical_velocity, ddns_get_spectrum, ddns_set_droplets, ddns_get_droplets
-  public :: ddns_get_spectra, ddns_get_monitors, ddns_get_spectrum_device
+  public :: ddns_get_spectra, ddns_get_monitors, ddns_get_spectrum_device, ddns_get_tuning
   public :: ddns_init_droplets, ddns_overlap_fix, ddns_step, ddns_get_collision_counts
   public :: ddns_get_collision_bins, ddns_get_collision_pairs, ddns_get_contact_records, ddns_get_timing
   public :: ddns_fft_r2c, ddns_fft_c2r, ddns_check
@@ -232,6 +232,12 @@ module ddns_mod
       real(c_double), intent(out) :: rec(*)
       integer(c_int), value :: cap
       integer(c_int), intent(out) :: n
+    end function
+    integer(c_int) function ddns_get_tuning(h, shapes, ms) bind(C, name="ddns_get_tuning")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), intent(out) :: shapes(*)
+      real(c_double), intent(out) :: ms(*)
     end function
     integer(c_int) function ddns_get_timing(h, t) bind(C, name="ddns_get_timing")
       import :: c_int, c_ptr, ddns_timing
