@@ -1,6 +1,8 @@
 """The Fortran host driver (parameter.d in, reference restart files out) on the GPU:
 10 steps in one run equal 5 steps, a restart through vel2D<idp>.0000 / part.0000
-(load_save.f90 formats), and 5 more steps."""
+(load_save.f90 formats), and 5 more steps; a restart from the per-rank files of a
+2 x 2 decomposition equals the restart from the one-domain files; the statistics and
+collision files in the reference's formats."""
 import subprocess
 from pathlib import Path
 
@@ -14,8 +16,10 @@ DRIVER = ROOT / "mpi_2d_domain_decomp_turb_particles_amd" / "turb_gpu"
 BASE = ROOT / "tests" / "golden" / "parameter.d_64_short"
 
 
-def _param(path, nstep, new_flow, idp, new_particle):
+def _param(path, nstep, new_flow, idp, new_particle, nprocY=None):
     lines = BASE.read_text().split("\n")
+    if nprocY is not None:
+        lines[2] = str(nprocY)
     lines[4] = str(nstep)
     lines[12] = ".TRUE." if new_flow else ".FALSE."
     lines[14] = str(idp)
@@ -36,8 +40,8 @@ def _read_records(path, count):
     return out
 
 
-def _run(cwd, param):
-    r = subprocess.run([str(DRIVER), str(param)], cwd=cwd, capture_output=True, text=True, timeout=300)
+def _run(cwd, param, *args):
+    r = subprocess.run([str(DRIVER), str(param), *args], cwd=cwd, capture_output=True, text=True, timeout=300)
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0 and "Successfully completed" in r.stdout
 
@@ -142,3 +146,77 @@ def test_collision_statistics_files(tmp_path):
             wr = ((vp[:, 1:4] - vp[:, 4:7]) * vp[:, 7:10]).sum(1)
             assert np.abs(wr - vp[:, 10]).max() < 5e-5 * (1 + np.abs(vp[:, 1:7]).max())
     assert det[:, 1:].sum() > 0
+
+
+def _write_records(path, recs):
+    """Fortran unformatted sequential records (4-byte length markers)."""
+    with open(path, "wb") as f:
+        for r in recs:
+            m = np.array([len(r)], dtype=np.int32).tobytes()
+            f.write(m)
+            f.write(r)
+            f.write(m)
+
+
+def split_velocity_file(src, dst_dir, idump, N, npy, npz):
+    """The per-rank files vel2D<idump>.<rank> that an nprocY x nprocZ run of the
+    reference writes (load_save.f90:2-55), made from a one-domain file src (six records
+    v(N+2, N+2, N)): rank id = iy + nprocY iz holds v(:, iy ly + 1 : iy ly + ly + lyext,
+    iz lz + 1 : iz lz + lz), ly = N / nprocY, lz = N / nprocZ, with the two extra rows
+    (lyext = 2) on the last rank along y (turb.f90:282-295)."""
+    recs = _read_records(src, 6)
+    L = N + 2
+    ly, lz = N // npy, N // npz
+    for rank in range(npy * npz):
+        iy, iz = rank % npy, rank // npy
+        lyx = 2 if iy == npy - 1 else 0
+        out = []
+        for r in recs:
+            v = np.frombuffer(r, dtype=np.float64).reshape((L, L, N), order="F")
+            blk = v[:, iy * ly: iy * ly + ly + lyx, iz * lz: iz * lz + lz]
+            out.append(blk.tobytes(order="F"))
+        _write_records(dst_dir / f"vel2D{idump}.{rank:04d}", out)
+
+
+def split_droplet_file(src, dst_dir, nranks):
+    """part.<rank> files (load_save.f90:99-124: the count, then yp(8, n), fvp(3, 0:2, n)
+    and fdp(3, 0:3, n) in one record) holding consecutive slices of the droplets of
+    the one-domain file src, so that reading them in rank order gives the same list."""
+    head, body = _read_records(src, 2)
+    n = int(np.frombuffer(head, dtype=np.int32)[0])
+    x = np.frombuffer(body, dtype=np.float64)
+    yp = x[: 8 * n].reshape((8, n), order="F")
+    fvp = x[8 * n: 17 * n].reshape((9, n), order="F")
+    fdp = x[17 * n: 29 * n].reshape((12, n), order="F")
+    cuts = np.linspace(0, n, nranks + 1).astype(int)
+    for rank in range(nranks):
+        a, b = cuts[rank], cuts[rank + 1]
+        rec = b"".join(np.asfortranarray(arr[:, a:b]).tobytes(order="F") for arr in (yp, fvp, fdp))
+        _write_records(dst_dir / f"part.{rank:04d}", [np.array([b - a], dtype=np.int32).tobytes(), rec])
+
+
+@pytest.mark.skipif(not DRIVER.exists(), reason="Fortran driver not built")
+def test_restart_from_four_rank_files(tmp_path):
+    """A restart from the files of a 2 x 2 decomposition (nprocY = nprocZ = 2: four
+    vel2D1.<rank> and part.<rank> files) resumes to exactly the state of the restart
+    from the one-domain files they were cut from."""
+    import shutil
+    a, b, c = tmp_path / "a", tmp_path / "b", tmp_path / "c"
+    for d in (a, b, c):
+        d.mkdir()
+    _param(a / "p5", 5, True, 0, True)
+    _run(a, a / "p5")
+    shutil.copy(a / "vel2D1.0000", b / "vel2D1.0000")
+    shutil.copy(a / "part.0000", b / "part.0000")
+    _param(b / "p5r", 5, False, 1, False)
+    _run(b, b / "p5r")
+    N = 64
+    split_velocity_file(a / "vel2D1.0000", c, 1, N, 2, 2)
+    split_droplet_file(a / "part.0000", c, 4)
+    assert sorted(p.name for p in c.iterdir()) == [f"part.{r:04d}" for r in range(4)] + \
+        [f"vel2D1.{r:04d}" for r in range(4)]
+    _param(c / "p5r", 5, False, 1, False, nprocY=2)
+    _run(c, c / "p5r", "fp64", "4")
+    for name, count in (("vel2D2.0000", 6), ("part.0000", 2)):
+        rb, rc = _read_records(b / name, count), _read_records(c / name, count)
+        assert all(x == y for x, y in zip(rb, rc)), name
