@@ -208,6 +208,39 @@ def cpu_baseline_and_statistics(N, npart, state, steps):
     return 1.0 / statistics.median(secs), statistics.median(secs), threads, stats
 
 
+def coalescence_1024(steps, warmup):
+    """The 1024^3 configuration: FP32-mixed, stochastic forcing, 5,000,000 droplets of
+    15 um growing by coalescence from the first step, HI to 50 radii, collisions.  nu for
+    k_max eta ~ 1.5 (eta ~ 4.8e-3, tau_K ~ 0.028 at u' ~ 1), dt at CFL ~ 0.1.  Reported
+    beside the headline, not as it: steps/s and the flow kernels' HBM rates."""
+    p = dd.default_params(N=1024, precision=dd.FP32_MIXED, dt=0.00025, rnu=0.0008, forcing=2,
+                          etk_DNS=0.00476, tauk_DNS=0.0283, npart=5_000_000, nset=1, radius=[15.0],
+                          hdi_included=1, hdi_trunc=50.0, nonoverlap=1, nstep_grow=1)
+    log("set-up 1024^3, 5M droplets")
+    with dd.Solver(p) as s:
+        t0 = time.perf_counter()
+        s.init_flow(dd.SPEC_K4_GAUSS, 123, 4.0, 1.0)
+        s.init_droplets(seed=290710)
+        setup = time.perf_counter() - t0
+        el, t, _ = timed_run(s, steps, warmup)
+        n = s.get_droplets()["yp"].shape[1]
+        alone = s.profile_flow(2)
+        tune = s.tuning()
+    kb = kernel_bytes(1024, dd.FP32_MIXED)
+    kms = dict(zip(["y_inv", "x_fused", "y_fwd", "pre", "z_stage", "sym"], t.flow_kernel_ms))
+    return {
+        "workload": "1024^3 DNS (stochastic forcing) + 5M droplets of 15 um, coalescence from step 1, HI 50 radii",
+        "steps": steps, "warmup": warmup, "steps_per_s": steps / el, "ms_per_step": 1e3 * el / steps,
+        "setup_s": setup, "droplets_after": n,
+        "phase_ms_per_step": {"flow": t.flow_ms / steps, "cells": t.cells_ms / steps, "interp": t.interp_ms / steps,
+                              "hdi": t.hdi_ms / steps, "advance": t.advance_ms / steps,
+                              "colldet": t.colldet_ms / steps, "overlap": t.overlap_ms / steps},
+        "flow_kernels_GBps_in_step": {k: kb[k] / (kms[k] / steps * 1e-3) / 1e9 for k in FFT_KERNELS if kms[k] > 0},
+        "flow_kernels_GBps_alone": {k: kb[k] / (alone[k] * 1e-3) / 1e9 for k in FFT_KERNELS if alone[k] > 0},
+        "tuning": {k: (v.tolist() if hasattr(v, "tolist") else v) for k, v in tune.items()},
+    }
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,10 +253,15 @@ def main(argv=None):
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--only-fp32", action="store_true",
                     help="time the FP32-mixed path only (for profiler runs)")
+    ap.add_argument("--workload", choices=["headline", "coalescence-1024"], default="headline",
+                    help="coalescence-1024: the 1024^3 + 5M-droplet configuration instead (one JSON line of its own)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU: the whole periodic domain lives on a single MI355X; --gpus {a.gpus} is not supported", file=sys.stderr)
         return 2
+    if a.workload == "coalescence-1024":
+        print(json.dumps(coalescence_1024(a.steps, a.warmup)))
+        return 0
     N, npart = a.N, a.npart
 
     # ---- FP32-mixed (the value)
@@ -241,6 +279,7 @@ def main(argv=None):
     flow_ms = t32.flow_ms / a.steps
     # the flow step alone (no droplet block beside it), after the timed run
     alone = s32.profile_flow(5)
+    tune = {k: (v.tolist() if hasattr(v, "tolist") else v) for k, v in s32.tuning().items()}
     copy_ms, copy_bytes = s32.profile_copy(20)
     s32.close()
     dom = max(FFT_KERNELS, key=lambda n: per_launch[n])
@@ -320,6 +359,7 @@ def main(argv=None):
                                "with the droplet block running beside it); frac_alone: the flow step alone",
                      "achieved_alone": achieved_alone, "frac_alone": achieved_alone / HBM_PEAK_GBS},
         "hbm_copy_GBps": copy_bytes / (copy_ms * 1e-3) / 1e9,
+        "tuned_shapes": tune,
         "collision_statistics": stats,
         "cpu_baseline": cpu,
     }
