@@ -605,7 +605,43 @@ __global__ void __launch_bounds__(256) k_zero_spans(ZeroSpans z)
         for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < z.words[k]; i += stride) z.p[k][i] = 0u;
 }
 
+// The collision search is queued by part_collide_search (no host wait), so the caller
+// can queue other streams' work while it runs; part_collide_finish then waits for the
+// search's counts, grows the candidate buffer and searches again when it overflowed,
+// sizes the contact records and queues the pair kernel.
+int part_collide_search(PartBuffers& pb, const PartConst& pc, hipStream_t s);
+int part_collide_finish(PartBuffers& pb, const PartConst& pc, hipStream_t s);
+
 int part_collide(PartBuffers& pb, const PartConst& pc, hipStream_t s)
+{
+    const int st = part_collide_search(pb, pc, s);
+    return st ? st : part_collide_finish(pb, pc, s);
+}
+
+static CollArgs collide_args(PartBuffers& pb, const PartConst& pc)
+{
+    CollArgs a{};
+    a.pos = pb.pos; a.vel = pb.vel; a.rad = pb.rad; a.gid = pb.gid; a.fvp0 = pb.fvp; a.dmove = pb.dmove; a.vp = pb.vp;
+    a.cell_start = pb.cell_start; a.sorted = pb.sorted; a.pk = pb.pk;
+    a.n = pc.n; a.cap = pb.cap; a.nset = pc.nset;
+    for (int i = 0; i < DDNS_MAXSET; ++i) a.radius[i] = pc.radius[i];
+    a.dt = pc.dt; a.st = pc.st; a.hdi = pc.hdi;
+    const double dstat = 20.0 * pc.radmax;                                 // 1264
+    a.g = make_geom(pb, pc, dstat, pc.hdi ? pc.d_hdi : pc.d_coll, pc.nx_coll);
+    a.nirbin = pb.nirbin; a.bins = pb.bins; a.counts = pb.counts; a.tested = pb.tested;
+    a.pairs = pb.pairs; a.pair_cap = pb.pair_cap; a.mult = pb.mult;
+    a.dmax = (unsigned long long*)(pb.flags + 32);
+    a.hit = pb.hit;
+    a.ncand = pb.flags + 40;
+    a.ncontact = pb.flags + 41;
+    a.cand = (int2*)pb.cand;
+    a.cand_cap = pb.cand_cap;
+    a.contact = pb.contact;
+    a.contact_cap = pb.contact_cap;
+    return a;
+}
+
+int part_collide_search(PartBuffers& pb, const PartConst& pc, hipStream_t s)
 {
     const size_t nb = (size_t)pc.nset * pc.nset * DDNS_NBIN;
     if (!pb.hit) DDNS_HIP(hipMalloc(&pb.hit, (size_t)pb.cap * sizeof(int)));
@@ -621,47 +657,44 @@ int part_collide(PartBuffers& pb, const PartConst& pc, hipStream_t s)
         add(pb.mult, (size_t)(pc.n + 1) * sizeof(int));
         add(pb.hit, (size_t)pc.n * sizeof(int));
         add(pb.flags + 32, 10 * sizeof(int));
+        add(pb.flags + 48, sizeof(int));               // relocation-draw overflow of this step
         hipLaunchKernelGGL(k_zero_spans, dim3(512), dim3(256), 0, s, z);
         DDNS_HIP(hipGetLastError());
     }
     if (pc.n == 0) return 0;
-    CollArgs a{};
-    a.pos = pb.pos; a.vel = pb.vel; a.rad = pb.rad; a.gid = pb.gid; a.fvp0 = pb.fvp; a.dmove = pb.dmove; a.vp = pb.vp;
-    a.cell_start = pb.cell_start; a.sorted = pb.sorted; a.pk = pb.pk;
-    a.n = pc.n; a.cap = pb.cap; a.nset = pc.nset;
-    for (int i = 0; i < DDNS_MAXSET; ++i) a.radius[i] = pc.radius[i];
-    a.dt = pc.dt; a.st = pc.st; a.hdi = pc.hdi;
-    const double dstat = 20.0 * pc.radmax;                                 // 1264
-    a.g = make_geom(pb, pc, dstat, pc.hdi ? pc.d_hdi : pc.d_coll, pc.nx_coll);
-    a.nirbin = pb.nirbin; a.bins = pb.bins; a.counts = pb.counts; a.tested = pb.tested;
-    a.pairs = pb.pairs; a.pair_cap = pb.pair_cap; a.mult = pb.mult;
-    unsigned long long* dmax = (unsigned long long*)(pb.flags + 32);
-    hipLaunchKernelGGL(k_extent_max, dim3(std::min(256, (pc.n + 255) / 256)), dim3(256), 0, s, (const double*)pb.dmove,
-                       (const double*)pb.rad, pc.n, pb.cap, dmax);
-    a.dmax = dmax;
     if (!pb.cand) {
         pb.cand_cap = std::max(1 << 16, 2 * pb.cap);
         DDNS_HIP(hipMalloc(&pb.cand, (size_t)pb.cand_cap * sizeof(int2)));
     }
-    a.hit = pb.hit;
-    a.ncand = pb.flags + 40;
-    a.ncontact = pb.flags + 41;
-    int found[2] = {0, 0};       // handed-on pairs, pairs in contact
+    if (!pb.found_host) DDNS_HIP(hipHostMalloc((void**)&pb.found_host, 4 * sizeof(int), hipHostMallocDefault));
+    CollArgs a = collide_args(pb, pc);
+    hipLaunchKernelGGL(k_extent_max, dim3(std::min(256, (pc.n + 255) / 256)), dim3(256), 0, s, (const double*)pb.dmove,
+                       (const double*)pb.rad, pc.n, pb.cap, (unsigned long long*)(pb.flags + 32));
+    hipLaunchKernelGGL(k_collide, dim3((pc.n + CT - 1) / CT), dim3(CT), 0, s, a);
+    DDNS_HIP(hipGetLastError());
+    // handed-on pairs and pairs in contact, read by part_collide_finish
+    DDNS_HIP(hipMemcpyAsync(pb.found_host, a.ncand, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    return 0;
+}
+
+int part_collide_finish(PartBuffers& pb, const PartConst& pc, hipStream_t s)
+{
+    if (pc.n == 0) return 0;
+    DDNS_HIP(hipStreamSynchronize(s));
+    int found[2] = {pb.found_host[0], pb.found_host[1]};
     int& ncand = found[0];
-    for (bool again = false;; again = true) {
-        a.cand = (int2*)pb.cand;
-        a.cand_cap = pb.cand_cap;
-        if (again) DDNS_HIP(hipMemsetAsync(a.ncand, 0, 2 * sizeof(int), s));
-        hipLaunchKernelGGL(k_collide, dim3((pc.n + CT - 1) / CT), dim3(CT), 0, s, a);
-        DDNS_HIP(hipGetLastError());
-        DDNS_HIP(hipMemcpyAsync(found, a.ncand, sizeof(found), hipMemcpyDeviceToHost, s));
-        DDNS_HIP(hipStreamSynchronize(s));
-        if (ncand <= pb.cand_cap) break;
+    while (ncand > pb.cand_cap) {
         // more pairs to hand on than the buffer holds: widen it and search again
         DDNS_HIP(hipFree(pb.cand));
         pb.cand_cap = ncand + ncand / 2;
         DDNS_HIP(hipMalloc(&pb.cand, (size_t)pb.cand_cap * sizeof(int2)));
+        CollArgs a = collide_args(pb, pc);
+        DDNS_HIP(hipMemsetAsync(a.ncand, 0, 2 * sizeof(int), s));
         DDNS_HIP(hipMemsetAsync(pb.tested, 0, sizeof(long long), s));
+        hipLaunchKernelGGL(k_collide, dim3((pc.n + CT - 1) / CT), dim3(CT), 0, s, a);
+        DDNS_HIP(hipGetLastError());
+        DDNS_HIP(hipMemcpyAsync(found, a.ncand, sizeof(found), hipMemcpyDeviceToHost, s));
+        DDNS_HIP(hipStreamSynchronize(s));
     }
     // the contact records of the pair kernel: the search has counted them
     if (found[1] > pb.contact_cap) {
@@ -669,8 +702,7 @@ int part_collide(PartBuffers& pb, const PartConst& pc, hipStream_t s)
         pb.contact_cap = std::max(4096, found[1] + found[1] / 2);
         DDNS_HIP(hipMalloc(&pb.contact, (size_t)pb.contact_cap * DDNS_CONTACT_REC * sizeof(double)));
     }
-    a.contact = pb.contact;
-    a.contact_cap = pb.contact_cap;
+    CollArgs a = collide_args(pb, pc);
     if (ncand > 0)
         hipLaunchKernelGGL(k_collide_pairs, dim3((ncand + 255) / 256), dim3(256), 0, s, a, ncand);
     DDNS_HIP(hipGetLastError());
@@ -975,6 +1007,38 @@ int part_read_counts(const PartBuffers& pb, const PartConst& pc, ddns_collision_
     cc->pairs_tested = t[0];
     cc->pairs_in_dstat = t[1];
     (void)ncoll;
+    return 0;
+}
+
+// The counts of part_read_counts through pinned memory: queued on the stream, read once
+// the caller has waited for it (no extra round trip).  A relocation that ran out of draws
+// (flags[48]) is reported here as a capacity error.
+int part_queue_counts(PartBuffers& pb, hipStream_t s)
+{
+    if (!pb.counts_host)
+        DDNS_HIP(hipHostMalloc((void**)&pb.counts_host, (512 + 8 + 4) * sizeof(int), hipHostMallocDefault));
+    DDNS_HIP(hipMemcpyAsync(pb.counts_host, pb.counts, 512 * sizeof(int), hipMemcpyDeviceToHost, s));
+    DDNS_HIP(hipMemcpyAsync(pb.counts_host + 512, pb.tested, 4 * sizeof(long long), hipMemcpyDeviceToHost, s));
+    DDNS_HIP(hipMemcpyAsync(pb.counts_host + 520, pb.flags + 48, sizeof(int), hipMemcpyDeviceToHost, s));
+    return 0;
+}
+
+int part_take_counts(const PartBuffers& pb, const PartConst& pc, ddns_collision_counts* cc)
+{
+    const int* h = pb.counts_host;
+    long long t[4];
+    std::memcpy(t, h + 512, sizeof(t));
+    cc->icc[0] = h[0]; cc->icc[1] = h[1]; cc->icc[2] = h[2];
+    for (int i = 0; i < DDNS_MAXSET * DDNS_MAXSET; ++i) { cc->colltype[i] = h[8 + i]; cc->nir[i] = h[108 + i]; }
+    cc->npairs = h[256];
+    cc->ncoll = h[256] + h[257];
+    cc->nset = pc.nset;
+    cc->pairs_tested = t[0];
+    cc->pairs_in_dstat = t[1];
+    if (h[520]) {
+        set_error("relocation: the removal list needs more draws than three per droplet slot");
+        return DDNS_ERR_CAPACITY;
+    }
     return 0;
 }
 

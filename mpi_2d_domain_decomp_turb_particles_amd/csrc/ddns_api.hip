@@ -792,8 +792,10 @@ static int droplet_block(ddns_handle* h, float* ms)
         h->cc.gmres_residual = res;
     }
     DDNS_HIP(hipEventRecord(h->ev[11], s));
-    st = queue_flow_rest(h, h->ev[11]);                                       // after the HI solve
-    if (st) return st;
+    // The droplet stream's next work is queued before the flow's launches, so it never
+    // waits for the host: the advance, the collision grid and the collision search.  The
+    // flow's forward transforms and update start when the HI solve is done (event 11), as
+    // before; they are queued while the search runs, ahead of its wait.
     st = part_advance(h->pb, pc, s);                                          // 574-576
     if (st) return st;
     h->new_particle = 0;
@@ -805,7 +807,11 @@ static int droplet_block(ddns_handle* h, float* ms)
         st = part_build_cells(h->pb, pc, coll_cell_width(h), s);
         if (st) return st;
     }
-    st = part_collide(h->pb, pc, s);                                          // 584
+    st = part_collide_search(h->pb, pc, s);                                   // 584
+    if (st) return st;
+    st = queue_flow_rest(h, h->ev[11]);                                       // after the HI solve
+    if (st) return st;
+    st = part_collide_finish(h->pb, pc, s);
     if (st) return st;
     const bool reloc = h->p.nonoverlap && h->istep < h->p.nstep_grow;         // 590
     invalidate_cells(h->pb);
@@ -829,6 +835,8 @@ static int droplet_block(ddns_handle* h, float* ms)
         h->cc.noverlap_sweeps = sw;
         h->cc.noverlap_relocated = rl;
     }
+    st = part_queue_counts(h->pb, s);
+    if (st) return st;
     DDNS_HIP(hipEventRecord(h->ev[14], s));
     DDNS_HIP(hipEventSynchronize(h->ev[14]));
     float t;
@@ -839,7 +847,7 @@ static int droplet_block(ddns_handle* h, float* ms)
     DDNS_HIP(hipEventElapsedTime(&t, h->ev[12], h->ev[13])); ms[4] += t;
     DDNS_HIP(hipEventElapsedTime(&t, h->ev[13], h->ev[14])); ms[5] += t;
     ddns_collision_counts c2{};
-    st = part_read_counts(h->pb, pc, &c2, s);
+    st = part_take_counts(h->pb, pc, &c2);
     if (st) return st;
     for (int k = 0; k < 3; ++k) h->cc.icc[k] = c2.icc[k];
     std::memcpy(h->cc.colltype, c2.colltype, sizeof(c2.colltype));

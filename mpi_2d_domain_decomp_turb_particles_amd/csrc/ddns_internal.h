@@ -337,6 +337,8 @@ struct PartBuffers {
     int pair_cap = 0;
     double* contact = nullptr;  // contact_cap * DDNS_CONTACT_REC: one record per pair in contact
     int contact_cap = 0;
+    int* found_host = nullptr;  // pinned: candidate and contact counts of the last collision search
+    int* counts_host = nullptr; // pinned: the step's counts (512 ints) and tested pairs (4 long long)
     double* draws = nullptr;    // relocation draws
     int draw_cap = 0;
     int* draw_off = nullptr;    // cap + 1
@@ -369,7 +371,15 @@ int part_build_cells(PartBuffers& pb, const PartConst& pc, double hmin, hipStrea
 int part_hdi(PartBuffers& pb, const PartConst& pc, int maxiter, double tol, int* iters, double* res, hipStream_t s);
 int part_advance(const PartBuffers& pb, const PartConst& pc, hipStream_t s);
 int part_collide(PartBuffers& pb, const PartConst& pc, hipStream_t s);
+// the collision search queued without a host wait, and its completion (waits for the
+// search's counts, then queues the pair kernel)
+int part_collide_search(PartBuffers& pb, const PartConst& pc, hipStream_t s);
+int part_collide_finish(PartBuffers& pb, const PartConst& pc, hipStream_t s);
+// queue the step's counts into pinned memory; read them after the stream was waited for
+int part_queue_counts(PartBuffers& pb, hipStream_t s);
+int part_take_counts(const PartBuffers& pb, const PartConst& pc, ddns_collision_counts* cc);
 int part_update_and_wrap(const PartBuffers& pb, const PartConst& pc, bool relocate, hipStream_t s);
+int relocate_async(PartBuffers& pb, const PartConst& pc, hipStream_t s);
 int part_overlap(PartBuffers& pb, const PartConst& pc, int* sweeps, int* relocated, hipStream_t s);
 int part_read_counts(const PartBuffers& pb, const PartConst& pc, ddns_collision_counts* cc, hipStream_t s);
 int part_read_bins(const PartBuffers& pb, const PartConst& pc, int* nirbin, double* const bins[7], hipStream_t s);

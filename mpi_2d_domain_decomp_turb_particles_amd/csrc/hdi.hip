@@ -95,7 +95,10 @@ __global__ void __launch_bounds__(NT_NEI) k_nei(const double4* __restrict__ pk, 
         });
         if (overlap) atomicOr(flags, 1);
         if (k > DDNS_MAXNEI) atomicOr(flags, 2);
-        cnt[t] = k;
+        // a row with more neighbours than the lists hold keeps what fits: the products that
+        // may run before the host has read the width flag stay inside the lists, and the
+        // host fills them again, wider, before it uses any result
+        cnt[t] = k < width ? k : width;
     }
     // the widest row of the wavefront, for the width check
     int km = k;
@@ -652,8 +655,10 @@ __device__ void ls_solve(double* sm, GmSmall L, int iter)
 }
 
 // ------------------------------------------------------------------ driver
+// NEI_LIST_PRECOMP (partlib.f90:3218-3371) queued without a host wait: the flags it
+// raises are read with the solve's first convergence check (lists_outcome).
 template <typename T>
-static int build_lists(PartBuffers& pb, const PartConst& pc, hipStream_t s)
+static int queue_lists(PartBuffers& pb, const PartConst& pc, hipStream_t s)
 {
     const int n = pc.n;
     if (!pb.nei_idx) {
@@ -661,24 +666,28 @@ static int build_lists(PartBuffers& pb, const PartConst& pc, hipStream_t s)
         DDNS_HIP(hipMalloc(&pb.nei_idx, (size_t)pb.ell_width * pb.cap * sizeof(int)));
         DDNS_HIP(hipMalloc(&pb.nei_m, (size_t)6 * pb.ell_width * pb.cap * sizeof(T)));
     }
-    for (;;) {
-        DDNS_HIP(hipMemsetAsync(pb.flags, 0, 8 * sizeof(int), s));
-        hipLaunchKernelGGL((k_nei<T>), dim3((n + NT_NEI - 1) / NT_NEI), dim3(NT_NEI), 0, s, (const double4*)pb.pk, pb.cell_start,
-                           n, pb.G, pb.hc, pc.pi2, pc.hdi_trunc, pb.ell_width, pb.nei_cnt, pb.nei_idx, (T*)pb.nei_m,
-                           pb.flags);
-        DDNS_HIP(hipGetLastError());
-        int f[8];
-        DDNS_HIP(hipMemcpyAsync(f, pb.flags, sizeof(f), hipMemcpyDeviceToHost, s));
-        DDNS_HIP(hipStreamSynchronize(s));
-        if (f[0] & 1) { set_error("inside NEI_LIST_PRECOMP: overlapping is not allowed"); return DDNS_ERR_OVERLAP; }
-        if (f[0] & 2) { set_error("more than 100 HI neighbours for a droplet"); return DDNS_ERR_NEIGHBOUR_OVERFLOW; }
-        if (f[4] <= pb.ell_width) return 0;
-        // some row has more neighbours than the lists hold: widen them and fill again
-        if (pb.nei_idx) { (void)hipFree(pb.nei_idx); (void)hipFree(pb.nei_m); pb.nei_idx = nullptr; pb.nei_m = nullptr; }
-        pb.ell_width = std::min(DDNS_MAXNEI, (f[4] + ELL_STEP - 1) / ELL_STEP * ELL_STEP);
-        DDNS_HIP(hipMalloc(&pb.nei_idx, (size_t)pb.ell_width * pb.cap * sizeof(int)));
-        DDNS_HIP(hipMalloc(&pb.nei_m, (size_t)6 * pb.ell_width * pb.cap * sizeof(T)));
-    }
+    DDNS_HIP(hipMemsetAsync(pb.flags, 0, 8 * sizeof(int), s));
+    hipLaunchKernelGGL((k_nei<T>), dim3((n + NT_NEI - 1) / NT_NEI), dim3(NT_NEI), 0, s, (const double4*)pb.pk, pb.cell_start,
+                       n, pb.G, pb.hc, pc.pi2, pc.hdi_trunc, pb.ell_width, pb.nei_cnt, pb.nei_idx, (T*)pb.nei_m,
+                       pb.flags);
+    DDNS_HIP(hipGetLastError());
+    return 0;
+}
+
+// The flags of queue_lists, once read: an error (overlapping droplets, more than 100
+// neighbours), 1 when some row had more neighbours than the lists hold (they have been
+// widened; fill them again), else 0.
+template <typename T>
+static int lists_outcome(PartBuffers& pb, const int* f)
+{
+    if (f[0] & 1) { set_error("inside NEI_LIST_PRECOMP: overlapping is not allowed"); return DDNS_ERR_OVERLAP; }
+    if (f[0] & 2) { set_error("more than 100 HI neighbours for a droplet"); return DDNS_ERR_NEIGHBOUR_OVERFLOW; }
+    if (f[4] <= pb.ell_width) return 0;
+    if (pb.nei_idx) { (void)hipFree(pb.nei_idx); (void)hipFree(pb.nei_m); pb.nei_idx = nullptr; pb.nei_m = nullptr; }
+    pb.ell_width = std::min(DDNS_MAXNEI, (f[4] + ELL_STEP - 1) / ELL_STEP * ELL_STEP);
+    DDNS_HIP(hipMalloc(&pb.nei_idx, (size_t)pb.ell_width * pb.cap * sizeof(int)));
+    DDNS_HIP(hipMalloc(&pb.nei_m, (size_t)6 * pb.ell_width * pb.cap * sizeof(T)));
+    return 1;
 }
 
 template <typename T>
@@ -693,8 +702,6 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
     if (maxiter + 1 > LT) { set_error("GMRES maxiter too large (at most 50)"); return DDNS_ERR_ARG; }
     GmSmall L{maxiter};
     if (L.total() > (size_t)LS_MAX) { set_error("GMRES maxiter too large (at most 50)"); return DDNS_ERR_ARG; }
-    int st = build_lists<T>(pb, pc, s);
-    if (st) return st;
     // Krylov storage
     if (!pb.gm_small) {
         DDNS_HIP(hipMalloc(&pb.gm_small, 16384 * sizeof(double)));
@@ -714,66 +721,87 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
     const unsigned nb3 = (unsigned)((n3 + HT - 1) / HT);
     const unsigned nbr = (unsigned)((n + HT - 1) / HT);
     const unsigned nred = (unsigned)std::min<long long>(RED_BLOCKS, (n3 + 4LL * RT - 1) / (4LL * RT));   // reduction grid
-    // x0 (pertvel) and w = V - U in cell order were written by the interpolation
-    hipLaunchKernelGGL((k_spmv<T, 0>), dim3(nbr), dim3(HT), 0, s, (const int*)pb.nei_cnt, (const int*)pb.nei_idx,
-                       (const T*)pb.nei_m, n, (const T*)nullptr, (const double*)nullptr, (T*)nullptr, (T*)nullptr,
-                       (const double*)pb.gm_x0, ax0, (const double*)pb.gm_w, pb.gm_b, (const double*)nullptr);
-    const int nr0 = (int)std::min<long long>(RED_BLOCKS, (n3 + HT - 1) / HT);
-    hipLaunchKernelGGL((k_residual0<T>), dim3(nr0), dim3(HT), 0, s, pb.gm_b, ax0, Q, n3, pb.gm_red);
-    // the tolerance, and the stop flag when the initial residual is below 1e-5
-    // (2866-2868): then no iteration runs and pertvel keeps its warm-start value
-    hipLaunchKernelGGL(k_gm_start, dim3(1), dim3(HT), 0, s, sm, (int)L.total(), tol, (const double*)pb.gm_red, nr0);
-    hipLaunchKernelGGL((k_scale<T>), dim3(nb3), dim3(HT), 0, s, Q, sm + 1, n3);
-    // Iterations are launched in batches; every kernel of an iteration returns at
-    // once when the least-squares update of an earlier one has set the converged
-    // flag sm[4], so the host checks it once per batch.  The first batch runs as many
-    // iterations as the last solve needed (consecutive steps converge alike), the
-    // later ones three at a time.
-    constexpr int BATCH = 3;
-    const double* stop = sm + 4;
-    double* Hd = sm + L.H0();
-    const int M1 = L.M1();
+    // The neighbour lists are filled and the solve started without a host wait; the
+    // first convergence check also reads the lists' flags, and when a row had more
+    // neighbours than the lists hold they are widened and the solve starts over (its
+    // inputs x0 and w are unchanged, and pertvel is written only at the end).
     int iter = 0;
     double st6[6] = {0, 0, 0, 0, 0, 0};
-    while (iter < maxiter && st6[4] == 0.0) {
-        const int last = std::min(maxiter, iter + (iter == 0 ? std::max(BATCH, pb.gm_last_iters) : BATCH));
-        for (int it = iter + 1; it <= last; ++it) {
-            T* v = work[it & 1];
-            if (it == 1)
-                hipLaunchKernelGGL((k_spmv<T, 1>), dim3(nbr), dim3(HT), 0, s, (const int*)pb.nei_cnt,
-                                   (const int*)pb.nei_idx, (const T*)pb.nei_m, n, (const T*)Q, (const double*)nullptr,
-                                   (T*)nullptr, v, (const double*)nullptr, (double*)nullptr, (const double*)nullptr,
-                                   (double*)nullptr, stop);
-            else    // q_it = v_(it-1) / H(it, it-1), formed inside the product
-                hipLaunchKernelGGL((k_spmv<T, 2>), dim3(nbr), dim3(HT), 0, s, (const int*)pb.nei_cnt,
-                                   (const int*)pb.nei_idx, (const T*)pb.nei_m, n, (const T*)work[(it - 1) & 1],
-                                   (const double*)(Hd + (size_t)M1 * (it - 2) + (it - 1)), Q + (size_t)(it - 1) * ld, v,
-                                   (const double*)nullptr, (double*)nullptr, (const double*)nullptr, (double*)nullptr,
-                                   stop);
-            // orthogonalise v against q_1..q_it: classical Gram-Schmidt, twice (the
-            // reference's modified Gram-Schmidt, 2886-2894, in two block passes)
-            double* hcol = Hd + (size_t)M1 * (it - 1);
-            hipLaunchKernelGGL((k_mdot<T>), dim3(nred), dim3(RT), 0, s, (const T*)Q, (const T*)v, it, n3, ld, part1, stop);
-            if (it <= 8)
-                hipLaunchKernelGGL((k_maxpy_mdot<T, 8>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, it, n3, ld,
-                                   (const double*)part1, hcol, part2, stop);
-            else if (it <= 12)
-                hipLaunchKernelGGL((k_maxpy_mdot<T, 12>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, it, n3, ld,
-                                   (const double*)part1, hcol, part2, stop);
-            else {
-                hipLaunchKernelGGL((k_maxpy<T>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, it, n3, ld,
-                                   (const double*)part1, hcol, 0, (double*)nullptr, stop);
-                hipLaunchKernelGGL((k_mdot<T>), dim3(nred), dim3(RT), 0, s, (const T*)Q, (const T*)v, it, n3, ld, part2,
-                                   stop);
+    for (int lists = 1; lists > 0;) {
+        const int sq = queue_lists<T>(pb, pc, s);
+        if (sq) return sq;
+        lists = 0;
+        // x0 (pertvel) and w = V - U in cell order were written by the interpolation
+        hipLaunchKernelGGL((k_spmv<T, 0>), dim3(nbr), dim3(HT), 0, s, (const int*)pb.nei_cnt, (const int*)pb.nei_idx,
+                           (const T*)pb.nei_m, n, (const T*)nullptr, (const double*)nullptr, (T*)nullptr, (T*)nullptr,
+                           (const double*)pb.gm_x0, ax0, (const double*)pb.gm_w, pb.gm_b, (const double*)nullptr);
+        const int nr0 = (int)std::min<long long>(RED_BLOCKS, (n3 + HT - 1) / HT);
+        hipLaunchKernelGGL((k_residual0<T>), dim3(nr0), dim3(HT), 0, s, pb.gm_b, ax0, Q, n3, pb.gm_red);
+        // the tolerance, and the stop flag when the initial residual is below 1e-5
+        // (2866-2868): then no iteration runs and pertvel keeps its warm-start value
+        hipLaunchKernelGGL(k_gm_start, dim3(1), dim3(HT), 0, s, sm, (int)L.total(), tol, (const double*)pb.gm_red, nr0);
+        hipLaunchKernelGGL((k_scale<T>), dim3(nb3), dim3(HT), 0, s, Q, sm + 1, n3);
+        // Iterations are launched in batches; every kernel of an iteration returns at
+        // once when the least-squares update of an earlier one has set the converged
+        // flag sm[4], so the host checks it once per batch.  The first batch runs as many
+        // iterations as the last solve needed (consecutive steps converge alike), the
+        // later ones three at a time.
+        constexpr int BATCH = 3;
+        const double* stop = sm + 4;
+        double* Hd = sm + L.H0();
+        const int M1 = L.M1();
+        iter = 0;
+        for (double& v : st6) v = 0.0;
+        bool first = true;
+        while (iter < maxiter && st6[4] == 0.0) {
+            const int last = std::min(maxiter, iter + (iter == 0 ? std::max(BATCH, pb.gm_last_iters) : BATCH));
+            for (int it = iter + 1; it <= last; ++it) {
+                T* v = work[it & 1];
+                if (it == 1)
+                    hipLaunchKernelGGL((k_spmv<T, 1>), dim3(nbr), dim3(HT), 0, s, (const int*)pb.nei_cnt,
+                                       (const int*)pb.nei_idx, (const T*)pb.nei_m, n, (const T*)Q, (const double*)nullptr,
+                                       (T*)nullptr, v, (const double*)nullptr, (double*)nullptr, (const double*)nullptr,
+                                       (double*)nullptr, stop);
+                else    // q_it = v_(it-1) / H(it, it-1), formed inside the product
+                    hipLaunchKernelGGL((k_spmv<T, 2>), dim3(nbr), dim3(HT), 0, s, (const int*)pb.nei_cnt,
+                                       (const int*)pb.nei_idx, (const T*)pb.nei_m, n, (const T*)work[(it - 1) & 1],
+                                       (const double*)(Hd + (size_t)M1 * (it - 2) + (it - 1)), Q + (size_t)(it - 1) * ld, v,
+                                       (const double*)nullptr, (double*)nullptr, (const double*)nullptr, (double*)nullptr,
+                                       stop);
+                // orthogonalise v against q_1..q_it: classical Gram-Schmidt, twice (the
+                // reference's modified Gram-Schmidt, 2886-2894, in two block passes)
+                double* hcol = Hd + (size_t)M1 * (it - 1);
+                hipLaunchKernelGGL((k_mdot<T>), dim3(nred), dim3(RT), 0, s, (const T*)Q, (const T*)v, it, n3, ld, part1, stop);
+                if (it <= 8)
+                    hipLaunchKernelGGL((k_maxpy_mdot<T, 8>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, it, n3, ld,
+                                       (const double*)part1, hcol, part2, stop);
+                else if (it <= 12)
+                    hipLaunchKernelGGL((k_maxpy_mdot<T, 12>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, it, n3, ld,
+                                       (const double*)part1, hcol, part2, stop);
+                else {
+                    hipLaunchKernelGGL((k_maxpy<T>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, it, n3, ld,
+                                       (const double*)part1, hcol, 0, (double*)nullptr, stop);
+                    hipLaunchKernelGGL((k_mdot<T>), dim3(nred), dim3(RT), 0, s, (const T*)Q, (const T*)v, it, n3, ld, part2,
+                                       stop);
+                }
+                hipLaunchKernelGGL((k_maxpy<T>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, it, n3, ld, (const double*)part2,
+                                   hcol, 1, normpart, stop);
+                hipLaunchKernelGGL(k_ls, dim3(1), dim3(LT), 0, s, sm, L, it, (const double*)normpart, (int)nred);
             }
-            hipLaunchKernelGGL((k_maxpy<T>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, it, n3, ld, (const double*)part2,
-                               hcol, 1, normpart, stop);
-            hipLaunchKernelGGL(k_ls, dim3(1), dim3(LT), 0, s, sm, L, it, (const double*)normpart, (int)nred);
+            DDNS_HIP(hipGetLastError());
+            iter = last;
+            // the first check also reads the neighbour lists' flags
+            int f[8];
+            if (first) DDNS_HIP(hipMemcpyAsync(f, pb.flags, sizeof(f), hipMemcpyDeviceToHost, s));
+            DDNS_HIP(hipMemcpyAsync(st6, sm, sizeof(st6), hipMemcpyDeviceToHost, s));
+            DDNS_HIP(hipStreamSynchronize(s));
+            if (first) {
+                first = false;
+                lists = lists_outcome<T>(pb, f);
+                if (lists < 0) return lists;
+                if (lists > 0) break;            // fill the widened lists and solve again
+            }
         }
-        DDNS_HIP(hipGetLastError());
-        iter = last;
-        DDNS_HIP(hipMemcpyAsync(st6, sm, sizeof(st6), hipMemcpyDeviceToHost, s));
-        DDNS_HIP(hipStreamSynchronize(s));
     }
     if (st6[4] < 0.0) return 0;     // initial residual below 1e-5: no iteration
     const bool done = st6[4] != 0.0;

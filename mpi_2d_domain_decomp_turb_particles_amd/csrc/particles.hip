@@ -154,6 +154,8 @@ void part_free(PartBuffers& pb)
     for (void* p : ps) if (p) (void)hipFree(p);
     void* sp[] = {pb.spare.cell_count, pb.spare.cell_start, pb.spare.cell_of, pb.spare.sorted, pb.spare.pk};
     for (void* p : sp) if (p) (void)hipFree(p);
+    if (pb.found_host) (void)hipHostFree(pb.found_host);
+    if (pb.counts_host) (void)hipHostFree(pb.counts_host);
     pb = PartBuffers{};
 }
 
@@ -453,7 +455,9 @@ __device__ double dev_ranff2(int* idum, int* iv, int* iy, int* idum2)
 }
 
 // state layout: [0] idum, [1] iy, [2] idum2, [3..34] iv
-__global__ void k_draws(int* st, double* draws, const int* total)
+// At most cap draws are made: a removal list that needs more sets *err (the host then
+// reports a capacity error) and is cut short.
+__global__ void k_draws(int* st, double* draws, const int* total, int cap, int* err)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     int idum = st[0], iy = st[1], idum2 = st[2];
@@ -461,7 +465,11 @@ __global__ void k_draws(int* st, double* draws, const int* total)
     // indexed register moves
     __shared__ int iv[32];
     for (int q = 0; q < 32; ++q) iv[q] = st[3 + q];
-    const int nd = 3 * (*total);
+    int nd = 3 * (*total);
+    if (nd > cap) {
+        atomicOr(err, 1);
+        nd = cap - cap % 3;
+    }
     for (int q = 0; q < nd; ++q) draws[q] = dev_ranff2(&idum, iv, &iy, &idum2);
     st[0] = idum; st[1] = iy; st[2] = idum2;
     for (int q = 0; q < 32; ++q) st[3 + q] = iv[q];
@@ -469,12 +477,14 @@ __global__ void k_draws(int* st, double* draws, const int* total)
 
 // A droplet that appears m times in the sorted removal list consumes 3m draws at
 // offset off; the last triple is the position it keeps (RELOCATE, 1994-2006).
-__global__ void k_apply_reloc(double* pos, const int* mult, const int* off, const double* draws, int n, int cap, double pi2)
+__global__ void k_apply_reloc(double* pos, const int* mult, const int* off, const double* draws, int n, int cap, double pi2,
+                              long long draw_cap)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int m = mult[i];
     if (m <= 0) return;
+    if (3 * ((long long)off[i] + m) > draw_cap) return;      // cut short by k_draws (capacity error)
     const double* d = draws + 3 * (size_t)(off[i] + m - 1);
     pos[i] = d[0] * pi2;
     pos[(size_t)cap + i] = d[1] * pi2 + 0.0 * pi2;
@@ -511,9 +521,34 @@ int relocate_by_mult(PartBuffers& pb, const PartConst& pc, int* total_host, hipS
         pb.draw_cap = 3 * total + 1024;
         DDNS_HIP(hipMalloc(&pb.draws, (size_t)pb.draw_cap * sizeof(double)));
     }
-    hipLaunchKernelGGL(k_draws, dim3(1), dim3(1), 0, s, pb.rng_state, pb.draws, pb.draw_off + pc.n);
+    hipLaunchKernelGGL(k_draws, dim3(1), dim3(1), 0, s, pb.rng_state, pb.draws, pb.draw_off + pc.n, pb.draw_cap,
+                       pb.flags + 48);
     hipLaunchKernelGGL(k_apply_reloc, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.pos, pb.mult, pb.draw_off, pb.draws,
-                       pc.n, pb.cap, pc.pi2);
+                       pc.n, pb.cap, pc.pi2, (long long)pb.draw_cap);
+    DDNS_HIP(hipGetLastError());
+    return 0;
+}
+
+// The relocation of the collision step (RELOCATE, partlib.f90:1984-2012) without a host
+// round trip: the draws are made for whatever the removal list holds, into a buffer of
+// three draws per droplet slot (each droplet relocated once; a step whose removal list
+// needs more sets the capacity flag flags[48], which the step reports as an error).
+// Droplets with no removal entry keep their positions.
+int relocate_async(PartBuffers& pb, const PartConst& pc, hipStream_t s)
+{
+    int st = exclusive_scan(pb.mult, pb.draw_off, pc.n, (int*)pb.scan_tmp, s);
+    if (st) return st;
+    invalidate_cells(pb);                 // relocated droplets move
+    const int need = 3 * std::max(pb.cap, 1 << 16);
+    if (pb.draw_cap < need) {
+        DDNS_HIP(hipFree(pb.draws));
+        pb.draw_cap = need;
+        DDNS_HIP(hipMalloc(&pb.draws, (size_t)pb.draw_cap * sizeof(double)));
+    }
+    hipLaunchKernelGGL(k_draws, dim3(1), dim3(1), 0, s, pb.rng_state, pb.draws, pb.draw_off + pc.n, pb.draw_cap,
+                       pb.flags + 48);
+    hipLaunchKernelGGL(k_apply_reloc, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.pos, pb.mult, pb.draw_off, pb.draws,
+                       pc.n, pb.cap, pc.pi2, (long long)pb.draw_cap);
     DDNS_HIP(hipGetLastError());
     return 0;
 }
@@ -524,8 +559,7 @@ int part_update_and_wrap(const PartBuffers& pb, const PartConst& pc, bool reloca
     hipLaunchKernelGGL(k_update_pos, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.pos, pb.dmove, pb.fvp, pb.fdp, pc.n, pb.cap);
     DDNS_HIP(hipGetLastError());
     if (relocate) {
-        int tot = 0;
-        int st = relocate_by_mult(const_cast<PartBuffers&>(pb), pc, &tot, s);
+        int st = relocate_async(const_cast<PartBuffers&>(pb), pc, s);
         if (st) return st;
     }
     hipLaunchKernelGGL(k_wrap, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.pos, pc.n, pb.cap, pc.pi2);
