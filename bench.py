@@ -226,6 +226,7 @@ def coalescence_1024(steps, warmup):
         n = s.get_droplets()["yp"].shape[1]
         alone = s.profile_flow(2)
         tune = s.tuning()
+        copy_ms, copy_bytes = s.profile_copy(5)
     kb = kernel_bytes(1024, dd.FP32_MIXED)
     kms = dict(zip(["y_inv", "x_fused", "y_fwd", "pre", "z_stage", "sym"], t.flow_kernel_ms))
     return {
@@ -238,6 +239,7 @@ def coalescence_1024(steps, warmup):
         "flow_kernels_GBps_in_step": {k: kb[k] / (kms[k] / steps * 1e-3) / 1e9 for k in FFT_KERNELS if kms[k] > 0},
         "flow_kernels_GBps_alone": {k: kb[k] / (alone[k] * 1e-3) / 1e9 for k in FFT_KERNELS if alone[k] > 0},
         "tuning": {k: (v.tolist() if hasattr(v, "tolist") else v) for k, v in tune.items()},
+        "hbm_copy_GBps": copy_bytes / (copy_ms * 1e-3) / 1e9,
     }
 
 

@@ -45,6 +45,11 @@ struct ddns_handle {
     // stream (the droplets read only the physical velocity of the first half)
     hipStream_t dstream = nullptr;
     bool flow_rest_queued = false;   // the flow's second half of this step is on stream
+    // z_gate: the second half is queued in two pieces, and its z stage waits for the
+    // droplet block's collision-pair launch (DDNS_Z_GATE=0 at ddns_create queues it in
+    // one piece); z_pending: the first piece is queued, the z stage not yet
+    bool z_gate = true;
+    bool z_pending = false;
     FlowBuffers fb;
     FlowConst fc{};
     bool a_valid = false;        // inverse pipeline holds the start of the next step
@@ -63,11 +68,11 @@ struct ddns_handle {
     // kernel boundaries, one set per parity of the step: the next step's first half may
     // be queued before this step's times are read
     hipEvent_t flev[2][4] = {};
-    hipEvent_t fev[2][8] = {};
-    // the two halves of the flow step as instantiated graphs, one per half and event set
-    // (step parity), captured by the first step that runs them; graph_mode 0 launches
+    hipEvent_t fev[2][FLOW_EVENTS] = {};
+    // the parts of the flow step (FlowPart) as instantiated graphs, one per part and event
+    // set (step parity), captured by the first step that runs them; graph_mode 0 launches
     // the kernels one by one (DDNS_GRAPH=0 in the environment at ddns_create)
-    hipGraphExec_t gflow[2][2] = {};
+    hipGraphExec_t gflow[4][2] = {};
     int graph_mode = 1;
     int par = 0;                 // parity of the step in progress
     bool want_next = false;      // a further step follows in this call
@@ -312,6 +317,7 @@ extern "C" int ddns_create(const ddns_params* pin, ddns_handle** out)
     int st = 0;
     if (hipGetDevice(&h->dev) != hipSuccess) { delete h; return fail(nullptr, DDNS_ERR_HIP, "hipGetDevice"); }
     if (const char* g = std::getenv("DDNS_GRAPH")) h->graph_mode = g[0] != '0';
+    if (const char* g = std::getenv("DDNS_Z_GATE")) h->z_gate = g[0] != '0';
     // the device address of the allocation-free spectrum's scratch, once per device
     st = spectrum_scratch_init();
     if (st) { delete h; return st; }
@@ -353,7 +359,7 @@ extern "C" int ddns_create(const ddns_params* pin, ddns_handle** out)
     for (int i = 0; i < 16; ++i)
         if (hipEventCreate(&h->ev[i]) != hipSuccess) { ddns_destroy(h); return fail(nullptr, DDNS_ERR_HIP, "event"); }
     for (int q = 0; q < 2; ++q) {
-        for (int i = 0; i < 8; ++i)
+        for (int i = 0; i < FLOW_EVENTS; ++i)
             if (hipEventCreate(&h->fev[q][i]) != hipSuccess) { ddns_destroy(h); return fail(nullptr, DDNS_ERR_HIP, "event"); }
         for (int i = 0; i < 4; ++i)
             if (hipEventCreate(&h->flev[q][i]) != hipSuccess) { ddns_destroy(h); return fail(nullptr, DDNS_ERR_HIP, "event"); }
@@ -372,9 +378,9 @@ extern "C" int ddns_destroy(ddns_handle* h)
     part_free(h->pb);
     for (int i = 0; i < 16; ++i) if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
     for (int q = 0; q < 2; ++q) {
-        for (int i = 0; i < 8; ++i) if (h->fev[q][i]) (void)hipEventDestroy(h->fev[q][i]);
+        for (int i = 0; i < FLOW_EVENTS; ++i) if (h->fev[q][i]) (void)hipEventDestroy(h->fev[q][i]);
         for (int i = 0; i < 4; ++i) if (h->flev[q][i]) (void)hipEventDestroy(h->flev[q][i]);
-        for (int a = 0; a < 2; ++a) if (h->gflow[a][q]) (void)hipGraphExecDestroy(h->gflow[a][q]);
+        for (int a = 0; a < 4; ++a) if (h->gflow[a][q]) (void)hipGraphExecDestroy(h->gflow[a][q]);
     }
     if (h->dstream) (void)hipStreamDestroy(h->dstream);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -675,22 +681,22 @@ static double cell_width(const ddns_handle* h)
     return w;
 }
 
-// Half `rest` of the flow step (0: inverse stages, 1: forward stages and update) with
-// the timing events of parity q.  In graph mode the half is captured once into a graph
-// (its kernels and their events) and replayed with one launch: the flow step takes no
-// decision on the host and its buffers never move, so every step replays the same
-// graph.  A failed capture turns graph mode off for the handle and the kernels run one
-// by one.
-static int run_flow_half(ddns_handle* h, int rest, int q)
+// Part `part` of the flow step (FlowPart: the inverse stages, the forward stages and
+// update, or one of the two pieces of the latter) with the timing events of parity q.
+// In graph mode the part is captured once into a graph (its kernels and their events)
+// and replayed with one launch: the flow step takes no decision on the host and its
+// buffers never move, so every step replays the same graph.  A failed capture turns
+// graph mode off for the handle and the kernels run one by one.
+static int run_flow_half(ddns_handle* h, int part, int q)
 {
     hipStream_t s = h->stream;
     if (h->graph_mode) {
-        hipGraphExec_t& gx = h->gflow[rest][q];
+        hipGraphExec_t& gx = h->gflow[part][q];
         if (!gx) {
             hipGraph_t g = nullptr;
             int st = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess ? 0 : DDNS_ERR_HIP;
             if (!st) {
-                st = flow_step(h->fb, h->fc, h->prec, rest != 0, s, h->fev[q], true);
+                st = flow_step(h->fb, h->fc, h->prec, part, s, h->fev[q], true);
                 const hipError_t e = hipStreamEndCapture(s, &g);
                 if (!st && e != hipSuccess) st = DDNS_ERR_HIP;
             }
@@ -702,7 +708,7 @@ static int run_flow_half(ddns_handle* h, int rest, int q)
             if (st) {
                 (void)hipGetLastError();
                 h->graph_mode = 0;
-                for (int a = 0; a < 2; ++a)
+                for (int a = 0; a < 4; ++a)
                     for (int b = 0; b < 2; ++b)
                         if (h->gflow[a][b]) { (void)hipGraphExecDestroy(h->gflow[a][b]); h->gflow[a][b] = nullptr; }
             }
@@ -712,7 +718,7 @@ static int run_flow_half(ddns_handle* h, int rest, int q)
             return 0;
         }
     }
-    return flow_step(h->fb, h->fc, h->prec, rest != 0, s, h->fev[q]);
+    return flow_step(h->fb, h->fc, h->prec, part, s, h->fev[q]);
 }
 
 // The first half of a step's flow (the curl and inverse z transforms when the state
@@ -727,7 +733,7 @@ static int queue_flow_first(ddns_handle* h, int q)
         if (st) return st;
         h->a_valid = true;
     }
-    const int st = run_flow_half(h, 0, q);
+    const int st = run_flow_half(h, FLOW_INV, q);
     if (st) return st;
     DDNS_HIP(hipEventRecord(h->flev[q][1], s));
     return 0;
@@ -738,23 +744,54 @@ static int queue_flow_first(ddns_handle* h, int q)
 // flow reads nothing of the droplets, so this may run beside any part of the block; it
 // is queued to start when the HI solve is done (event after): the collision search,
 // the pair work and the overlap sweep that follow are latency-bound and leave HBM to it,
-// while the GMRES kernels before stream the Krylov basis themselves.
+// while the GMRES kernels before stream the Krylov basis themselves.  With z_gate only
+// the forward y stage and pre are queued here, and queue_flow_z adds the z stage later.
+static int queue_flow_next(ddns_handle* h, hipEvent_t after);
+static int queue_flow_z(ddns_handle* h, hipEvent_t after);
+
 static int queue_flow_rest(ddns_handle* h, hipEvent_t after)
 {
-    if (h->flow_rest_queued) return 0;
+    if (h->flow_rest_queued) return h->z_pending ? queue_flow_z(h, nullptr) : 0;
     hipStream_t s = h->stream;
     const int q = h->par;
     if (after) DDNS_HIP(hipStreamWaitEvent(s, after, 0));
     DDNS_HIP(hipEventRecord(h->flev[q][2], s));
-    int st = run_flow_half(h, 1, q);
+    h->flow_rest_queued = true;
+    if (h->z_gate && after) {
+        h->z_pending = true;
+        return run_flow_half(h, FLOW_FWD_Y, q);
+    }
+    int st = run_flow_half(h, FLOW_FWD, q);
     if (st) return st;
     DDNS_HIP(hipEventRecord(h->flev[q][3], s));
-    h->flow_rest_queued = true;
-    // The next step's first half can follow at once: it overwrites the physical velocity,
-    // and the gather that reads it comes before event after, which this queue has waited
-    // for.  It then runs beside the rest of this droplet block.
+    return queue_flow_next(h, after);
+}
+
+// The z stage and the symmetry pass of a second half queued in two pieces, after event
+// `after` of the droplet stream (the collision-pair launch).  A pair kernel queued while
+// the z stage ran made no progress until the z stage ended (35 us alone, 400-460 us
+// then, in a quarter of the steps at 256^3 with 1M droplets); forcing the order cost the
+// flow less than that on average.
+static int queue_flow_z(ddns_handle* h, hipEvent_t after)
+{
+    if (!h->z_pending) return 0;
+    hipStream_t s = h->stream;
+    const int q = h->par;
+    if (after) DDNS_HIP(hipStreamWaitEvent(s, after, 0));
+    int st = run_flow_half(h, FLOW_FWD_Z, q);
+    if (st) return st;
+    DDNS_HIP(hipEventRecord(h->flev[q][3], s));
+    h->z_pending = false;
+    return queue_flow_next(h, after);
+}
+
+// The next step's first half can follow at once: it overwrites the physical velocity,
+// and the gather that reads it comes before the event the second half waited for.  It
+// then runs beside the rest of this droplet block.
+static int queue_flow_next(ddns_handle* h, hipEvent_t after)
+{
     if (h->want_next && after) {
-        st = queue_flow_first(h, 1 - q);
+        const int st = queue_flow_first(h, 1 - h->par);
         if (st) return st;
         h->next_queued = true;
     }
@@ -813,6 +850,11 @@ static int droplet_block(ddns_handle* h, float* ms)
     if (st) return st;
     st = part_collide_finish(h->pb, pc, s);
     if (st) return st;
+    if (h->z_pending) {
+        DDNS_HIP(hipEventRecord(h->ev[6], s));
+        st = queue_flow_z(h, h->ev[6]);
+        if (st) return st;
+    }
     const bool reloc = h->p.nonoverlap && h->istep < h->p.nstep_grow;         // 590
     invalidate_cells(h->pb);
     st = part_update_and_wrap(h->pb, pc, reloc, s);                           // 586, 590, 608
@@ -872,6 +914,7 @@ static int step_fail(ddns_handle* h, int st)
     h->a_valid = false;
     h->next_queued = false;
     h->flow_rest_queued = false;
+    h->z_pending = false;
     return keep(h, st);
 }
 
@@ -914,7 +957,7 @@ extern "C" int ddns_step(ddns_handle* h, int nsteps)
         // its end, and reports its whole time as flow time, without the per-kernel split.
         if (!flow_only_graph) {
             DDNS_HIP(hipEventSynchronize(h->flev[q][3]));
-            const int pairs[6][2] = {{0, 1}, {1, 2}, {3, 4}, {4, 5}, {5, 6}, {6, 7}};
+            const int pairs[6][2] = {{0, 1}, {1, 2}, {3, 4}, {4, 5}, {8, 6}, {6, 7}};
             for (int i = 0; i < 6; ++i) {
                 float t;
                 DDNS_HIP(hipEventElapsedTime(&t, h->fev[q][pairs[i][0]], h->fev[q][pairs[i][1]]));

@@ -244,9 +244,14 @@ struct FlowConst {
 int flow_alloc(FlowBuffers& fb, int N, int precision);
 void flow_free(FlowBuffers& fb);
 int flow_prologue(const FlowBuffers& fb, const FlowConst& fc, int precision, hipStream_t s);
-// capturing: the stream is being captured into a graph (the events become graph nodes)
-int flow_step(const FlowBuffers& fb, const FlowConst& fc, int precision, bool last, hipStream_t s,
-              hipEvent_t* ev /* 8 events or nullptr */, bool capturing = false);
+// Parts of the flow step: the inverse stages (first half), the whole forward half, or the
+// forward half in two pieces (forward y + pre, then z + sym) so that a wait can sit
+// between them.  capturing: the stream is being captured into a graph (the events
+// become graph nodes).
+enum FlowPart { FLOW_INV = 0, FLOW_FWD = 1, FLOW_FWD_Y = 2, FLOW_FWD_Z = 3 };
+int flow_step(const FlowBuffers& fb, const FlowConst& fc, int precision, int part, hipStream_t s,
+              hipEvent_t* ev /* FLOW_EVENTS events or nullptr */, bool capturing = false);
+constexpr int FLOW_EVENTS = 9;
 int flow_upload_state(const FlowBuffers& fb, int N, int precision, const double* const src[6], hipStream_t s);
 int flow_download_state(const FlowBuffers& fb, int N, int precision, double* const dst[6], hipStream_t s);
 int flow_download_phys(const FlowBuffers& fb, int N, int precision, double* ux, double* uy, double* uz, hipStream_t s);

@@ -1247,11 +1247,13 @@ template <typename T, int N>
 static int step_impl(const FlowBuffers& fb, const FlowConst& fc, hipStream_t s, hipEvent_t* ev, int part,
                      unsigned evflags)
 {
-    // part 0: inverse y + fused x (produces the physical velocity);
-    // part 1: forward y + pre + z + sym (closes the step, opens the next).
+    // FLOW_INV: inverse y + fused x (produces the physical velocity);
+    // FLOW_FWD: forward y + pre + z + sym (closes the step, opens the next), or its two
+    // pieces FLOW_FWD_Y (forward y + pre) and FLOW_FWD_Z (z + sym).  Events: 0-2 around
+    // the first half, 3 forward y, 4 pre, 5 end of pre, 8 start of z, 6 end of z, 7 end.
     // evflags = hipEventRecordExternal while the stream is being captured into a graph:
     // the timing events then become event-record nodes of the graph
-    if (part == 0) {
+    if (part == FLOW_INV) {
         if (ev) DDNS_HIP(hipEventRecordWithFlags(ev[0], s, evflags));
         int st = launch_y<T, N>(true, true, fb.A, 6, fb, s);
         if (st) return st;
@@ -1261,14 +1263,19 @@ static int step_impl(const FlowBuffers& fb, const FlowConst& fc, hipStream_t s, 
         if (ev) DDNS_HIP(hipEventRecordWithFlags(ev[2], s, evflags));
         return 0;
     }
-    if (ev) DDNS_HIP(hipEventRecordWithFlags(ev[3], s, evflags));
-    void* bb[3] = {fb.B[0], fb.B[1], fb.B[2]};
-    int st = launch_y<T, N>(false, false, bb, 3, fb, s);
-    if (st) return st;
-    if (ev) DDNS_HIP(hipEventRecordWithFlags(ev[4], s, evflags));
-    st = launch_pre<T, N>(fb, fc, s);
-    if (st) return st;
-    if (ev) DDNS_HIP(hipEventRecordWithFlags(ev[5], s, evflags));
+    int st = 0;
+    if (part != FLOW_FWD_Z) {
+        if (ev) DDNS_HIP(hipEventRecordWithFlags(ev[3], s, evflags));
+        void* bb[3] = {fb.B[0], fb.B[1], fb.B[2]};
+        st = launch_y<T, N>(false, false, bb, 3, fb, s);
+        if (st) return st;
+        if (ev) DDNS_HIP(hipEventRecordWithFlags(ev[4], s, evflags));
+        st = launch_pre<T, N>(fb, fc, s);
+        if (st) return st;
+        if (ev) DDNS_HIP(hipEventRecordWithFlags(ev[5], s, evflags));
+        if (part == FLOW_FWD_Y) return 0;
+    }
+    if (ev) DDNS_HIP(hipEventRecordWithFlags(ev[8], s, evflags));
     st = launch_z<T, N>(KZ_FULL, fb, fc, s);
     if (st) return st;
     if (ev) DDNS_HIP(hipEventRecordWithFlags(ev[6], s, evflags));
@@ -1297,11 +1304,9 @@ int flow_prologue(const FlowBuffers& fb, const FlowConst& fc, int precision, hip
     return st;
 }
 
-int flow_step(const FlowBuffers& fb, const FlowConst& fc, int precision, bool last, hipStream_t s, hipEvent_t* ev,
+int flow_step(const FlowBuffers& fb, const FlowConst& fc, int precision, int part, hipStream_t s, hipEvent_t* ev,
               bool capturing)
 {
-    // 'last' selects the part: false = first half (inverse), true = second half (forward + update)
-    const int part = last ? 1 : 0;
     const unsigned evflags = capturing ? hipEventRecordExternal : hipEventRecordDefault;
     int st = 0;
     if (precision == DDNS_FP64) { DDNS_DISPATCH_N(fc.N, st = (step_impl<double, NN>(fb, fc, s, ev, part, evflags))) }
@@ -1803,13 +1808,13 @@ int flow_diagnostics(FlowBuffers& fb, int N, int nek, double rnu, int precision,
 template <typename T, int N>
 static int profile_impl(const FlowBuffers& fb, const FlowConst& fc, int k, double* ms, int nk, hipStream_t s)
 {
-    hipEvent_t ev[8];
-    for (int i = 0; i < 8; ++i) DDNS_HIP(hipEventCreate(&ev[i]));
+    hipEvent_t ev[FLOW_EVENTS];
+    for (int i = 0; i < FLOW_EVENTS; ++i) DDNS_HIP(hipEventCreate(&ev[i]));
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int it = 0; it < k; ++it) {
-        int st = step_impl<T, N>(fb, fc, s, ev, 0, hipEventRecordDefault);
+        int st = step_impl<T, N>(fb, fc, s, ev, FLOW_INV, hipEventRecordDefault);
         if (st) return st;
-        st = step_impl<T, N>(fb, fc, s, ev, 1, hipEventRecordDefault);
+        st = step_impl<T, N>(fb, fc, s, ev, FLOW_FWD, hipEventRecordDefault);
         if (st) return st;
         DDNS_HIP(hipEventSynchronize(ev[7]));
         float t;
@@ -1817,12 +1822,12 @@ static int profile_impl(const FlowBuffers& fb, const FlowConst& fc, int k, doubl
         DDNS_HIP(hipEventElapsedTime(&t, ev[1], ev[2])); acc[1] += t;   // x fused
         DDNS_HIP(hipEventElapsedTime(&t, ev[3], ev[4])); acc[2] += t;   // y forward
         DDNS_HIP(hipEventElapsedTime(&t, ev[4], ev[5])); acc[3] += t;   // pre
-        DDNS_HIP(hipEventElapsedTime(&t, ev[5], ev[6])); acc[4] += t;   // z stage
+        DDNS_HIP(hipEventElapsedTime(&t, ev[8], ev[6])); acc[4] += t;   // z stage
         DDNS_HIP(hipEventElapsedTime(&t, ev[6], ev[7])); acc[5] += t;   // sym
         DDNS_HIP(hipEventElapsedTime(&t, ev[0], ev[7])); acc[6] += t;   // whole
     }
     for (int i = 0; i < nk && i < 7; ++i) ms[i] = acc[i] / k;
-    for (int i = 0; i < 8; ++i) (void)hipEventDestroy(ev[i]);
+    for (int i = 0; i < FLOW_EVENTS; ++i) (void)hipEventDestroy(ev[i]);
     return 0;
 }
 

@@ -194,15 +194,15 @@ __device__ __forceinline__ void load_node(const double* p, double& a, double& b,
 constexpr int IG = 6;              // lanes per droplet
 constexpr int IPW = 64 / IG;       // droplets per wavefront
 
-// With HI on it also writes, in cell order, the inputs of the HI solve: the warm start
-// x0 = pertvel and the velocity difference w = V - U, so the solve reads them
-// contiguously.
+// In cell order (order != null) it writes the result in cell order, us[c * n + t], and
+// k_interp_scatter takes it to the droplets: the gather then shares L2 with nothing but
+// the field, instead of with three droplet arrays read and one written a line per
+// droplet each (which halved its L2 hit rate).  Without a cell order it writes vp.
 template <typename T>
 __global__ void __launch_bounds__(PT) k_interp(const T* __restrict__ u, int N, double pi2,
                                                const double* __restrict__ pos, const int* __restrict__ order,
                                                const double4* __restrict__ pk, double* __restrict__ vp, int n, int cap,
-                                               const double* __restrict__ vel, const double* __restrict__ pert,
-                                               double* __restrict__ x0, double* __restrict__ w)
+                                               double* __restrict__ us)
 {
     const int lane = threadIdx.x & 63;
     const int wave = (xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
@@ -258,17 +258,43 @@ __global__ void __launch_bounds__(PT) k_interp(const T* __restrict__ u, int N, d
         s0 += p0; s1 += p1; s2 += p2;
     }
     if (active && a == 0) {
-        const size_t C = (size_t)cap, M = (size_t)n;
-        vp[i] = s0;
-        vp[C + i] = s1;
-        vp[2 * C + i] = s2;
-        if (x0) {
-            const double s3[3] = {s0, s1, s2};
+        if (us) {
+            const size_t M = (size_t)n;
+            us[t] = s0;
+            us[M + t] = s1;
+            us[2 * M + t] = s2;
+        } else {
+            const size_t C = (size_t)cap;
+            vp[i] = s0;
+            vp[C + i] = s1;
+            vp[2 * C + i] = s2;
+        }
+    }
+}
+
+// The gathered velocities, in cell order in us, to the droplets (vp in the reference's
+// order).  With HI on it also writes, in cell order, the inputs of the HI solve: the
+// warm start x0 = pertvel and the velocity difference w = V - U, so the solve reads
+// them contiguously; w may be us itself (each element is read before it is written).
+__global__ void __launch_bounds__(PT) k_interp_scatter(const int* __restrict__ order, const double* us, double* vp,
+                                                       int n, int cap, const double* __restrict__ vel,
+                                                       const double* __restrict__ pert, double* __restrict__ x0,
+                                                       double* w)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = order[t];
+    const size_t C = (size_t)cap, M = (size_t)n;
+    double s3[3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                x0[c * M + t] = pert[3 * (size_t)i + c];
-                w[c * M + t] = vel[c * C + i] - s3[c];
-            }
+    for (int c = 0; c < 3; ++c) s3[c] = us[c * M + t];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) vp[c * C + i] = s3[c];
+    if (x0) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            x0[c * M + t] = pert[3 * (size_t)i + c];
+            w[c * M + t] = vel[c * C + i] - s3[c];
         }
     }
 }
@@ -278,23 +304,22 @@ int part_interp(PartBuffers& pb, const PartConst& pc, const void* uphys, hipStre
     if (pc.n == 0) return 0;
     const int* order = pb.G > 0 ? pb.sorted : nullptr;
     const double4* pk = order ? pb.pk : nullptr;
-    if (pc.hdi) {
-        // the HI solve works in the cell order of this grid
-        if (!order) { set_error("HI needs the cell grid before the interpolation"); return DDNS_ERR_ARG; }
-        if (!pb.gm_x0) {
-            DDNS_HIP(hipMalloc(&pb.gm_x0, 3 * (size_t)pb.cap * sizeof(double)));
-            DDNS_HIP(hipMalloc(&pb.gm_w, 3 * (size_t)pb.cap * sizeof(double)));
-        }
-    }
-    double* x0 = pc.hdi ? pb.gm_x0 : nullptr;
-    double* w = pc.hdi ? pb.gm_w : nullptr;
+    // the HI solve works in the cell order of this grid
+    if (pc.hdi && !order) { set_error("HI needs the cell grid before the interpolation"); return DDNS_ERR_ARG; }
+    if (order && !pb.gm_w) DDNS_HIP(hipMalloc(&pb.gm_w, 3 * (size_t)pb.cap * sizeof(double)));
+    if (pc.hdi && !pb.gm_x0) DDNS_HIP(hipMalloc(&pb.gm_x0, 3 * (size_t)pb.cap * sizeof(double)));
+    double* us = order ? pb.gm_w : nullptr;       // cell-order result, then w in place
     const unsigned nb = (unsigned)(((long long)pc.n + IPW * (PT / 64) - 1) / (IPW * (PT / 64)));
     if (pc.precision == DDNS_FP64)
         hipLaunchKernelGGL(k_interp<double>, dim3(nb), dim3(PT), 0, s, (const double*)uphys, pc.N, pc.pi2,
-                           pb.pos, order, pk, pb.vp, pc.n, pb.cap, (const double*)pb.vel, (const double*)pb.pert, x0, w);
+                           pb.pos, order, pk, pb.vp, pc.n, pb.cap, us);
     else
         hipLaunchKernelGGL(k_interp<float>, dim3(nb), dim3(PT), 0, s, (const float*)uphys, pc.N, pc.pi2,
-                           pb.pos, order, pk, pb.vp, pc.n, pb.cap, (const double*)pb.vel, (const double*)pb.pert, x0, w);
+                           pb.pos, order, pk, pb.vp, pc.n, pb.cap, us);
+    if (order)
+        hipLaunchKernelGGL(k_interp_scatter, dim3(nblk(pc.n)), dim3(PT), 0, s, order, (const double*)us, pb.vp, pc.n,
+                           pb.cap, (const double*)pb.vel, (const double*)pb.pert, pc.hdi ? pb.gm_x0 : nullptr,
+                           pc.hdi ? pb.gm_w : nullptr);
     DDNS_HIP(hipGetLastError());
     return 0;
 }

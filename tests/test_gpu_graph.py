@@ -61,3 +61,25 @@ def test_step_with_droplets_graph_equals_eager(ddns, monkeypatch):
     for a, b in zip(g[2], e[2]):
         assert np.array_equal(a, b)
     assert all(x > 0 for x in g[3].flow_kernel_ms[:5])     # the graph's timing events
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+def test_z_stage_gated_on_pair_launch_equals_default(ddns, monkeypatch, graph):
+    """DDNS_Z_GATE=1 (the default) queues the flow's second half in two pieces, the z
+    stage after the droplet block's collision-pair launch; DDNS_Z_GATE=0 queues it in
+    one.  Only the order of launches changes."""
+    N = 32
+    p = ddns.default_params(N=N, npart=40000, nset=2, radius=[60.0, 90.0], forcing=1, dt=0.004, rnu=0.02,
+                            nonoverlap=1, hdi_included=1, hdi_trunc=10.0)
+    v = random_spectral(N, seed=9, kmax=6, amp=4.0)
+    monkeypatch.setenv("DDNS_Z_GATE", "1")
+    g = _run(ddns, monkeypatch, graph, p, v, 4, True)
+    monkeypatch.setenv("DDNS_Z_GATE", "0")
+    e = _run(ddns, monkeypatch, graph, p, v, 4, True)
+    for a, b in zip(g[0][0] + g[0][1], e[0][0] + e[0][1]):
+        assert np.array_equal(a, b)
+    for k in ("yp", "fvp", "fdp", "pertvel"):
+        assert np.array_equal(g[1][k], e[1][k]), k
+    for a, b in zip(g[2], e[2]):
+        assert np.array_equal(a, b)
+    assert all(x > 0 for x in g[3].flow_kernel_ms[:5])
