@@ -110,7 +110,10 @@ __global__ void __launch_bounds__(NT_NEI) k_nei(const double4* __restrict__ pk, 
 // y = x + M x (AFUN_PRECOMP).  MODE 0: x = x0 in double, y0d = A x0 and b = M w.
 // MODE 1: x is a basis vector.  MODE 2: x = v / h for the previous work vector v; the
 // row's own entries of x are also stored as the next basis vector (q = v / h,
-// partlib.f90:2897).
+// partlib.f90:2897).  In modes 1 and 2 the Arnoldi step runs on M, y = M x: A = I + M
+// spans the same Krylov spaces and its Hessenberg matrix is M's plus the identity
+// (added to H(it, it) with the first projection), so the projections no longer cancel
+// the large identity part of A x against q_it.
 template <typename T, int MODE>
 __global__ void __launch_bounds__(HT) k_spmv(const int* __restrict__ cnt, const int* __restrict__ idx,
                                              const T* __restrict__ mm, int n, const T* __restrict__ x,
@@ -152,7 +155,7 @@ __global__ void __launch_bounds__(HT) k_spmv(const int* __restrict__ cnt, const 
         y0d[t] = r0 + s0; y0d[N + t] = r1 + s1; y0d[2 * N + t] = r2 + s2;
         b[t] = b0; b[N + t] = b1; b[2 * N + t] = b2;
     } else {
-        y[t] = (T)(r0 + s0); y[N + t] = (T)(r1 + s1); y[2 * N + t] = (T)(r2 + s2);
+        y[t] = (T)r0; y[N + t] = (T)r1; y[2 * N + t] = (T)r2;   // M x: the identity is added to H
         if (MODE == 2) { qout[t] = (T)s0; qout[N + t] = (T)s1; qout[2 * N + t] = (T)s2; }
     }
 }
@@ -227,7 +230,9 @@ __global__ void k_scale(T* q, const double* den, long long n)
     if (i < n) q[i] = (T)((double)q[i] / *den);
 }
 
-// ---- Arnoldi step: classical Gram-Schmidt applied twice.  Each projection kernel
+// ---- Arnoldi step: classical Gram-Schmidt applied twice, the second pass's update
+// only where the first pass lost more than 1 - 1/sqrt(2) of the vector's norm (the
+// DGKS criterion: otherwise its corrections are below rounding).  Each projection kernel
 // runs on a fixed grid of P <= RED_BLOCKS work-groups and writes one partial sum per
 // work-group and column (a fixed tree inside the work-group); the kernel that needs
 // the sums adds the P partials itself, in a fixed order, at its start.  Every result
@@ -258,7 +263,8 @@ __device__ __forceinline__ void block_sums(double (&acc)[J], double (*sh)[J], do
 
 // hs[j] = sum over b < P of part[j * P + b] for j < k (lane-strided in order, then a
 // fixed wavefront tree), the same in every work-group.  Work-group 0 also sets
-// (add = 0) or increments (add = 1) the Hessenberg column hcol.
+// (add = 0, with A's identity added to H(k, k)) or increments (add = 1) the Hessenberg
+// column hcol.
 __device__ void sum_partials(const double* __restrict__ part, int P, int k, double* hs, double* hcol, int add)
 {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -268,7 +274,7 @@ __device__ void sum_partials(const double* __restrict__ part, int P, int k, doub
         acc = wave_sum(acc);
         if (lane == 0) {
             hs[j] = acc;
-            if (blockIdx.x == 0) hcol[j] = add ? hcol[j] + acc : acc;
+            if (blockIdx.x == 0) hcol[j] = add ? hcol[j] + acc : acc + (j == k - 1 ? 1.0 : 0.0);
         }
     }
     __syncthreads();
@@ -294,26 +300,30 @@ __device__ __forceinline__ void vst(T* p, const T (&x)[kvec<T>::W])
     *(V*)p = *(const V*)x;
 }
 
-// partials of Q_j . v for j < k
+// partials of Q_j . v for j < k, and of v . v as column k
 template <typename T>
 __global__ void __launch_bounds__(RT) k_mdot(const T* __restrict__ Q, const T* __restrict__ v, int k, long long n3,
                                              long long ld, double* __restrict__ part, const double* __restrict__ stop)
 {
     if (*stop != 0.0) return;
     constexpr int W = kvec<T>::W;
-    __shared__ double sh[RW][MD_J];
-    __shared__ double red[MD_J];
+    __shared__ double sh[RW][MD_J + 1];
+    __shared__ double red[MD_J + 1];
     const int tid = threadIdx.x;
     const long long nv = (n3 + W - 1) / W;
     for (int j0 = 0; j0 < k; j0 += MD_J) {
-        double acc[MD_J];
+        double acc[MD_J + 1];
 #pragma unroll
-        for (int jj = 0; jj < MD_J; ++jj) acc[jj] = 0.0;
+        for (int jj = 0; jj <= MD_J; ++jj) acc[jj] = 0.0;
         for (long long iv = (long long)blockIdx.x * RT + tid; iv < nv; iv += (long long)gridDim.x * RT) {
             const long long i = iv * W;
             if (i + W <= n3) {
                 T x[W];
                 vld(v + i, x);
+                if (j0 == 0) {
+#pragma unroll
+                    for (int e = 0; e < W; ++e) acc[MD_J] += (double)x[e] * (double)x[e];
+                }
 #pragma unroll
                 for (int jj = 0; jj < MD_J; ++jj)
                     if (j0 + jj < k) {
@@ -323,19 +333,23 @@ __global__ void __launch_bounds__(RT) k_mdot(const T* __restrict__ Q, const T* _
                         for (int e = 0; e < W; ++e) acc[jj] += (double)q[e] * (double)x[e];
                     }
             } else {
-                for (long long e = i; e < n3; ++e)
+                for (long long e = i; e < n3; ++e) {
+                    if (j0 == 0) acc[MD_J] += (double)v[e] * (double)v[e];
 #pragma unroll
                     for (int jj = 0; jj < MD_J; ++jj)
                         if (j0 + jj < k) acc[jj] += (double)Q[(size_t)(j0 + jj) * ld + e] * (double)v[e];
+                }
             }
         }
-        block_sums<MD_J>(acc, sh, red);
+        block_sums<MD_J + 1>(acc, sh, red);
         if (tid < MD_J && j0 + tid < k) part[(size_t)(j0 + tid) * gridDim.x + blockIdx.x] = red[tid];
+        if (j0 == 0 && tid == 0) part[(size_t)k * gridDim.x + blockIdx.x] = red[MD_J];
     }
 }
 
 // First pass's update fused with the second pass's projections, for k <= KC columns:
 // h = sum of part_in (first pass), v -= sum_j h_j Q_j, partials of Q_j . v into part_out
+// and of v . v (the updated v) as its column k
 template <typename T, int KC>
 __global__ void __launch_bounds__(RT) k_maxpy_mdot(T* __restrict__ v, const T* __restrict__ Q, int k, long long n3,
                                                    long long ld, const double* __restrict__ part_in,
@@ -345,13 +359,13 @@ __global__ void __launch_bounds__(RT) k_maxpy_mdot(T* __restrict__ v, const T* _
     if (*stop != 0.0) return;
     constexpr int W = kvec<T>::W;
     __shared__ double hs[KC];
-    __shared__ double sh[RW][KC];
-    __shared__ double red[KC];
+    __shared__ double sh[RW][KC + 1];
+    __shared__ double red[KC + 1];
     const int tid = threadIdx.x;
     sum_partials(part_in, gridDim.x, k, hs, hcol, 0);
-    double acc[KC];
+    double acc[KC + 1];
 #pragma unroll
-    for (int jj = 0; jj < KC; ++jj) acc[jj] = 0.0;
+    for (int jj = 0; jj <= KC; ++jj) acc[jj] = 0.0;
     const long long nv = (n3 + W - 1) / W;
     for (long long iv = (long long)blockIdx.x * RT + tid; iv < nv; iv += (long long)gridDim.x * RT) {
         const long long i = iv * W;
@@ -370,7 +384,10 @@ __global__ void __launch_bounds__(RT) k_maxpy_mdot(T* __restrict__ v, const T* _
                     for (int e = 0; e < W; ++e) x[e] -= hs[jj] * (double)q[jj][e];
                 }
 #pragma unroll
-            for (int e = 0; e < W; ++e) xv[e] = (T)x[e];
+            for (int e = 0; e < W; ++e) {
+                xv[e] = (T)x[e];
+                acc[KC] += (double)xv[e] * (double)xv[e];
+            }
             vst(v + i, xv);
 #pragma unroll
             for (int jj = 0; jj < KC; ++jj)
@@ -384,29 +401,124 @@ __global__ void __launch_bounds__(RT) k_maxpy_mdot(T* __restrict__ v, const T* _
                 for (int jj = 0; jj < k; ++jj) x -= hs[jj] * (double)Q[(size_t)jj * ld + e];
                 const T xt = (T)x;
                 v[e] = xt;
+                acc[KC] += (double)xt * (double)xt;
 #pragma unroll
                 for (int jj = 0; jj < KC; ++jj)
                     if (jj < k) acc[jj] += (double)Q[(size_t)jj * ld + e] * (double)xt;
             }
         }
     }
-    block_sums<KC>(acc, sh, red);
+    block_sums<KC + 1>(acc, sh, red);
     if (tid < k) part_out[(size_t)tid * gridDim.x + blockIdx.x] = red[tid];
+    if (tid == 0) part_out[(size_t)k * gridDim.x + blockIdx.x] = red[KC];
 }
 
-// h = sum of part_in, v -= sum_j h[j] Q_j; with normpart, also the partials of |v|^2
+// sum over b < P of part[b] (lane-strided in order, then the wavefront tree), the same
+// in every work-group; all threads get it
+__device__ __forceinline__ double sum_one(const double* __restrict__ part, int P, double* sh1)
+{
+    if (threadIdx.x < 64) {
+        double acc = 0.0;
+        for (int b = threadIdx.x; b < P; b += 64) acc += part[b];
+        acc = wave_sum(acc);
+        if (threadIdx.x == 0) *sh1 = acc;
+    }
+    __syncthreads();
+    return *sh1;
+}
+
+// The product of an Arnoldi step fused with the first pass's projections, for k <= KC
+// columns: y = x + M x as k_spmv<T, MODE> (MODE 1: x = Q_0; MODE 2: x = v / h, stored as
+// Q_(k-1)), and the partials of Q_j . y for j < k and of y . y (column k).  The grid is
+// the projection kernels' (P <= RED_BLOCKS work-groups); work-group b takes a contiguous
+// run of rows, the runs placed so that neighbouring ones share an XCD.
+template <typename T, int MODE, int KC>
+__global__ void __launch_bounds__(RT) k_spmv_mdot(const int* __restrict__ cnt, const int* __restrict__ idx,
+                                                  const T* __restrict__ mm, int n, const T* __restrict__ x,
+                                                  const double* __restrict__ hx, T* __restrict__ qout,
+                                                  T* __restrict__ y, const T* __restrict__ Q, long long ld, int k,
+                                                  double* __restrict__ part, const double* __restrict__ stop)
+{
+    if (*stop != 0.0) return;   // converged in an earlier iteration of this batch
+    __shared__ double sh[RW][KC + 1];
+    __shared__ double red[KC + 1];
+    const size_t N = (size_t)n;
+    const double h = MODE == 2 ? *hx : 1.0;
+    auto xv = [&](size_t q) -> double {
+        if (MODE == 1) return (double)x[q];
+        return (double)(T)((double)x[q] / h);
+    };
+    double acc[KC + 1];
+#pragma unroll
+    for (int jj = 0; jj <= KC; ++jj) acc[jj] = 0.0;
+    const int lb = xcd_chunk(blockIdx.x, gridDim.x);
+    const int per = (n + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int t1 = min(n, (lb + 1) * per);
+    for (int t = lb * per + threadIdx.x; t < t1; t += RT) {
+        const int kk = cnt[t];
+        double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+        for (int sl = 0; sl < kk; ++sl) {
+            const size_t q = (size_t)sl * N + t;
+            const int j = idx[q];
+            const T* m = mm + (size_t)(6 * sl) * N + t;
+            const double m0 = m[0], m1 = m[N], m2 = m[2 * N], m3 = m[3 * N], m4 = m[4 * N], m5 = m[5 * N];
+            const double v0 = xv(j), v1 = xv(N + j), v2 = xv(2 * N + j);
+            r0 = r0 + m0 * v0 + m1 * v1 + m2 * v2;
+            r1 = r1 + m1 * v0 + m3 * v1 + m4 * v2;
+            r2 = r2 + m2 * v0 + m4 * v1 + m5 * v2;
+        }
+        const double s0 = xv(t), s1 = xv(N + t), s2 = xv(2 * N + t);
+        const T y0 = (T)r0, y1 = (T)r1, y2 = (T)r2;   // M x: the identity is added to H
+        y[t] = y0; y[N + t] = y1; y[2 * N + t] = y2;
+        if (MODE == 2) { qout[t] = (T)s0; qout[N + t] = (T)s1; qout[2 * N + t] = (T)s2; }
+        const double d0 = (double)y0, d1 = (double)y1, d2 = (double)y2;
+        acc[KC] += d0 * d0 + d1 * d1 + d2 * d2;
+#pragma unroll
+        for (int jj = 0; jj < KC; ++jj)
+            if (jj < k) {
+                double q0, q1, q2;
+                if (jj == k - 1) {      // this row of x itself (Q_0 in MODE 1, the new column in MODE 2)
+                    q0 = (double)(T)s0; q1 = (double)(T)s1; q2 = (double)(T)s2;
+                } else {
+                    const T* qc = Q + (size_t)jj * ld;
+                    q0 = (double)qc[t]; q1 = (double)qc[N + t]; q2 = (double)qc[2 * N + t];
+                }
+                acc[jj] += q0 * d0 + q1 * d1 + q2 * d2;
+            }
+    }
+    block_sums<KC + 1>(acc, sh, red);
+    if ((int)threadIdx.x < k) part[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = red[threadIdx.x];
+    if (threadIdx.x == 0) part[(size_t)k * gridDim.x + blockIdx.x] = red[KC];
+}
+
+// h = sum of part_in, v -= sum_j h[j] Q_j; with normpart, also the partials of |v|^2.
+// With gate (the second pass's update): gate[0 .. P) and gate[P .. 2P) hold the partials
+// of |w|^2 before the first pass and |v|^2 after it; when the first pass kept at least
+// 1/sqrt(2) of the norm (|v|^2 >= |w|^2 / 2, the DGKS criterion) the second pass's
+// corrections are below rounding and its update is skipped: v, hcol stay, and normpart
+// takes the partials of |v|^2 already summed.
 template <typename T>
 __global__ void __launch_bounds__(RT) k_maxpy(T* __restrict__ v, const T* __restrict__ Q, int k, long long n3,
                                               long long ld, const double* __restrict__ part_in,
                                               double* __restrict__ hcol, int add, double* __restrict__ normpart,
-                                              const double* __restrict__ stop)
+                                              const double* __restrict__ stop, const double* __restrict__ gate_w,
+                                              const double* __restrict__ gate_v)
 {
     if (*stop != 0.0) return;
     constexpr int W = kvec<T>::W;
     __shared__ double hs[64];
     __shared__ double sh[RW][1];
     __shared__ double red[1];
+    __shared__ double g1[2];
     const int tid = threadIdx.x;
+    if (gate_w) {
+        const double nw = sum_one(gate_w, gridDim.x, &g1[0]);
+        const double nv2 = sum_one(gate_v, gridDim.x, &g1[1]);
+        if (nv2 >= 0.5 * nw) {
+            if (normpart && tid == 0) normpart[blockIdx.x] = gate_v[blockIdx.x];
+            return;
+        }
+    }
     sum_partials(part_in, gridDim.x, k, hs, hcol, add);
     double nacc[1] = {0.0};
     const long long nv = (n3 + W - 1) / W;
@@ -757,21 +869,34 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
             const int last = std::min(maxiter, iter + (iter == 0 ? std::max(BATCH, pb.gm_last_iters) : BATCH));
             for (int it = iter + 1; it <= last; ++it) {
                 T* v = work[it & 1];
+                // v = A q_it, q_it = v_(it-1) / H(it, it-1) formed inside the product; up to
+                // 12 columns the first pass's projections are summed in the same kernel
+                const T* xin = it == 1 ? (const T*)Q : (const T*)work[(it - 1) & 1];
+                const double* hx = it == 1 ? (const double*)nullptr : (const double*)(Hd + (size_t)M1 * (it - 2) + (it - 1));
+                T* qo = it == 1 ? (T*)nullptr : Q + (size_t)(it - 1) * ld;
+                const int* nc = (const int*)pb.nei_cnt;
+                const int* ni = (const int*)pb.nei_idx;
+                const T* nm = (const T*)pb.nei_m;
                 if (it == 1)
-                    hipLaunchKernelGGL((k_spmv<T, 1>), dim3(nbr), dim3(HT), 0, s, (const int*)pb.nei_cnt,
-                                       (const int*)pb.nei_idx, (const T*)pb.nei_m, n, (const T*)Q, (const double*)nullptr,
-                                       (T*)nullptr, v, (const double*)nullptr, (double*)nullptr, (const double*)nullptr,
-                                       (double*)nullptr, stop);
-                else    // q_it = v_(it-1) / H(it, it-1), formed inside the product
-                    hipLaunchKernelGGL((k_spmv<T, 2>), dim3(nbr), dim3(HT), 0, s, (const int*)pb.nei_cnt,
-                                       (const int*)pb.nei_idx, (const T*)pb.nei_m, n, (const T*)work[(it - 1) & 1],
-                                       (const double*)(Hd + (size_t)M1 * (it - 2) + (it - 1)), Q + (size_t)(it - 1) * ld, v,
+                    hipLaunchKernelGGL((k_spmv_mdot<T, 1, 8>), dim3(nred), dim3(RT), 0, s, nc, ni, nm, n, xin, hx, qo, v,
+                                       (const T*)Q, ld, it, part1, stop);
+                else if (it <= 8)
+                    hipLaunchKernelGGL((k_spmv_mdot<T, 2, 8>), dim3(nred), dim3(RT), 0, s, nc, ni, nm, n, xin, hx, qo, v,
+                                       (const T*)Q, ld, it, part1, stop);
+                else if (it <= 12)
+                    hipLaunchKernelGGL((k_spmv_mdot<T, 2, 12>), dim3(nred), dim3(RT), 0, s, nc, ni, nm, n, xin, hx, qo,
+                                       v, (const T*)Q, ld, it, part1, stop);
+                else {
+                    hipLaunchKernelGGL((k_spmv<T, 2>), dim3(nbr), dim3(HT), 0, s, nc, ni, nm, n, xin, hx, qo, v,
                                        (const double*)nullptr, (double*)nullptr, (const double*)nullptr, (double*)nullptr,
                                        stop);
+                    hipLaunchKernelGGL((k_mdot<T>), dim3(nred), dim3(RT), 0, s, (const T*)Q, (const T*)v, it, n3, ld, part1,
+                                       stop);
+                }
                 // orthogonalise v against q_1..q_it: classical Gram-Schmidt, twice (the
-                // reference's modified Gram-Schmidt, 2886-2894, in two block passes)
+                // reference's modified Gram-Schmidt, 2886-2894, in two block passes), the
+                // second pass's update only where the DGKS criterion asks for it
                 double* hcol = Hd + (size_t)M1 * (it - 1);
-                hipLaunchKernelGGL((k_mdot<T>), dim3(nred), dim3(RT), 0, s, (const T*)Q, (const T*)v, it, n3, ld, part1, stop);
                 if (it <= 8)
                     hipLaunchKernelGGL((k_maxpy_mdot<T, 8>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, it, n3, ld,
                                        (const double*)part1, hcol, part2, stop);
@@ -780,12 +905,15 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
                                        (const double*)part1, hcol, part2, stop);
                 else {
                     hipLaunchKernelGGL((k_maxpy<T>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, it, n3, ld,
-                                       (const double*)part1, hcol, 0, (double*)nullptr, stop);
+                                       (const double*)part1, hcol, 0, (double*)nullptr, stop, (const double*)nullptr,
+                                       (const double*)nullptr);
                     hipLaunchKernelGGL((k_mdot<T>), dim3(nred), dim3(RT), 0, s, (const T*)Q, (const T*)v, it, n3, ld, part2,
                                        stop);
                 }
+                // the second pass's update, skipped where the first kept 1/sqrt(2) of |v|
                 hipLaunchKernelGGL((k_maxpy<T>), dim3(nred), dim3(RT), 0, s, v, (const T*)Q, it, n3, ld, (const double*)part2,
-                                   hcol, 1, normpart, stop);
+                                   hcol, 1, normpart, stop, (const double*)(part1 + (size_t)it * nred),
+                                   (const double*)(part2 + (size_t)it * nred));
                 hipLaunchKernelGGL(k_ls, dim3(1), dim3(LT), 0, s, sm, L, it, (const double*)normpart, (int)nred);
             }
             DDNS_HIP(hipGetLastError());
