@@ -573,7 +573,6 @@ __global__ void __launch_bounds__(CT) k_overlap(OverArgs a)
 }
 
 // ------------------------------------------------------------------ host side
-int exclusive_scan(const int* in, int* out, int n, int* tmp, hipStream_t s);
 int relocate_by_mult(PartBuffers& pb, const PartConst& pc, int* total_host, hipStream_t s);
 
 static PairGeom make_geom(const PartBuffers& pb, const PartConst& pc, double dstat, double dist, int range)
@@ -736,34 +735,45 @@ int part_read_contacts(const PartBuffers& pb, double* rec, int cap, int* n, hipS
     return 0;
 }
 
-int part_overlap(PartBuffers& pb, const PartConst& pc, int* sweeps, int* relocated, hipStream_t s)
+// the largest radius into rbig[1] (relocation moves droplets, it does not resize them)
+static int overlap_setup(PartBuffers& pb, const PartConst& pc, hipStream_t s)
 {
-    *sweeps = 0;
-    *relocated = 0;
-    if (pc.n == 0) return 0;
-    const double d_overl = 2.0 * pc.radmax;                                // 2063
-    // the largest radius ([1]; relocation moves droplets, it does not resize them)
     unsigned long long* rbig = (unsigned long long*)(pb.flags + 36);
     DDNS_HIP(hipMemsetAsync(rbig, 0, 2 * sizeof(unsigned long long), s));
     hipLaunchKernelGGL(k_extent_max, dim3(std::min(256, (pc.n + 255) / 256)), dim3(256), 0, s, (const double*)nullptr,
                        (const double*)pb.rad, pc.n, pb.cap, rbig);
-    for (int it = 0; it < 10000; ++it) {
-        // two droplets in adjacent reference cells (the window of partlib.f90:2091) are
-        // less than 2 dx apart per component, so cells of width >= 2 dx keep every such
-        // pair within the 27-cell neighbourhood
-        int st = part_build_cells(pb, pc, std::max(d_overl, 2.0 * pc.pi2 / pc.N), s);
+    return 0;
+}
+
+// One sweep's search: the overlap grid, and the per-droplet overlap counts in mult
+static int overlap_search(PartBuffers& pb, const PartConst& pc, hipStream_t s)
+{
+    const double d_overl = 2.0 * pc.radmax;                                // 2063
+    // two droplets in adjacent reference cells (the window of partlib.f90:2091) are
+    // less than 2 dx apart per component, so cells of width >= 2 dx keep every such
+    // pair within the 27-cell neighbourhood
+    int st = part_build_cells(pb, pc, std::max(d_overl, 2.0 * pc.pi2 / pc.N), s);
+    if (st) return st;
+    OverArgs a{};
+    a.pos = pb.pos; a.rad = pb.rad; a.pk = pb.pk; a.cell_start = pb.cell_start; a.sorted = pb.sorted;
+    a.n = pc.n; a.cap = pb.cap;
+    const int jmax = 1 + (int)(d_overl * (double)pc.N / pc.pi2);
+    a.g = make_geom(pb, pc, d_overl, d_overl, jmax);
+    a.rbig = (unsigned long long*)(pb.flags + 36) + 1;
+    a.cnt = pb.mult;
+    a.total = pb.flags + 1;
+    DDNS_HIP(hipMemsetAsync(pb.flags + 1, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_overlap, dim3((pc.n + CT - 1) / CT), dim3(CT), 0, s, a);
+    DDNS_HIP(hipGetLastError());
+    return 0;
+}
+
+// sweeps from the current positions until one relocates nothing, each relocation waited for
+static int overlap_loop(PartBuffers& pb, const PartConst& pc, int* sweeps, int* relocated, hipStream_t s)
+{
+    for (int it = *sweeps; it < 10000; ++it) {
+        int st = overlap_search(pb, pc, s);
         if (st) return st;
-        OverArgs a{};
-        a.pos = pb.pos; a.rad = pb.rad; a.pk = pb.pk; a.cell_start = pb.cell_start; a.sorted = pb.sorted;
-        a.n = pc.n; a.cap = pb.cap;
-        const int jmax = 1 + (int)(d_overl * (double)pc.N / pc.pi2);
-        a.g = make_geom(pb, pc, d_overl, d_overl, jmax);
-        a.rbig = rbig + 1;
-        a.cnt = pb.mult;
-        a.total = pb.flags + 1;
-        DDNS_HIP(hipMemsetAsync(pb.flags + 1, 0, sizeof(int), s));
-        hipLaunchKernelGGL(k_overlap, dim3((pc.n + CT - 1) / CT), dim3(CT), 0, s, a);
-        DDNS_HIP(hipGetLastError());
         int total = 0;
         st = relocate_by_mult(pb, pc, &total, s);
         if (st) return st;
@@ -773,6 +783,59 @@ int part_overlap(PartBuffers& pb, const PartConst& pc, int* sweeps, int* relocat
     }
     set_error("OVERLAP_DET did not terminate after 10000 sweeps");
     return DDNS_ERR_OVERLAP;
+}
+
+int part_overlap(PartBuffers& pb, const PartConst& pc, int* sweeps, int* relocated, hipStream_t s)
+{
+    *sweeps = 0;
+    *relocated = 0;
+    if (pc.n == 0) return 0;
+    int st = overlap_setup(pb, pc, s);
+    if (st) return st;
+    return overlap_loop(pb, pc, sweeps, relocated, s);
+}
+
+// The step's sweep without a host wait where it finds nothing to relocate (the usual
+// case): the first sweep and its relocation are queued, the draws made for whatever the
+// count is (as in the collision step's relocation: none for a count of 0), and the count
+// is copied to pinned memory for the step's final wait.
+int part_overlap_queue(PartBuffers& pb, const PartConst& pc, hipStream_t s)
+{
+    pb.overlap_queued = false;
+    if (pc.n == 0) return 0;
+    int st = overlap_setup(pb, pc, s);
+    if (st) return st;
+    st = overlap_search(pb, pc, s);
+    if (st) return st;
+    const bool kept = pb.cells_valid, spare_kept = pb.spare.valid;
+    st = relocate_async(pb, pc, s);          // invalidates the grids: positions may move
+    if (st) return st;
+    pb.overlap_cells_kept = kept;
+    pb.overlap_spare_kept = spare_kept;
+    if (!pb.counts_host)
+        DDNS_HIP(hipHostMalloc((void**)&pb.counts_host, (512 + 8 + 4) * sizeof(int), hipHostMallocDefault));
+    DDNS_HIP(hipMemcpyAsync(pb.counts_host + 521, pb.draw_off + pc.n, sizeof(int), hipMemcpyDeviceToHost, s));
+    pb.overlap_queued = true;
+    return 0;
+}
+
+// After the caller's wait: a first sweep that relocated nothing leaves the positions and
+// the grids as they were; one that did is followed by the sweeps of part_overlap.
+int part_overlap_finish(PartBuffers& pb, const PartConst& pc, int* sweeps, int* relocated, hipStream_t s)
+{
+    *sweeps = 0;
+    *relocated = 0;
+    if (!pb.overlap_queued) return 0;
+    pb.overlap_queued = false;
+    const int total = pb.counts_host[521];
+    if (total == 0) {
+        pb.cells_valid = pb.overlap_cells_kept;
+        pb.spare.valid = pb.overlap_spare_kept;
+        return 0;
+    }
+    *sweeps = 1;
+    *relocated = total;
+    return overlap_loop(pb, pc, sweeps, relocated, s);
 }
 
 // canonical pair list: sort the packed keys (min gid, max gid, type)

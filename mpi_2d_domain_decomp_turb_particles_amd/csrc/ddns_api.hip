@@ -768,7 +768,8 @@ static int queue_flow_rest(ddns_handle* h, hipEvent_t after)
 }
 
 // The z stage and the symmetry pass of a second half queued in two pieces, after event
-// `after` of the droplet stream (the collision-pair launch).  A pair kernel queued while
+// `after` of the droplet stream (the collision-pair launch); the droplet block queues
+// the next step's first half after it with queue_flow_next.  A pair kernel queued while
 // the z stage ran made no progress until the z stage ended (35 us alone, 400-460 us
 // then, in a quarter of the steps at 256^3 with 1M droplets); forcing the order cost the
 // flow less than that on average.
@@ -782,7 +783,7 @@ static int queue_flow_z(ddns_handle* h, hipEvent_t after)
     if (st) return st;
     DDNS_HIP(hipEventRecord(h->flev[q][3], s));
     h->z_pending = false;
-    return queue_flow_next(h, after);
+    return 0;
 }
 
 // The next step's first half can follow at once: it overwrites the physical velocity,
@@ -850,7 +851,8 @@ static int droplet_block(ddns_handle* h, float* ms)
     if (st) return st;
     st = part_collide_finish(h->pb, pc, s);
     if (st) return st;
-    if (h->z_pending) {
+    const bool gated = h->z_pending;
+    if (gated) {
         DDNS_HIP(hipEventRecord(h->ev[6], s));
         st = queue_flow_z(h, h->ev[6]);
         if (st) return st;
@@ -869,18 +871,34 @@ static int droplet_block(ddns_handle* h, float* ms)
         pc.n = h->n;
         if (h->n < h->p.npart / 2) return fail(h, DDNS_ERR_DROPLET_LOSS, "npart_t < npart/2 (turb.f90:600-604)");
     }
-    DDNS_HIP(hipEventRecord(h->ev[13], s));
-    if (h->p.nonoverlap) {                                                    // 610
-        int sw = 0, rl = 0;
-        st = part_overlap(h->pb, pc, &sw, &rl, s);
+    // the next step's first half, queued on the flow stream behind the z stage once the
+    // droplet work above is queued (a graph launch holds the host for tens of us)
+    if (gated) {
+        st = queue_flow_next(h, h->ev[6]);
         if (st) return st;
-        h->cc.noverlap_sweeps = sw;
-        h->cc.noverlap_relocated = rl;
+    }
+    DDNS_HIP(hipEventRecord(h->ev[13], s));
+    // OVERLAP_DET (610): the first sweep is queued; its count is read with the step's
+    // counts, and only a sweep that relocated droplets is followed by more
+    if (h->p.nonoverlap) {
+        st = part_overlap_queue(h->pb, pc, s);
+        if (st) return st;
     }
     st = part_queue_counts(h->pb, s);
     if (st) return st;
     DDNS_HIP(hipEventRecord(h->ev[14], s));
     DDNS_HIP(hipEventSynchronize(h->ev[14]));
+    if (h->p.nonoverlap) {
+        int sw = 0, rl = 0;
+        st = part_overlap_finish(h->pb, pc, &sw, &rl, s);
+        if (st) return st;
+        h->cc.noverlap_sweeps = sw;
+        h->cc.noverlap_relocated = rl;
+        if (sw > 0) {                    // the further sweeps belong to the overlap phase
+            DDNS_HIP(hipEventRecord(h->ev[14], s));
+            DDNS_HIP(hipEventSynchronize(h->ev[14]));
+        }
+    }
     float t;
     DDNS_HIP(hipEventElapsedTime(&t, h->ev[8], h->ev[9])); ms[0] += t;
     DDNS_HIP(hipEventElapsedTime(&t, h->ev[9], h->ev[10])); ms[1] += t;

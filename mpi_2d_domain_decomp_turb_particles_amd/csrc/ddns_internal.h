@@ -304,6 +304,11 @@ struct PartBuffers {
     double4* pk = nullptr;      // cap: (x, y, z, radius) of sorted[p], refreshed with the cells
     int* cell_rank = nullptr;   // cap: place of a droplet among those of its cell (build scratch)
     bool cells_valid = false;   // the grid above was built on the current positions
+    long long cell_count_zero = 0;  // cell_count[0 .. this) is zero (the last build's scan clears it)
+    // the step's first overlap sweep is queued and its count not yet read; the validity
+    // of the grids before its relocation, restored when it relocated nothing
+    bool overlap_queued = false;
+    bool overlap_cells_kept = false, overlap_spare_kept = false;
     // a second grid, kept from the last build of another width while it stays valid:
     // a search at that width then swaps it in instead of rebuilding
     struct CellSet {
@@ -312,6 +317,7 @@ struct PartBuffers {
         int *cell_count = nullptr, *cell_start = nullptr, *cell_of = nullptr, *sorted = nullptr;
         double4* pk = nullptr;
         bool valid = false;
+        long long count_zero = 0;
     } spare;
     void* scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
@@ -382,10 +388,17 @@ int part_collide_search(PartBuffers& pb, const PartConst& pc, hipStream_t s);
 int part_collide_finish(PartBuffers& pb, const PartConst& pc, hipStream_t s);
 // queue the step's counts into pinned memory; read them after the stream was waited for
 int part_queue_counts(PartBuffers& pb, hipStream_t s);
+// exclusive scan of n ints into out[0..n-1], the total into out[n] (particles.hip);
+// clear_in leaves in[0..n-1] zero (in must not alias out then)
+int exclusive_scan(const int* in, int* out, int n, int* tmp, hipStream_t s, bool clear_in = false);
 int part_take_counts(const PartBuffers& pb, const PartConst& pc, ddns_collision_counts* cc);
 int part_update_and_wrap(const PartBuffers& pb, const PartConst& pc, bool relocate, hipStream_t s);
 int relocate_async(PartBuffers& pb, const PartConst& pc, hipStream_t s);
 int part_overlap(PartBuffers& pb, const PartConst& pc, int* sweeps, int* relocated, hipStream_t s);
+// the step's overlap sweep: the first sweep queued without a host wait; the rest, if it
+// relocated droplets, after the caller has waited for the stream
+int part_overlap_queue(PartBuffers& pb, const PartConst& pc, hipStream_t s);
+int part_overlap_finish(PartBuffers& pb, const PartConst& pc, int* sweeps, int* relocated, hipStream_t s);
 int part_read_counts(const PartBuffers& pb, const PartConst& pc, ddns_collision_counts* cc, hipStream_t s);
 int part_read_bins(const PartBuffers& pb, const PartConst& pc, int* nirbin, double* const bins[7], hipStream_t s);
 int part_read_pairs(PartBuffers& pb, const PartConst& pc, int64_t* triples, int cap, int* npairs, hipStream_t s);

@@ -30,7 +30,6 @@ constexpr int HT = 256;
 constexpr int RED_BLOCKS = 256;    // reduction grids: at most one work-group per CU
 constexpr int ELL_STEP = 8;        // the ELL width grows in multiples of this
 
-int exclusive_scan(const int* in, int* out, int n, int* tmp, hipStream_t s);
 
 // deterministic wavefront sum: lane 0 holds the butterfly result
 __device__ __forceinline__ double wave_sum(double x)

@@ -23,69 +23,136 @@ constexpr int CELL_GMAX = 256;   // finest pair-search grid per dimension
 static inline unsigned nblk(long long n, int t = PT) { return (unsigned)((n + t - 1) / t); }
 
 // ------------------------------------------------------------------ scans
-// exclusive scan of n ints (in may alias out); writes the total to out[n]
-__global__ void k_scan_block(const int* in, int* out, int* bsum, int n)
+// Exclusive scan of n ints in two launches: the totals of tiles of E * SCAN_T ints, then
+// each tile scanned with the sum of the totals before it (every work-group adds those
+// itself, at most a few thousand ints from L2).  No work-group waits for another.
+// Work-groups are small (256 threads, no LDS tile): a scan runs beside the flow kernels,
+// and a work-group of 1024 threads was seen to wait for a whole flow kernel to drain.
+constexpr int SCAN_T = 256;
+constexpr int SCAN_W = SCAN_T / 64;
+
+// the work-group's sum of x, returned to every thread
+__device__ __forceinline__ int block_sum_int(int x, int* sh)
 {
-    __shared__ int s[1024];
-    const int base = blockIdx.x * 1024;
-    const int t = threadIdx.x;
-    int v[4];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) sh[wv] = x;
+    __syncthreads();
+    int t = 0;
+#pragma unroll
+    for (int w = 0; w < SCAN_W; ++w) t += sh[w];
+    __syncthreads();
+    return t;
+}
+
+// thread tid's E consecutive ints of the tile (zero past n)
+template <int E>
+__device__ __forceinline__ void scan_load(const int* in, long long i0, int n, int (&v)[E])
+{
+    if (i0 + E <= n) {
+#pragma unroll
+        for (int q = 0; q < E; q += 4) {
+            const int4 x = *(const int4*)(in + i0 + q);
+            v[q] = x.x; v[q + 1] = x.y; v[q + 2] = x.z; v[q + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < E; ++q) v[q] = i0 + q < n ? in[i0 + q] : 0;
+    }
+}
+
+template <int E>
+__global__ void __launch_bounds__(SCAN_T) k_scan_tiles(const int* __restrict__ in, int* __restrict__ bsum, int n)
+{
+    __shared__ int sh[SCAN_W];
+    int v[E];
+    scan_load<E>(in, ((long long)blockIdx.x * SCAN_T + threadIdx.x) * E, n, v);
     int tot = 0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int i = base + t * 4 + q;
-        v[q] = i < n ? in[i] : 0;
-        tot += v[q];
-    }
-    s[t] = tot;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        const int x = t >= off ? s[t - off] : 0;
-        __syncthreads();
-        s[t] += x;
-        __syncthreads();
-    }
-    int run = s[t] - tot;
+    for (int q = 0; q < E; ++q) tot += v[q];
+    tot = block_sum_int(tot, sh);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// in may alias out: a thread reads its ints before it writes them.  zin (= in, not
+// aliasing out, or null): the input is cleared as it is read.
+template <int E>
+__global__ void __launch_bounds__(SCAN_T) k_scan_apply(const int* in, int* out, const int* __restrict__ bsum, int n,
+                                                       int* zin)
+{
+    __shared__ int sh[SCAN_W];
+    __shared__ int wsum[SCAN_W];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int pre = 0;
+    for (int b = tid; b < (int)blockIdx.x; b += SCAN_T) pre += bsum[b];
+    pre = block_sum_int(pre, sh);
+    const long long i0 = ((long long)blockIdx.x * SCAN_T + tid) * E;
+    int v[E];
+    scan_load<E>(in, i0, n, v);
+    if (zin) {
+        if (i0 + E <= n) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int i = base + t * 4 + q;
-        if (i < n) out[i] = run;
+            for (int q = 0; q < E; q += 4) *(int4*)(zin + i0 + q) = make_int4(0, 0, 0, 0);
+        } else {
+            for (int q = 0; q < E; ++q)
+                if (i0 + q < n) zin[i0 + q] = 0;
+        }
+    }
+    int tot = 0;
+#pragma unroll
+    for (int q = 0; q < E; ++q) tot += v[q];
+    int incl = tot;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    int run = pre + incl - tot;
+#pragma unroll
+    for (int w = 0; w < SCAN_W; ++w)
+        if (w < wv) run += wsum[w];
+    int r[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        r[q] = run;
         run += v[q];
     }
-    if (t == 255) bsum[blockIdx.x] = s[255];
-}
-
-__global__ void k_scan_add(int* out, const int* bsum, int n, int* total_out, const int* total_in)
-{
-    const int i = blockIdx.x * 1024 + threadIdx.x * 4;
-    const int add = bsum[blockIdx.x];
+    if (i0 + E <= n) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-        if (i + q < n) out[i + q] += add;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *total_out = *total_in;
-}
-
-__global__ void k_scan_total(const int* bsum, int* total_out)
-{
-    if (threadIdx.x == 0) *total_out = bsum[0];
-}
-
-// out[0..n-1] = exclusive scan of in, out[n] = total (in may alias out).  Block sums
-// are scanned recursively, so every level is a full-width launch; tmp holds
-// scan_tmp_ints(n) ints.
-int exclusive_scan(const int* in, int* out, int n, int* tmp, hipStream_t s)
-{
-    const int nb = (n + 1023) / 1024;
-    hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(256), 0, s, in, out, tmp, n);
-    if (nb == 1) {
-        hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, s, (const int*)tmp, out + n);
+        for (int q = 0; q < E; q += 4) *(int4*)(out + i0 + q) = make_int4(r[q], r[q + 1], r[q + 2], r[q + 3]);
     } else {
-        const int st = exclusive_scan(tmp, tmp, nb, tmp + nb + 1, s);
-        if (st) return st;
-        hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, s, out, (const int*)tmp, n, out + n, (const int*)(tmp + nb));
+        for (int q = 0; q < E; ++q)
+            if (i0 + q < n) out[i0 + q] = r[q];
     }
+    if (blockIdx.x == gridDim.x - 1 && tid == SCAN_T - 1) out[n] = run;
+}
+
+template <int E>
+static int scan_launch(const int* in, int* out, int n, int* tmp, int* zin, hipStream_t s)
+{
+    const unsigned nb = (unsigned)(((long long)n + (long long)E * SCAN_T - 1) / ((long long)E * SCAN_T));
+    hipLaunchKernelGGL(k_scan_tiles<E>, dim3(nb), dim3(SCAN_T), 0, s, in, tmp, n);
+    hipLaunchKernelGGL(k_scan_apply<E>, dim3(nb), dim3(SCAN_T), 0, s, in, out, (const int*)tmp, n, zin);
     DDNS_HIP(hipGetLastError());
     return 0;
+}
+
+// out[0..n-1] = exclusive scan of in, out[n] = total (in may alias out); tmp holds
+// scan_tmp_ints(n) ints; in and out 16-byte aligned.  clear_in (in not aliasing out):
+// in[0..n-1] is left zero.
+int exclusive_scan(const int* in, int* out, int n, int* tmp, hipStream_t s, bool clear_in)
+{
+    int* zin = clear_in ? const_cast<int*>(in) : nullptr;
+    if (n <= 0) {
+        hipLaunchKernelGGL(k_scan_apply<4>, dim3(1), dim3(SCAN_T), 0, s, in, out, (const int*)tmp, 0, zin);
+        DDNS_HIP(hipGetLastError());
+        return 0;
+    }
+    if ((long long)n <= 16LL * SCAN_T * 1024) return scan_launch<16>(in, out, n, tmp, zin, s);
+    return scan_launch<32>(in, out, n, tmp, zin, s);
 }
 
 // ints of scratch exclusive_scan needs for n elements
@@ -658,6 +725,7 @@ static void swap_cells(PartBuffers& pb)
     std::swap(pb.G, o.G);
     std::swap(pb.hc, o.hc);
     std::swap(pb.cell_count, o.cell_count);
+    std::swap(pb.cell_count_zero, o.count_zero);
     std::swap(pb.cell_start, o.cell_start);
     std::swap(pb.cell_of, o.cell_of);
     std::swap(pb.sorted, o.sorted);
@@ -714,12 +782,19 @@ int part_build_cells(PartBuffers& pb, const PartConst& pc, double hmin, hipStrea
     pb.cells_valid = false;
     pb.G = G;
     pb.hc = pc.pi2 / G;
-    DDNS_HIP(hipMemsetAsync(pb.cell_count, 0, (ncell + 1) * sizeof(int), s));
+    // the counts are left zero by the scan of the last build into this buffer
+    if (pb.cell_count_zero < ncell + 1) {
+        DDNS_HIP(hipMemsetAsync(pb.cell_count, 0, (ncell + 1) * sizeof(int), s));
+        pb.cell_count_zero = ncell + 1;
+    }
+    const long long zero = pb.cell_count_zero;
+    pb.cell_count_zero = 0;
     if (pc.n > 0)
         hipLaunchKernelGGL(k_cell_key, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.pos, pb.cell_of, pb.cell_rank, pb.cell_count, pc.n,
                            pb.cap, G, pb.hc);
-    int st = exclusive_scan(pb.cell_count, pb.cell_start, (int)ncell, (int*)pb.scan_tmp, s);
+    int st = exclusive_scan(pb.cell_count, pb.cell_start, (int)ncell, (int*)pb.scan_tmp, s, true);
     if (st) return st;
+    pb.cell_count_zero = zero;
     if (pc.n > 0) {
         hipLaunchKernelGGL(k_cell_fill, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.cell_of, pb.cell_rank, pb.cell_start,
                            pb.sorted, pb.pos, pb.rad, pb.pk, pc.n, pb.cap);
