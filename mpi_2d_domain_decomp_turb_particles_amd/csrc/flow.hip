@@ -466,10 +466,21 @@ __global__ void __launch_bounds__(NT) k_y(KyArgs a)
     __shared__ C lds[N * LDR];
     __shared__ C tws[N];
     const int tid = threadIdx.x;
-    const int kx0 = blockIdx.x * TC;
-    const int z = blockIdx.y;
+    // tiles whose rows share a 128-byte line (P of them, side by side in kx) go to linear
+    // ids L, L+8, ..., i.e. to one XCD, where the second finds the line in L2
+    constexpr int P = (TC * (int)sizeof(C) >= 128) ? 1 : 128 / (TC * (int)sizeof(C));
+    int kxb = blockIdx.x, z = blockIdx.y, fi = blockIdx.z;
+    if constexpr (P > 1) {
+        if ((M / TC) % P == 0 && (gridDim.y * gridDim.z) % 8 == 0) {
+            const int t = xcd_tile<P>(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+            kxb = t % gridDim.x;
+            z = (t / gridDim.x) % gridDim.y;
+            fi = t / (gridDim.x * gridDim.y);
+        }
+    }
+    const int kx0 = kxb * TC;
     stage_twiddles<C, N, NT>(tws, (const C*)a.tw, tid);
-    C* f = (C*)a.F[blockIdx.z];
+    C* f = (C*)a.F[fi];
     const size_t base = (size_t)z * N * M + kx0;
     tile_load<C, N, TC, LDR, NT>(lds, f + base, M, tid);
     __syncthreads();
