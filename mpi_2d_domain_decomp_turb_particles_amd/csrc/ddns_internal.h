@@ -210,7 +210,7 @@ struct FlowBuffers {
     void* O[3] = {nullptr, nullptr, nullptr};  // AB2 history [kz][ky][kx]
     void* A[6] = {};                            // inverse pipeline: u(3), w(3)
     void* B[3] = {};                            // forward pipeline: (u x w)
-    void* uphys = nullptr;                      // [z][y][x][4]: u, v, w, unused
+    void* uphys = nullptr;                      // [z][y][x][3]: u, v, w
     void* tw = nullptr;                         // N twiddles exp(-2 pi i m / N)
     void* twh = nullptr;                        // N/2 twiddles for the half-length x FFT
     double* low = nullptr;                      // pre-kernel scratch (1024): ff[2], energies, forcing table, DFT sums

@@ -574,14 +574,20 @@ __device__ __forceinline__ void r2c_unpack_store(const cpx<T>* lds, const cpx<T>
     }
 }
 
-__device__ __forceinline__ void store_node(float* p, float a, float b, float c)
+// two neighbouring nodes (x = 2m, 2m + 1) of the physical velocity, three components
+// each, 6 values in a row: three 2-vector stores (p is 2-vector aligned)
+__device__ __forceinline__ void store_node_pair(float* p, float a0, float b0, float c0, float a1, float b1, float c1)
 {
-    *(float4*)p = make_float4(a, b, c, 0.f);
+    ((float2*)p)[0] = make_float2(a0, b0);
+    ((float2*)p)[1] = make_float2(c0, a1);
+    ((float2*)p)[2] = make_float2(b1, c1);
 }
-__device__ __forceinline__ void store_node(double* p, double a, double b, double c)
+__device__ __forceinline__ void store_node_pair(double* p, double a0, double b0, double c0, double a1, double b1,
+                                                double c1)
 {
-    ((double2*)p)[0] = make_double2(a, b);
-    ((double2*)p)[1] = make_double2(c, 0.0);
+    ((double2*)p)[0] = make_double2(a0, b0);
+    ((double2*)p)[1] = make_double2(c0, a1);
+    ((double2*)p)[2] = make_double2(b1, c1);
 }
 
 // Fused x stage: six C2R (u, w), physical u for the droplets, u x w, three R2C.
@@ -632,16 +638,16 @@ __global__ void __launch_bounds__(NT) k_x_fused(KxArgs a)
         }
         __syncthreads();
     }
-    // physical velocity for the droplet gather: [line][x][4], one vector store per node
+    // physical velocity for the droplet gather: [line][x][3], the two nodes of a complex
+    // value in three vector stores
     {
-        T* up = (T*)a.uphys + line0 * (size_t)N * 4;
+        T* up = (T*)a.uphys + line0 * (size_t)N * 3;
 #pragma unroll
         for (int q = 0; q < QP; ++q) {
             const int e = tid + NT * q;
             const int line = e / M, m = e % M;
-            T* p = up + ((size_t)line * N + 2 * m) * 4;
-            store_node(p, ph[0][q].x, ph[1][q].x, ph[2][q].x);
-            store_node(p + 4, ph[0][q].y, ph[1][q].y, ph[2][q].y);
+            T* p = up + ((size_t)line * N + 2 * m) * 3;
+            store_node_pair(p, ph[0][q].x, ph[1][q].x, ph[2][q].x, ph[0][q].y, ph[1][q].y, ph[2][q].y);
         }
     }
     // nonlinear term u x w (turb.f90:634-637) in natural component order, into the
@@ -1059,8 +1065,8 @@ int flow_alloc(FlowBuffers& fb, int N, int precision)
         DDNS_HIP(hipMemset(fb.O[i], 0, fe * cs));
     }
     for (int i = 0; i < 6; ++i) DDNS_HIP(hipMalloc(&fb.A[i], fe * cs));
-    DDNS_HIP(hipMalloc(&fb.uphys, (size_t)N * N * N * 4 * rs));
-    DDNS_HIP(hipMemset(fb.uphys, 0, (size_t)N * N * N * 4 * rs));
+    DDNS_HIP(hipMalloc(&fb.uphys, (size_t)N * N * N * 3 * rs));
+    DDNS_HIP(hipMemset(fb.uphys, 0, (size_t)N * N * N * 3 * rs));
     // twiddles exp(-2 pi i m / L), computed in double and rounded once
     for (int pass = 0; pass < 2; ++pass) {
         const int L = pass == 0 ? N : N / 2;
@@ -1391,13 +1397,13 @@ template <typename T>
 static int download_phys_t(const void* dev, int N, double* ux, double* uy, double* uz, hipStream_t s)
 {
     const size_t n3 = (size_t)N * N * N;
-    std::vector<T> h(n3 * 4);
+    std::vector<T> h(n3 * 3);
     DDNS_HIP(hipMemcpyAsync(h.data(), dev, h.size() * sizeof(T), hipMemcpyDeviceToHost, s));
     DDNS_HIP(hipStreamSynchronize(s));
     double* d[3] = {ux, uy, uz};
     for (int c = 0; c < 3; ++c)
         if (d[c])
-            for (size_t i = 0; i < n3; ++i) d[c][i] = (double)h[4 * i + c];
+            for (size_t i = 0; i < n3; ++i) d[c][i] = (double)h[3 * i + c];
     return 0;
 }
 

@@ -239,22 +239,22 @@ __device__ __forceinline__ void lagrange6(double z, double* bg)
     bg[5] = (4.0 * z - 5.0 * z3 + z5) / 120.0;
 }
 
-// The physical velocity is stored node-major, [z][y][x][4] (u, v, w, unused), so one
-// node is one 16-byte (float) or 32-byte (double) access.
+// The physical velocity is stored node-major, [z][y][x][3] (u, v, w), so one node is one
+// 12-byte (float) or three 8-byte (double) loads, and the six nodes of a stencil row are
+// one contiguous run of 72 (144) bytes.
 __device__ __forceinline__ void load_node(const float* p, double& a, double& b, double& c)
 {
-    const float4 v = *(const float4*)p;
+    const float3 v = *(const float3*)p;
     a = v.x; b = v.y; c = v.z;
 }
 __device__ __forceinline__ void load_node(const double* p, double& a, double& b, double& c)
 {
-    const double2 v = *(const double2*)p;
-    a = v.x; b = v.y; c = p[2];
+    a = p[0]; b = p[1]; c = p[2];
 }
 
 // Six lanes per droplet, one per x offset of the 6 x 6 x 6 stencil: for each of the
 // 36 (y, z) stencil rows the six lanes read the row's six nodes as one contiguous run,
-// so a wavefront load touches about 15 cache lines instead of one line per lane.  Ten
+// so a wavefront load touches about 10 cache lines instead of one line per lane.  Ten
 // droplets share a wavefront (lanes 60..63 idle), taken in cell order (sorted[]) so
 // that they read overlapping rows.  Each lane sums its 36 nodes with the weights
 // bg(x) bg(y) bg(z) of partlib.f90:509; the six partial sums are added in x order.
@@ -295,7 +295,7 @@ __global__ void __launch_bounds__(PT) k_interp(const T* __restrict__ u, int N, d
         double wxa = wx[0];
 #pragma unroll
         for (int m = 1; m < 6; ++m) wxa = a == m ? wx[m] : wxa;
-        const size_t ix = (size_t)(((nx - 2 + a) % N + N) % N) * 4;
+        const size_t ix = (size_t)(((nx - 2 + a) % N + N) % N) * 3;
         size_t rowy[6], rowz[6];
 #pragma unroll
         for (int m = 0; m < 6; ++m) {
@@ -307,7 +307,7 @@ __global__ void __launch_bounds__(PT) k_interp(const T* __restrict__ u, int N, d
 #pragma unroll
             for (int b2 = 0; b2 < 6; ++b2) {
                 double u0, u1, u2;
-                load_node(u + (rowz[d] + rowy[b2]) * 4 + ix, u0, u1, u2);
+                load_node(u + (rowz[d] + rowy[b2]) * 3 + ix, u0, u1, u2);
                 const double w = wxa * wy[b2] * wz[d];
                 acc0 += u0 * w;
                 acc1 += u1 * w;
