@@ -620,7 +620,7 @@ int part_collide(PartBuffers& pb, const PartConst& pc, hipStream_t s)
 static CollArgs collide_args(PartBuffers& pb, const PartConst& pc)
 {
     CollArgs a{};
-    a.pos = pb.pos; a.vel = pb.vel; a.rad = pb.rad; a.gid = pb.gid; a.fvp0 = pb.fvp; a.dmove = pb.dmove; a.vp = pb.vp;
+    a.pos = pb.pos; a.vel = pb.vel; a.rad = pb.rad; a.gid = pb.gid; a.fvp0 = pb.fvp + (size_t)pb.vcol(0) * pb.cap; a.dmove = pb.dmove; a.vp = pb.vp;
     a.cell_start = pb.cell_start; a.sorted = pb.sorted; a.pk = pb.pk;
     a.n = pc.n; a.cap = pb.cap; a.nset = pc.nset;
     for (int i = 0; i < DDNS_MAXSET; ++i) a.radius[i] = pc.radius[i];

@@ -567,6 +567,8 @@ extern "C" int ddns_set_droplets(ddns_handle* h, int n, const double* yp, const 
     DDNS_HIP(hipMemcpy(h->pb.fvp, fv.data(), fv.size() * 8, hipMemcpyHostToDevice));
     DDNS_HIP(hipMemcpy(h->pb.fdp, fd.data(), fd.size() * 8, hipMemcpyHostToDevice));
     DDNS_HIP(hipMemcpy(h->pb.pert, pv.data(), pv.size() * 8, hipMemcpyHostToDevice));
+    h->pb.hv_v = h->pb.hv_d = 0;                 // the levels were written in order
+    h->pb.v0_stale = h->pb.d0_stale = false;
     invalidate_cells(h->pb);
     h->n = n;
     h->pc.n = n;
@@ -591,14 +593,18 @@ extern "C" int ddns_get_droplets(ddns_handle* h, int* n, double* yp, double* fvp
     DDNS_HIP(hipMemcpy(pv.data(), h->pb.pert, pv.size() * 8, hipMemcpyDeviceToHost));
     DDNS_HIP(hipMemcpy(vv.data(), h->pb.vp, vv.size() * 8, hipMemcpyDeviceToHost));
     DDNS_HIP(hipMemcpy(dm.data(), h->pb.dmove, dm.size() * 8, hipMemcpyDeviceToHost));
+    // history level t in its ring column (level 0 reads as level 1 after a shift)
+    int vc[3], dc[4];
+    for (int t = 0; t < 3; ++t) vc[t] = h->pb.vcol(t == 0 && h->pb.v0_stale ? 1 : t);
+    for (int t = 0; t < 4; ++t) dc[t] = h->pb.dcol(t == 0 && h->pb.d0_stale ? 1 : t);
     for (int i = 0; i < m; ++i) {
         for (int c = 0; c < 3; ++c) {
             if (yp) { yp[8 * (size_t)i + c] = pos[(size_t)c * cap + i]; yp[8 * (size_t)i + 3 + c] = vel[(size_t)c * cap + i]; }
             if (pertvel) pertvel[3 * (size_t)i + c] = pv[3 * (size_t)i + c];
             if (vp) vp[3 * (size_t)i + c] = vv[(size_t)c * cap + i];
             if (dmove) dmove[3 * (size_t)i + c] = dm[(size_t)c * cap + i];
-            for (int t = 0; t < 3; ++t) if (fvp) fvp[9 * (size_t)i + 3 * t + c] = fv[(size_t)(3 * t + c) * cap + i];
-            for (int t = 0; t < 4; ++t) if (fdp) fdp[12 * (size_t)i + 3 * t + c] = fd[(size_t)(3 * t + c) * cap + i];
+            for (int t = 0; t < 3; ++t) if (fvp) fvp[9 * (size_t)i + 3 * t + c] = fv[(size_t)(vc[t] + c) * cap + i];
+            for (int t = 0; t < 4; ++t) if (fdp) fdp[12 * (size_t)i + 3 * t + c] = fd[(size_t)(dc[t] + c) * cap + i];
         }
         if (yp) { yp[8 * (size_t)i + 6] = rad[i]; yp[8 * (size_t)i + 7] = gid[i]; }
     }

@@ -304,6 +304,15 @@ struct PartBuffers {
     double4* pk = nullptr;      // cap: (x, y, z, radius) of sorted[p], refreshed with the cells
     int* cell_rank = nullptr;   // cap: place of a droplet among those of its cell (build scratch)
     bool cells_valid = false;   // the grid above was built on the current positions
+    // History levels as rings: level t of fvp (3 levels) is column block (t + hv_v) % 3,
+    // of fdp (4 levels) (t + hv_d) % 4, so the step's history shift (level t+1 = level t)
+    // only turns the rings.  After a shift level 0 holds the oldest level until it is
+    // written again (fvp by the history step, fdp by the advance); until then it reads as
+    // level 1, which the shift copied it to (v0_stale, d0_stale).
+    int hv_v = 0, hv_d = 0;
+    bool v0_stale = false, d0_stale = false;
+    int vcol(int t) const { return 3 * ((t + hv_v) % 3); }   // first column of fvp level t
+    int dcol(int t) const { return 3 * ((t + hv_d) % 4); }   // first column of fdp level t
     long long cell_count_zero = 0;  // cell_count[0 .. this) is zero (the last build's scan clears it)
     // the step's first overlap sweep is queued and its count not yet read; the validity
     // of the grids before its relocation, restored when it relocated nothing
@@ -377,10 +386,10 @@ int part_alloc(PartBuffers& pb, int cap, int nset, int G);
 void part_free(PartBuffers& pb);
 // with HI on, also writes the HI solve's x0 and w in cell order (after part_history_fvp)
 int part_interp(PartBuffers& pb, const PartConst& pc, const void* uphys, hipStream_t s);
-int part_history_fvp(const PartBuffers& pb, const PartConst& pc, hipStream_t s);
+int part_history_fvp(PartBuffers& pb, const PartConst& pc, hipStream_t s);
 int part_build_cells(PartBuffers& pb, const PartConst& pc, double hmin, hipStream_t s);
 int part_hdi(PartBuffers& pb, const PartConst& pc, int maxiter, double tol, int* iters, double* res, hipStream_t s);
-int part_advance(const PartBuffers& pb, const PartConst& pc, hipStream_t s);
+int part_advance(PartBuffers& pb, const PartConst& pc, hipStream_t s);
 int part_collide(PartBuffers& pb, const PartConst& pc, hipStream_t s);
 // the collision search queued without a host wait, and its completion (waits for the
 // search's counts, then queues the pair kernel)
@@ -392,8 +401,9 @@ int part_queue_counts(PartBuffers& pb, hipStream_t s);
 // clear_in leaves in[0..n-1] zero (in must not alias out then)
 int exclusive_scan(const int* in, int* out, int n, int* tmp, hipStream_t s, bool clear_in = false);
 int part_take_counts(const PartBuffers& pb, const PartConst& pc, ddns_collision_counts* cc);
-int part_update_and_wrap(const PartBuffers& pb, const PartConst& pc, bool relocate, hipStream_t s);
-int relocate_async(PartBuffers& pb, const PartConst& pc, hipStream_t s);
+int part_update_and_wrap(PartBuffers& pb, const PartConst& pc, bool relocate, hipStream_t s);
+// wrap: the drawn positions get the PERIODIC wrap (the step's wrap follows its relocation)
+int relocate_async(PartBuffers& pb, const PartConst& pc, hipStream_t s, bool wrap = false);
 int part_overlap(PartBuffers& pb, const PartConst& pc, int* sweeps, int* relocated, hipStream_t s);
 // the step's overlap sweep: the first sweep queued without a host wait; the rest, if it
 // relocated droplets, after the caller has waited for the stream

@@ -393,7 +393,8 @@ int part_interp(PartBuffers& pb, const PartConst& pc, const void* uphys, hipStre
 
 // ------------------------------------------------------------------ histories
 // PART_HISTORY_FVP, partlib.f90:562-592
-__global__ void k_hist_fvp(double* vel, const double* rad, double* fvp, int n, int cap, int newp, double Sv)
+__global__ void k_hist_fvp(double* vel, const double* rad, double* fvp, int n, int cap, int newp, double Sv, int v0,
+                           int v1, int v2)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -406,17 +407,18 @@ __global__ void k_hist_fvp(double* vel, const double* rad, double* fvp, int n, i
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const double v = vel[(size_t)c * cap + i];
-        fvp[(size_t)c * cap + i] = v;
-        if (newp) { fvp[(size_t)(3 + c) * cap + i] = v; fvp[(size_t)(6 + c) * cap + i] = v; }
+        fvp[(size_t)(v0 + c) * cap + i] = v;
+        if (newp) { fvp[(size_t)(v1 + c) * cap + i] = v; fvp[(size_t)(v2 + c) * cap + i] = v; }
     }
 }
 
-int part_history_fvp(const PartBuffers& pb, const PartConst& pc, hipStream_t s)
+int part_history_fvp(PartBuffers& pb, const PartConst& pc, hipStream_t s)
 {
     if (pc.n == 0) return 0;
     hipLaunchKernelGGL(k_hist_fvp, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.vel, pb.rad, pb.fvp, pc.n, pb.cap,
-                       pc.new_particle, pc.Sv_const);
+                       pc.new_particle, pc.Sv_const, pb.vcol(0), pb.vcol(1), pb.vcol(2));
     DDNS_HIP(hipGetLastError());
+    pb.v0_stale = false;
     return 0;
 }
 
@@ -426,7 +428,7 @@ int part_history_fvp(const PartBuffers& pb, const PartConst& pc, hipStream_t s)
 // gravity along +x, displacement dmove.
 __global__ void k_advance(double* vel, const double* rad, double* fvp, double* fdp, const double* vp,
                           const double* pert, double* dmove, int n, int cap, int newp, int hdi,
-                          int ndrag, double dt, double rnu, double Sv, double tau2)
+                          int ndrag, double dt, double rnu, double Sv, double tau2, int4 vc, int4 dc)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -435,8 +437,8 @@ __global__ void k_advance(double* vel, const double* rad, double* fvp, double* f
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         f0[c] = hdi ? vp[c * C + i] + pert[3 * (size_t)i + c] : vp[c * C + i];
-        fdp[c * C + i] = f0[c];
-        if (newp) { fdp[(3 + c) * C + i] = f0[c]; fdp[(6 + c) * C + i] = f0[c]; fdp[(9 + c) * C + i] = f0[c]; }
+        fdp[(dc.x + c) * C + i] = f0[c];
+        if (newp) { fdp[(dc.y + c) * C + i] = f0[c]; fdp[(dc.z + c) * C + i] = f0[c]; fdp[(dc.w + c) * C + i] = f0[c]; }
     }
     const double a = rad[i];
     double FSN = 1.;
@@ -455,55 +457,51 @@ __global__ void k_advance(double* vel, const double* rad, double* fvp, double* f
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const double unitv = c == 0 ? 1.0 : 0.0;
-        const double fv0 = fvp[c * C + i], fv1 = fvp[(3 + c) * C + i], fv2 = fvp[(6 + c) * C + i];
+        const double fv0 = fvp[(vc.x + c) * C + i], fv1 = fvp[(vc.y + c) * C + i], fv2 = fvp[(vc.z + c) * C + i];
         const double tempv = 19.0 * fv0 - 5.0 * fv1 + fv2;
-        const double tempd = 55.0 * f0[c] - 59.0 * fdp[(3 + c) * C + i] + 37.0 * fdp[(6 + c) * C + i] - 9.0 * fdp[(9 + c) * C + i];
+        const double tempd = 55.0 * f0[c] - 59.0 * fdp[(dc.y + c) * C + i] + 37.0 * fdp[(dc.z + c) * C + i] -
+                             9.0 * fdp[(dc.w + c) * C + i];
         const double vnew = (vel[c * C + i] + dt * unitv * wp_vel / taup * FSN - dtau * tempv + dtau * tempd) / (1.0 + 9.0 * dtau);
         vel[c * C + i] = vnew;
         dmove[c * C + i] = dtc * (9.0 * vnew + 19.0 * fv0 - 5.0 * fv1 + fv2);
     }
 }
 
-int part_advance(const PartBuffers& pb, const PartConst& pc, hipStream_t s)
+int part_advance(PartBuffers& pb, const PartConst& pc, hipStream_t s)
 {
     if (pc.n == 0) return 0;
+    const int4 vc = make_int4(pb.vcol(0), pb.vcol(1), pb.vcol(2), 0);
+    const int4 dc = make_int4(pb.dcol(0), pb.dcol(1), pb.dcol(2), pb.dcol(3));
     hipLaunchKernelGGL(k_advance, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.vel, pb.rad, pb.fvp, pb.fdp, pb.vp, pb.pert,
-                       pb.dmove, pc.n, pb.cap, pc.new_particle, pc.hdi, pc.ndrag, pc.dt, pc.rnu, pc.Sv_const, pc.tau2);
+                       pb.dmove, pc.n, pb.cap, pc.new_particle, pc.hdi, pc.ndrag, pc.dt, pc.rnu, pc.Sv_const, pc.tau2,
+                       vc, dc);
     DDNS_HIP(hipGetLastError());
+    pb.d0_stale = false;
     return 0;
 }
 
 // ------------------------------------------------------------------ position update, relocation, wrap
 // UPDATE_PART_POSITION (741-751)
-__global__ void k_update_pos(double* pos, const double* dmove, double* fvp, double* fdp, int n, int cap)
+// PERIODIC wrap (1142-1151) of one coordinate
+__device__ __forceinline__ double wrap_periodic(double y, double pi2)
+{
+    if (y >= pi2) y = y - pi2;
+    if (y < 0.0) y = y + pi2;
+    return y;
+}
+
+// UPDATE_PART_POSITION (730-757): the position, with the PERIODIC wrap of 608 applied at
+// once (the relocation between them wraps the positions it draws itself); the history
+// shift turns the rings (part_update_and_wrap)
+__global__ void k_update_pos(double* pos, const double* dmove, int n, int cap, double pi2)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const size_t C = (size_t)cap;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        fvp[(6 + c) * C + i] = fvp[(3 + c) * C + i];
-        fvp[(3 + c) * C + i] = fvp[c * C + i];
-        fdp[(9 + c) * C + i] = fdp[(6 + c) * C + i];
-        fdp[(6 + c) * C + i] = fdp[(3 + c) * C + i];
-        fdp[(3 + c) * C + i] = fdp[c * C + i];
-        pos[c * C + i] = pos[c * C + i] + dmove[c * C + i];
-    }
+    for (int c = 0; c < 3; ++c) pos[c * C + i] = wrap_periodic(pos[c * C + i] + dmove[c * C + i], pi2);
 }
 
-// PERIODIC wrap (1142-1151)
-__global__ void k_wrap(double* pos, int n, int cap, double pi2)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        double y = pos[(size_t)c * cap + i];
-        if (y >= pi2) y = y - pi2;
-        if (y < 0.0) y = y + pi2;
-        pos[(size_t)c * cap + i] = y;
-    }
-}
 
 // ranff2, accesories.f90:107-147, sequential on one lane: the relocation draws of
 // RELOCATE / OVERLAP_DET come from one stream in the reference's order.
@@ -570,7 +568,7 @@ __global__ void k_draws(int* st, double* draws, const int* total, int cap, int* 
 // A droplet that appears m times in the sorted removal list consumes 3m draws at
 // offset off; the last triple is the position it keeps (RELOCATE, 1994-2006).
 __global__ void k_apply_reloc(double* pos, const int* mult, const int* off, const double* draws, int n, int cap, double pi2,
-                              long long draw_cap)
+                              long long draw_cap, int wrap)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -578,9 +576,12 @@ __global__ void k_apply_reloc(double* pos, const int* mult, const int* off, cons
     if (m <= 0) return;
     if (3 * ((long long)off[i] + m) > draw_cap) return;      // cut short by k_draws (capacity error)
     const double* d = draws + 3 * (size_t)(off[i] + m - 1);
-    pos[i] = d[0] * pi2;
-    pos[(size_t)cap + i] = d[1] * pi2 + 0.0 * pi2;
-    pos[2 * (size_t)cap + i] = d[2] * pi2 + 0.0 * pi2;
+    double y[3] = {d[0] * pi2, d[1] * pi2 + 0.0 * pi2, d[2] * pi2 + 0.0 * pi2};
+    if (wrap)
+        for (int c = 0; c < 3; ++c) y[c] = wrap_periodic(y[c], pi2);
+    pos[i] = y[0];
+    pos[(size_t)cap + i] = y[1];
+    pos[2 * (size_t)cap + i] = y[2];
 }
 
 __global__ void k_set_rng(int* st, int seed)
@@ -616,7 +617,7 @@ int relocate_by_mult(PartBuffers& pb, const PartConst& pc, int* total_host, hipS
     hipLaunchKernelGGL(k_draws, dim3(1), dim3(1), 0, s, pb.rng_state, pb.draws, pb.draw_off + pc.n, pb.draw_cap,
                        pb.flags + 48);
     hipLaunchKernelGGL(k_apply_reloc, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.pos, pb.mult, pb.draw_off, pb.draws,
-                       pc.n, pb.cap, pc.pi2, (long long)pb.draw_cap);
+                       pc.n, pb.cap, pc.pi2, (long long)pb.draw_cap, 0);
     DDNS_HIP(hipGetLastError());
     return 0;
 }
@@ -626,7 +627,7 @@ int relocate_by_mult(PartBuffers& pb, const PartConst& pc, int* total_host, hipS
 // three draws per droplet slot (each droplet relocated once; a step whose removal list
 // needs more sets the capacity flag flags[48], which the step reports as an error).
 // Droplets with no removal entry keep their positions.
-int relocate_async(PartBuffers& pb, const PartConst& pc, hipStream_t s)
+int relocate_async(PartBuffers& pb, const PartConst& pc, hipStream_t s, bool wrap)
 {
     int st = exclusive_scan(pb.mult, pb.draw_off, pc.n, (int*)pb.scan_tmp, s);
     if (st) return st;
@@ -640,22 +641,25 @@ int relocate_async(PartBuffers& pb, const PartConst& pc, hipStream_t s)
     hipLaunchKernelGGL(k_draws, dim3(1), dim3(1), 0, s, pb.rng_state, pb.draws, pb.draw_off + pc.n, pb.draw_cap,
                        pb.flags + 48);
     hipLaunchKernelGGL(k_apply_reloc, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.pos, pb.mult, pb.draw_off, pb.draws,
-                       pc.n, pb.cap, pc.pi2, (long long)pb.draw_cap);
+                       pc.n, pb.cap, pc.pi2, (long long)pb.draw_cap, wrap ? 1 : 0);
     DDNS_HIP(hipGetLastError());
     return 0;
 }
 
-int part_update_and_wrap(const PartBuffers& pb, const PartConst& pc, bool relocate, hipStream_t s)
+int part_update_and_wrap(PartBuffers& pb, const PartConst& pc, bool relocate, hipStream_t s)
 {
     if (pc.n == 0) return 0;
-    hipLaunchKernelGGL(k_update_pos, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.pos, pb.dmove, pb.fvp, pb.fdp, pc.n, pb.cap);
+    hipLaunchKernelGGL(k_update_pos, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.pos, pb.dmove, pc.n, pb.cap, pc.pi2);
     DDNS_HIP(hipGetLastError());
+    // fvp(2) = fvp(1), fvp(1) = fvp(0); fdp(3..1) = fdp(2..0): level t+1 takes level t's columns
+    pb.hv_v = (pb.hv_v + 2) % 3;
+    pb.hv_d = (pb.hv_d + 3) % 4;
+    pb.v0_stale = true;
+    pb.d0_stale = true;
     if (relocate) {
-        int st = relocate_async(const_cast<PartBuffers&>(pb), pc, s);
+        int st = relocate_async(pb, pc, s, true);
         if (st) return st;
     }
-    hipLaunchKernelGGL(k_wrap, dim3(nblk(pc.n)), dim3(PT), 0, s, pb.pos, pc.n, pb.cap, pc.pi2);
-    DDNS_HIP(hipGetLastError());
     return 0;
 }
 
