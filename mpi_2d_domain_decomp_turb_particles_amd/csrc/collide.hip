@@ -465,11 +465,13 @@ __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
 
 // Second half: one thread per handed-on pair redoes the pair's geometry exactly as the
 // search did and runs the statistics and collision tests.
-__global__ void __launch_bounds__(256) k_collide_pairs(CollArgs a, int ncand)
+// The kernel is queued right behind the search and reads the search's count on the
+// device (the host checks it for overflow afterwards); a grid-stride loop over the pairs.
+__global__ void __launch_bounds__(256) k_collide_pairs(CollArgs a)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncand = min(*a.ncand, a.cand_cap);
     long long instat = 0;
-    if (r < ncand) {
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < ncand; r += gridDim.x * blockDim.x) {
         const int2 rec = a.cand[r];
         const int t = rec.x, p = rec.y & ((1 << 27) - 1), kc = (int)((unsigned)rec.y >> 27);   // kc < 27: 5 bits
         const int ia = a.sorted[t];
@@ -510,9 +512,10 @@ __global__ void __launch_bounds__(256) k_collide_pairs(CollArgs a, int ncand)
             }
         }
     }
-    const unsigned long long m = __ballot(instat != 0);
-    if (m && (threadIdx.x & 63) == __ffsll((long long)m) - 1)
-        atomicAdd((unsigned long long*)(a.tested + 1), (unsigned long long)__popcll(m));
+    // pairs in the statistics window: one atomic per wavefront
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) instat += __shfl_xor(instat, o, 64);
+    if ((threadIdx.x & 63) == 0 && instat) atomicAdd((unsigned long long*)(a.tested + 1), (unsigned long long)instat);
 }
 
 // OVERLAP_DET sweep (2076-2157): count, per droplet as ih1, the overlapping pairs
@@ -604,10 +607,11 @@ __global__ void __launch_bounds__(256) k_zero_spans(ZeroSpans z)
         for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < z.words[k]; i += stride) z.p[k][i] = 0u;
 }
 
-// The collision search is queued by part_collide_search (no host wait), so the caller
-// can queue other streams' work while it runs; part_collide_finish then waits for the
-// search's counts, grows the candidate buffer and searches again when it overflowed,
-// sizes the contact records and queues the pair kernel.
+// The collision search and the pair kernel behind it are queued by part_collide_search
+// (no host wait), so the caller can queue other streams' work while they run;
+// part_collide_finish then waits for them and reads the search's counts: when the
+// candidates or the contact records overflowed their buffers, it widens them and runs
+// both again (the search clears everything the pair kernel writes).
 int part_collide_search(PartBuffers& pb, const PartConst& pc, hipStream_t s);
 int part_collide_finish(PartBuffers& pb, const PartConst& pc, hipStream_t s);
 
@@ -666,46 +670,46 @@ int part_collide_search(PartBuffers& pb, const PartConst& pc, hipStream_t s)
         DDNS_HIP(hipMalloc(&pb.cand, (size_t)pb.cand_cap * sizeof(int2)));
     }
     if (!pb.found_host) DDNS_HIP(hipHostMalloc((void**)&pb.found_host, 4 * sizeof(int), hipHostMallocDefault));
+    if (!pb.contact) {
+        pb.contact_cap = 4096;
+        DDNS_HIP(hipMalloc(&pb.contact, (size_t)pb.contact_cap * DDNS_CONTACT_REC * sizeof(double)));
+    }
     CollArgs a = collide_args(pb, pc);
     hipLaunchKernelGGL(k_extent_max, dim3(std::min(256, (pc.n + 255) / 256)), dim3(256), 0, s, (const double*)pb.dmove,
                        (const double*)pb.rad, pc.n, pb.cap, (unsigned long long*)(pb.flags + 32));
     hipLaunchKernelGGL(k_collide, dim3((pc.n + CT - 1) / CT), dim3(CT), 0, s, a);
-    DDNS_HIP(hipGetLastError());
     // handed-on pairs and pairs in contact, read by part_collide_finish
     DDNS_HIP(hipMemcpyAsync(pb.found_host, a.ncand, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    // the pair kernel, sized by the last step's count (a grid-stride loop covers more)
+    const int nwg = std::max(256, std::min(4096, (int)((pb.last_ncand + pb.last_ncand / 4 + 255) / 256)));
+    hipLaunchKernelGGL(k_collide_pairs, dim3(nwg), dim3(256), 0, s, a);
+    DDNS_HIP(hipGetLastError());
     return 0;
 }
 
 int part_collide_finish(PartBuffers& pb, const PartConst& pc, hipStream_t s)
 {
     if (pc.n == 0) return 0;
-    DDNS_HIP(hipStreamSynchronize(s));
-    int found[2] = {pb.found_host[0], pb.found_host[1]};
-    int& ncand = found[0];
-    while (ncand > pb.cand_cap) {
-        // more pairs to hand on than the buffer holds: widen it and search again
-        DDNS_HIP(hipFree(pb.cand));
-        pb.cand_cap = ncand + ncand / 2;
-        DDNS_HIP(hipMalloc(&pb.cand, (size_t)pb.cand_cap * sizeof(int2)));
-        CollArgs a = collide_args(pb, pc);
-        DDNS_HIP(hipMemsetAsync(a.ncand, 0, 2 * sizeof(int), s));
-        DDNS_HIP(hipMemsetAsync(pb.tested, 0, sizeof(long long), s));
-        hipLaunchKernelGGL(k_collide, dim3((pc.n + CT - 1) / CT), dim3(CT), 0, s, a);
-        DDNS_HIP(hipGetLastError());
-        DDNS_HIP(hipMemcpyAsync(found, a.ncand, sizeof(found), hipMemcpyDeviceToHost, s));
+    for (;;) {
         DDNS_HIP(hipStreamSynchronize(s));
+        const int ncand = pb.found_host[0], ncontact = pb.found_host[1];
+        pb.last_ncand = ncand;
+        if (ncand <= pb.cand_cap && ncontact <= pb.contact_cap) return 0;
+        // more pairs to hand on, or more contact records, than the buffers hold: widen
+        // them and run the search and the pair kernel again
+        if (ncand > pb.cand_cap) {
+            DDNS_HIP(hipFree(pb.cand));
+            pb.cand_cap = ncand + ncand / 2;
+            DDNS_HIP(hipMalloc(&pb.cand, (size_t)pb.cand_cap * sizeof(int2)));
+        }
+        if (ncontact > pb.contact_cap) {
+            if (pb.contact) DDNS_HIP(hipFree(pb.contact));
+            pb.contact_cap = std::max(4096, ncontact + ncontact / 2);
+            DDNS_HIP(hipMalloc(&pb.contact, (size_t)pb.contact_cap * DDNS_CONTACT_REC * sizeof(double)));
+        }
+        const int st = part_collide_search(pb, pc, s);
+        if (st) return st;
     }
-    // the contact records of the pair kernel: the search has counted them
-    if (found[1] > pb.contact_cap) {
-        if (pb.contact) DDNS_HIP(hipFree(pb.contact));
-        pb.contact_cap = std::max(4096, found[1] + found[1] / 2);
-        DDNS_HIP(hipMalloc(&pb.contact, (size_t)pb.contact_cap * DDNS_CONTACT_REC * sizeof(double)));
-    }
-    CollArgs a = collide_args(pb, pc);
-    if (ncand > 0)
-        hipLaunchKernelGGL(k_collide_pairs, dim3((ncand + 255) / 256), dim3(256), 0, s, a, ncand);
-    DDNS_HIP(hipGetLastError());
-    return 0;
 }
 
 // The last detection's contact records, buffer(1:12, ij) of partlib.f90:1431-1437, in the

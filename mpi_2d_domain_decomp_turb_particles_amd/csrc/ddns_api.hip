@@ -837,9 +837,10 @@ static int droplet_block(ddns_handle* h, float* ms)
     }
     DDNS_HIP(hipEventRecord(h->ev[11], s));
     // The droplet stream's next work is queued before the flow's launches, so it never
-    // waits for the host: the advance, the collision grid and the collision search.  The
-    // flow's forward transforms and update start when the HI solve is done (event 11), as
-    // before; they are queued while the search runs, ahead of its wait.
+    // waits for the host: the advance, the collision grid, the collision search and the
+    // pair kernel.  The flow's forward y stage and pre start when the HI solve is done
+    // (event 11), its z stage when the pair kernel has run (event 6); both are queued while
+    // the search runs, ahead of the host's check of the search's counts.
     st = part_advance(h->pb, pc, s);                                          // 574-576
     if (st) return st;
     h->new_particle = 0;
@@ -851,18 +852,18 @@ static int droplet_block(ddns_handle* h, float* ms)
         st = part_build_cells(h->pb, pc, coll_cell_width(h), s);
         if (st) return st;
     }
-    st = part_collide_search(h->pb, pc, s);                                   // 584
+    st = part_collide_search(h->pb, pc, s);                                   // 584, search and pairs
     if (st) return st;
+    if (h->z_gate) DDNS_HIP(hipEventRecord(h->ev[6], s));                     // the pair kernel is queued
     st = queue_flow_rest(h, h->ev[11]);                                       // after the HI solve
-    if (st) return st;
-    st = part_collide_finish(h->pb, pc, s);
     if (st) return st;
     const bool gated = h->z_pending;
     if (gated) {
-        DDNS_HIP(hipEventRecord(h->ev[6], s));
         st = queue_flow_z(h, h->ev[6]);
         if (st) return st;
     }
+    st = part_collide_finish(h->pb, pc, s);                                   // overflow check
+    if (st) return st;
     const bool reloc = h->p.nonoverlap && h->istep < h->p.nstep_grow;         // 590
     invalidate_cells(h->pb);
     st = part_update_and_wrap(h->pb, pc, reloc, s);                           // 586, 590, 608

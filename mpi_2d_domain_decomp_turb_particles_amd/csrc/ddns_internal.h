@@ -347,6 +347,7 @@ struct PartBuffers {
     int* mult = nullptr;        // cap: removal-list multiplicity
     void* cand = nullptr;       // cand_cap int2: pairs handed from the collision search to the pair kernel
     int cand_cap = 0;
+    int last_ncand = 0;         // the last search's count: the size of the next pair-kernel grid
     int* hit = nullptr;         // cap: droplet already counted as ih1 of a collision
     int* flags = nullptr;       // device flags / counters (64 ints)
     long long* tested = nullptr;
