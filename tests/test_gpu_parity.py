@@ -173,6 +173,35 @@ def test_collision_pairs_bit_identical(ddns, oracle_mod, hdi):
         assert np.allclose(ro[:, 8:], rg[:, 8:], rtol=0, atol=1e-13)
 
 
+def test_collision_contact_overflow_matches_oracle(ddns, oracle_mod):
+    """5000 pairs placed 1.01 (a + b) apart (inside the 2 % contact shell): more contact
+    records than the first buffer holds (4096), so the search and the pair kernel run a
+    second time with a wider one; the result is the oracle's."""
+    N = 64
+    npair = 5000
+    p = _params(ddns, N=N, npart=2 * npair, nset=2, radius=[60.0, 90.0], dt=0.01)
+    with oracle_mod.Oracle(p) as o, ddns.Solver(p) as s:
+        rad = o.scaling()["radius"][:2]
+        yp, dm, v0, vn, vp = _collision_inputs(2 * npair, 11, rad, dt=0.01)
+        rng = np.random.default_rng(12)
+        u = rng.normal(size=(3, npair))
+        u /= np.linalg.norm(u, axis=0)
+        a, b = yp[6, 0::2], yp[6, 1::2]
+        yp[0:3, 1::2] = np.mod(yp[0:3, 0::2] + 1.01 * (a + b) * u, 2 * np.pi)
+        o.collision_detect(yp, dm, v0, vn, vp)
+        s.collision_detect(yp, dm, v0, vn, vp)
+        co, cg = o.collision_counts(), s.collision_counts()
+        assert sum(co.nir) > 4096
+        assert list(co.nir) == list(cg.nir)
+        assert list(co.icc) == list(cg.icc)
+        assert co.pairs_in_dstat == cg.pairs_in_dstat
+        assert np.array_equal(o.collision_pairs(), s.collision_pairs())
+        ro, rg = o.contact_records(), s.contact_records()
+        assert rg.shape == ro.shape
+        assert np.array_equal(ro[:, :8], rg[:, :8])
+        assert np.allclose(ro[:, 8:], rg[:, 8:], rtol=0, atol=1e-13)
+
+
 def test_droplet_steps_match_oracle(ddns, oracle_mod):
     N = 32
     p = _params(ddns, N=N, npart=3000, nset=2, radius=[20.0, 40.0], forcing=1, dt=0.002, rnu=0.02, nonoverlap=1)
