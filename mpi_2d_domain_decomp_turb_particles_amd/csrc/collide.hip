@@ -691,7 +691,7 @@ int part_collide_finish(PartBuffers& pb, const PartConst& pc, hipStream_t s)
 {
     if (pc.n == 0) return 0;
     for (;;) {
-        DDNS_HIP(hipStreamSynchronize(s));
+        DDNS_HIP(wait_stream(pb, s));
         const int ncand = pb.found_host[0], ncontact = pb.found_host[1];
         pb.last_ncand = ncand;
         if (ncand <= pb.cand_cap && ncontact <= pb.contact_cap) return 0;

@@ -894,7 +894,7 @@ static int droplet_block(ddns_handle* h, float* ms)
     st = part_queue_counts(h->pb, s);
     if (st) return st;
     DDNS_HIP(hipEventRecord(h->ev[14], s));
-    DDNS_HIP(hipEventSynchronize(h->ev[14]));
+    DDNS_HIP(spin_wait(h->ev[14]));
     if (h->p.nonoverlap) {
         int sw = 0, rl = 0;
         st = part_overlap_finish(h->pb, pc, &sw, &rl, s);
@@ -903,7 +903,7 @@ static int droplet_block(ddns_handle* h, float* ms)
         h->cc.noverlap_relocated = rl;
         if (sw > 0) {                    // the further sweeps belong to the overlap phase
             DDNS_HIP(hipEventRecord(h->ev[14], s));
-            DDNS_HIP(hipEventSynchronize(h->ev[14]));
+            DDNS_HIP(spin_wait(h->ev[14]));
         }
     }
     float t;

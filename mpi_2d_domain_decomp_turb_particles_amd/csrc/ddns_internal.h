@@ -304,6 +304,8 @@ struct PartBuffers {
     double4* pk = nullptr;      // cap: (x, y, z, radius) of sorted[p], refreshed with the cells
     int* cell_rank = nullptr;   // cap: place of a droplet among those of its cell (build scratch)
     bool cells_valid = false;   // the grid above was built on the current positions
+    hipEvent_t sync_ev = nullptr;   // waits of the droplet block (wait_stream)
+    double* gm_host = nullptr;      // pinned: GMRES's small-problem head [0..5] and the list flags [8..11]
     // History levels as rings: level t of fvp (3 levels) is column block (t + hv_v) % 3,
     // of fdp (4 levels) (t + hv_d) % 4, so the step's history shift (level t+1 = level t)
     // only turns the rings.  After a shift level 0 holds the oldest level until it is
@@ -384,6 +386,26 @@ struct PartConst {
 };
 
 int part_alloc(PartBuffers& pb, int cap, int nset, int G);
+// Waits for event e by polling it: after a blocking wait the host thread ran again tens
+// of microseconds after the GPU had finished, and the stream waited that long for the
+// launches that follow.
+inline hipError_t spin_wait(hipEvent_t e)
+{
+    for (;;) {
+        const hipError_t r = hipEventQuery(e);
+        if (r != hipErrorNotReady) return r;
+    }
+}
+// all work queued on s so far, waited for by polling (pb.sync_ev, made on first use)
+inline hipError_t wait_stream(PartBuffers& pb, hipStream_t s)
+{
+    if (!pb.sync_ev) {
+        const hipError_t r = hipEventCreateWithFlags(&pb.sync_ev, hipEventDisableTiming);
+        if (r != hipSuccess) return r;
+    }
+    const hipError_t r = hipEventRecord(pb.sync_ev, s);
+    return r != hipSuccess ? r : spin_wait(pb.sync_ev);
+}
 void part_free(PartBuffers& pb);
 // with HI on, also writes the HI solve's x0 and w in cell order (after part_history_fvp)
 int part_interp(PartBuffers& pb, const PartConst& pc, const void* uphys, hipStream_t s);

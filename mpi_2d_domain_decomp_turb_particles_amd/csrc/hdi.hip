@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 namespace ddns {
 
@@ -917,11 +918,16 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
             }
             DDNS_HIP(hipGetLastError());
             iter = last;
-            // the first check also reads the neighbour lists' flags
+            // the first check also reads the neighbour lists' flags (pinned memory: the two
+            // copies are queued together and waited for once)
+            if (!pb.gm_host) DDNS_HIP(hipHostMalloc((void**)&pb.gm_host, 16 * sizeof(double), hipHostMallocDefault));
             int f[8];
-            if (first) DDNS_HIP(hipMemcpyAsync(f, pb.flags, sizeof(f), hipMemcpyDeviceToHost, s));
-            DDNS_HIP(hipMemcpyAsync(st6, sm, sizeof(st6), hipMemcpyDeviceToHost, s));
-            DDNS_HIP(hipStreamSynchronize(s));
+            int* fh = (int*)(pb.gm_host + 8);
+            if (first) DDNS_HIP(hipMemcpyAsync(fh, pb.flags, sizeof(f), hipMemcpyDeviceToHost, s));
+            DDNS_HIP(hipMemcpyAsync(pb.gm_host, sm, sizeof(st6), hipMemcpyDeviceToHost, s));
+            DDNS_HIP(wait_stream(pb, s));
+            std::memcpy(st6, pb.gm_host, sizeof(st6));
+            if (first) std::memcpy(f, fh, sizeof(f));
             if (first) {
                 first = false;
                 lists = lists_outcome<T>(pb, f);

@@ -221,6 +221,8 @@ void part_free(PartBuffers& pb)
     for (void* p : ps) if (p) (void)hipFree(p);
     void* sp[] = {pb.spare.cell_count, pb.spare.cell_start, pb.spare.cell_of, pb.spare.sorted, pb.spare.pk};
     for (void* p : sp) if (p) (void)hipFree(p);
+    if (pb.sync_ev) (void)hipEventDestroy(pb.sync_ev);
+    if (pb.gm_host) (void)hipHostFree(pb.gm_host);
     if (pb.found_host) (void)hipHostFree(pb.found_host);
     if (pb.counts_host) (void)hipHostFree(pb.counts_host);
     pb = PartBuffers{};
