@@ -803,7 +803,7 @@ int part_overlap(PartBuffers& pb, const PartConst& pc, int* sweeps, int* relocat
 // case): the first sweep and its relocation are queued, the draws made for whatever the
 // count is (as in the collision step's relocation: none for a count of 0), and the count
 // is copied to pinned memory for the step's final wait.
-int part_overlap_queue(PartBuffers& pb, const PartConst& pc, hipStream_t s)
+int part_overlap_queue(PartBuffers& pb, const PartConst& pc, hipStream_t s, double next_width)
 {
     pb.overlap_queued = false;
     if (pc.n == 0) return 0;
@@ -811,6 +811,12 @@ int part_overlap_queue(PartBuffers& pb, const PartConst& pc, hipStream_t s)
     if (st) return st;
     st = overlap_search(pb, pc, s);
     if (st) return st;
+    // the next step's grid on the same positions, while the flow's z stage runs (a
+    // relocation by this sweep invalidates it with the sweep's own grid)
+    if (next_width > 0.0) {
+        st = part_build_cells(pb, pc, next_width, s);
+        if (st) return st;
+    }
     const bool kept = pb.cells_valid, spare_kept = pb.spare.valid;
     st = relocate_async(pb, pc, s);          // invalidates the grids: positions may move
     if (st) return st;

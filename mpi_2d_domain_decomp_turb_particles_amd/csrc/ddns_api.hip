@@ -888,7 +888,7 @@ static int droplet_block(ddns_handle* h, float* ms)
     // OVERLAP_DET (610): the first sweep is queued; its count is read with the step's
     // counts, and only a sweep that relocated droplets is followed by more
     if (h->p.nonoverlap) {
-        st = part_overlap_queue(h->pb, pc, s);
+        st = part_overlap_queue(h->pb, pc, s, cell_width(h));
         if (st) return st;
     }
     st = part_queue_counts(h->pb, s);

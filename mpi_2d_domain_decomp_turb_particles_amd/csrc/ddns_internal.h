@@ -430,7 +430,9 @@ int relocate_async(PartBuffers& pb, const PartConst& pc, hipStream_t s, bool wra
 int part_overlap(PartBuffers& pb, const PartConst& pc, int* sweeps, int* relocated, hipStream_t s);
 // the step's overlap sweep: the first sweep queued without a host wait; the rest, if it
 // relocated droplets, after the caller has waited for the stream
-int part_overlap_queue(PartBuffers& pb, const PartConst& pc, hipStream_t s);
+// next_width > 0: the grid of that width (the next step's first) is built too, on the
+// positions the sweep saw, and kept if the sweep moves nothing
+int part_overlap_queue(PartBuffers& pb, const PartConst& pc, hipStream_t s, double next_width = 0.0);
 int part_overlap_finish(PartBuffers& pb, const PartConst& pc, int* sweeps, int* relocated, hipStream_t s);
 int part_read_counts(const PartBuffers& pb, const PartConst& pc, ddns_collision_counts* cc, hipStream_t s);
 int part_read_bins(const PartBuffers& pb, const PartConst& pc, int* nirbin, double* const bins[7], hipStream_t s);
