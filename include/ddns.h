@@ -332,6 +332,36 @@ int ddns_profile_flow(ddns_handle* h, int k, double* ms_per_kernel, int nkernels
  * flow state is kept: the next step rebuilds its work buffers from u-hat. */
 int ddns_profile_copy(ddns_handle* h, int reps, double* ms_per_launch, double* bytes_per_launch);
 
+/* ---- batched realisations ---------------------------------------------------------- */
+
+/* R independent realisations advanced together (no counterpart in the reference, which
+ * runs one realisation per job: the step turb.f90:492-707 over the state of one
+ * parameter.d).  params[r] is realisation r's parameter set; all share the grid (N <= 256),
+ * the precision, dt and the viscosity, and each keeps its own forcing kind and amplitudes
+ * (forcing, force, OU time scale and variance), seeds, droplets, HI and collision settings.
+ * Every flow stage of a step is one launch over all realisations, replayed from HIP
+ * graphs; each realisation's droplet block runs as a single handle's does, beside the
+ * flow, the blocks spread over worker threads (DDNS_BATCH_THREADS, default 8). */
+typedef struct ddns_batch ddns_batch;
+int ddns_batch_create(const ddns_params* params, int nreal, ddns_batch** out);
+int ddns_batch_destroy(ddns_batch* b);
+int ddns_batch_size(const ddns_batch* b);
+/* Realisation r as a handle, for ddns_init_flow / set_flow / get_flow / get_spectrum /
+ * init_droplets / set_droplets / get_droplets / the collision getters / the forcing state.
+ * It belongs to the batch: ddns_step and ddns_destroy refuse it. */
+ddns_handle* ddns_batch_member(ddns_batch* b, int r);
+/* nsteps steps of every realisation (turb.f90:492-707 each).  Returns the first error of
+ * any realisation (ddns_batch_last_error names it), else DDNS_WARN_GMRES when some
+ * realisation's GMRES stopped short of its tolerance, else DDNS_OK. */
+int ddns_batch_step(ddns_batch* b, int nsteps);
+/* Timings of the last ddns_batch_step: flow_ms and flow_kernel_ms for the batched flow
+ * launches, total_ms for the call; the droplet phases summed over the realisations (each
+ * member's own are in its ddns_get_timing). */
+int ddns_batch_get_timing(ddns_batch* b, ddns_timing* out);
+/* As ddns_get_tuning, for the tile shapes timed on the batch's fields at creation. */
+int ddns_batch_get_tuning(ddns_batch* b, int* shapes, double* ms);
+const char* ddns_batch_last_error(const ddns_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

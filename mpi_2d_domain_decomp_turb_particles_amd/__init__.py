@@ -278,3 +278,130 @@ class Solver:
         ms, by = C.c_double(), C.c_double()
         self._check(self.lib.ddns_profile_copy(self.h, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
+
+
+def stokes_radius_um(p: Params, st: float) -> float:
+    """Droplet radius (um) whose Stokes number tau_p / tau_k is st, with the response time
+    and Kolmogorov time of PART_INIT's scaling (partlib.f90:104-150):
+    tau_p = 2 rho_w a^2 / (9 rho_a nu), tau_k = sqrt(nu / eps), a in cm."""
+    tauk = np.sqrt(p.viscosity_CLOUD / p.ediss_CLOUD)
+    a_cm = np.sqrt(st * tauk * 9.0 * p.rho_air * p.viscosity_CLOUD / (2.0 * p.rho_water))
+    return float(a_cm * 1.0e4)
+
+
+def batched_realisations(R: int = 48, N: int = 64, precision: int = FP32_MIXED, seed: int = 2024,
+                         nmin: int = 5000, nmax: int = 200000, st_min: float = 0.2, st_max: float = 3.0,
+                         mean_neighbours: float = 24.0, hdi: bool = True):
+    """Parameter sets of R independent realisations for a Batch: the reference's N^3
+    defaults, seeds that differ per realisation (flow iseed_m0, forcing iseedf, relocation
+    draws), deterministic forcing on the even realisations and stochastic (OU) forcing on
+    the odd ones, droplet counts drawn uniformly from [nmin, nmax], one droplet size per
+    realisation with Stokes numbers evenly spaced over [st_min, st_max].  With HI, each
+    realisation's truncation radius is 50 droplet radii, or less where that would give a
+    droplet more than mean_neighbours neighbours on average, so every HI row stays well
+    inside the reference's limit of 100 neighbours."""
+    rng = np.random.default_rng(seed)
+    counts = rng.integers(nmin, nmax + 1, size=R)
+    sts = np.linspace(st_min, st_max, R) if R > 1 else np.array([st_min])
+    vol = (2.0 * np.pi) ** 3
+    out = []
+    for r in range(R):
+        p = default_params(N=N, precision=precision)
+        p.forcing = 1 if r % 2 == 0 else 2
+        p.iseed_m0 = 1000 + r
+        p.iseedf = 232300 + 17 * r
+        p.relocate_seed = r
+        p.npart = int(counts[r])
+        p.nset = 1
+        p.radius[0] = stokes_radius_um(p, float(sts[r]))
+        if hdi:
+            etk_cloud = (p.viscosity_CLOUD ** 3 / p.ediss_CLOUD) ** 0.25
+            a_code = p.radius[0] * (p.etk_DNS / etk_cloud) / 1.0e4
+            reach = (mean_neighbours * vol / (p.npart * 4.0 / 3.0 * np.pi)) ** (1.0 / 3.0)
+            p.hdi_included, p.nonoverlap = 1, 1
+            p.hdi_trunc = float(min(50.0, reach / a_code))
+        out.append(p)
+    return out
+
+
+class _Member(Solver):
+    """One realisation of a Batch: a Solver over the member handle, owned by the batch
+    (closing it does nothing; stepping it is refused by the library)."""
+
+    def __init__(self, batch: "Batch", r: int):
+        self.lib = batch.lib
+        self.p = batch.params[r]
+        self.N = self.p.nx
+        h = self.lib.ddns_batch_member(batch.b, r)
+        if not h:
+            raise DDNSError(DDNS_ERR_ARG, f"no realisation {r}")
+        self.h = C.c_void_p(h)
+        self.nek = self.lib.ddns_nek(self.h)
+        self._batch = batch
+
+    def close(self):
+        self.h = None
+
+
+class Batch:
+    """R independent realisations of one grid, precision, dt and viscosity advanced
+    together: every flow stage is one launch over all of them, and each realisation's
+    droplet block runs as a Solver's does.  ``member(r)`` gives realisation r's state
+    (flow, forcing, droplets, collision statistics) as a Solver."""
+
+    def __init__(self, params):
+        self.lib = load_library()
+        self.params = list(params)
+        arr = (Params * len(self.params))(*self.params)
+        b = C.c_void_p()
+        st = self.lib.ddns_batch_create(arr, len(self.params), C.byref(b))
+        if st != 0:
+            raise DDNSError(st, self.lib.ddns_batch_last_error(None).decode())
+        self.b = b
+        self.R = len(self.params)
+        self._members = [_Member(self, r) for r in range(self.R)]
+
+    def member(self, r: int) -> Solver:
+        return self._members[r]
+
+    def __len__(self):
+        return self.R
+
+    def close(self):
+        if self.b:
+            for m in self._members:
+                m.close()
+            self.lib.ddns_batch_destroy(self.b)
+            self.b = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st: int) -> int:
+        if st < 0:
+            raise DDNSError(st, self.lib.ddns_batch_last_error(self.b).decode())
+        return st
+
+    def step(self, nsteps: int = 1) -> int:
+        return self._check(self.lib.ddns_batch_step(self.b, nsteps))
+
+    def timing(self) -> Timing:
+        t = Timing()
+        self._check(self.lib.ddns_batch_get_timing(self.b, C.byref(t)))
+        return t
+
+    def tuning(self):
+        sh = np.zeros(4, dtype=np.int32)
+        ms = np.zeros(6)
+        self._check(self.lib.ddns_batch_get_tuning(self.b, sh.ctypes.data_as(C.POINTER(C.c_int)), _dp(ms)))
+        return {"ky_tc": int(sh[0]), "ky_tc_default": int(sh[1]), "kz_nt": int(sh[2]),
+                "kz_nt_default": int(sh[3]), "ky_ms": ms[:3].copy(), "kz_ms": ms[3:].copy()}

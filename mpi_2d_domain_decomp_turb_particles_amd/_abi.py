@@ -180,4 +180,12 @@ SIGNATURES = {
     "ddns_collision_detect": (C.c_int, [_P, C.c_int, _D, _D, _D, _D, _D]),
     "ddns_profile_flow": (C.c_int, [_P, C.c_int, _D, C.c_int]),
     "ddns_profile_copy": (C.c_int, [_P, C.c_int, _D, _D]),
+    "ddns_batch_create": (C.c_int, [C.POINTER(Params), C.c_int, C.POINTER(_P)]),
+    "ddns_batch_destroy": (C.c_int, [_P]),
+    "ddns_batch_size": (C.c_int, [_P]),
+    "ddns_batch_member": (_P, [_P, C.c_int]),
+    "ddns_batch_step": (C.c_int, [_P, C.c_int]),
+    "ddns_batch_get_timing": (C.c_int, [_P, C.POINTER(Timing)]),
+    "ddns_batch_get_tuning": (C.c_int, [_P, _I, _D]),
+    "ddns_batch_last_error": (C.c_char_p, [_P]),
 }

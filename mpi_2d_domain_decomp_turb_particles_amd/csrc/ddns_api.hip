@@ -7,6 +7,7 @@
 #include "ddns_internal.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -14,6 +15,7 @@
 #include <fstream>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace ddns {
@@ -77,6 +79,11 @@ struct ddns_handle {
     int par = 0;                 // parity of the step in progress
     bool want_next = false;      // a further step follows in this call
     bool next_queued = false;    // its first half is already on the flow stream
+    // A member of a batch (ddns_batch): fb is a view of one realisation of the batch's
+    // flow buffers (not owned), the batch runs the flow, and the member's step is only its
+    // droplet block, whose gather waits for the batch's event phys_ev
+    bool member = false;
+    hipEvent_t phys_ev = nullptr;
     std::string err;
 };
 
@@ -288,7 +295,10 @@ extern "C" int ddns_read_parameter_file(const char* path, ddns_params* p)
 }
 
 // ------------------------------------------------------------------ lifetime
-extern "C" int ddns_create(const ddns_params* pin, ddns_handle** out)
+// A handle with its own flow buffers (view == nullptr), or a batch member whose flow
+// buffers are the given view of the batch's (nothing allocated or tuned for the flow).
+static void destroy_impl(ddns_handle* h);
+static int create_impl(const ddns_params* pin, ddns_handle** out, const FlowBuffers* view)
 {
     if (!pin || !out) return fail(nullptr, DDNS_ERR_ARG, "null argument");
     *out = nullptr;
@@ -322,19 +332,32 @@ extern "C" int ddns_create(const ddns_params* pin, ddns_handle** out)
     st = spectrum_scratch_init();
     if (st) { delete h; return st; }
     if (hipStreamCreate(&h->stream) != hipSuccess) { delete h; return fail(nullptr, DDNS_ERR_HIP, "stream"); }
-    if (hipStreamCreate(&h->dstream) != hipSuccess) { ddns_destroy(h); return fail(nullptr, DDNS_ERR_HIP, "stream"); }
-    st = flow_alloc(h->fb, N, h->prec);
-    if (st) { ddns_destroy(h); return st; }
+    if (hipStreamCreate(&h->dstream) != hipSuccess) { destroy_impl(h); return fail(nullptr, DDNS_ERR_HIP, "stream"); }
+    if (view) {
+        h->fb = *view;
+        h->member = true;
+        // the member's own scratch of the spectrum (the diagnostics allocate theirs on first use)
+        if (hipMalloc(&h->fb.spec_part, ((size_t)SPEC_BLOCKS + 1) * SPEC_NEK_MAX * sizeof(double)) != hipSuccess) {
+            h->fb.spec_part = nullptr;
+            destroy_impl(h);
+            return fail(nullptr, DDNS_ERR_HIP, "hipMalloc");
+        }
+    } else {
+        st = flow_alloc(h->fb, N, h->prec);
+        if (st) { destroy_impl(h); return st; }
+    }
     h->fc.N = N; h->fc.nek = h->nek; h->fc.forcing = h->p.forcing; h->fc.dt = h->p.dt; h->fc.rnu = h->p.rnu;
     h->fc.force0 = h->p.force[0]; h->fc.force1 = h->p.force[1];
     h->fc.ou_tf = h->p.ou_tf; h->fc.ou_var = h->p.ou_var; h->fc.pi2 = pi2_ref();
-    st = flow_tune(h->fb, h->fc, h->prec, h->stream);      // tile shapes, as FFTW_MEASURE plans (mpifft.f90:36-56)
-    if (st) { ddns_destroy(h); return st; }
+    if (!view) {
+        st = flow_tune(h->fb, h->fc, h->prec, h->stream);  // tile shapes, as FFTW_MEASURE plans (mpifft.f90:36-56)
+        if (st) { destroy_impl(h); return st; }
+    }
     // stochastic forcing starts from rest with iseedf = -iseedf (turb.f90:441-447)
     {
         int stt[34] = {0};
         stt[0] = -h->p.iseedf;
-        if (hipMemcpy(h->fb.sth_state, stt, sizeof(stt), hipMemcpyHostToDevice) != hipSuccess) { ddns_destroy(h); return fail(nullptr, DDNS_ERR_HIP, "copy"); }
+        if (hipMemcpy(h->fb.sth_state, stt, sizeof(stt), hipMemcpyHostToDevice) != hipSuccess) { destroy_impl(h); return fail(nullptr, DDNS_ERR_HIP, "copy"); }
     }
     // droplets
     h->pc.N = N;
@@ -348,33 +371,48 @@ extern "C" int ddns_create(const ddns_params* pin, ddns_handle** out)
         const double pi2 = h->pc.pi2;
         if (pi2 <= h->pc.d_coll || pi2 <= 2.0 * h->pc.radmax || (h->p.hdi_included && pi2 <= h->pc.d_hdi) ||
             2 * h->pc.nx_max + 1 >= N) {
-            ddns_destroy(h);
+            destroy_impl(h);
             return fail(nullptr, DDNS_ERR_DOMAIN, "cell/halo too thin for the collision or HI radius (partlib.f90:181-220)");
         }
     }
     st = part_alloc(h->pb, cap, h->p.nset, 1);
-    if (st) { ddns_destroy(h); return st; }
+    if (st) { destroy_impl(h); return st; }
     st = part_set_rng(h->pb, h->p.relocate_seed, h->stream);
-    if (st) { ddns_destroy(h); return st; }
+    if (st) { destroy_impl(h); return st; }
     for (int i = 0; i < 16; ++i)
-        if (hipEventCreate(&h->ev[i]) != hipSuccess) { ddns_destroy(h); return fail(nullptr, DDNS_ERR_HIP, "event"); }
+        if (hipEventCreate(&h->ev[i]) != hipSuccess) { destroy_impl(h); return fail(nullptr, DDNS_ERR_HIP, "event"); }
     for (int q = 0; q < 2; ++q) {
         for (int i = 0; i < FLOW_EVENTS; ++i)
-            if (hipEventCreate(&h->fev[q][i]) != hipSuccess) { ddns_destroy(h); return fail(nullptr, DDNS_ERR_HIP, "event"); }
+            if (hipEventCreate(&h->fev[q][i]) != hipSuccess) { destroy_impl(h); return fail(nullptr, DDNS_ERR_HIP, "event"); }
         for (int i = 0; i < 4; ++i)
-            if (hipEventCreate(&h->flev[q][i]) != hipSuccess) { ddns_destroy(h); return fail(nullptr, DDNS_ERR_HIP, "event"); }
+            if (hipEventCreate(&h->flev[q][i]) != hipSuccess) { destroy_impl(h); return fail(nullptr, DDNS_ERR_HIP, "event"); }
     }
-    if (hipStreamSynchronize(h->stream) != hipSuccess) { ddns_destroy(h); return fail(nullptr, DDNS_ERR_HIP, "sync"); }
+    if (hipStreamSynchronize(h->stream) != hipSuccess) { destroy_impl(h); return fail(nullptr, DDNS_ERR_HIP, "sync"); }
     *out = h;
     return DDNS_OK;
 }
 
+extern "C" int ddns_create(const ddns_params* pin, ddns_handle** out) { return create_impl(pin, out, nullptr); }
+
 extern "C" int ddns_destroy(ddns_handle* h)
 {
     if (!h) return DDNS_OK;
+    if (h->member) return fail(h, DDNS_ERR_ARG, "a batch member is destroyed with its batch");
+    destroy_impl(h);
+    return DDNS_OK;
+}
+
+static void destroy_impl(ddns_handle* h)
+{
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->dstream) (void)hipStreamSynchronize(h->dstream);
-    flow_free(h->fb);
+    if (!h->member) {
+        flow_free(h->fb);
+    } else {                     // a view: only its own scratch
+        for (int q = 0; q < 8; ++q) if (h->fb.dg[q]) (void)hipFree(h->fb.dg[q]);
+        if (h->fb.dg_part) (void)hipFree(h->fb.dg_part);
+        if (h->fb.spec_part) (void)hipFree(h->fb.spec_part);
+    }
     part_free(h->pb);
     for (int i = 0; i < 16; ++i) if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
     for (int q = 0; q < 2; ++q) {
@@ -385,7 +423,6 @@ extern "C" int ddns_destroy(ddns_handle* h)
     if (h->dstream) (void)hipStreamDestroy(h->dstream);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
-    return DDNS_OK;
 }
 
 extern "C" const char* ddns_last_error(const ddns_handle* h)
@@ -822,7 +859,7 @@ static int droplet_block(ddns_handle* h, float* ms)
     if (st) return st;
     // the binning and the history ran beside the flow's first half; the gather needs
     // the physical velocity it writes
-    DDNS_HIP(hipStreamWaitEvent(s, h->flev[h->par][1], 0));
+    DDNS_HIP(hipStreamWaitEvent(s, h->member ? h->phys_ev : h->flev[h->par][1], 0));
     st = part_interp(h->pb, pc, h->fb.uphys, s);                              // 553-556
     if (st) return st;
     DDNS_HIP(hipEventRecord(h->ev[10], s));
@@ -855,8 +892,10 @@ static int droplet_block(ddns_handle* h, float* ms)
     st = part_collide_search(h->pb, pc, s);                                   // 584, search and pairs
     if (st) return st;
     if (h->z_gate) DDNS_HIP(hipEventRecord(h->ev[6], s));                     // the pair kernel is queued
-    st = queue_flow_rest(h, h->ev[11]);                                       // after the HI solve
-    if (st) return st;
+    if (!h->member) {                              // a batch queues its flow itself
+        st = queue_flow_rest(h, h->ev[11]);                                   // after the HI solve
+        if (st) return st;
+    }
     const bool gated = h->z_pending;
     if (gated) {
         st = queue_flow_z(h, h->ev[6]);
@@ -946,6 +985,7 @@ static int step_fail(ddns_handle* h, int st)
 extern "C" int ddns_step(ddns_handle* h, int nsteps)
 {
     if (!h || nsteps < 0) return DDNS_ERR_ARG;
+    if (h->member) return fail(h, DDNS_ERR_ARG, "a batch member steps with its batch (ddns_batch_step)");
     hipStream_t s = h->stream;
     int status = 0, st;
     float pms[6] = {0, 0, 0, 0, 0, 0};
@@ -1129,4 +1169,304 @@ extern "C" int ddns_profile_copy(ddns_handle* h, int reps, double* ms_per_launch
     // the copy overwrites the inverse pipeline, which the next step would start from
     h->a_valid = false;
     return keep(h, flow_profile_copy(h->fb, h->N, h->prec, reps, ms_per_launch, bytes_per_launch, h->stream));
+}
+
+// ------------------------------------------------------------------ batches
+// R independent realisations of the same grid, precision, time step and viscosity (the
+// reference runs each as its own job; turb.f90:492-707 is the step of one).  The batch
+// holds the flow fields of all of them side by side and runs every flow stage as one
+// launch over all realisations, replayed from two HIP graphs per step.  Each realisation
+// is a member handle: a view of its flow fields plus its own droplets, forcing kind and
+// random streams, reached through ddns_batch_member for state transfers and diagnostics.
+// A step queues the batch's inverse stages, then its forward stages and update (they
+// touch nothing the droplets read), and runs the members' droplet blocks on worker
+// threads, each on its own streams, beside the flow.
+struct ddns_batch {
+    int R = 0, N = 0, prec = 0, nek = 0;
+    std::vector<ddns_handle*> m;
+    FlowBuffers fb;
+    FlowConst fc{};
+    hipStream_t stream = nullptr;
+    hipEvent_t flev[4] = {};               // inverse half start, end; forward half start, end
+    hipEvent_t fev[FLOW_EVENTS] = {};
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    hipGraphExec_t gflow[2] = {};          // FLOW_INV, FLOW_FWD
+    int graph_mode = 1;
+    int threads = 8;                       // worker threads of the droplet blocks
+    ddns_timing timing{};
+    std::string err;
+};
+
+static int bfail(ddns_batch* b, int code, const std::string& msg)
+{
+    set_error(msg);
+    if (b) b->err = msg;
+    return code;
+}
+
+extern "C" int ddns_batch_destroy(ddns_batch* b)
+{
+    if (!b) return DDNS_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (ddns_handle* h : b->m)
+        if (h) destroy_impl(h);
+    flow_free(b->fb);
+    for (auto& e : b->flev) if (e) (void)hipEventDestroy(e);
+    for (auto& e : b->fev) if (e) (void)hipEventDestroy(e);
+    if (b->t0) (void)hipEventDestroy(b->t0);
+    if (b->t1) (void)hipEventDestroy(b->t1);
+    for (auto& g : b->gflow) if (g) (void)hipGraphExecDestroy(g);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return DDNS_OK;
+}
+
+extern "C" int ddns_batch_create(const ddns_params* params, int nreal, ddns_batch** out)
+{
+    if (!params || !out || nreal < 1) return bfail(nullptr, DDNS_ERR_ARG, "null argument or no realisation");
+    *out = nullptr;
+    const ddns_params& p0 = params[0];
+    const int N = p0.nx;
+    if (N > 256) return bfail(nullptr, DDNS_ERR_ARG, "batched realisations need N <= 256 (the fused z stage)");
+    for (int r = 1; r < nreal; ++r) {
+        const ddns_params& p = params[r];
+        if (p.nx != N || p.ny != p0.ny || p.nz != p0.nz || p.precision != p0.precision || p.dt != p0.dt ||
+            p.rnu != p0.rnu)
+            return bfail(nullptr, DDNS_ERR_ARG, "the realisations of a batch share grid, precision, dt and viscosity");
+    }
+    ddns_batch* b = new ddns_batch();
+    b->R = nreal;
+    b->N = N;
+    b->prec = p0.precision;
+    b->nek = (int)(N / 2 - 1.5);
+    if (const char* g = std::getenv("DDNS_GRAPH")) b->graph_mode = g[0] != '0';
+    if (const char* t = std::getenv("DDNS_BATCH_THREADS")) b->threads = std::max(1, std::atoi(t));
+    int st = 0;
+    if (hipStreamCreate(&b->stream) != hipSuccess) { ddns_batch_destroy(b); return bfail(nullptr, DDNS_ERR_HIP, "stream"); }
+    for (auto& e : b->flev) if (hipEventCreate(&e) != hipSuccess) st = DDNS_ERR_HIP;
+    for (auto& e : b->fev) if (hipEventCreate(&e) != hipSuccess) st = DDNS_ERR_HIP;
+    if (hipEventCreate(&b->t0) != hipSuccess || hipEventCreate(&b->t1) != hipSuccess) st = DDNS_ERR_HIP;
+    if (st) { ddns_batch_destroy(b); return bfail(nullptr, st, "event"); }
+    // the shape checks of ddns_create run on the first member below; the flow buffers of
+    // all realisations, then the tile shapes timed on them
+    if (N != p0.ny || N != p0.nz || N < 32 || (N & (N - 1))) { ddns_batch_destroy(b); return bfail(nullptr, DDNS_ERR_ARG, "grid must be cubic, a power of two, 32..256"); }
+    if (b->prec != DDNS_FP64 && b->prec != DDNS_FP32_MIXED) { ddns_batch_destroy(b); return bfail(nullptr, DDNS_ERR_ARG, "precision must be DDNS_FP64 or DDNS_FP32_MIXED"); }
+    st = flow_alloc(b->fb, N, b->prec, nreal);
+    if (st) { ddns_batch_destroy(b); return st; }
+    b->fc.N = N; b->fc.nek = b->nek; b->fc.forcing = -1; b->fc.dt = p0.dt; b->fc.rnu = p0.rnu;
+    b->fc.pi2 = pi2_ref();
+    st = flow_tune(b->fb, b->fc, b->prec, b->stream);
+    if (st) { ddns_batch_destroy(b); return st; }
+    b->m.assign(nreal, nullptr);
+    for (int r = 0; r < nreal; ++r) {
+        const FlowBuffers v = flow_view(b->fb, r, b->prec);
+        st = create_impl(&params[r], &b->m[r], &v);
+        if (st) { ddns_batch_destroy(b); return st; }
+        const ddns_handle* h = b->m[r];
+        st = flow_set_realisation(b->fb, r, h->p.forcing, h->p.force[0], h->p.force[1], h->p.ou_tf, h->p.ou_var);
+        if (st) { ddns_batch_destroy(b); return st; }
+    }
+    *out = b;
+    return DDNS_OK;
+}
+
+extern "C" ddns_handle* ddns_batch_member(ddns_batch* b, int r)
+{
+    if (!b || r < 0 || r >= b->R) return nullptr;
+    return b->m[r];
+}
+
+extern "C" int ddns_batch_size(const ddns_batch* b) { return b ? b->R : 0; }
+
+extern "C" const char* ddns_batch_last_error(const ddns_batch* b)
+{
+    if (b && !b->err.empty()) return b->err.c_str();
+    return g_err.c_str();
+}
+
+// one half of the batch's flow step (FLOW_INV or FLOW_FWD), from its graph when it has one
+static int batch_flow_half(ddns_batch* b, int part)
+{
+    hipStream_t s = b->stream;
+    const int gi = part == FLOW_INV ? 0 : 1;
+    if (b->graph_mode) {
+        hipGraphExec_t& gx = b->gflow[gi];
+        if (!gx) {
+            hipGraph_t g = nullptr;
+            int st = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess ? 0 : DDNS_ERR_HIP;
+            if (!st) {
+                st = flow_step(b->fb, b->fc, b->prec, part, s, b->fev, true);
+                const hipError_t e = hipStreamEndCapture(s, &g);
+                if (!st && e != hipSuccess) st = DDNS_ERR_HIP;
+            }
+            if (!st && hipGraphInstantiate(&gx, g, nullptr, nullptr, 0) != hipSuccess) {
+                gx = nullptr;
+                st = DDNS_ERR_HIP;
+            }
+            if (g) (void)hipGraphDestroy(g);
+            if (st) {
+                (void)hipGetLastError();
+                b->graph_mode = 0;
+                for (auto& x : b->gflow) if (x) { (void)hipGraphExecDestroy(x); x = nullptr; }
+            }
+        }
+        if (gx) {
+            DDNS_HIP(hipGraphLaunch(gx, s));
+            return 0;
+        }
+    }
+    return flow_step(b->fb, b->fc, b->prec, part, s, b->fev);
+}
+
+// after a failed step: every stream idle, and the next step rebuilds the inverse pipeline
+static int batch_fail(ddns_batch* b, int st)
+{
+    (void)hipStreamSynchronize(b->stream);
+    for (ddns_handle* h : b->m) {
+        (void)hipStreamSynchronize(h->dstream);
+        (void)hipStreamSynchronize(h->stream);
+        h->a_valid = false;
+    }
+    if (st != DDNS_OK && b->err.empty()) b->err = g_err;
+    return st;
+}
+
+extern "C" int ddns_batch_step(ddns_batch* b, int nsteps)
+{
+    if (!b || nsteps < 0) return DDNS_ERR_ARG;
+    b->err.clear();
+    hipStream_t s = b->stream;
+    const int R = b->R;
+    int status = 0, st;
+    double flow_ms = 0, kms[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<std::array<float, 6>> pms(R);
+    for (auto& a : pms) a.fill(0.f);
+    std::vector<int> rc(R, 0);
+    DDNS_HIP(hipEventRecord(b->t0, s));
+    for (int k = 0; k < nsteps; ++k) {
+        bool prologue = false;
+        for (ddns_handle* h : b->m) {
+            h->istep += 1;                                                    // turb.f90:492
+            if (!h->a_valid) prologue = true;
+        }
+        // the inverse pipeline of every realisation (rebuilt from u-hat when any member's
+        // state was set since the last step), the inverse y and x stages and the physical
+        // velocity; then the forward stages and the update, which read nothing the
+        // droplet blocks write and write nothing they read
+        DDNS_HIP(hipEventRecord(b->flev[0], s));
+        if (prologue) {
+            st = flow_prologue(b->fb, b->fc, b->prec, s);
+            if (st) return batch_fail(b, st);
+        }
+        st = batch_flow_half(b, FLOW_INV);
+        if (st) return batch_fail(b, st);
+        DDNS_HIP(hipEventRecord(b->flev[1], s));
+        DDNS_HIP(hipEventRecord(b->flev[2], s));
+        st = batch_flow_half(b, FLOW_FWD);
+        if (st) return batch_fail(b, st);
+        DDNS_HIP(hipEventRecord(b->flev[3], s));
+        for (ddns_handle* h : b->m) {
+            h->a_valid = true;
+            h->phys_valid = true;
+            h->phys_ev = b->flev[1];
+        }
+        // the droplet blocks (546-623), members spread over the worker threads
+        std::fill(rc.begin(), rc.end(), 0);
+        auto work = [&](int w, int nw) {
+            for (int r = w; r < R; r += nw) {
+                ddns_handle* h = b->m[r];
+                h->cc = ddns_collision_counts{};
+                if (!(h->p.npart > 0 && h->n > 0)) continue;
+                int q = droplet_block(h, pms[r].data());
+                if (q >= 0 && h->n < h->p.npart / 2)
+                    q = fail(h, DDNS_ERR_DROPLET_LOSS, "npart_t < npart/2 (turb.f90:600-604)");
+                rc[r] = q;
+            }
+        };
+        const int nw = std::min(b->threads, R);
+        if (nw <= 1) {
+            work(0, 1);
+        } else {
+            std::vector<std::thread> pool;
+            for (int w = 0; w < nw; ++w) pool.emplace_back(work, w, nw);
+            for (auto& t : pool) t.join();
+        }
+        for (int r = 0; r < R; ++r) {
+            if (rc[r] < 0) {
+                b->err = "realisation " + std::to_string(r) + ": " + ddns_last_error(b->m[r]);
+                return batch_fail(b, rc[r]);
+            }
+            if (rc[r] > 0 && status == 0) status = rc[r];
+        }
+        // the next step's inverse half overwrites the physical velocity: it waits for
+        // every droplet stream
+        for (ddns_handle* h : b->m) {
+            DDNS_HIP(hipEventRecord(h->ev[5], h->dstream));
+            DDNS_HIP(hipStreamWaitEvent(s, h->ev[5], 0));
+        }
+        DDNS_HIP(hipEventSynchronize(b->flev[3]));
+        const int pairs[6][2] = {{0, 1}, {1, 2}, {3, 4}, {4, 5}, {8, 6}, {6, 7}};
+        for (int i = 0; i < 6; ++i) {
+            float t;
+            DDNS_HIP(hipEventElapsedTime(&t, b->fev[pairs[i][0]], b->fev[pairs[i][1]]));
+            kms[i] += t;
+        }
+        float t1, t2;
+        DDNS_HIP(hipEventElapsedTime(&t1, b->flev[0], b->flev[1]));
+        DDNS_HIP(hipEventElapsedTime(&t2, b->flev[2], b->flev[3]));
+        flow_ms += t1 + t2;
+        for (ddns_handle* h : b->m) h->time += h->p.dt;                       // 693
+    }
+    DDNS_HIP(hipEventRecord(b->t1, s));
+    DDNS_HIP(hipEventSynchronize(b->t1));
+    float tot = 0;
+    DDNS_HIP(hipEventElapsedTime(&tot, b->t0, b->t1));
+    b->timing = ddns_timing{};
+    b->timing.flow_ms = flow_ms;
+    b->timing.total_ms = tot;
+    b->timing.nsteps = nsteps;
+    for (int q = 0; q < 6; ++q) b->timing.flow_kernel_ms[q] = kms[q];
+    for (int r = 0; r < R; ++r) {
+        ddns_handle* h = b->m[r];
+        h->timing = ddns_timing{};
+        h->timing.cells_ms = pms[r][0];
+        h->timing.interp_ms = pms[r][1];
+        h->timing.hdi_ms = pms[r][2];
+        h->timing.advance_ms = pms[r][3];
+        h->timing.colldet_ms = pms[r][4];
+        h->timing.overlap_ms = pms[r][5];
+        h->timing.nsteps = nsteps;
+        // the batch-wide sums: droplet phases added over the realisations
+        b->timing.cells_ms += pms[r][0];
+        b->timing.interp_ms += pms[r][1];
+        b->timing.hdi_ms += pms[r][2];
+        b->timing.advance_ms += pms[r][3];
+        b->timing.colldet_ms += pms[r][4];
+        b->timing.overlap_ms += pms[r][5];
+    }
+    return status;
+}
+
+extern "C" int ddns_batch_get_timing(ddns_batch* b, ddns_timing* out)
+{
+    if (!b || !out) return DDNS_ERR_ARG;
+    *out = b->timing;
+    return DDNS_OK;
+}
+
+extern "C" int ddns_batch_get_tuning(ddns_batch* b, int* shapes, double* ms)
+{
+    if (!b) return DDNS_ERR_ARG;
+    if (shapes) {
+        shapes[0] = b->fb.ky_tc;
+        shapes[1] = flow_default_ky_tc(b->N, b->prec);
+        shapes[2] = b->fb.kz_nt;
+        shapes[3] = flow_default_kz_nt(b->N, b->prec);
+    }
+    if (ms)
+        for (int k = 0; k < 3; ++k) {
+            ms[k] = b->fb.ky_ms[k];
+            ms[3 + k] = b->fb.kz_ms[k];
+        }
+    return DDNS_OK;
 }

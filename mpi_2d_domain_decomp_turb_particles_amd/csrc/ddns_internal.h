@@ -223,7 +223,16 @@ struct FlowBuffers {
     float ky_ms[3] = {0, 0, 0};                 // the tuner's time per y stage pair of each candidate (0: not valid)
     int kz_nt = 0;                              // threads of the fused z stage chosen by flow_tune (0: the default)
     float kz_ms[3] = {0, 0, 0};                 // the tuner's time per fused z stage of each candidate (0: not valid)
+    // Independent realisations held side by side (a batch): every field above holds R
+    // realisations, realisation r at r * rstride complex values (uphys: r * pstride reals,
+    // low: r * 1024 doubles, sth_state: r * 64 ints, sth_b: r * 512 doubles), and every
+    // flow launch covers all of them.  R = 1, rstride = pstride = 0 for one realisation.
+    int R = 1;
+    size_t rstride = 0, pstride = 0;
 };
+// The buffers of realisation r of a batch as a one-realisation FlowBuffers (pointers
+// into the batch's arrays; nothing owned), for the transfers and diagnostics.
+FlowBuffers flow_view(const FlowBuffers& fb, int r, int precision);
 
 // E(k) shell sums (spectrum.hip): work-groups of the partial-sum kernel and the most
 // shells any grid has (nek = 510 at N = 1024)
@@ -241,7 +250,11 @@ struct FlowConst {
 };
 
 // launch wrappers (flow.hip)
-int flow_alloc(FlowBuffers& fb, int N, int precision);
+int flow_alloc(FlowBuffers& fb, int N, int precision, int R = 1);
+// the forcing kind and amplitudes of realisation r into its low-mode table (batches run
+// with FlowConst::forcing = -1 and read them there)
+int flow_set_realisation(const FlowBuffers& fb, int r, int forcing, double force0, double force1, double ou_tf,
+                         double ou_var);
 void flow_free(FlowBuffers& fb);
 int flow_prologue(const FlowBuffers& fb, const FlowConst& fc, int precision, hipStream_t s);
 // Parts of the flow step: the inverse stages (first half), the whole forward half, or the

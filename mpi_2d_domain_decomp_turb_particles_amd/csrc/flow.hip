@@ -115,6 +115,21 @@ template <typename T, int N> constexpr int kz_ldr()
 
 __device__ __forceinline__ int wave_index(int i, int N) { return i <= N / 2 ? i : i - N; }
 
+// Batched realisations: realisation r of a field starts r * rstride complex values after
+// realisation 0 (rstride = 0 for one realisation), its low-mode table r * LOW_STRIDE
+// doubles after, and the physical velocity r * pstride reals after.  With forcing < 0
+// every realisation takes its forcing kind and amplitudes from its own low-mode table
+// (LOW_PAR onwards).
+constexpr int LOW_STRIDE = 1024;
+constexpr int LOW_PAR = 1000;       // [0] forcing kind, [1..2] force(1:2), [3] OU time scale, [4] OU variance
+constexpr int STH_STATE_STRIDE = 64;
+constexpr int STH_B_STRIDE = 512;
+
+__device__ __forceinline__ int real_forcing(int forcing, const double* lowr)
+{
+    return forcing >= 0 ? forcing : (int)lowr[LOW_PAR];
+}
+
 struct KzArgs {
     const void* B[3];
     void* U[3];
@@ -125,12 +140,15 @@ struct KzArgs {
     double dt, rnu;
     int nek, forcing;
     double scale;      // plain modes: output scale
+    size_t rstride;    // complex values between realisations of one field
 };
 
 struct KyArgs {
     void* F[6];
     const void* tw;
     double scale;
+    int nf, R;         // fields per realisation, realisations: grid z runs over nf * R
+    size_t rstride;
 };
 
 struct KxArgs {
@@ -139,6 +157,8 @@ struct KxArgs {
     void* uphys;
     const void* tw;    // N entries (post/pre-processing twiddles)
     const void* twh;   // N/2 entries (half-length FFT)
+    size_t rstride;    // grid y runs over the realisations
+    size_t pstride;    // reals between realisations of the physical velocity
 };
 
 // ------------------------------------------------------------------ spectral update
@@ -218,7 +238,7 @@ __global__ void __launch_bounds__((kz_threads<T, N, MODE>())) k_z(KzArgs a)
     stage_twiddles<C, N, NTK>(tws, (const C*)a.tw, tid);
     const C* tw = tws;
     const size_t rowstride = (size_t)N * M;
-    const size_t base = (size_t)ky * M + kx0;
+    const size_t base = (size_t)blockIdx.y * a.rstride + (size_t)ky * M + kx0;
 
     if constexpr (MODE == KZ_PLAIN_FWD || MODE == KZ_PLAIN_INV) {
         const C* src = (const C*)a.B[0];
@@ -361,17 +381,26 @@ __global__ void __launch_bounds__(NTZ) k_z_full(KzArgs a, int ntiles)
     stage_twiddles<C, N, NTZ>(tws, (const C*)a.tw, tid);
     const C* tw = tws;
     const size_t rowstride = (size_t)N * M;
+    // ntiles counts the tiles of every realisation: tile = r * TPR + local tile
+    constexpr int TPR = NBX * N;
+    auto tile_base = [&](int tl) {
+        const int r = tl / TPR, lt = tl - r * TPR;
+        return (size_t)r * a.rstride + (size_t)(lt / NBX) * M + (lt % NBX) * TC;
+    };
     using TR = TileRegs<C, N, TC, NTZ>;
     TR nb[3];
     int tile = blockIdx.x;
     if (tile < ntiles) {
-        const size_t base = (size_t)(tile / NBX) * M + (tile % NBX) * TC;
+        const size_t base = tile_base(tile);
 #pragma unroll
         for (int q = 0; q < 3; ++q) tile_fetch<C, N, TC, NTZ>(nb[q], (const C*)a.B[q] + base, rowstride, tid);
     }
     for (; tile < ntiles; tile += gridDim.x) {
-        const int kx0 = (tile % NBX) * TC, ky = tile / NBX;
-        const size_t base = (size_t)ky * M + kx0;
+        const int rr = tile / TPR, lt = tile - rr * TPR;
+        const int kx0 = (lt % NBX) * TC, ky = lt / NBX;
+        const size_t base = (size_t)rr * a.rstride + (size_t)ky * M + kx0;
+        const double* lowr = a.low + (size_t)rr * LOW_STRIDE;
+        const int forcing = real_forcing(a.forcing, lowr);
         if (!NEXT_B && tile != (int)blockIdx.x) {
 #pragma unroll
             for (int q = 0; q < 3; ++q) tile_fetch<C, N, TC, NTZ>(nb[q], (const C*)a.B[q] + base, rowstride, tid);
@@ -399,7 +428,7 @@ __global__ void __launch_bounds__(NTZ) k_z_full(KzArgs a, int ntiles)
             C nn[3], vv[3], nd[3];
 #pragma unroll
             for (int q = 0; q < 3; ++q) nn[q] = cscale(lds[q][n * LDR + c], inv_n);
-            update_mode<T>(kx0 + c, ky, n, N, a.nek, a.forcing, (T)a.dt, (T)a.rnu, a.low, nn, u3, o3, vv, nd);
+            update_mode<T>(kx0 + c, ky, n, N, a.nek, forcing, (T)a.dt, (T)a.rnu, lowr, nn, u3, o3, vv, nd);
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
                 ((C*)a.U[q])[g] = vv[q];
@@ -426,7 +455,7 @@ __global__ void __launch_bounds__(NTZ) k_z_full(KzArgs a, int ntiles)
         asm volatile("" ::: "memory");
         const int nt = tile + (int)gridDim.x;
         if (NEXT_B && nt < ntiles) {
-            const size_t nbase = (size_t)(nt / NBX) * M + (nt % NBX) * TC;
+            const size_t nbase = tile_base(nt);
 #pragma unroll
             for (int q = 0; q < 3; ++q) tile_fetch<C, N, TC, NTZ>(nb[q], (const C*)a.B[q] + nbase, rowstride, tid);
         }
@@ -480,7 +509,8 @@ __global__ void __launch_bounds__(NT) k_y(KyArgs a)
     }
     const int kx0 = kxb * TC;
     stage_twiddles<C, N, NT>(tws, (const C*)a.tw, tid);
-    C* f = (C*)a.F[fi];
+    const int rr = fi / a.nf;
+    C* f = (C*)a.F[fi - rr * a.nf] + (size_t)rr * a.rstride;
     const size_t base = (size_t)z * N * M + kx0;
     tile_load<C, N, TC, LDR, NT>(lds, f + base, M, tid);
     __syncthreads();
@@ -608,21 +638,22 @@ __global__ void __launch_bounds__(NT) k_x_fused(KxArgs a)
     __shared__ C tws[N], twhs[M];
     const int tid = threadIdx.x;
     const size_t line0 = (size_t)blockIdx.x * TL;
+    const size_t roff = (size_t)blockIdx.y * a.rstride;     // this realisation's fields
     stage_twiddles<C, N, NT>(tws, (const C*)a.tw, tid);
     stage_twiddles<C, M, NT>(twhs, (const C*)a.twh, tid);
     const C* tw = tws;
     const C* twh = twhs;
     C ph[6][QP];
     TileRegs<C, TL, M, NT> next0, next1;
-    tile_fetch<C, TL, M, NT>(next0, (const C*)a.A[0] + line0 * M, M, tid);
-    tile_fetch<C, TL, M, NT>(next1, (const C*)a.A[1] + line0 * M, M, tid);
+    tile_fetch<C, TL, M, NT>(next0, (const C*)a.A[0] + roff + line0 * M, M, tid);
+    tile_fetch<C, TL, M, NT>(next1, (const C*)a.A[1] + roff + line0 * M, M, tid);
 #pragma unroll
     for (int f = 0; f < 6; f += 2) {
         tile_put<C, TL, M, LDR, NT, true>(lds, next0, tid);
         tile_put<C, TL, M, LDR, NT, true>(lds + TS, next1, tid);
         if (f < 4) {
-            tile_fetch<C, TL, M, NT>(next0, (const C*)a.A[f + 2] + line0 * M, M, tid);
-            tile_fetch<C, TL, M, NT>(next1, (const C*)a.A[f + 3] + line0 * M, M, tid);
+            tile_fetch<C, TL, M, NT>(next0, (const C*)a.A[f + 2] + roff + line0 * M, M, tid);
+            tile_fetch<C, TL, M, NT>(next1, (const C*)a.A[f + 3] + roff + line0 * M, M, tid);
         }
         __syncthreads();
         c2r_pack<T, N, TL, LDR>(lds, tw, tid);
@@ -641,7 +672,7 @@ __global__ void __launch_bounds__(NT) k_x_fused(KxArgs a)
     // physical velocity for the droplet gather: [line][x][3], the two nodes of a complex
     // value in three vector stores
     {
-        T* up = (T*)a.uphys + line0 * (size_t)N * 3;
+        T* up = (T*)a.uphys + (size_t)blockIdx.y * a.pstride + line0 * (size_t)N * 3;
 #pragma unroll
         for (int q = 0; q < QP; ++q) {
             const int e = tid + NT * q;
@@ -666,7 +697,7 @@ __global__ void __launch_bounds__(NT) k_x_fused(KxArgs a)
     fft_tile<T, M, false, TL, LDR, NT, 3, TS>(lds, twh, tid);
     const T scale = (T)1 / (T)N;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) r2c_unpack_store<T, N, TL, LDR>(lds + c * TS, tw, (C*)a.B[c], line0, scale, tid);
+    for (int c = 0; c < 3; ++c) r2c_unpack_store<T, N, TL, LDR>(lds + c * TS, tw, (C*)a.B[c] + roff, line0, scale, tid);
 }
 
 // Stand-alone x transforms of one field (kernel-level parity of mpifft3DRC/CR).
@@ -717,7 +748,8 @@ struct PreArgs {
     int* sth_state;
     double* sth_b;
     double dt, rnu, force0, force1, ou_tf, ou_var, pi2;
-    int nek, forcing;
+    int nek, forcing;  // forcing < 0: each realisation's kind and amplitudes from its table
+    size_t rstride;    // complex values between realisations of one field
 };
 
 __device__ double dev_ranff(int* idum, int* iy, int* iv)
@@ -766,7 +798,9 @@ __global__ void __launch_bounds__(NT) k_pre_dft(PreArgs a)
     const int id = blockIdx.x, comp = id / 75, mi = id % 75;
     const int kx = mi % 3, kyw = (mi / 3) % 5 - 2, kzw = mi / 15 - 2;
     const int ky = (kyw + N) % N, kzi = (kzw + N) % N;
-    const C* b = (const C*)a.B[comp];
+    double* low = a.low + (size_t)blockIdx.y * LOW_STRIDE;     // realisation blockIdx.y
+    if (real_forcing(a.forcing, low) != 1) return;
+    const C* b = (const C*)a.B[comp] + (size_t)blockIdx.y * a.rstride;
     const C* tw = (const C*)a.tw;
     double sr = 0.0, si = 0.0;
     for (int z = threadIdx.x; z < N; z += NT) {
@@ -787,8 +821,8 @@ __global__ void __launch_bounds__(NT) k_pre_dft(PreArgs a)
         double r = 0.0, i = 0.0;
 #pragma unroll
         for (int w = 0; w < NT / 64; ++w) { r += wr[w]; i += wi[w]; }
-        a.low[LOW_DFT + id] = r / N;
-        a.low[LOW_DFT + 225 + id] = i / N;
+        low[LOW_DFT + id] = r / N;
+        low[LOW_DFT + 225 + id] = i / N;
     }
 }
 
@@ -798,16 +832,21 @@ __global__ void __launch_bounds__(NT) k_pre(PreArgs a)
     using C = cpx<T>;
     constexpr int M = N / 2;
     const int tid = threadIdx.x;
-    if (a.forcing == 2) {
+    const int rr = blockIdx.x;                                       // realisation
+    double* low = a.low + (size_t)rr * LOW_STRIDE;
+    const int forcing = real_forcing(a.forcing, low);
+    const bool own = a.forcing < 0;
+    if (forcing == 2) {
         if (tid != 0) return;
-        int* st = a.sth_state;
+        int* st = a.sth_state + (size_t)rr * STH_STATE_STRIDE;
         int idum = st[0], iy = st[1];
         // shuffle table in LDS: a private array indexed by a data-dependent j is
         // lowered to indexed register moves, which gives no protection at all
         // against a stray index
         __shared__ int iv[32];
         for (int i = 0; i < 32; ++i) iv[i] = st[2 + i];
-        const double dt = a.dt, tf = a.ou_tf, var = a.ou_var, pi2 = a.pi2;
+        const double dt = a.dt, tf = own ? low[LOW_PAR + 3] : a.ou_tf, var = own ? low[LOW_PAR + 4] : a.ou_var;
+        const double pi2 = a.pi2;
         double fr[3][150];
         for (int mc = 0; mc < 3; ++mc)                                   // flowlib.f90:405-419
             for (int k = 0; k < 5; ++k)
@@ -837,8 +876,8 @@ __global__ void __launch_bounds__(NT) k_pre(PreArgs a)
             fr[mc][0] = 0.0;
             fr[mc][1] = 0.0;
         }
-        double* b = a.sth_b;
-        double* fa = a.low + 8;
+        double* b = a.sth_b + (size_t)rr * STH_B_STRIDE;
+        double* fa = low + 8;
         const double smallx = 1.0e-18, xkf = sqrt(8.0);
         for (int k = 0; k < 5; ++k)                                      // 447-478
             for (int j = 0; j < 5; ++j)
@@ -884,13 +923,13 @@ __global__ void __launch_bounds__(NT) k_pre(PreArgs a)
         for (int i = 0; i < 32; ++i) st[2 + i] = iv[i];
         return;
     }
-    if (a.forcing != 1) return;
+    if (forcing != 1) return;
     // 75 low modes: kx in 0..2, ky, kz in -2..2; 225 (mode, component) DFT sums
     __shared__ double nre[225], nim[225];
     __shared__ double emode[75];
     if (tid < 225) {                      // the z-DFT sums of k_pre_dft
-        nre[tid] = a.low[LOW_DFT + tid];
-        nim[tid] = a.low[LOW_DFT + 225 + tid];
+        nre[tid] = low[LOW_DFT + tid];
+        nim[tid] = low[LOW_DFT + 225 + tid];
     }
     __syncthreads();
     if (tid < 75) {
@@ -900,8 +939,8 @@ __global__ void __launch_bounds__(NT) k_pre(PreArgs a)
         C nn[3], uu[3], oo[3], vv[3], nd[3];
         for (int q = 0; q < 3; ++q) {
             nn[q] = mk<C>((T)nre[q * 75 + tid], (T)nim[q * 75 + tid]);
-            uu[q] = ((const C*)a.U[q])[g];
-            oo[q] = ((const C*)a.O[q])[g];
+            uu[q] = ((const C*)a.U[q])[(size_t)rr * a.rstride + g];
+            oo[q] = ((const C*)a.O[q])[(size_t)rr * a.rstride + g];
         }
         // the update without the forcing factor (ff = 1 here)
         double one[2] = {1.0, 1.0};
@@ -923,16 +962,16 @@ __global__ void __launch_bounds__(NT) k_pre(PreArgs a)
             if (emode[i] > 0) e1 += emode[i];
             else e2 -= emode[i];
         }
-        a.low[0] = sqrt(a.force0 / e1);
-        a.low[1] = sqrt(a.force1 / e2);
-        a.low[2] = e1;
-        a.low[3] = e2;
+        low[0] = sqrt((own ? low[LOW_PAR + 1] : a.force0) / e1);
+        low[1] = sqrt((own ? low[LOW_PAR + 2] : a.force1) / e2);
+        low[2] = e1;
+        low[3] = e2;
     }
 }
 
 // ------------------------------------------------------------------ kx = 0 plane of u-hat
 template <typename T, int N>
-__global__ void k_sym(void* u0, void* u1, void* u2)
+__global__ void k_sym(void* u0, void* u1, void* u2, size_t rstride)
 {
     using C = cpx<T>;
     constexpr int M = N / 2;
@@ -944,7 +983,7 @@ __global__ void k_sym(void* u0, void* u1, void* u2)
     void* fs[3] = {u0, u1, u2};
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        C* f = (C*)fs[q];
+        C* f = (C*)fs[q] + (size_t)blockIdx.y * rstride;     // realisation blockIdx.y
         const size_t gp = ((size_t)kz * N + ky) * M, gm = ((size_t)mz * N + my) * M;
         const C p = f[gp], m = f[gm];
         const C s = mk<C>((T)0.5 * (p.x + m.x), (T)0.5 * (p.y - m.y));
@@ -984,10 +1023,12 @@ __device__ __forceinline__ double2 init_gauss(unsigned long long key, int c, int
 // One thread per mode: the three Gaussian components, the Hermitian kx = 0 plane
 // ((g(k) + conj g(-k)) / 2, computed from both draws so the two halves agree), the
 // Nyquist planes and the mean set to zero, the projection onto k.u = 0 (flowlib.f90:
-// 94-101); then the shell energy (kx = 0 modes at half weight) and the shell's mode
-// count, 1 per mode with kx > 0 and 1/2 on the kx = 0 plane (flowlib.f90:107-118).
+// 94-101); then twice the shell's mode count, 2 per mode with kx > 0 and 1 on the kx = 0
+// plane (flowlib.f90:107-118).  The shell energies are summed afterwards by the E(k)
+// reduction (kx = 0 modes at half weight), whose fixed order makes the field the same
+// bits on every run.
 template <typename T, int N>
-__global__ void k_init_modes(void* u0, void* u1, void* u2, unsigned long long key, int nek, double* ek, double* cnt)
+__global__ void k_init_modes(void* u0, void* u1, void* u2, unsigned long long key, int nek, unsigned long long* cnt2)
 {
     using C = cpx<T>;
     constexpr int M = N / 2;
@@ -1012,18 +1053,16 @@ __global__ void k_init_modes(void* u0, void* u1, void* u2, unsigned long long ke
     const double pr = (k[0] * g[0].x + k[1] * g[1].x + k[2] * g[2].x) / k2;
     const double pi = (k[0] * g[0].y + k[1] * g[1].y + k[2] * g[2].y) / k2;
     void* u[3] = {u0, u1, u2};
-    double e = 0.0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const double x = g[c].x - k[c] * pr, y = g[c].y - k[c] * pi;
-        const C v = mk<C>((T)x, (T)y);
-        ((C*)u[c])[t] = v;
-        e += (double)v.x * v.x + (double)v.y * v.y;
+        ((C*)u[c])[t] = mk<C>((T)x, (T)y);
     }
     const int ik2 = (int)(sqrt(k2) + 0.5);
     if (ik2 < 1 || ik2 > nek) return;
-    atomicAdd(ek + ik2 - 1, kx == 0 ? 0.5 * e : e);
-    atomicAdd(cnt + ik2 - 1, kx == 0 ? 0.5 : 1.0);
+    // twice the shell's mode count (kx = 0 modes count one half): integer sums, the same
+    // in any order; the shell energies come from the fixed-tree E(k) reduction
+    atomicAdd(cnt2 + ik2 - 1, kx == 0 ? 1ull : 2ull);
 }
 
 // the shell factors (flowlib.f90:119-137), and zero above nek (139-141)
@@ -1052,11 +1091,15 @@ __global__ void k_init_scale(void* u0, void* u1, void* u2, int nek, const double
 // ================================================================== host side
 static size_t field_elems(int N) { return (size_t)N * N * (N / 2); }
 
-int flow_alloc(FlowBuffers& fb, int N, int precision)
+int flow_alloc(FlowBuffers& fb, int N, int precision, int R)
 {
     const size_t cs = (precision == DDNS_FP64) ? sizeof(double2) : sizeof(float2);
     const size_t rs = (precision == DDNS_FP64) ? sizeof(double) : sizeof(float);
-    const size_t fe = field_elems(N);
+    if (R < 1) R = 1;
+    fb.R = R;
+    fb.rstride = R > 1 ? field_elems(N) : 0;
+    fb.pstride = R > 1 ? (size_t)N * N * N * 3 : 0;
+    const size_t fe = field_elems(N) * R;
     for (int i = 0; i < 3; ++i) {
         DDNS_HIP(hipMalloc(&fb.U[i], fe * cs));
         DDNS_HIP(hipMalloc(&fb.O[i], fe * cs));
@@ -1065,8 +1108,8 @@ int flow_alloc(FlowBuffers& fb, int N, int precision)
         DDNS_HIP(hipMemset(fb.O[i], 0, fe * cs));
     }
     for (int i = 0; i < 6; ++i) DDNS_HIP(hipMalloc(&fb.A[i], fe * cs));
-    DDNS_HIP(hipMalloc(&fb.uphys, (size_t)N * N * N * 3 * rs));
-    DDNS_HIP(hipMemset(fb.uphys, 0, (size_t)N * N * N * 3 * rs));
+    DDNS_HIP(hipMalloc(&fb.uphys, (size_t)N * N * N * 3 * rs * R));
+    DDNS_HIP(hipMemset(fb.uphys, 0, (size_t)N * N * N * 3 * rs * R));
     // twiddles exp(-2 pi i m / L), computed in double and rounded once
     for (int pass = 0; pass < 2; ++pass) {
         const int L = pass == 0 ? N : N / 2;
@@ -1086,13 +1129,38 @@ int flow_alloc(FlowBuffers& fb, int N, int precision)
             DDNS_HIP(hipMemcpy(*dst, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
         }
     }
-    DDNS_HIP(hipMalloc(&fb.low, 1024 * sizeof(double)));
-    DDNS_HIP(hipMemset(fb.low, 0, 1024 * sizeof(double)));
-    DDNS_HIP(hipMalloc(&fb.sth_state, 64 * sizeof(int)));
-    DDNS_HIP(hipMemset(fb.sth_state, 0, 64 * sizeof(int)));
-    DDNS_HIP(hipMalloc(&fb.sth_b, 450 * sizeof(double)));
-    DDNS_HIP(hipMemset(fb.sth_b, 0, 450 * sizeof(double)));
+    DDNS_HIP(hipMalloc(&fb.low, (size_t)LOW_STRIDE * R * sizeof(double)));
+    DDNS_HIP(hipMemset(fb.low, 0, (size_t)LOW_STRIDE * R * sizeof(double)));
+    DDNS_HIP(hipMalloc(&fb.sth_state, (size_t)STH_STATE_STRIDE * R * sizeof(int)));
+    DDNS_HIP(hipMemset(fb.sth_state, 0, (size_t)STH_STATE_STRIDE * R * sizeof(int)));
+    DDNS_HIP(hipMalloc(&fb.sth_b, (size_t)STH_B_STRIDE * R * sizeof(double)));
+    DDNS_HIP(hipMemset(fb.sth_b, 0, (size_t)STH_B_STRIDE * R * sizeof(double)));
     DDNS_HIP(hipMalloc(&fb.spec_part, ((size_t)SPEC_BLOCKS + 1) * SPEC_NEK_MAX * sizeof(double)));
+    return 0;
+}
+
+FlowBuffers flow_view(const FlowBuffers& fb, int r, int precision)
+{
+    const size_t cs = (precision == DDNS_FP64) ? sizeof(double2) : sizeof(float2);
+    const size_t rs = (precision == DDNS_FP64) ? sizeof(double) : sizeof(float);
+    FlowBuffers v{};
+    auto off = [&](void* p) { return p ? (void*)((char*)p + (size_t)r * fb.rstride * cs) : nullptr; };
+    for (int i = 0; i < 3; ++i) { v.U[i] = off(fb.U[i]); v.O[i] = off(fb.O[i]); v.B[i] = off(fb.B[i]); }
+    for (int i = 0; i < 6; ++i) v.A[i] = off(fb.A[i]);
+    v.uphys = (char*)fb.uphys + (size_t)r * fb.pstride * rs;
+    v.tw = fb.tw; v.twh = fb.twh;
+    v.low = fb.low + (size_t)r * LOW_STRIDE;
+    v.sth_state = fb.sth_state + (size_t)r * STH_STATE_STRIDE;
+    v.sth_b = fb.sth_b + (size_t)r * STH_B_STRIDE;
+    v.ky_tc = fb.ky_tc; v.kz_nt = fb.kz_nt;
+    return v;
+}
+
+int flow_set_realisation(const FlowBuffers& fb, int r, int forcing, double force0, double force1, double ou_tf,
+                         double ou_var)
+{
+    const double par[5] = {(double)forcing, force0, force1, ou_tf, ou_var};
+    DDNS_HIP(hipMemcpy(fb.low + (size_t)r * LOW_STRIDE + LOW_PAR, par, sizeof(par), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -1117,11 +1185,14 @@ static int launch_z(int mode, const FlowBuffers& fb, const FlowConst& fc, hipStr
     for (int i = 0; i < 6; ++i) a.A[i] = fb.A[i];
     a.tw = fb.tw; a.low = fb.low; a.dt = fc.dt; a.rnu = fc.rnu; a.nek = fc.nek; a.forcing = fc.forcing;
     a.scale = scale;
-    if (mode == KZ_PLAIN_FWD || mode == KZ_PLAIN_INV) { a.B[0] = src; a.U[0] = dst; }
+    a.rstride = fb.rstride;
+    const int R = fb.R;
+    if (mode == KZ_PLAIN_FWD || mode == KZ_PLAIN_INV) { a.B[0] = src; a.U[0] = dst; a.rstride = 0; }
     constexpr int TC = kz_tc<T, N>();
     dim3 grid((N / 2 / TC) * N);
     const char* env = (mode == KZ_FULL || mode == KZ_CURL_INV) ? std::getenv("DDNS_Z_SPLIT") : nullptr;
-    const bool split = env && (env[0] == '0' || env[0] == '1') ? env[0] == '1' : kz_split_default<T, N>();
+    // a batch (R > 1) always runs the fused z stage: the split kernels take one realisation
+    const bool split = R == 1 && (env && (env[0] == '0' || env[0] == '1') ? env[0] == '1' : kz_split_default<T, N>());
     if (split && (mode == KZ_FULL || mode == KZ_CURL_INV)) {
         const unsigned gp = (unsigned)(N / 2 / kz_tc_plain<T, N>() * N);
         const unsigned ge = (unsigned)(((size_t)N * N * (N / 2) + 255) / 256);
@@ -1154,7 +1225,7 @@ static int launch_z(int mode, const FlowBuffers& fb, const FlowConst& fc, hipStr
         int dev = 0, ncu = 0;
         DDNS_HIP(hipGetDevice(&dev));
         DDNS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        const int ntiles = (int)grid.x;
+        const int ntiles = (int)grid.x * R;
         const dim3 g(std::max(1, std::min(ncu, ntiles)));
         constexpr int N0 = kz_nt_cand<T, N>(0), N1 = kz_nt_cand<T, N>(1), N2 = kz_nt_cand<T, N>(2);
         if constexpr (kz_nt_fits<T, N, N1>()) {
@@ -1175,6 +1246,7 @@ static int launch_z(int mode, const FlowBuffers& fb, const FlowConst& fc, hipStr
         DDNS_HIP(hipGetLastError());
         return 0;
     }
+    if (mode == KZ_CURL_INV) grid.y = R;
     switch (mode) {
     case KZ_CURL_INV: hipLaunchKernelGGL((k_z<T, N, KZ_CURL_INV>), grid, dim3(NT), 0, s, a); break;
     case KZ_PLAIN_FWD: hipLaunchKernelGGL((k_z<T, N, KZ_PLAIN_FWD>), grid, dim3(kz_threads<T, N, KZ_PLAIN_FWD>()), 0, s, a); break;
@@ -1187,7 +1259,7 @@ static int launch_z(int mode, const FlowBuffers& fb, const FlowConst& fc, hipStr
 template <typename T, int N, int TC>
 static void launch_y_tc(bool inv, bool sym, const KyArgs& a, int nf, hipStream_t s)
 {
-    dim3 grid(N / 2 / TC, N, nf);
+    dim3 grid(N / 2 / TC, N, nf * a.R);
     if (inv && sym) hipLaunchKernelGGL((k_y<T, N, true, true, TC>), grid, dim3(NT), 0, s, a);
     else if (inv) hipLaunchKernelGGL((k_y<T, N, true, false, TC>), grid, dim3(NT), 0, s, a);
     else hipLaunchKernelGGL((k_y<T, N, false, false, TC>), grid, dim3(NT), 0, s, a);
@@ -1214,6 +1286,9 @@ static int launch_y(bool inv, bool sym, void* const* f, int nf, const FlowBuffer
     for (int i = 0; i < nf; ++i) a.F[i] = f[i];
     a.tw = fb.tw;
     a.scale = 1.0 / N;
+    a.nf = nf;
+    a.R = fb.R;
+    a.rstride = fb.rstride;
     const int tc = fb.ky_tc;
     bool done = false;
     if (tc == ky_cand<T, N>(0)) done = launch_y_cand<T, N, 0>(inv, sym, a, nf, s);
@@ -1230,7 +1305,8 @@ static int launch_x(const FlowBuffers& fb, hipStream_t s)
     for (int i = 0; i < 6; ++i) a.A[i] = fb.A[i];
     for (int i = 0; i < 3; ++i) a.B[i] = fb.B[i];
     a.uphys = fb.uphys; a.tw = fb.tw; a.twh = fb.twh;
-    dim3 grid((unsigned)((size_t)N * N / kxf_lines<N>()));
+    a.rstride = fb.rstride; a.pstride = fb.pstride;
+    dim3 grid((unsigned)((size_t)N * N / kxf_lines<N>()), fb.R);
     hipLaunchKernelGGL((k_x_fused<T, N>), grid, dim3(NT), 0, s, a);
     DDNS_HIP(hipGetLastError());
     return 0;
@@ -1239,14 +1315,15 @@ static int launch_x(const FlowBuffers& fb, hipStream_t s)
 template <typename T, int N>
 static int launch_pre(const FlowBuffers& fb, const FlowConst& fc, hipStream_t s)
 {
-    if (fc.forcing != 1 && fc.forcing != 2) return 0;
+    if (fc.forcing != 1 && fc.forcing != 2 && fc.forcing >= 0) return 0;
     PreArgs a{};
     for (int i = 0; i < 3; ++i) { a.B[i] = fb.B[i]; a.U[i] = fb.U[i]; a.O[i] = fb.O[i]; }
     a.tw = fb.tw; a.low = fb.low; a.sth_state = fb.sth_state; a.sth_b = fb.sth_b;
     a.dt = fc.dt; a.rnu = fc.rnu; a.force0 = fc.force0; a.force1 = fc.force1;
     a.ou_tf = fc.ou_tf; a.ou_var = fc.ou_var; a.pi2 = fc.pi2; a.nek = fc.nek; a.forcing = fc.forcing;
-    if (fc.forcing == 1) hipLaunchKernelGGL((k_pre_dft<T, N>), dim3(225), dim3(NT), 0, s, a);
-    hipLaunchKernelGGL((k_pre<T, N>), dim3(1), dim3(NT), 0, s, a);
+    a.rstride = fb.rstride;
+    if (fc.forcing == 1 || fc.forcing < 0) hipLaunchKernelGGL((k_pre_dft<T, N>), dim3(225, fb.R), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL((k_pre<T, N>), dim3(fb.R), dim3(NT), 0, s, a);
     DDNS_HIP(hipGetLastError());
     return 0;
 }
@@ -1255,7 +1332,8 @@ template <typename T, int N>
 static int launch_sym(const FlowBuffers& fb, hipStream_t s)
 {
     const int tot = N * N;
-    hipLaunchKernelGGL((k_sym<T, N>), dim3((tot + 255) / 256), dim3(256), 0, s, fb.U[0], fb.U[1], fb.U[2]);
+    hipLaunchKernelGGL((k_sym<T, N>), dim3((tot + 255) / 256, fb.R), dim3(256), 0, s, fb.U[0], fb.U[1], fb.U[2],
+                       fb.rstride);
     DDNS_HIP(hipGetLastError());
     return 0;
 }
@@ -1420,15 +1498,21 @@ static int init_impl(FlowBuffers& fb, int nek, int kind, int seed, double k0, do
 {
     const size_t tot = field_elems(N);
     const unsigned g = (unsigned)((tot + 255) / 256);
-    double* d = nullptr;                       // shell energies, counts, factors (3 nek)
-    DDNS_HIP(hipMalloc(&d, 3 * (size_t)nek * sizeof(double)));
-    DDNS_HIP(hipMemsetAsync(d, 0, 2 * (size_t)nek * sizeof(double), s));
+    double* d = nullptr;                       // twice the mode counts (integers), then the factors (2 nek)
+    DDNS_HIP(hipMalloc(&d, 2 * (size_t)nek * sizeof(double)));
+    DDNS_HIP(hipMemsetAsync(d, 0, (size_t)nek * sizeof(double), s));
     const unsigned long long key = (unsigned long long)(long long)seed;
-    hipLaunchKernelGGL((k_init_modes<T, N>), dim3(g), dim3(256), 0, s, fb.U[0], fb.U[1], fb.U[2], key, nek, d, d + nek);
+    hipLaunchKernelGGL((k_init_modes<T, N>), dim3(g), dim3(256), 0, s, fb.U[0], fb.U[1], fb.U[2], key, nek,
+                       (unsigned long long*)d);
     DDNS_HIP(hipGetLastError());
+    std::vector<unsigned long long> c2(nek);
     std::vector<double> h(2 * (size_t)nek), fac(nek, 1.0);
-    DDNS_HIP(hipMemcpyAsync(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-    DDNS_HIP(hipStreamSynchronize(s));
+    DDNS_HIP(hipMemcpyAsync(c2.data(), d, c2.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    // shell energies of the unscaled field (kx = 0 modes at half weight), by the same
+    // fixed-order reduction as E(k): the same bits on every run
+    int sst = flow_spectrum(fb, N, nek, sizeof(T) == 8 ? DDNS_FP64 : DDNS_FP32_MIXED, h.data(), s);
+    if (sst) { (void)hipFree(d); return sst; }
+    for (int i = 0; i < nek; ++i) h[nek + i] = 0.5 * (double)c2[i];
     const double* e_t = h.data();
     const double* enumv = h.data() + nek;
     const double pi = 4. * std::atan(1.);
@@ -1454,9 +1538,9 @@ static int init_impl(FlowBuffers& fb, int nek, int kind, int seed, double k0, do
                                         : 0.0;
         }
     }
-    DDNS_HIP(hipMemcpyAsync(d + 2 * nek, fac.data(), nek * sizeof(double), hipMemcpyHostToDevice, s));
+    DDNS_HIP(hipMemcpyAsync(d + nek, fac.data(), nek * sizeof(double), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL((k_init_scale<T, N>), dim3(g), dim3(256), 0, s, fb.U[0], fb.U[1], fb.U[2], nek,
-                       (const double*)(d + 2 * nek));
+                       (const double*)(d + nek));
     DDNS_HIP(hipGetLastError());
     // the history starts equal to the velocity (turb.f90:452-454)
     for (int c = 0; c < 3; ++c)
@@ -1943,7 +2027,7 @@ template <typename T, int N>
 static int tune_impl(FlowBuffers& fb, const FlowConst& fc, hipStream_t s)
 {
     static std::mutex mu;
-    static std::map<int, TuneRecord> cache;          // key: N, per precision (one map per T)
+    static std::map<int, TuneRecord> cache;          // key: N * 4096 + R, per precision (one map per T)
     fb.ky_tc = ky_tc<T, N>();
     fb.kz_nt = kz_nt<T, N>();
     for (int k = 0; k < 3; ++k) fb.ky_ms[k] = fb.kz_ms[k] = 0.f;
@@ -1965,7 +2049,7 @@ static int tune_impl(FlowBuffers& fb, const FlowConst& fc, hipStream_t s)
         if (t[0] == '0') return 0;
     {
         std::lock_guard<std::mutex> lock(mu);
-        auto it = cache.find(N);
+        auto it = cache.find(N * 4096 + fb.R);        // per grid and number of realisations
         if (it != cache.end()) {
             fb.ky_tc = it->second.ky_tc;
             fb.kz_nt = it->second.kz_nt;
@@ -1976,7 +2060,7 @@ static int tune_impl(FlowBuffers& fb, const FlowConst& fc, hipStream_t s)
             return 0;
         }
     }
-    const size_t bytes = field_elems(N) * sizeof(cpx<T>);
+    const size_t bytes = field_elems(N) * sizeof(cpx<T>) * fb.R;
     for (int q = 0; q < 6; ++q) DDNS_HIP(hipMemsetAsync(fb.A[q], 0, bytes, s));
     for (int q = 0; q < 3; ++q) DDNS_HIP(hipMemsetAsync(fb.B[q], 0, bytes, s));
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1986,7 +2070,7 @@ static int tune_impl(FlowBuffers& fb, const FlowConst& fc, hipStream_t s)
         DDNS_HIP(hipMemsetAsync(fb.U[q], 0, bytes, s));
         DDNS_HIP(hipMemsetAsync(fb.O[q], 0, bytes, s));
     }
-    DDNS_HIP(hipMemsetAsync(fb.low, 0, 1024 * sizeof(double), s));
+    DDNS_HIP(hipMemsetAsync(fb.low, 0, (size_t)LOW_STRIDE * fb.R * sizeof(double), s));
     float best = 0.f;
     int best_tc = ky_tc<T, N>(), best_nt = kz_nt<T, N>();
     int st = 0;
@@ -2025,7 +2109,7 @@ static int tune_impl(FlowBuffers& fb, const FlowConst& fc, hipStream_t s)
         DDNS_HIP(hipMemsetAsync(fb.U[q], 0, bytes, s));
         DDNS_HIP(hipMemsetAsync(fb.O[q], 0, bytes, s));
     }
-    DDNS_HIP(hipMemsetAsync(fb.low, 0, 1024 * sizeof(double), s));
+    DDNS_HIP(hipMemsetAsync(fb.low, 0, (size_t)LOW_STRIDE * fb.R * sizeof(double), s));
     DDNS_HIP(hipStreamSynchronize(s));
     fb.ky_tc = best_tc;
     fb.kz_nt = best_nt;
@@ -2037,7 +2121,7 @@ static int tune_impl(FlowBuffers& fb, const FlowConst& fc, hipStream_t s)
         r.ky_ms[k] = fb.ky_ms[k];
         r.kz_ms[k] = fb.kz_ms[k];
     }
-    cache[N] = r;
+    cache[N * 4096 + fb.R] = r;
     return 0;
 }
 
