@@ -139,6 +139,7 @@ struct CollArgs {
     double* contact;       // DDNS_CONTACT_REC per pair in contact, count at counts[258]
     int contact_cap;
     int* ncontact;         // pairs in contact found by the search (sizes contact)
+    int* far;              // set when a droplet's reach exceeds the cell width (the walk would miss partners)
 };
 
 // The largest displacement length |dmove| of the step (out[0], when dmove is given) and
@@ -400,6 +401,9 @@ __device__ __forceinline__ void collide_search(const CollArgs& a, int t, int lan
     const double rbig = __longlong_as_double((long long)a.dmax[1]);
     const double reach = fmax(10. * (ra + rbig), ra + rbig + dmove_a + dmax);
     const double reach2 = reach * reach * (1. + 1e-9);
+    // the walk covers the neighbour cells only: a reach beyond the cell width is reported,
+    // and the search is repeated on wider cells
+    if (reach + 1e-9 * g.hc > g.hc) atomicOr(a.far, 1);
     for_each_candidate(a.cell_start, g.G, cc, a.pk, a.sorted, [&](int p, int kc, const double4& cb, int ib) {
         if (p == t) return;
         int wx, wy, wz;
@@ -637,6 +641,7 @@ static CollArgs collide_args(PartBuffers& pb, const PartConst& pc)
     a.hit = pb.hit;
     a.ncand = pb.flags + 40;
     a.ncontact = pb.flags + 41;
+    a.far = pb.flags + 42;
     a.cand = (int2*)pb.cand;
     a.cand_cap = pb.cand_cap;
     a.contact = pb.contact;
@@ -659,7 +664,7 @@ int part_collide_search(PartBuffers& pb, const PartConst& pc, hipStream_t s)
         add(pb.tested, 4 * sizeof(long long));
         add(pb.mult, (size_t)(pc.n + 1) * sizeof(int));
         add(pb.hit, (size_t)pc.n * sizeof(int));
-        add(pb.flags + 32, 10 * sizeof(int));
+        add(pb.flags + 32, 11 * sizeof(int));
         add(pb.flags + 48, sizeof(int));               // relocation-draw overflow of this step
         hipLaunchKernelGGL(k_zero_spans, dim3(512), dim3(256), 0, s, z);
         DDNS_HIP(hipGetLastError());
@@ -679,7 +684,7 @@ int part_collide_search(PartBuffers& pb, const PartConst& pc, hipStream_t s)
                        (const double*)pb.rad, pc.n, pb.cap, (unsigned long long*)(pb.flags + 32));
     hipLaunchKernelGGL(k_collide, dim3((pc.n + CT - 1) / CT), dim3(CT), 0, s, a);
     // handed-on pairs and pairs in contact, read by part_collide_finish
-    DDNS_HIP(hipMemcpyAsync(pb.found_host, a.ncand, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    DDNS_HIP(hipMemcpyAsync(pb.found_host, a.ncand, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
     // the pair kernel, sized by the last step's count (a grid-stride loop covers more)
     const int nwg = std::max(256, std::min(4096, (int)((pb.last_ncand + pb.last_ncand / 4 + 255) / 256)));
     hipLaunchKernelGGL(k_collide_pairs, dim3(nwg), dim3(256), 0, s, a);
@@ -694,6 +699,7 @@ int part_collide_finish(PartBuffers& pb, const PartConst& pc, hipStream_t s)
         DDNS_HIP(wait_stream(pb, s));
         const int ncand = pb.found_host[0], ncontact = pb.found_host[1];
         pb.last_ncand = ncand;
+        if (pb.found_host[2]) return DDNS_COLL_REGRID;     // cells too narrow for a droplet's reach
         if (ncand <= pb.cand_cap && ncontact <= pb.contact_cap) return 0;
         // more pairs to hand on, or more contact records, than the buffers hold: widen
         // them and run the search and the pair kernel again

@@ -84,6 +84,9 @@ struct ddns_handle {
     // droplet block, whose gather waits for the batch's event phys_ev
     bool member = false;
     hipEvent_t phys_ev = nullptr;
+    // the collision search once found a droplet whose reach exceeded its cells: the cells
+    // stay at least two grid spacings wide from then on
+    bool coll_wide = false;
     std::string err;
 };
 
@@ -708,13 +711,23 @@ extern "C" int ddns_init_droplets(ddns_handle* h, int seed, int radii_kind, doub
 }
 
 // ------------------------------------------------------------------ the step
-// Cell widths of the pair searches.  Collisions need the statistics radius 20 a_max
-// and two grid cells (the reference's adjacent-cell window); HI needs its truncation
-// radius, usually much wider, so it gets its own, coarser grid.
+// Cell widths of the pair searches.  A collision-search walk visits the neighbour cells,
+// so a cell must be as wide as a droplet's reach: the statistics window 10 (a + a_max) <=
+// 20 a_max, or the collision window 2 a_max plus both displacements of the step.  Two grid
+// spacings cover any step's displacement, and so does the statistics window at the usual
+// step sizes: where the droplets are dense the cells are only as wide as 20 a_max, with at
+// most about one (Morton-numbered) cell per four droplets so that the cell scan stays
+// small; the search reports any droplet whose reach exceeds the cells (collide_regrid).
+// HI needs its truncation radius, usually much wider, so it gets its own, coarser grid.
 static double coll_cell_width(const ddns_handle* h)
 {
     const double dx = h->pc.pi2 / h->N;
-    return std::max(20.0 * h->pc.radmax, 2.0 * dx);
+    const double wide = std::max(20.0 * h->pc.radmax, 2.0 * dx);
+    if (h->coll_wide) return wide;
+    int b = 0;
+    while (b < 8 && (1LL << (3 * (b + 1))) <= 4LL * std::max(1, h->n)) ++b;
+    const double fine = std::max(20.0 * h->pc.radmax * (1.0 + 1e-6), h->pc.pi2 / (double)(1 << b));
+    return std::min(wide, fine);
 }
 
 static double cell_width(const ddns_handle* h)
@@ -842,13 +855,16 @@ static int queue_flow_next(ddns_handle* h, hipEvent_t after)
     return 0;
 }
 
-static int droplet_block(ddns_handle* h, float* ms)
+// The droplet block (turb.f90:546-623) in three pieces: droplet_first up to the gather
+// (binning, history, interpolation), the HI solve, droplet_rest from the advance on.
+// A batch runs the HI solves of all its realisations together between the two.
+static int droplet_first(ddns_handle* h)
 {
     PartConst& pc = h->pc;
     pc.n = h->n;
     pc.new_particle = h->new_particle;
     hipStream_t s = h->dstream;
-    int status = 0, st;
+    int st;
     DDNS_HIP(hipEventRecord(h->ev[8], s));
     st = part_build_cells(h->pb, pc, cell_width(h), s);                       // HEADLIST (binning)
     if (st) return st;
@@ -864,14 +880,51 @@ static int droplet_block(ddns_handle* h, float* ms)
     if (st) return st;
     DDNS_HIP(hipEventRecord(h->ev[10], s));
     h->cc = ddns_collision_counts{};
+    return 0;
+}
+
+static int droplet_rest(ddns_handle* h, float* ms);
+
+// The collision search found a droplet whose reach exceeds the width of its cells (a
+// displacement, or a radius grown by coalescence, beyond what the width allowed): the
+// cells are made wider and the search runs again on the same positions.
+static int collide_regrid(ddns_handle* h, hipStream_t s)
+{
+    for (int tries = 0; tries < 3; ++tries) {
+        h->coll_wide = true;
+        const double w = std::max(coll_cell_width(h), 2.0 * h->pb.hc);
+        int st = part_build_cells(h->pb, h->pc, w, s);
+        if (st) return st;
+        st = part_collide_search(h->pb, h->pc, s);
+        if (st) return st;
+        st = part_collide_finish(h->pb, h->pc, s);
+        if (st != DDNS_COLL_REGRID) return st;
+    }
+    return fail(h, DDNS_ERR_DOMAIN, "a droplet's collision reach exceeds the cells of the pair search");
+}
+
+static int droplet_block(ddns_handle* h, float* ms)
+{
+    int status = 0;
+    int st = droplet_first(h);
+    if (st) return st;
     if (h->p.hdi_included) {                                                  // 571-572
         int it = 0; double res = 0;
-        st = part_hdi(h->pb, pc, h->p.gmres_maxiter, h->p.gmres_tol, &it, &res, s);
+        st = part_hdi(h->pb, h->pc, h->p.gmres_maxiter, h->p.gmres_tol, &it, &res, h->dstream);
         if (st < 0) return st;
         if (st > 0) status = st;
         h->cc.gmres_iters = it;
         h->cc.gmres_residual = res;
     }
+    st = droplet_rest(h, ms);
+    return st < 0 ? st : status;
+}
+
+static int droplet_rest(ddns_handle* h, float* ms)
+{
+    PartConst& pc = h->pc;
+    hipStream_t s = h->dstream;
+    int status = 0, st;
     DDNS_HIP(hipEventRecord(h->ev[11], s));
     // The droplet stream's next work is queued before the flow's launches, so it never
     // waits for the host: the advance, the collision grid, the collision search and the
@@ -902,6 +955,7 @@ static int droplet_block(ddns_handle* h, float* ms)
         if (st) return st;
     }
     st = part_collide_finish(h->pb, pc, s);                                   // overflow check
+    if (st == DDNS_COLL_REGRID) st = collide_regrid(h, s);
     if (st) return st;
     const bool reloc = h->p.nonoverlap && h->istep < h->p.nstep_grow;         // 590
     invalidate_cells(h->pb);
@@ -1144,6 +1198,7 @@ extern "C" int ddns_collision_detect(ddns_handle* h, int n, const double* yp, co
     st = part_build_cells(h->pb, h->pc, cell_width(h), h->stream);
     if (st) return keep(h, st);
     st = part_collide(h->pb, h->pc, h->stream);
+    if (st == DDNS_COLL_REGRID) st = collide_regrid(h, h->stream);
     if (st) return keep(h, st);
     ddns_collision_counts c2{};
     st = part_read_counts(h->pb, h->pc, &c2, h->stream);
@@ -1193,6 +1248,12 @@ struct ddns_batch {
     hipGraphExec_t gflow[2] = {};          // FLOW_INV, FLOW_FWD
     int graph_mode = 1;
     int threads = 8;                       // worker threads of the droplet blocks
+    // the HI solves of all realisations as one batched GMRES (DDNS_BATCH_HI=0: each
+    // realisation solves its own inside its droplet block)
+    bool merged_hi = true;
+    hipStream_t hstream = nullptr;
+    hipEvent_t hi_ev = nullptr;
+    HdiBatch hb;
     ddns_timing timing{};
     std::string err;
 };
@@ -1216,6 +1277,10 @@ extern "C" int ddns_batch_destroy(ddns_batch* b)
     if (b->t0) (void)hipEventDestroy(b->t0);
     if (b->t1) (void)hipEventDestroy(b->t1);
     for (auto& g : b->gflow) if (g) (void)hipGraphExecDestroy(g);
+    if (b->hstream) (void)hipStreamSynchronize(b->hstream);
+    part_hdi_batch_free(b->hb);
+    if (b->hi_ev) (void)hipEventDestroy(b->hi_ev);
+    if (b->hstream) (void)hipStreamDestroy(b->hstream);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return DDNS_OK;
@@ -1241,8 +1306,13 @@ extern "C" int ddns_batch_create(const ddns_params* params, int nreal, ddns_batc
     b->nek = (int)(N / 2 - 1.5);
     if (const char* g = std::getenv("DDNS_GRAPH")) b->graph_mode = g[0] != '0';
     if (const char* t = std::getenv("DDNS_BATCH_THREADS")) b->threads = std::max(1, std::atoi(t));
+    if (const char* t = std::getenv("DDNS_BATCH_HI")) b->merged_hi = t[0] != '0';
     int st = 0;
-    if (hipStreamCreate(&b->stream) != hipSuccess) { ddns_batch_destroy(b); return bfail(nullptr, DDNS_ERR_HIP, "stream"); }
+    if (hipStreamCreate(&b->stream) != hipSuccess || hipStreamCreate(&b->hstream) != hipSuccess ||
+        hipEventCreateWithFlags(&b->hi_ev, hipEventDisableTiming) != hipSuccess) {
+        ddns_batch_destroy(b);
+        return bfail(nullptr, DDNS_ERR_HIP, "stream");
+    }
     for (auto& e : b->flev) if (hipEventCreate(&e) != hipSuccess) st = DDNS_ERR_HIP;
     for (auto& e : b->fev) if (hipEventCreate(&e) != hipSuccess) st = DDNS_ERR_HIP;
     if (hipEventCreate(&b->t0) != hipSuccess || hipEventCreate(&b->t1) != hipSuccess) st = DDNS_ERR_HIP;
@@ -1370,14 +1440,85 @@ extern "C" int ddns_batch_step(ddns_batch* b, int nsteps)
             h->phys_valid = true;
             h->phys_ev = b->flev[1];
         }
-        // the droplet blocks (546-623), members spread over the worker threads
+        // the droplet blocks (546-623).  With the merged HI solve: each realisation's
+        // binning, history, gather and HI start are queued from this thread (no host
+        // waits), the GMRES iterations of all of them run as one batched solve, and the
+        // rest of each block runs on the worker threads
         std::fill(rc.begin(), rc.end(), 0);
+        std::vector<char> active(R, 0), merged(R, 0);
+        for (int r = 0; r < R; ++r) {
+            ddns_handle* h = b->m[r];
+            h->cc = ddns_collision_counts{};
+            active[r] = h->p.npart > 0 && h->n > 0;
+        }
+        if (b->merged_hi) {
+            int maxiter = -1;
+            std::vector<PartBuffers*> pbs;
+            std::vector<const PartConst*> pcs;
+            std::vector<int> idx;
+            for (int r = 0; r < R; ++r) {
+                if (!active[r]) continue;
+                ddns_handle* h = b->m[r];
+                st = droplet_first(h);
+                if (st) { b->err = "realisation " + std::to_string(r) + ": " + ddns_last_error(h); return batch_fail(b, st); }
+                if (!h->p.hdi_included) continue;
+                if (maxiter < 0) maxiter = h->p.gmres_maxiter;
+                if (h->p.gmres_maxiter != maxiter) continue;        // solved on its own below
+                st = part_hdi_begin(h->pb, h->pc, maxiter, h->p.gmres_tol, h->dstream);
+                if (st) { b->err = "realisation " + std::to_string(r) + ": " + g_err; return batch_fail(b, st); }
+                DDNS_HIP(hipEventRecord(h->ev[4], h->dstream));
+                DDNS_HIP(hipStreamWaitEvent(b->hstream, h->ev[4], 0));
+                merged[r] = 1;
+                pbs.push_back(&h->pb);
+                pcs.push_back(&h->pc);
+                idx.push_back(r);
+            }
+            const int nh = (int)idx.size();
+            std::vector<int> its(nh), sts(nh), redo(nh);
+            std::vector<double> res(nh);
+            if (nh > 0) {
+                st = part_hdi_batch(b->hb, pbs.data(), pcs.data(), nh, maxiter, its.data(), res.data(), sts.data(),
+                                    redo.data(), b->hstream);
+                if (st) {
+                    for (int i = 0; i < nh; ++i)
+                        if (sts[i] < 0) b->err = "realisation " + std::to_string(idx[i]) + ": " + g_err;
+                    return batch_fail(b, st);
+                }
+                DDNS_HIP(hipEventRecord(b->hi_ev, b->hstream));
+            }
+            for (int i = 0; i < nh; ++i) {
+                ddns_handle* h = b->m[idx[i]];
+                DDNS_HIP(hipStreamWaitEvent(h->dstream, b->hi_ev, 0));
+                if (redo[i]) {                 // lists widened: the whole solve again, alone
+                    int it = 0; double rs = 0;
+                    const int q = part_hdi(h->pb, h->pc, h->p.gmres_maxiter, h->p.gmres_tol, &it, &rs, h->dstream);
+                    if (q < 0) { b->err = "realisation " + std::to_string(idx[i]) + ": " + g_err; return batch_fail(b, q); }
+                    its[i] = it; res[i] = rs; sts[i] = q;
+                }
+                h->cc.gmres_iters = its[i];
+                h->cc.gmres_residual = res[i];
+                if (sts[i] > 0) rc[idx[i]] = sts[i];
+            }
+        }
         auto work = [&](int w, int nw) {
             for (int r = w; r < R; r += nw) {
                 ddns_handle* h = b->m[r];
-                h->cc = ddns_collision_counts{};
-                if (!(h->p.npart > 0 && h->n > 0)) continue;
-                int q = droplet_block(h, pms[r].data());
+                if (!active[r]) continue;
+                int q;
+                if (!b->merged_hi) {
+                    q = droplet_block(h, pms[r].data());
+                } else {
+                    q = 0;
+                    if (h->p.hdi_included && !merged[r]) {
+                        int it = 0; double rs = 0;
+                        q = part_hdi(h->pb, h->pc, h->p.gmres_maxiter, h->p.gmres_tol, &it, &rs, h->dstream);
+                        h->cc.gmres_iters = it;
+                        h->cc.gmres_residual = rs;
+                    }
+                    const int warn = q > 0 ? q : rc[r];
+                    if (q >= 0) q = droplet_rest(h, pms[r].data());
+                    if (q >= 0) q = warn;
+                }
                 if (q >= 0 && h->n < h->p.npart / 2)
                     q = fail(h, DDNS_ERR_DROPLET_LOSS, "npart_t < npart/2 (turb.f90:600-604)");
                 rc[r] = q;

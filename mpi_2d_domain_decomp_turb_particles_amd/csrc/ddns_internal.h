@@ -425,11 +425,31 @@ int part_interp(PartBuffers& pb, const PartConst& pc, const void* uphys, hipStre
 int part_history_fvp(PartBuffers& pb, const PartConst& pc, hipStream_t s);
 int part_build_cells(PartBuffers& pb, const PartConst& pc, double hmin, hipStream_t s);
 int part_hdi(PartBuffers& pb, const PartConst& pc, int maxiter, double tol, int* iters, double* res, hipStream_t s);
+// The start of a solve (neighbour lists, right-hand side, first Krylov vector) queued on s.
+int part_hdi_begin(PartBuffers& pb, const PartConst& pc, int maxiter, double tol, hipStream_t s);
+// The iterations of R started solves at once on stream s (every kernel one launch over
+// the R systems), then each solution scattered to its droplets.  iters, res, status (0,
+// DDNS_WARN_GMRES, or an error of the neighbour lists) per system; redo[r] = 1 where the
+// lists had to be widened: that system is to be solved again on its own.
+struct HdiBatch {
+    void* table = nullptr;          // device: one descriptor per system
+    double* heads = nullptr;        // device: 16 doubles per system
+    void* table_host = nullptr;     // pinned staging of the descriptors
+    double* heads_host = nullptr;   // pinned copy of the heads
+    int cap = 0;
+};
+int part_hdi_batch(HdiBatch& hb, PartBuffers* const* pbs, const PartConst* const* pcs, int R, int maxiter,
+                   int* iters, double* res, int* status, int* redo, hipStream_t s);
+void part_hdi_batch_free(HdiBatch& hb);
 int part_advance(PartBuffers& pb, const PartConst& pc, hipStream_t s);
 int part_collide(PartBuffers& pb, const PartConst& pc, hipStream_t s);
 // the collision search queued without a host wait, and its completion (waits for the
 // search's counts, then queues the pair kernel)
 int part_collide_search(PartBuffers& pb, const PartConst& pc, hipStream_t s);
+// part_collide_finish / part_collide: some droplet's reach (its statistics window or
+// its collision window) exceeds the cell width, so the walk may have missed partners:
+// the caller builds wider cells and searches again
+constexpr int DDNS_COLL_REGRID = 2;
 int part_collide_finish(PartBuffers& pb, const PartConst& pc, hipStream_t s);
 // queue the step's counts into pinned memory; read them after the stream was waited for
 int part_queue_counts(PartBuffers& pb, hipStream_t s);

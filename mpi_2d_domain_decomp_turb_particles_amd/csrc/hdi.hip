@@ -115,15 +115,15 @@ __global__ void __launch_bounds__(NT_NEI) k_nei(const double4* __restrict__ pk, 
 // (added to H(it, it) with the first projection), so the projections no longer cancel
 // the large identity part of A x against q_it.
 template <typename T, int MODE>
-__global__ void __launch_bounds__(HT) k_spmv(const int* __restrict__ cnt, const int* __restrict__ idx,
+__device__ __forceinline__ void k_spmv_body(const int* __restrict__ cnt, const int* __restrict__ idx,
                                              const T* __restrict__ mm, int n, const T* __restrict__ x,
                                              const double* __restrict__ hx, T* __restrict__ qout, T* __restrict__ y,
                                              const double* __restrict__ x0d, double* __restrict__ y0d,
                                              const double* __restrict__ w, double* __restrict__ b,
-                                             const double* __restrict__ stop)
+                                             const double* __restrict__ stop, int bx, int nbx)
 {
     if (stop && *stop != 0.0) return;   // converged in an earlier iteration of this batch
-    const int t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int t = xcd_chunk(bx, nbx) * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const size_t N = (size_t)n;
     const double h = MODE == 2 ? *hx : 1.0;
@@ -158,6 +158,16 @@ __global__ void __launch_bounds__(HT) k_spmv(const int* __restrict__ cnt, const 
         y[t] = (T)r0; y[N + t] = (T)r1; y[2 * N + t] = (T)r2;   // M x: the identity is added to H
         if (MODE == 2) { qout[t] = (T)s0; qout[N + t] = (T)s1; qout[2 * N + t] = (T)s2; }
     }
+}
+template <typename T, int MODE>
+__global__ void __launch_bounds__(HT) k_spmv(const int* __restrict__ cnt, const int* __restrict__ idx,
+                                             const T* __restrict__ mm, int n, const T* __restrict__ x,
+                                             const double* __restrict__ hx, T* __restrict__ qout, T* __restrict__ y,
+                                             const double* __restrict__ x0d, double* __restrict__ y0d,
+                                             const double* __restrict__ w, double* __restrict__ b,
+                                             const double* __restrict__ stop)
+{
+    k_spmv_body<T, MODE>(cnt, idx, mm, n, x, hx, qout, y, x0d, y0d, w, b, stop, blockIdx.x, gridDim.x);
 }
 
 // ------------------------------------------------------------------ vector kernels
@@ -265,7 +275,7 @@ __device__ __forceinline__ void block_sums(double (&acc)[J], double (*sh)[J], do
 // fixed wavefront tree), the same in every work-group.  Work-group 0 also sets
 // (add = 0, with A's identity added to H(k, k)) or increments (add = 1) the Hessenberg
 // column hcol.
-__device__ void sum_partials(const double* __restrict__ part, int P, int k, double* hs, double* hcol, int add)
+__device__ void sum_partials(const double* __restrict__ part, int P, int k, double* hs, double* hcol, int add, int bx)
 {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int j = wv; j < k; j += RW) {
@@ -274,7 +284,7 @@ __device__ void sum_partials(const double* __restrict__ part, int P, int k, doub
         acc = wave_sum(acc);
         if (lane == 0) {
             hs[j] = acc;
-            if (blockIdx.x == 0) hcol[j] = add ? hcol[j] + acc : acc + (j == k - 1 ? 1.0 : 0.0);
+            if (bx == 0) hcol[j] = add ? hcol[j] + acc : acc + (j == k - 1 ? 1.0 : 0.0);
         }
     }
     __syncthreads();
@@ -302,8 +312,8 @@ __device__ __forceinline__ void vst(T* p, const T (&x)[kvec<T>::W])
 
 // partials of Q_j . v for j < k, and of v . v as column k
 template <typename T>
-__global__ void __launch_bounds__(RT) k_mdot(const T* __restrict__ Q, const T* __restrict__ v, int k, long long n3,
-                                             long long ld, double* __restrict__ part, const double* __restrict__ stop)
+__device__ __forceinline__ void k_mdot_body(const T* __restrict__ Q, const T* __restrict__ v, int k, long long n3,
+                                             long long ld, double* __restrict__ part, const double* __restrict__ stop, int bx, int nbx)
 {
     if (*stop != 0.0) return;
     constexpr int W = kvec<T>::W;
@@ -315,7 +325,7 @@ __global__ void __launch_bounds__(RT) k_mdot(const T* __restrict__ Q, const T* _
         double acc[MD_J + 1];
 #pragma unroll
         for (int jj = 0; jj <= MD_J; ++jj) acc[jj] = 0.0;
-        for (long long iv = (long long)blockIdx.x * RT + tid; iv < nv; iv += (long long)gridDim.x * RT) {
+        for (long long iv = (long long)bx * RT + tid; iv < nv; iv += (long long)nbx * RT) {
             const long long i = iv * W;
             if (i + W <= n3) {
                 T x[W];
@@ -342,19 +352,25 @@ __global__ void __launch_bounds__(RT) k_mdot(const T* __restrict__ Q, const T* _
             }
         }
         block_sums<MD_J + 1>(acc, sh, red);
-        if (tid < MD_J && j0 + tid < k) part[(size_t)(j0 + tid) * gridDim.x + blockIdx.x] = red[tid];
-        if (j0 == 0 && tid == 0) part[(size_t)k * gridDim.x + blockIdx.x] = red[MD_J];
+        if (tid < MD_J && j0 + tid < k) part[(size_t)(j0 + tid) * nbx + bx] = red[tid];
+        if (j0 == 0 && tid == 0) part[(size_t)k * nbx + bx] = red[MD_J];
     }
+}
+template <typename T>
+__global__ void __launch_bounds__(RT) k_mdot(const T* __restrict__ Q, const T* __restrict__ v, int k, long long n3,
+                                             long long ld, double* __restrict__ part, const double* __restrict__ stop)
+{
+    k_mdot_body<T>(Q, v, k, n3, ld, part, stop, blockIdx.x, gridDim.x);
 }
 
 // First pass's update fused with the second pass's projections, for k <= KC columns:
 // h = sum of part_in (first pass), v -= sum_j h_j Q_j, partials of Q_j . v into part_out
 // and of v . v (the updated v) as its column k
 template <typename T, int KC>
-__global__ void __launch_bounds__(RT) k_maxpy_mdot(T* __restrict__ v, const T* __restrict__ Q, int k, long long n3,
+__device__ __forceinline__ void k_maxpy_mdot_body(T* __restrict__ v, const T* __restrict__ Q, int k, long long n3,
                                                    long long ld, const double* __restrict__ part_in,
                                                    double* __restrict__ hcol, double* __restrict__ part_out,
-                                                   const double* __restrict__ stop)
+                                                   const double* __restrict__ stop, int bx, int nbx)
 {
     if (*stop != 0.0) return;
     constexpr int W = kvec<T>::W;
@@ -362,12 +378,12 @@ __global__ void __launch_bounds__(RT) k_maxpy_mdot(T* __restrict__ v, const T* _
     __shared__ double sh[RW][KC + 1];
     __shared__ double red[KC + 1];
     const int tid = threadIdx.x;
-    sum_partials(part_in, gridDim.x, k, hs, hcol, 0);
+    sum_partials(part_in, nbx, k, hs, hcol, 0, bx);
     double acc[KC + 1];
 #pragma unroll
     for (int jj = 0; jj <= KC; ++jj) acc[jj] = 0.0;
     const long long nv = (n3 + W - 1) / W;
-    for (long long iv = (long long)blockIdx.x * RT + tid; iv < nv; iv += (long long)gridDim.x * RT) {
+    for (long long iv = (long long)bx * RT + tid; iv < nv; iv += (long long)nbx * RT) {
         const long long i = iv * W;
         if (i + W <= n3) {
             T q[KC][W];
@@ -409,8 +425,16 @@ __global__ void __launch_bounds__(RT) k_maxpy_mdot(T* __restrict__ v, const T* _
         }
     }
     block_sums<KC + 1>(acc, sh, red);
-    if (tid < k) part_out[(size_t)tid * gridDim.x + blockIdx.x] = red[tid];
-    if (tid == 0) part_out[(size_t)k * gridDim.x + blockIdx.x] = red[KC];
+    if (tid < k) part_out[(size_t)tid * nbx + bx] = red[tid];
+    if (tid == 0) part_out[(size_t)k * nbx + bx] = red[KC];
+}
+template <typename T, int KC>
+__global__ void __launch_bounds__(RT) k_maxpy_mdot(T* __restrict__ v, const T* __restrict__ Q, int k, long long n3,
+                                                   long long ld, const double* __restrict__ part_in,
+                                                   double* __restrict__ hcol, double* __restrict__ part_out,
+                                                   const double* __restrict__ stop)
+{
+    k_maxpy_mdot_body<T, KC>(v, Q, k, n3, ld, part_in, hcol, part_out, stop, blockIdx.x, gridDim.x);
 }
 
 // sum over b < P of part[b] (lane-strided in order, then the wavefront tree), the same
@@ -433,11 +457,11 @@ __device__ __forceinline__ double sum_one(const double* __restrict__ part, int P
 // the projection kernels' (P <= RED_BLOCKS work-groups); work-group b takes a contiguous
 // run of rows, the runs placed so that neighbouring ones share an XCD.
 template <typename T, int MODE, int KC>
-__global__ void __launch_bounds__(RT) k_spmv_mdot(const int* __restrict__ cnt, const int* __restrict__ idx,
+__device__ __forceinline__ void k_spmv_mdot_body(const int* __restrict__ cnt, const int* __restrict__ idx,
                                                   const T* __restrict__ mm, int n, const T* __restrict__ x,
                                                   const double* __restrict__ hx, T* __restrict__ qout,
                                                   T* __restrict__ y, const T* __restrict__ Q, long long ld, int k,
-                                                  double* __restrict__ part, const double* __restrict__ stop)
+                                                  double* __restrict__ part, const double* __restrict__ stop, int bx, int nbx)
 {
     if (*stop != 0.0) return;   // converged in an earlier iteration of this batch
     __shared__ double sh[RW][KC + 1];
@@ -451,8 +475,8 @@ __global__ void __launch_bounds__(RT) k_spmv_mdot(const int* __restrict__ cnt, c
     double acc[KC + 1];
 #pragma unroll
     for (int jj = 0; jj <= KC; ++jj) acc[jj] = 0.0;
-    const int lb = xcd_chunk(blockIdx.x, gridDim.x);
-    const int per = (n + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int lb = xcd_chunk(bx, nbx);
+    const int per = (n + (int)nbx - 1) / (int)nbx;
     const int t1 = min(n, (lb + 1) * per);
     for (int t = lb * per + threadIdx.x; t < t1; t += RT) {
         const int kk = cnt[t];
@@ -487,8 +511,17 @@ __global__ void __launch_bounds__(RT) k_spmv_mdot(const int* __restrict__ cnt, c
             }
     }
     block_sums<KC + 1>(acc, sh, red);
-    if ((int)threadIdx.x < k) part[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = red[threadIdx.x];
-    if (threadIdx.x == 0) part[(size_t)k * gridDim.x + blockIdx.x] = red[KC];
+    if ((int)threadIdx.x < k) part[(size_t)threadIdx.x * nbx + bx] = red[threadIdx.x];
+    if (threadIdx.x == 0) part[(size_t)k * nbx + bx] = red[KC];
+}
+template <typename T, int MODE, int KC>
+__global__ void __launch_bounds__(RT) k_spmv_mdot(const int* __restrict__ cnt, const int* __restrict__ idx,
+                                                  const T* __restrict__ mm, int n, const T* __restrict__ x,
+                                                  const double* __restrict__ hx, T* __restrict__ qout,
+                                                  T* __restrict__ y, const T* __restrict__ Q, long long ld, int k,
+                                                  double* __restrict__ part, const double* __restrict__ stop)
+{
+    k_spmv_mdot_body<T, MODE, KC>(cnt, idx, mm, n, x, hx, qout, y, Q, ld, k, part, stop, blockIdx.x, gridDim.x);
 }
 
 // h = sum of part_in, v -= sum_j h[j] Q_j; with normpart, also the partials of |v|^2.
@@ -498,11 +531,11 @@ __global__ void __launch_bounds__(RT) k_spmv_mdot(const int* __restrict__ cnt, c
 // corrections are below rounding and its update is skipped: v, hcol stay, and normpart
 // takes the partials of |v|^2 already summed.
 template <typename T>
-__global__ void __launch_bounds__(RT) k_maxpy(T* __restrict__ v, const T* __restrict__ Q, int k, long long n3,
+__device__ __forceinline__ void k_maxpy_body(T* __restrict__ v, const T* __restrict__ Q, int k, long long n3,
                                               long long ld, const double* __restrict__ part_in,
                                               double* __restrict__ hcol, int add, double* __restrict__ normpart,
                                               const double* __restrict__ stop, const double* __restrict__ gate_w,
-                                              const double* __restrict__ gate_v)
+                                              const double* __restrict__ gate_v, int bx, int nbx)
 {
     if (*stop != 0.0) return;
     constexpr int W = kvec<T>::W;
@@ -512,17 +545,17 @@ __global__ void __launch_bounds__(RT) k_maxpy(T* __restrict__ v, const T* __rest
     __shared__ double g1[2];
     const int tid = threadIdx.x;
     if (gate_w) {
-        const double nw = sum_one(gate_w, gridDim.x, &g1[0]);
-        const double nv2 = sum_one(gate_v, gridDim.x, &g1[1]);
+        const double nw = sum_one(gate_w, nbx, &g1[0]);
+        const double nv2 = sum_one(gate_v, nbx, &g1[1]);
         if (nv2 >= 0.5 * nw) {
-            if (normpart && tid == 0) normpart[blockIdx.x] = gate_v[blockIdx.x];
+            if (normpart && tid == 0) normpart[bx] = gate_v[bx];
             return;
         }
     }
-    sum_partials(part_in, gridDim.x, k, hs, hcol, add);
+    sum_partials(part_in, nbx, k, hs, hcol, add, bx);
     double nacc[1] = {0.0};
     const long long nv = (n3 + W - 1) / W;
-    for (long long iv = (long long)blockIdx.x * RT + tid; iv < nv; iv += (long long)gridDim.x * RT) {
+    for (long long iv = (long long)bx * RT + tid; iv < nv; iv += (long long)nbx * RT) {
         const long long i = iv * W;
         if (i + W <= n3) {
             T xv[W];
@@ -556,7 +589,16 @@ __global__ void __launch_bounds__(RT) k_maxpy(T* __restrict__ v, const T* __rest
     }
     if (!normpart) return;
     block_sums<1>(nacc, sh, red);
-    if (tid == 0) normpart[blockIdx.x] = red[0];
+    if (tid == 0) normpart[bx] = red[0];
+}
+template <typename T>
+__global__ void __launch_bounds__(RT) k_maxpy(T* __restrict__ v, const T* __restrict__ Q, int k, long long n3,
+                                              long long ld, const double* __restrict__ part_in,
+                                              double* __restrict__ hcol, int add, double* __restrict__ normpart,
+                                              const double* __restrict__ stop, const double* __restrict__ gate_w,
+                                              const double* __restrict__ gate_v)
+{
+    k_maxpy_body<T>(v, Q, k, n3, ld, part_in, hcol, add, normpart, stop, gate_w, gate_v, blockIdx.x, gridDim.x);
 }
 
 // pertvel = x0 + Q y (2951-2956), scattered back to the reference's droplet order
@@ -640,8 +682,8 @@ __device__ __forceinline__ void copy_in(double* sm, const double* __restrict__ g
     }
 }
 
-__global__ void __launch_bounds__(LT) k_ls(double* gsm, GmSmall L, int iter, const double* __restrict__ normpart,
-                                          int P)
+__device__ __forceinline__ void k_ls_body(double* gsm, GmSmall L, int iter, const double* __restrict__ normpart,
+                                          int P, int bx, int nbx)
 {
     // the part of the small problem this iteration touches is copied to LDS, solved
     // there, and the changed part copied back
@@ -675,6 +717,12 @@ __global__ void __launch_bounds__(LT) k_ls(double* gsm, GmSmall L, int iter, con
     copy_range(gsm, sm, L.R0(), L.R0() + M1 * iter);
     copy_range(gsm, sm, L.Q0(), L.Q0() + M1 * (iter + 1));
     copy_range(gsm, sm, L.B0(), L.total());
+}
+
+__global__ void __launch_bounds__(LT) k_ls(double* gsm, GmSmall L, int iter, const double* __restrict__ normpart,
+                                          int P)
+{
+    k_ls_body(gsm, L, iter, normpart, P, blockIdx.x, gridDim.x);
 }
 
 // All LT lanes call it.  Lane r owns row r + 1 of the triangular factor and of the
@@ -802,15 +850,14 @@ static int lists_outcome(PartBuffers& pb, const int* f)
     return 1;
 }
 
+// The solve's start (2820-2868) queued on s without a host wait: the neighbour lists,
+// b = M w and A x0, r0 = b - A x0 into the first Krylov column with the norms, the
+// small problem cleared, the tolerance and the stop flag, and q_1 = r0 / |r0|.
 template <typename T>
-static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double tol, int* iters_out, double* res_out,
-                    hipStream_t s)
+static int hdi_begin(PartBuffers& pb, const PartConst& pc, int maxiter, double tol, hipStream_t s)
 {
     const int n = pc.n;
     const long long n3 = 3LL * n;
-    *iters_out = 0;
-    *res_out = 0.0;
-    if (n == 0) return 0;
     if (maxiter + 1 > LT) { set_error("GMRES maxiter too large (at most 50)"); return DDNS_ERR_ARG; }
     GmSmall L{maxiter};
     if (L.total() > (size_t)LS_MAX) { set_error("GMRES maxiter too large (at most 50)"); return DDNS_ERR_ARG; }
@@ -822,37 +869,67 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
         DDNS_HIP(hipMalloc(&pb.gm_v, 3 * (size_t)pb.cap * sizeof(double)));
         DDNS_HIP(hipMalloc(&pb.gm_Q, (size_t)(maxiter + 2) * ((3LL * pb.cap + 63) / 64 * 64) * sizeof(T)));
     }
-    double* part1 = pb.gm_red;                              // first-pass projections
-    double* part2 = part1 + (size_t)64 * RED_BLOCKS;        // second-pass projections
-    double* normpart = part2 + (size_t)64 * RED_BLOCKS;
     double* sm = pb.gm_small;
     double* ax0 = (double*)pb.gm_v;
     T* Q = (T*)pb.gm_Q;
-    const long long ld = (n3 + 63) / 64 * 64;     // column stride of the Krylov vectors
-    T* work[2] = {Q + (size_t)maxiter * ld, Q + (size_t)(maxiter + 1) * ld};
     const unsigned nb3 = (unsigned)((n3 + HT - 1) / HT);
     const unsigned nbr = (unsigned)((n + HT - 1) / HT);
-    const unsigned nred = (unsigned)std::min<long long>(RED_BLOCKS, (n3 + 4LL * RT - 1) / (4LL * RT));   // reduction grid
+    const int sq = queue_lists<T>(pb, pc, s);
+    if (sq) return sq;
+    // x0 (pertvel) and w = V - U in cell order were written by the interpolation
+    hipLaunchKernelGGL((k_spmv<T, 0>), dim3(nbr), dim3(HT), 0, s, (const int*)pb.nei_cnt, (const int*)pb.nei_idx,
+                       (const T*)pb.nei_m, n, (const T*)nullptr, (const double*)nullptr, (T*)nullptr, (T*)nullptr,
+                       (const double*)pb.gm_x0, ax0, (const double*)pb.gm_w, pb.gm_b, (const double*)nullptr);
+    const int nr0 = (int)std::min<long long>(RED_BLOCKS, (n3 + HT - 1) / HT);
+    hipLaunchKernelGGL((k_residual0<T>), dim3(nr0), dim3(HT), 0, s, pb.gm_b, ax0, Q, n3, pb.gm_red);
+    // the tolerance, and the stop flag when the initial residual is below 1e-5
+    // (2866-2868): then no iteration runs and pertvel keeps its warm-start value
+    hipLaunchKernelGGL(k_gm_start, dim3(1), dim3(HT), 0, s, sm, (int)L.total(), tol, (const double*)pb.gm_red, nr0);
+    hipLaunchKernelGGL((k_scale<T>), dim3(nb3), dim3(HT), 0, s, Q, sm + 1, n3);
+    DDNS_HIP(hipGetLastError());
+    return 0;
+}
+
+// work-groups of the projection kernels of a system of n droplets
+static int hdi_nred(int n)
+{
+    const long long n3 = 3LL * n;
+    return (int)std::min<long long>(RED_BLOCKS, (n3 + 4LL * RT - 1) / (4LL * RT));
+}
+
+template <typename T>
+static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double tol, int* iters_out, double* res_out,
+                    hipStream_t s)
+{
+    const int n = pc.n;
+    const long long n3 = 3LL * n;
+    *iters_out = 0;
+    *res_out = 0.0;
+    if (n == 0) return 0;
+    GmSmall L{maxiter};
+    const long long ld = (n3 + 63) / 64 * 64;     // column stride of the Krylov vectors
+    const unsigned nbr = (unsigned)((n + HT - 1) / HT);
+    const unsigned nred = (unsigned)hdi_nred(n);   // reduction grid
     // The neighbour lists are filled and the solve started without a host wait; the
     // first convergence check also reads the lists' flags, and when a row had more
     // neighbours than the lists hold they are widened and the solve starts over (its
     // inputs x0 and w are unchanged, and pertvel is written only at the end).
     int iter = 0;
     double st6[6] = {0, 0, 0, 0, 0, 0};
+    double* sm = nullptr;
+    T* Q = nullptr;
     for (int lists = 1; lists > 0;) {
-        const int sq = queue_lists<T>(pb, pc, s);
+        // the Krylov storage is allocated by the first hdi_begin of the handle: every
+        // pointer into it is taken after the call
+        const int sq = hdi_begin<T>(pb, pc, maxiter, tol, s);
         if (sq) return sq;
         lists = 0;
-        // x0 (pertvel) and w = V - U in cell order were written by the interpolation
-        hipLaunchKernelGGL((k_spmv<T, 0>), dim3(nbr), dim3(HT), 0, s, (const int*)pb.nei_cnt, (const int*)pb.nei_idx,
-                           (const T*)pb.nei_m, n, (const T*)nullptr, (const double*)nullptr, (T*)nullptr, (T*)nullptr,
-                           (const double*)pb.gm_x0, ax0, (const double*)pb.gm_w, pb.gm_b, (const double*)nullptr);
-        const int nr0 = (int)std::min<long long>(RED_BLOCKS, (n3 + HT - 1) / HT);
-        hipLaunchKernelGGL((k_residual0<T>), dim3(nr0), dim3(HT), 0, s, pb.gm_b, ax0, Q, n3, pb.gm_red);
-        // the tolerance, and the stop flag when the initial residual is below 1e-5
-        // (2866-2868): then no iteration runs and pertvel keeps its warm-start value
-        hipLaunchKernelGGL(k_gm_start, dim3(1), dim3(HT), 0, s, sm, (int)L.total(), tol, (const double*)pb.gm_red, nr0);
-        hipLaunchKernelGGL((k_scale<T>), dim3(nb3), dim3(HT), 0, s, Q, sm + 1, n3);
+        sm = pb.gm_small;
+        Q = (T*)pb.gm_Q;
+        double* part1 = pb.gm_red;                          // first-pass projections
+        double* part2 = part1 + (size_t)64 * RED_BLOCKS;    // second-pass projections
+        double* normpart = part2 + (size_t)64 * RED_BLOCKS;
+        T* work[2] = {Q + (size_t)maxiter * ld, Q + (size_t)(maxiter + 1) * ld};
         // Iterations are launched in batches; every kernel of an iteration returns at
         // once when the least-squares update of an earlier one has set the converged
         // flag sm[4], so the host checks it once per batch.  The first batch runs as many
@@ -955,6 +1032,280 @@ int part_hdi(PartBuffers& pb, const PartConst& pc, int maxiter, double tol, int*
 {
     if (pc.precision == DDNS_FP64) return hdi_impl<double>(pb, pc, maxiter, tol, iters, res, s);
     return hdi_impl<float>(pb, pc, maxiter, tol, iters, res, s);
+}
+
+int part_hdi_begin(PartBuffers& pb, const PartConst& pc, int maxiter, double tol, hipStream_t s)
+{
+    if (pc.n == 0) return 0;
+    if (pc.precision == DDNS_FP64) return hdi_begin<double>(pb, pc, maxiter, tol, s);
+    return hdi_begin<float>(pb, pc, maxiter, tol, s);
+}
+
+// ------------------------------------------------------------------ batched solve
+// The GMRES iterations of several independent systems (one per realisation of a batch)
+// run together: each kernel of an iteration is one launch whose work-group row y works
+// on system y.  Every system keeps its own Krylov basis, small problem, stop flag and
+// reduction partials, and its reductions run on the grid of nred work-groups its own
+// solve would use, so each system gets the same bits as when solved alone; a converged
+// system's work-groups return at once, as in a solve of its own.
+template <typename T>
+struct GmDesc {
+    const int* cnt;
+    const int* idx;
+    const T* mm;
+    const int* flags;          // the neighbour lists' flags (read at the first check)
+    T* Q;                      // maxiter Krylov columns, then the two work vectors
+    double* sm;                // the small problem (GmSmall layout)
+    double* part1;             // first-pass projection partials
+    double* part2;             // second-pass projection partials
+    double* normpart;          // |v|^2 partials
+    long long n3, ld;
+    int n, nred, nbr;          // nred = 0: the system takes no part (any more)
+};
+
+template <typename T>
+struct GmIter {
+    const T* xin;
+    const double* hx;
+    T* qo;
+    T* v;
+    double* hcol;
+    const double* stop;
+};
+
+// the vectors and Hessenberg entries of iteration it, as hdi_impl forms them
+template <typename T>
+__device__ __forceinline__ GmIter<T> gm_iter(const GmDesc<T>& g, GmSmall L, int it)
+{
+    GmIter<T> r;
+    T* w0 = g.Q + (size_t)L.maxiter * g.ld;
+    T* w1 = w0 + g.ld;
+    double* Hd = g.sm + L.H0();
+    const size_t M1 = L.M1();
+    r.v = (it & 1) ? w1 : w0;
+    r.xin = it == 1 ? g.Q : (((it - 1) & 1) ? w1 : w0);
+    r.hx = it == 1 ? nullptr : Hd + M1 * (it - 2) + (it - 1);
+    r.qo = it == 1 ? nullptr : g.Q + (size_t)(it - 1) * g.ld;
+    r.hcol = Hd + M1 * (it - 1);
+    r.stop = g.sm + 4;
+    return r;
+}
+
+template <typename T, int MODE, int KC>
+__global__ void __launch_bounds__(RT) kb_spmv_mdot(const GmDesc<T>* __restrict__ d, GmSmall L, int it)
+{
+    const GmDesc<T>& g = d[blockIdx.y];
+    if ((int)blockIdx.x >= g.nred) return;
+    const GmIter<T> q = gm_iter(g, L, it);
+    k_spmv_mdot_body<T, MODE, KC>(g.cnt, g.idx, g.mm, g.n, q.xin, q.hx, q.qo, q.v, g.Q, g.ld, it, g.part1, q.stop,
+                                  blockIdx.x, g.nred);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(HT) kb_spmv2(const GmDesc<T>* __restrict__ d, GmSmall L, int it)
+{
+    const GmDesc<T>& g = d[blockIdx.y];
+    if (g.nred == 0 || (int)blockIdx.x >= g.nbr) return;
+    const GmIter<T> q = gm_iter(g, L, it);
+    k_spmv_body<T, 2>(g.cnt, g.idx, g.mm, g.n, q.xin, q.hx, q.qo, q.v, nullptr, nullptr, nullptr, nullptr, q.stop,
+                      blockIdx.x, g.nbr);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(RT) kb_mdot(const GmDesc<T>* __restrict__ d, GmSmall L, int it, int second)
+{
+    const GmDesc<T>& g = d[blockIdx.y];
+    if ((int)blockIdx.x >= g.nred) return;
+    const GmIter<T> q = gm_iter(g, L, it);
+    k_mdot_body<T>(g.Q, q.v, it, g.n3, g.ld, second ? g.part2 : g.part1, q.stop, blockIdx.x, g.nred);
+}
+
+template <typename T, int KC>
+__global__ void __launch_bounds__(RT) kb_maxpy_mdot(const GmDesc<T>* __restrict__ d, GmSmall L, int it)
+{
+    const GmDesc<T>& g = d[blockIdx.y];
+    if ((int)blockIdx.x >= g.nred) return;
+    const GmIter<T> q = gm_iter(g, L, it);
+    k_maxpy_mdot_body<T, KC>(q.v, g.Q, it, g.n3, g.ld, g.part1, q.hcol, g.part2, q.stop, blockIdx.x, g.nred);
+}
+
+// second = 0: the first pass's update (k > 12); 1: the second pass's DGKS-gated update
+template <typename T>
+__global__ void __launch_bounds__(RT) kb_maxpy(const GmDesc<T>* __restrict__ d, GmSmall L, int it, int second)
+{
+    const GmDesc<T>& g = d[blockIdx.y];
+    if ((int)blockIdx.x >= g.nred) return;
+    const GmIter<T> q = gm_iter(g, L, it);
+    if (second)
+        k_maxpy_body<T>(q.v, g.Q, it, g.n3, g.ld, g.part2, q.hcol, 1, g.normpart, q.stop,
+                        g.part1 + (size_t)it * g.nred, g.part2 + (size_t)it * g.nred, blockIdx.x, g.nred);
+    else
+        k_maxpy_body<T>(q.v, g.Q, it, g.n3, g.ld, g.part1, q.hcol, 0, nullptr, q.stop, nullptr, nullptr, blockIdx.x,
+                        g.nred);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(LT) kb_ls(const GmDesc<T>* __restrict__ d, GmSmall L, int it)
+{
+    const GmDesc<T>& g = d[blockIdx.x];
+    if (g.nred == 0) return;
+    k_ls_body(g.sm, L, it, g.normpart, g.nred, 0, 1);
+}
+
+// each system's small-problem head sm[0..5] (and its lists' flags) into 16 doubles
+template <typename T>
+__global__ void kb_heads(const GmDesc<T>* __restrict__ d, double* out, int with_flags)
+{
+    const GmDesc<T>& g = d[blockIdx.x];
+    double* o = out + 16 * (size_t)blockIdx.x;
+    const int i = threadIdx.x;
+    if (!g.sm) {                       // no droplets, no solve: reads as "no iteration"
+        if (i < 6) o[i] = i == 4 ? -1.0 : 0.0;
+        if (i < 8) ((int*)(o + 8))[i] = 0;
+        return;
+    }
+    if (i < 6) o[i] = g.sm[i];
+    if (with_flags && i < 8) ((int*)(o + 8))[i] = g.flags[i];
+}
+
+void part_hdi_batch_free(HdiBatch& hb)
+{
+    if (hb.table) (void)hipFree(hb.table);
+    if (hb.heads) (void)hipFree(hb.heads);
+    if (hb.table_host) (void)hipHostFree(hb.table_host);
+    if (hb.heads_host) (void)hipHostFree(hb.heads_host);
+    hb = HdiBatch{};
+}
+
+template <typename T>
+static int hdi_batch_impl(HdiBatch& hb, PartBuffers* const* pbs, const PartConst* const* pcs, int R, int maxiter,
+                          int* iters, double* res, int* status, int* redo, hipStream_t s)
+{
+    if (R <= 0) return 0;
+    GmSmall L{maxiter};
+    if (hb.cap < R) {
+        part_hdi_batch_free(hb);
+        DDNS_HIP(hipMalloc(&hb.table, (size_t)R * sizeof(GmDesc<T>)));
+        DDNS_HIP(hipMalloc(&hb.heads, (size_t)R * 16 * sizeof(double)));
+        DDNS_HIP(hipHostMalloc(&hb.table_host, (size_t)R * sizeof(GmDesc<double>), hipHostMallocDefault));
+        DDNS_HIP(hipHostMalloc((void**)&hb.heads_host, (size_t)R * 16 * sizeof(double), hipHostMallocDefault));
+        hb.cap = R;
+    }
+    GmDesc<T>* th = (GmDesc<T>*)hb.table_host;
+    int nredmax = 0, nbrmax = 0, first = 3;
+    for (int r = 0; r < R; ++r) {
+        PartBuffers& pb = *pbs[r];
+        const int n = pcs[r]->n;
+        iters[r] = 0; res[r] = 0.0; status[r] = 0; redo[r] = 0;
+        GmDesc<T> g{};
+        if (n == 0 || !pb.gm_small) {  // nothing to solve (part_hdi_begin queued nothing)
+            th[r] = g;
+            continue;
+        }
+        g.cnt = pb.nei_cnt; g.idx = pb.nei_idx; g.mm = (const T*)pb.nei_m; g.flags = pb.flags;
+        g.Q = (T*)pb.gm_Q; g.sm = pb.gm_small;
+        g.part1 = pb.gm_red; g.part2 = g.part1 + (size_t)64 * RED_BLOCKS; g.normpart = g.part2 + (size_t)64 * RED_BLOCKS;
+        g.n3 = 3LL * n; g.ld = (g.n3 + 63) / 64 * 64; g.n = n;
+        g.nred = hdi_nred(n); g.nbr = (n + HT - 1) / HT;
+        th[r] = g;
+        nredmax = std::max(nredmax, g.nred);
+        nbrmax = std::max(nbrmax, g.nbr);
+        first = std::max(first, pb.gm_last_iters);
+    }
+    const GmDesc<T>* td = (const GmDesc<T>*)hb.table;
+    DDNS_HIP(hipMemcpyAsync(hb.table, th, (size_t)R * sizeof(GmDesc<T>), hipMemcpyHostToDevice, s));
+    constexpr int BATCH = 3;
+    std::vector<double> st6((size_t)R * 6, 0.0);
+    std::vector<char> done(R, 0);
+    int iter = 0;
+    bool firstcheck = true;
+    while (iter < maxiter) {
+        bool any = false;
+        for (int r = 0; r < R; ++r) any = any || (!done[r] && !redo[r]);
+        if (!any) break;
+        const int last = std::min(maxiter, iter + (iter == 0 ? first : BATCH));
+        const dim3 g(nredmax, R);
+        for (int it = iter + 1; it <= last; ++it) {
+            if (it == 1) hipLaunchKernelGGL((kb_spmv_mdot<T, 1, 8>), g, dim3(RT), 0, s, td, L, it);
+            else if (it <= 8) hipLaunchKernelGGL((kb_spmv_mdot<T, 2, 8>), g, dim3(RT), 0, s, td, L, it);
+            else if (it <= 12) hipLaunchKernelGGL((kb_spmv_mdot<T, 2, 12>), g, dim3(RT), 0, s, td, L, it);
+            else {
+                hipLaunchKernelGGL((kb_spmv2<T>), dim3(nbrmax, R), dim3(HT), 0, s, td, L, it);
+                hipLaunchKernelGGL((kb_mdot<T>), g, dim3(RT), 0, s, td, L, it, 0);
+            }
+            if (it <= 8) hipLaunchKernelGGL((kb_maxpy_mdot<T, 8>), g, dim3(RT), 0, s, td, L, it);
+            else if (it <= 12) hipLaunchKernelGGL((kb_maxpy_mdot<T, 12>), g, dim3(RT), 0, s, td, L, it);
+            else {
+                hipLaunchKernelGGL((kb_maxpy<T>), g, dim3(RT), 0, s, td, L, it, 0);
+                hipLaunchKernelGGL((kb_mdot<T>), g, dim3(RT), 0, s, td, L, it, 1);
+            }
+            hipLaunchKernelGGL((kb_maxpy<T>), g, dim3(RT), 0, s, td, L, it, 1);
+            hipLaunchKernelGGL((kb_ls<T>), dim3(R), dim3(LT), 0, s, td, L, it);
+        }
+        DDNS_HIP(hipGetLastError());
+        iter = last;
+        // one check for every system: the heads (and, the first time, the lists' flags)
+        hipLaunchKernelGGL((kb_heads<T>), dim3(R), dim3(64), 0, s, td, hb.heads, firstcheck ? 1 : 0);
+        DDNS_HIP(hipMemcpyAsync(hb.heads_host, hb.heads, (size_t)R * 16 * sizeof(double), hipMemcpyDeviceToHost, s));
+        DDNS_HIP(hipStreamSynchronize(s));
+        bool retable = false;
+        for (int r = 0; r < R; ++r) {
+            if (redo[r] || !th[r].sm) {
+                done[r] = 1;
+                st6[6 * (size_t)r + 4] = -1.0;
+                continue;
+            }
+            const double* h = hb.heads_host + 16 * (size_t)r;
+            for (int k = 0; k < 6; ++k) st6[6 * (size_t)r + k] = h[k];
+            if (firstcheck) {
+                int f[8];
+                std::memcpy(f, h + 8, sizeof(f));
+                // the system leaves the merged launches before its lists are reallocated
+                if (f[0] & 3 || f[4] > pbs[r]->ell_width) {
+                    th[r].nred = 0;
+                    th[r].nbr = 0;
+                    retable = true;
+                }
+                const int lo = lists_outcome<T>(*pbs[r], f);
+                if (lo < 0) {
+                    status[r] = lo;
+                    DDNS_HIP(hipMemcpyAsync(hb.table, th, (size_t)R * sizeof(GmDesc<T>), hipMemcpyHostToDevice, s));
+                    DDNS_HIP(hipStreamSynchronize(s));
+                    return lo;
+                }
+                if (lo > 0) redo[r] = 1;        // widened: solved again on its own
+            }
+            done[r] = st6[6 * (size_t)r + 4] != 0.0;
+        }
+        if (retable) DDNS_HIP(hipMemcpyAsync(hb.table, th, (size_t)R * sizeof(GmDesc<T>), hipMemcpyHostToDevice, s));
+        firstcheck = false;
+    }
+    for (int r = 0; r < R; ++r) {
+        if (redo[r]) continue;
+        PartBuffers& pb = *pbs[r];
+        const double* h = &st6[6 * (size_t)r];
+        if (h[4] < 0.0) continue;      // initial residual below 1e-5: no iteration
+        const bool ok = h[4] != 0.0;
+        const int used = (int)h[5];
+        pb.gm_last_iters = used;
+        const int n = pcs[r]->n;
+        const long long ld = (3LL * n + 63) / 64 * 64;
+        hipLaunchKernelGGL((k_solution<T>), dim3((n + HT - 1) / HT), dim3(HT), 0, s, pb.gm_x0, (const T*)pb.gm_Q, ld,
+                           pb.gm_small + L.Y0(), used, (const int*)pb.sorted, pb.pert, n, pb.cap);
+        iters[r] = used;
+        res[r] = h[2];
+        if (!ok && h[2] > h[3]) status[r] = DDNS_WARN_GMRES;                // 2980-2982
+    }
+    DDNS_HIP(hipGetLastError());
+    return 0;
+}
+
+int part_hdi_batch(HdiBatch& hb, PartBuffers* const* pbs, const PartConst* const* pcs, int R, int maxiter,
+                   int* iters, double* res, int* status, int* redo, hipStream_t s)
+{
+    if (R <= 0) return 0;
+    if (pcs[0]->precision == DDNS_FP64) return hdi_batch_impl<double>(hb, pbs, pcs, R, maxiter, iters, res, status, redo, s);
+    return hdi_batch_impl<float>(hb, pbs, pcs, R, maxiter, iters, res, status, redo, s);
 }
 
 }  // namespace ddns
