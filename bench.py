@@ -243,7 +243,7 @@ def coalescence_1024(steps, warmup):
     }
 
 
-def batched_48x64(steps, warmup, R=48, N=64):
+def batched_48x64(steps, warmup, R=48, N=64, compare=True):
     """The batched configuration: 48 independent 64^3 realisations (different seeds,
     deterministic and stochastic forcing alternating), FP32-mixed, droplet counts drawn
     uniformly from 5,000-200,000 per realisation, one droplet size per realisation with
@@ -270,10 +270,10 @@ def batched_48x64(steps, warmup, R=48, N=64):
         t = b.timing()
         tune = b.tuning()
         iters = [b.member(r).collision_counts().gmres_iters for r in range(R)]
-    log(f"batched: {1e3 * el / steps:.2f} ms per step; handles one after another")
+    log(f"batched: {1e3 * el / steps:.2f} ms per step")
     seq_steps = max(2, steps // 4)
     seq = 0.0
-    for r, p in enumerate(params):
+    for r, p in enumerate(params if compare else []):
         with dd.Solver(p) as s:
             s.init_flow(dd.SPEC_K4_GAUSS, p.iseed_m0, 4.0, 1.0)
             s.init_droplets(seed=290710 + r)
@@ -281,7 +281,7 @@ def batched_48x64(steps, warmup, R=48, N=64):
             t2 = time.perf_counter()
             s.step(seq_steps)
             seq += time.perf_counter() - t2
-    seq_ms = 1e3 * seq / seq_steps
+    seq_ms = 1e3 * seq / seq_steps if compare else None
     ms = 1e3 * el / steps
     return {
         "metric": "realisation-steps/s, 48 x 64^3 realisations with 5k-200k droplets each, FP32-mixed, one GPU",
@@ -299,7 +299,7 @@ def batched_48x64(steps, warmup, R=48, N=64):
             "advance": t.advance_ms / steps, "colldet": t.colldet_ms / steps, "overlap": t.overlap_ms / steps},
         "gmres_iterations": {"min": min(iters), "max": max(iters)},
         "separate_handles_ms_per_step": seq_ms,
-        "speedup_vs_separate_handles": seq_ms / ms,
+        "speedup_vs_separate_handles": seq_ms / ms if compare else None,
         "tuning": {k: (v.tolist() if hasattr(v, "tolist") else v) for k, v in tune.items()},
     }
 
@@ -314,6 +314,8 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="batch-48x64: skip the timing of the same realisations as separate handles")
     ap.add_argument("--only-fp32", action="store_true",
                     help="time the FP32-mixed path only (for profiler runs)")
     ap.add_argument("--workload", choices=["headline", "coalescence-1024", "batch-48x64"], default="headline",
@@ -327,7 +329,7 @@ def main(argv=None):
         print(json.dumps(coalescence_1024(a.steps, a.warmup)))
         return 0
     if a.workload == "batch-48x64":
-        print(json.dumps(batched_48x64(a.steps, a.warmup)))
+        print(json.dumps(batched_48x64(a.steps, a.warmup, compare=not a.no_compare)))
         return 0
     N, npart = a.N, a.npart
 

@@ -96,6 +96,9 @@ module ddns_mod
   public :: ddns_init_droplets, ddns_overlap_fix, ddns_step, ddns_get_collision_counts
   public :: ddns_get_collision_bins, ddns_get_collision_pairs, ddns_get_contact_records, ddns_get_timing
   public :: ddns_fft_r2c, ddns_fft_c2r, ddns_check
+  ! batched realisations: R parameter sets advanced together, one member handle each
+  public :: ddns_batch_create, ddns_batch_destroy, ddns_batch_size, ddns_batch_member
+  public :: ddns_batch_step, ddns_batch_get_timing
 
   interface
     integer(c_int) function ddns_params_default(p) bind(C, name="ddns_params_default")
@@ -253,6 +256,35 @@ module ddns_mod
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h
       real(c_double), intent(inout) :: v(*)
+    end function
+    integer(c_int) function ddns_batch_create(params, nreal, b) bind(C, name="ddns_batch_create")
+      import :: c_int, c_ptr, ddns_params
+      type(ddns_params), intent(in) :: params(*)
+      integer(c_int), value :: nreal
+      type(c_ptr), intent(out) :: b
+    end function
+    integer(c_int) function ddns_batch_destroy(b) bind(C, name="ddns_batch_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+    end function
+    integer(c_int) function ddns_batch_size(b) bind(C, name="ddns_batch_size")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+    end function
+    type(c_ptr) function ddns_batch_member(b, r) bind(C, name="ddns_batch_member")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: r
+    end function
+    integer(c_int) function ddns_batch_step(b, nsteps) bind(C, name="ddns_batch_step")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: nsteps
+    end function
+    integer(c_int) function ddns_batch_get_timing(b, t) bind(C, name="ddns_batch_get_timing")
+      import :: c_int, c_ptr, ddns_timing
+      type(c_ptr), value :: b
+      type(ddns_timing), intent(out) :: t
     end function
     function c_strlen(s) bind(C, name="strlen")
       import :: c_ptr, c_size_t
