@@ -724,8 +724,13 @@ static double coll_cell_width(const ddns_handle* h)
     const double dx = h->pc.pi2 / h->N;
     const double wide = std::max(20.0 * h->pc.radmax, 2.0 * dx);
     if (h->coll_wide) return wide;
+    static const long long per = [] {
+        const char* e = std::getenv("DDNS_COLL_CELLS");     // Morton cells allowed per droplet
+        const long long v = e ? std::atoll(e) : 4;
+        return v > 0 ? v : 4;
+    }();
     int b = 0;
-    while (b < 8 && (1LL << (3 * (b + 1))) <= 4LL * std::max(1, h->n)) ++b;
+    while (b < 8 && (1LL << (3 * (b + 1))) <= per * std::max(1, h->n)) ++b;
     const double fine = std::max(20.0 * h->pc.radmax * (1.0 + 1e-6), h->pc.pi2 / (double)(1 << b));
     return std::min(wide, fine);
 }
