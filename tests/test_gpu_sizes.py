@@ -67,6 +67,51 @@ def test_split_z_stage_matches_fused_512(ddns, monkeypatch):
     assert rel_l2(out[1], out[0]) < 1e-13
 
 
+def test_split_z_stage_with_droplets_and_hi_512(ddns, monkeypatch):
+    """A 512^3 step with droplets, HI and collisions beside the z stage, with the separate
+    z kernels (the default at this size) and with the fused one: the same flow, and the
+    droplets, their disturbance velocities and the collision counts agree."""
+    kw = dict(N=N, forcing=1, dt=0.001, rnu=0.002, precision=0, npart=50000, radius=[40.0],
+              hdi_included=1, hdi_trunc=20.0)
+    out = []
+    for split in ("1", "0"):
+        monkeypatch.setenv("DDNS_Z_SPLIT", split)
+        with ddns.Solver(ddns.default_params(**kw)) as s:
+            s.init_flow(ddns.SPEC_K4_GAUSS, seed=11, k0=4.0, urms=1.0)
+            s.init_droplets(seed=290710)
+            s.step(3)
+            c = s.collision_counts()
+            out.append((s.get_flow()[0], s.get_droplets(), c.gmres_iters, list(c.icc)))
+    (u1, d1, it1, icc1), (u0, d0, it0, icc0) = out
+    print(f"512^3 with 50000 droplets: split against fused velocity {rel_l2(u1, u0):.2e}, GMRES {it1} / {it0}", flush=True)
+    assert rel_l2(u1, u0) < 1e-13
+    assert it1 == it0 >= 1 and icc1 == icc0
+    assert np.abs(d1["yp"] - d0["yp"]).max() < 1e-10
+    assert np.abs(d1["pertvel"] - d0["pertvel"]).max() < 1e-9 * max(1e-30, np.abs(d0["pertvel"]).max())
+
+
+def test_fft_roundtrip_1024_fp64(ddns):
+    """The largest FP64 plans: the plain 1024-point z transforms (whose padded tile and
+    twiddles fill the 160 KiB of LDS exactly) and the y and x stages, through the
+    transform pair on a field of three separable modes."""
+    n = 1024
+    L = n + 2
+    x = 2 * np.pi * np.arange(n) / n
+    phys = np.zeros((L, L, n), order="F")
+    phys[:n, :n, :] = (np.cos(511 * x + 0.3)[:, None, None] + 0.5 * np.sin(3 * x + 1.1)[None, :, None]
+                       + 0.25 * np.cos(400 * x)[None, None, :])
+    with ddns.Solver(ddns.default_params(N=n, precision=0)) as s:
+        spec = s.fft_r2c(phys)
+        back = s.fft_c2r(spec)
+    err = np.abs(back[:n, :n, :] - phys[:n, :n, :]).max()
+    print(f"1024^3 FP64 transform pair: max error {err:.2e}", flush=True)
+    assert err < 1e-12
+    # the forward transform holds the three modes and nothing else: at most two complex
+    # coefficients (four real entries) per mode
+    nz = int(np.count_nonzero(np.abs(spec) > 1e-9))
+    assert 3 <= nz <= 12, nz
+
+
 def test_flow_1024_fp32_mixed():
     """The largest grid (1024^3, the 5M-droplet configuration's flow) through the device
     initial field and two steps: the z stage as separate kernels, 1024-point plans.  The
