@@ -6,7 +6,7 @@ realisation exactly what a handle of its own gives on the same inputs:
   FP64 and FP32-mixed: the same bits after 6 steps, and the same forcing random state;
 - droplets with HI, collisions and relocation, three realisations with uneven droplet
   counts and a droplet size each, FP64: the same flow, droplets, GMRES iterations and
-  collision counts after 3 steps;
+  collision counts after 3 steps, with the HI solves batched and solved one by one;
 - the 48-realisation 64^3 configuration (uneven counts 5k-200k, Stokes numbers 0.2-3,
   mixed forcing, HI, FP32-mixed): 2 steps run, and three of its realisations, re-run as
   handles of their own, agree bit for bit.
@@ -60,7 +60,11 @@ def test_batch_flow_equals_single_handles(ddns, prec):
             assert np.array_equal(fs[0], fs1[0])
 
 
-def test_batch_droplets_hi_equal_single_handles(ddns):
+@pytest.mark.parametrize("merged", ["1", "0"])
+def test_batch_droplets_hi_equal_single_handles(ddns, monkeypatch, merged):
+    """merged = 1: the HI systems of all realisations in one batched GMRES; 0: each
+    realisation solves its own inside its droplet block (DDNS_BATCH_HI=0)."""
+    monkeypatch.setenv("DDNS_BATCH_HI", merged)
     N = 64
     spec = [(3000, 30.0, 1), (9000, 20.0, 2), (1500, 45.0, 1)]   # droplets, radius (um), forcing
     params = []
