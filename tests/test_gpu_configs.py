@@ -144,3 +144,51 @@ def test_config_1024_stochastic_5M_coalescence(ddns):
     assert _overlapping_pairs(yp) == 0
     assert m.vdiv2 <= 1e-5 * m.vort2
     assert st1[1] != st0[1] or not np.array_equal(st1[3], st0[3])
+
+
+def test_config_64_plain_fp64_twenty_steps(ddns, oracle_mod):
+    """The reference's own CPU-runnable case at its size: 64^3 FP64 (parameter.d_64_det_1:
+    deterministic forcing of shells 1-2, nu = 0.026), the k^4 exp(-2 (k/k0)^2) start
+    (k0 = 4, u_rms = 1), dt at a CFL number near 0.3, 20 steps, 10,000 monodisperse droplets
+    of Stokes number 1 at uniform positions starting with the fluid's velocity, gravity and
+    collision detection, no HI.  Against the oracle: the flow to 1e-11, the collision
+    counts of every step identical, the droplets to 1e-10, the last step's pair list and
+    statistics-window counts identical and its bin sums to 1e-9.  (This dilute case meets
+    no collision in 20 steps; some 800 pairs a step fall in the statistics window.)"""
+    p = ddns.read_parameter_file(GOLD / "parameter.d_64_det_1")
+    p.precision = ddns.FP64
+    p.npart, p.nset = 10_000, 1
+    p.radius[0] = ddns.stokes_radius_um(p, 1.0)
+    p.hdi_included, p.nonoverlap = 0, 0
+    p.dt = 0.004
+    with ddns.Solver(p) as s:
+        s.init_flow(ddns.SPEC_K4_GAUSS, 2024, 4.0, 1.0)
+        v0, h0 = s.get_flow()
+        cfl = s.monitors().cfl
+    assert 0.25 < cfl < 0.4, cfl
+    with ddns.Solver(p) as s, oracle_mod.Oracle(p) as o:
+        s.set_flow(v0, h0)
+        o.set_flow(v0, h0)
+        s.init_droplets(seed=290710)
+        o.init_droplets(seed=290710)
+        total = 0
+        for _ in range(20):
+            s.step(1)
+            o.step(1)
+            cg, co = s.collision_counts(), o.collision_counts()
+            assert list(cg.icc) == list(co.icc) and cg.ncoll == co.ncoll and cg.npairs == co.npairs
+            assert list(cg.colltype) == list(co.colltype) and list(cg.nir) == list(co.nir)
+            assert cg.pairs_in_dstat == co.pairs_in_dstat
+            total += co.ncoll
+        err = rel_l2(s.get_flow()[0], o.get_flow()[0])
+        dg, do = s.get_droplets(), o.get_droplets()
+        pg, po = s.collision_pairs(), o.collision_pairs()
+        bg, bo = s.collision_bins(), o.collision_bins()
+    print(f"64^3 FP64, 10k droplets of St 1, 20 steps: velocity rel L2 {err:.2e} against the oracle, "
+          f"{total} collisions, droplets max |dx| {np.abs(dg['yp'] - do['yp']).max():.2e}", flush=True)
+    assert err < 1e-11
+    assert np.abs(dg["yp"] - do["yp"]).max() < 1e-10
+    assert np.array_equal(pg, po)
+    assert np.array_equal(bg["nirbin"], bo["nirbin"]) and bg["nirbin"].sum() > 0
+    for k in ("wrbin", "wxbin", "wybin", "wzbin", "urbin", "thetabin", "wrtanbin"):
+        assert np.allclose(bg[k], bo[k], rtol=1e-9, atol=1e-12 * max(1.0, np.abs(bo[k]).max())), k
