@@ -976,12 +976,6 @@ static int droplet_rest(ddns_handle* h, float* ms)
         pc.n = h->n;
         if (h->n < h->p.npart / 2) return fail(h, DDNS_ERR_DROPLET_LOSS, "npart_t < npart/2 (turb.f90:600-604)");
     }
-    // the next step's first half, queued on the flow stream behind the z stage once the
-    // droplet work above is queued (a graph launch holds the host for tens of us)
-    if (gated) {
-        st = queue_flow_next(h, h->ev[6]);
-        if (st) return st;
-    }
     DDNS_HIP(hipEventRecord(h->ev[13], s));
     // OVERLAP_DET (610): the first sweep is queued; its count is read with the step's
     // counts, and only a sweep that relocated droplets is followed by more
@@ -992,6 +986,13 @@ static int droplet_rest(ddns_handle* h, float* ms)
     st = part_queue_counts(h->pb, s);
     if (st) return st;
     DDNS_HIP(hipEventRecord(h->ev[14], s));
+    // the next step's first half, queued on the flow stream behind the z stage once the
+    // droplet block's work is queued: a graph launch holds the host for tens of us, which
+    // the droplet stream now spends on the overlap sweep instead of idling
+    if (gated) {
+        st = queue_flow_next(h, h->ev[6]);
+        if (st) return st;
+    }
     DDNS_HIP(spin_wait(h->ev[14]));
     if (h->p.nonoverlap) {
         int sw = 0, rl = 0;

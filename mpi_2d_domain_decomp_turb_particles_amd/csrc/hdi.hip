@@ -601,14 +601,17 @@ __global__ void __launch_bounds__(RT) k_maxpy(T* __restrict__ v, const T* __rest
     k_maxpy_body<T>(v, Q, k, n3, ld, part_in, hcol, add, normpart, stop, gate_w, gate_v, blockIdx.x, gridDim.x);
 }
 
-// pertvel = x0 + Q y (2951-2956), scattered back to the reference's droplet order
+// pertvel = x0 + Q y (2951-2956), scattered back to the reference's droplet order.  The
+// number of Krylov columns is the small problem's count of the iterations that ran
+// (*itp), read on the device, so the kernel can be queued before the host has seen it.
 template <typename T>
-__global__ void k_solution(const double* x0, const T* Q, long long ld, const double* y, int iters, const int* sorted,
-                           double* pert, int n, int cap)
+__global__ void k_solution(const double* x0, const T* Q, long long ld, const double* y, const double* itp,
+                           const int* sorted, double* pert, int n, int cap)
 {
     // one droplet per thread: its three components go out as one 24-byte record
     const int t = blockIdx.x * HT + threadIdx.x;
     if (t >= n) return;
+    const int iters = (int)*itp;
     double out[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -1002,7 +1005,15 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
             int* fh = (int*)(pb.gm_host + 8);
             if (first) DDNS_HIP(hipMemcpyAsync(fh, pb.flags, sizeof(f), hipMemcpyDeviceToHost, s));
             DDNS_HIP(hipMemcpyAsync(pb.gm_host, sm, sizeof(st6), hipMemcpyDeviceToHost, s));
-            DDNS_HIP(wait_stream(pb, s));
+            if (!pb.sync_ev) DDNS_HIP(hipEventCreateWithFlags(&pb.sync_ev, hipEventDisableTiming));
+            DDNS_HIP(hipEventRecord(pb.sync_ev, s));
+            // the solution as far as this batch got (pertvel = x0 where no iteration ran),
+            // queued behind the copies: when the batch converged, the usual case, it runs
+            // while the host reads the check and queues what follows; a later batch is
+            // followed by its own
+            hipLaunchKernelGGL((k_solution<T>), dim3(nbr), dim3(HT), 0, s, pb.gm_x0, (const T*)Q, ld, sm + L.Y0(),
+                               (const double*)(sm + 5), (const int*)pb.sorted, pb.pert, n, pb.cap);
+            DDNS_HIP(spin_wait(pb.sync_ev));
             std::memcpy(st6, pb.gm_host, sizeof(st6));
             if (first) std::memcpy(f, fh, sizeof(f));
             if (first) {
@@ -1017,11 +1028,9 @@ static int hdi_impl(PartBuffers& pb, const PartConst& pc, int maxiter, double to
     const bool done = st6[4] != 0.0;
     const int used = (int)st6[5];   // the last iteration that ran
     pb.gm_last_iters = used;
-    hipLaunchKernelGGL((k_solution<T>), dim3(nbr), dim3(HT), 0, s, pb.gm_x0, (const T*)Q, ld, sm + L.Y0(), used,
-                       (const int*)pb.sorted, pb.pert, n, pb.cap);
     DDNS_HIP(hipGetLastError());
-    // the residual and its bound were final when the last batch was checked: the
-    // solution kernel reads the small problem but does not change it
+    // the solution was queued behind the last check; the residual and its bound were
+    // final then: the solution kernel reads the small problem but does not change it
     *iters_out = used;
     *res_out = st6[2];
     if (!done && st6[2] > st6[3]) return DDNS_WARN_GMRES;                   // 2980-2982
@@ -1291,7 +1300,8 @@ static int hdi_batch_impl(HdiBatch& hb, PartBuffers* const* pbs, const PartConst
         const int n = pcs[r]->n;
         const long long ld = (3LL * n + 63) / 64 * 64;
         hipLaunchKernelGGL((k_solution<T>), dim3((n + HT - 1) / HT), dim3(HT), 0, s, pb.gm_x0, (const T*)pb.gm_Q, ld,
-                           pb.gm_small + L.Y0(), used, (const int*)pb.sorted, pb.pert, n, pb.cap);
+                           pb.gm_small + L.Y0(), (const double*)(pb.gm_small + 5), (const int*)pb.sorted, pb.pert, n,
+                           pb.cap);
         iters[r] = used;
         res[r] = h[2];
         if (!ok && h[2] > h[3]) status[r] = DDNS_WARN_GMRES;                // 2980-2982
