@@ -146,3 +146,46 @@ def test_initial_field_is_reproducible(ddns, prec):
             s.init_flow(ddns.SPEC_K4_GAUSS, 77, 4.0, 1.0)
             out.append(s.get_flow()[0])
     assert _same(out[0], out[1])
+
+
+def test_batch_mixed_members_equal_single_handles(ddns):
+    """A batch whose realisations differ in what their droplet blocks do: no droplets, droplets
+    without HI (collisions and relocation only), HI with the usual GMRES limit, and HI with
+    another limit (solved on its own, outside the batched GMRES).  Each gives what a handle
+    of its own gives."""
+    N = 64
+    base = dict(N=N, precision=ddns.FP64, forcing=1)
+    params = [
+        ddns.default_params(**base),
+        ddns.default_params(**base, npart=4000, radius=[50.0], nonoverlap=1, relocate_seed=1),
+        ddns.default_params(**base, npart=3000, radius=[30.0], hdi_included=1, hdi_trunc=20.0, relocate_seed=2),
+        ddns.default_params(**base, npart=2000, radius=[35.0], hdi_included=1, hdi_trunc=20.0, relocate_seed=3,
+                            gmres_maxiter=30),
+    ]
+    steps = 3
+    single = []
+    for r, p in enumerate(params):
+        with ddns.Solver(p) as s:
+            s.init_flow(ddns.SPEC_K4_GAUSS, 700 + r, 4.0, 1.0)
+            if p.npart:
+                s.init_droplets(seed=1000 + r)
+            s.step(steps)
+            single.append((s.get_flow()[0], s.get_droplets() if p.npart else None, s.collision_counts()))
+    with ddns.Batch(params) as b:
+        for r, p in enumerate(params):
+            m = b.member(r)
+            m.init_flow(ddns.SPEC_K4_GAUSS, 700 + r, 4.0, 1.0)
+            if p.npart:
+                m.init_droplets(seed=1000 + r)
+        b.step(steps)
+        for r, p in enumerate(params):
+            m = b.member(r)
+            u1, d1, c1 = single[r]
+            assert _same(m.get_flow()[0], u1), f"flow of realisation {r} differs"
+            if p.npart:
+                d, c = m.get_droplets(), m.collision_counts()
+                for k in ("yp", "fvp", "fdp", "pertvel"):
+                    assert np.array_equal(d[k], d1[k]), f"{k} of realisation {r} differs"
+                assert c.gmres_iters == c1.gmres_iters and list(c.icc) == list(c1.icc)
+                if p.hdi_included:
+                    assert c.gmres_iters >= 1
