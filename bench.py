@@ -17,6 +17,7 @@ The CPU baseline times the oracle on a bounded sample of the same workload.
 from __future__ import annotations
 
 import argparse
+import functools
 import json
 import os
 import statistics
@@ -64,6 +65,26 @@ def solver_from(state, precision, N, npart):
     return s
 
 
+@functools.lru_cache(maxsize=None)
+def sphere_fraction(N: int) -> float:
+    """Fraction of the stored modes (kx < N/2, all ky, kz) inside the dealiasing sphere:
+    shell int(|k| + 1/2) <= nek = N/2 - 1.5, the mean mode counted inside.  Beyond it the
+    state and the stored nonlinear term are zero, and the z stage neither reads nor writes
+    them once a step has."""
+    nek = int(N / 2 - 1.5)
+    w = np.arange(N)
+    w = np.where(w <= N // 2, w, w - N).astype(np.float64)
+    r2 = (w[:, None] ** 2 + w[None, :] ** 2)
+    inside = 0
+    for kx in range(N // 2):
+        k2 = r2 + kx * kx
+        if kx == 0:
+            k2 = k2.copy()
+            k2[0, 0] = 1e-5
+        inside += int(np.count_nonzero(np.floor(np.sqrt(k2) + 0.5) <= nek))
+    return inside / (N * N * (N // 2))
+
+
 def kernel_bytes(N: int, prec: int):
     """Compulsory HBM bytes per launch of each flow kernel (every input read once,
     every output written once).  C = one complex field N*N*(N/2), R = one real field."""
@@ -71,14 +92,16 @@ def kernel_bytes(N: int, prec: int):
     rs = cs // 2
     C = N * N * (N // 2) * cs
     R = N * N * N * rs
+    f = sphere_fraction(N)
     return {
         "y_inv": 12 * C,                    # 6 fields read + written
         "x_fused": 6 * C + 3 * C + 3 * R,   # read u, w; write u x w and the physical u
         "y_fwd": 6 * C,
         "pre": 0,
-        # below N = 512 one fused kernel reads N, u, o (9C) and writes u, o and the six
-        # inverse outputs (12C); from 512 the separate kernels move 36C
-        "z_stage": (21 if N < 512 else 36) * C,
+        # below N = 512 one fused kernel reads N (3C) and writes the six inverse outputs
+        # (6C); it reads and writes u, o (12C) only inside the dealiasing sphere (fraction
+        # f).  From 512 the separate kernels move 24C + 12fC (36C when f = 1)
+        "z_stage": int(((9 if N < 512 else 24) + 12 * f) * C),
         "sym": 0,
     }
 

@@ -115,6 +115,20 @@ template <typename T, int N> constexpr int kz_ldr()
 
 __device__ __forceinline__ int wave_index(int i, int N) { return i <= N / 2 ? i : i - N; }
 
+// Is mode (kx, ky, kz) inside the dealiasing sphere (turb.f90:405, 648-650)?  update_mode's
+// own shell index, in its precision: outside it the update sets both the new u-hat and
+// the stored nonlinear term to zero (the mean mode counts as inside: its u-hat is zero
+// but its nonlinear term is kept).
+template <typename T>
+__device__ __forceinline__ bool mode_in_sphere(int kx, int ky, int kz, int N, int nek)
+{
+    const int kyw = wave_index(ky, N), kzw = wave_index(kz, N);
+    const T fx = (T)kx, fy = (T)kyw, fz = (T)kzw;
+    T k2 = fx * fx + fy * fy + fz * fz;
+    if (kx == 0 && kyw == 0 && kzw == 0) k2 = (T)1.e-5;
+    return (int)(sqrt(k2) + (T)0.5) <= nek;
+}
+
 // Batched realisations: realisation r of a field starts r * rstride complex values after
 // realisation 0 (rstride = 0 for one realisation), its low-mode table r * LOW_STRIDE
 // doubles after, and the physical velocity r * pstride reals after.  With forcing < 0
@@ -122,6 +136,11 @@ __device__ __forceinline__ int wave_index(int i, int N) { return i <= N / 2 ? i 
 // (LOW_PAR onwards).
 constexpr int LOW_STRIDE = 1024;
 constexpr int LOW_PAR = 1000;       // [0] forcing kind, [1..2] force(1:2), [3] OU time scale, [4] OU variance
+// [LOW_ZOUT] = 1 once a step has written the state's modes beyond the dealiasing sphere,
+// which every update sets to zero: the z stage then neither loads nor stores them (u-hat
+// and the AB2 history stay zero there).  Cleared by every upload of the state; set by the
+// step's k_sym, which runs after the z stage.
+constexpr int LOW_ZOUT = LOW_PAR + 5;
 constexpr int STH_STATE_STRIDE = 64;
 constexpr int STH_B_STRIDE = 512;
 
@@ -300,18 +319,22 @@ __global__ void __launch_bounds__(256) k_z_update(KzArgs a)
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)N * N * M) return;
     const int kx = (int)(t % M), ky = (int)((t / M) % N), kz = (int)(t / ((size_t)N * M));
+    // beyond the dealiasing sphere the state is zero once a step has written it
+    const bool in = a.low[LOW_ZOUT] == 0.0 || mode_in_sphere<T>(kx, ky, kz, N, a.nek);
     C nn[3], u3[3], o3[3], vv[3], nd[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         nn[q] = ((const C*)a.B[q])[t];
-        u3[q] = ((const C*)a.U[q])[t];
-        o3[q] = ((const C*)a.O[q])[t];
+        u3[q] = in ? ((const C*)a.U[q])[t] : mk<C>((T)0, (T)0);
+        o3[q] = in ? ((const C*)a.O[q])[t] : mk<C>((T)0, (T)0);
     }
     update_mode<T>(kx, ky, kz, N, a.nek, a.forcing, (T)a.dt, (T)a.rnu, a.low, nn, u3, o3, vv, nd);
+    if (in) {
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        ((C*)a.U[q])[t] = vv[q];
-        ((C*)a.O[q])[t] = nd[q];
+        for (int q = 0; q < 3; ++q) {
+            ((C*)a.U[q])[t] = vv[q];
+            ((C*)a.O[q])[t] = nd[q];
+        }
     }
     curl_to<T>(a, t, kx, ky, kz, N, vv);
 }
@@ -401,6 +424,8 @@ __global__ void __launch_bounds__(NTZ) k_z_full(KzArgs a, int ntiles)
         const size_t base = (size_t)rr * a.rstride + (size_t)ky * M + kx0;
         const double* lowr = a.low + (size_t)rr * LOW_STRIDE;
         const int forcing = real_forcing(a.forcing, lowr);
+        // beyond the dealiasing sphere the state is zero once a step has written it
+        const bool zout = lowr[LOW_ZOUT] != 0.0;
         if (!NEXT_B && tile != (int)blockIdx.x) {
 #pragma unroll
             for (int q = 0; q < 3; ++q) tile_fetch<C, N, TC, NTZ>(nb[q], (const C*)a.B[q] + base, rowstride, tid);
@@ -413,10 +438,11 @@ __global__ void __launch_bounds__(NTZ) k_z_full(KzArgs a, int ntiles)
         for (int k = 0; k < EPTP; ++k) {
             const int e = tid + k * NTZ, n = e / TC, c = e % TC;
             const size_t g = base + (size_t)n * rowstride + c;
+            const bool ld = !zout || mode_in_sphere<T>(kx0 + c, ky, n, N, a.nek);
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
-                uu[k][q] = ((const C*)a.U[q])[g];
-                oo[k][q] = ((const C*)a.O[q])[g];
+                uu[k][q] = ld ? ((const C*)a.U[q])[g] : mk<C>((T)0, (T)0);
+                oo[k][q] = ld ? ((const C*)a.O[q])[g] : mk<C>((T)0, (T)0);
             }
         }
         __syncthreads();
@@ -429,10 +455,13 @@ __global__ void __launch_bounds__(NTZ) k_z_full(KzArgs a, int ntiles)
 #pragma unroll
             for (int q = 0; q < 3; ++q) nn[q] = cscale(lds[q][n * LDR + c], inv_n);
             update_mode<T>(kx0 + c, ky, n, N, a.nek, forcing, (T)a.dt, (T)a.rnu, lowr, nn, u3, o3, vv, nd);
+            const bool wr = !zout || mode_in_sphere<T>(kx0 + c, ky, n, N, a.nek);
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
-                ((C*)a.U[q])[g] = vv[q];
-                ((C*)a.O[q])[g] = nd[q];
+                if (wr) {
+                    ((C*)a.U[q])[g] = vv[q];
+                    ((C*)a.O[q])[g] = nd[q];
+                }
                 lds[q][n * LDR + c] = vv[q];
             }
         };
@@ -442,11 +471,12 @@ __global__ void __launch_bounds__(NTZ) k_z_full(KzArgs a, int ntiles)
         for (int k = EPTP; k < EPT; ++k) {
             const int e = tid + k * NTZ, n = e / TC, c = e % TC;
             const size_t g = base + (size_t)n * rowstride + c;
+            const bool ld = !zout || mode_in_sphere<T>(kx0 + c, ky, n, N, a.nek);
             C u3[3], o3[3];
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
-                u3[q] = ((const C*)a.U[q])[g];
-                o3[q] = ((const C*)a.O[q])[g];
+                u3[q] = ld ? ((const C*)a.U[q])[g] : mk<C>((T)0, (T)0);
+                o3[q] = ld ? ((const C*)a.O[q])[g] : mk<C>((T)0, (T)0);
             }
             update(k, u3, o3);
         }
@@ -971,8 +1001,10 @@ __global__ void __launch_bounds__(NT) k_pre(PreArgs a)
 
 // ------------------------------------------------------------------ kx = 0 plane of u-hat
 template <typename T, int N>
-__global__ void k_sym(void* u0, void* u1, void* u2, size_t rstride)
+__global__ void k_sym(void* u0, void* u1, void* u2, size_t rstride, double* low)
 {
+    // the z stage before this kernel wrote every mode beyond the dealiasing sphere
+    if (blockIdx.x == 0 && threadIdx.x == 0) low[(size_t)blockIdx.y * LOW_STRIDE + LOW_ZOUT] = 1.0;
     using C = cpx<T>;
     constexpr int M = N / 2;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1333,7 +1365,7 @@ static int launch_sym(const FlowBuffers& fb, hipStream_t s)
 {
     const int tot = N * N;
     hipLaunchKernelGGL((k_sym<T, N>), dim3((tot + 255) / 256, fb.R), dim3(256), 0, s, fb.U[0], fb.U[1], fb.U[2],
-                       fb.rstride);
+                       fb.rstride, fb.low);
     DDNS_HIP(hipGetLastError());
     return 0;
 }
@@ -1447,8 +1479,20 @@ static int download_spectral(const void* dev, int N, double* v, hipStream_t s)
     return 0;
 }
 
+// the state's modes beyond the dealiasing sphere may be nonzero again (an upload, a new
+// field): the next z stage loads and stores them all
+static int mark_dense(const FlowBuffers& fb)
+{
+    static const double zero = 0.0;
+    DDNS_HIP(hipMemcpy(fb.low + LOW_ZOUT, &zero, sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
 int flow_upload_state(const FlowBuffers& fb, int N, int precision, const double* const src[6], hipStream_t s)
 {
+    DDNS_HIP(hipStreamSynchronize(s));
+    const int sm = mark_dense(fb);
+    if (sm) return sm;
     for (int f = 0; f < 6; ++f) {
         if (!src[f]) continue;
         void* dev = f < 3 ? fb.U[f] : fb.O[f - 3];
@@ -1554,6 +1598,9 @@ int flow_init(FlowBuffers& fb, int N, int nek, int precision, int kind, int seed
               hipStream_t s)
 {
     int st = 0;
+    DDNS_HIP(hipStreamSynchronize(s));
+    st = mark_dense(fb);
+    if (st) return st;
     if (precision == DDNS_FP64) { DDNS_DISPATCH_N(N, st = (init_impl<double, NN>(fb, nek, kind, seed, k0, urms, force0, s))) }
     else { DDNS_DISPATCH_N(N, st = (init_impl<float, NN>(fb, nek, kind, seed, k0, urms, force0, s))) }
     return st;
