@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(256) k_extent_max(const double* __restrict__ d
 __device__ __forceinline__ void collide_pair(const CollArgs& a, int ia, const double (&xa)[3], double ra, double dmove_a,
                                              int ib, const double4& cb, int wy, int wz, const double (&xb)[3],
                                              const double (&xd)[3], double dnij0, int di, int dj, int dk,
-                                             long long& instat, int& ncollip1)
+                                             long long& instat, int& ncollip1, double* bins, int* nirbin)
 {
     const size_t C = (size_t)a.cap;
     const int nbin = DDNS_NBIN, ns = a.nset;
@@ -216,36 +216,36 @@ __device__ __forceinline__ void collide_pair(const CollArgs& a, int ia, const do
             findindex(a.radius, ns, ra, rb, &i1, &i2);
             const int bi = (i1 - 1) + ns * ((i2 - 1) + ns * ir);
             const size_t nb = (size_t)ns * ns * nbin;
-            atomicAdd(a.nirbin + bi, 1);
+            atomicAdd(nirbin + bi, 1);
             if (i2 == i1) {
-                atomicAdd(a.bins + 1 * nb + bi, va[0] - vb[0]);
-                atomicAdd(a.bins + 2 * nb + bi, va[1] - vb[1]);
-                atomicAdd(a.bins + 3 * nb + bi, va[2] - vb[2]);
+                atomicAdd(bins + 1 * nb + bi, va[0] - vb[0]);
+                atomicAdd(bins + 2 * nb + bi, va[1] - vb[1]);
+                atomicAdd(bins + 3 * nb + bi, va[2] - vb[2]);
             }
             if (ra > rb) {
-                atomicAdd(a.bins + 1 * nb + bi, va[0] - vb[0]);
-                atomicAdd(a.bins + 2 * nb + bi, va[1] - vb[1]);
-                atomicAdd(a.bins + 3 * nb + bi, va[2] - vb[2]);
+                atomicAdd(bins + 1 * nb + bi, va[0] - vb[0]);
+                atomicAdd(bins + 2 * nb + bi, va[1] - vb[1]);
+                atomicAdd(bins + 3 * nb + bi, va[2] - vb[2]);
             } else if (ra < rb) {
-                atomicAdd(a.bins + 1 * nb + bi, vb[0] - va[0]);
-                atomicAdd(a.bins + 2 * nb + bi, vb[1] - va[1]);
-                atomicAdd(a.bins + 3 * nb + bi, vb[2] - va[2]);
+                atomicAdd(bins + 1 * nb + bi, vb[0] - va[0]);
+                atomicAdd(bins + 2 * nb + bi, vb[1] - va[1]);
+                atomicAdd(bins + 3 * nb + bi, vb[2] - va[2]);
             }
-            atomicAdd(a.bins + 0 * nb + bi, Wr);
+            atomicAdd(bins + 0 * nb + bi, Wr);
             const double du0 = a.vp[ia] - a.vp[ib], du1 = a.vp[C + ia] - a.vp[C + ib], du2 = a.vp[2 * C + ia] - a.vp[2 * C + ib];
-            atomicAdd(a.bins + 4 * nb + bi, du0 * xh[0] + du1 * xh[1] + du2 * xh[2]);
+            atomicAdd(bins + 4 * nb + bi, du0 * xh[0] + du1 * xh[1] + du2 * xh[2]);
             const double aaa = (va[0] - vb[0]) * xh[0] + (va[1] - vb[1]) * xh[1] + (va[2] - vb[2]) * xh[2];
             const double bbb = sqrt((va[0] - vb[0]) * (va[0] - vb[0]) + (va[1] - vb[1]) * (va[1] - vb[1]) +
                                     (va[2] - vb[2]) * (va[2] - vb[2]));
             if (bbb != 0.) {
                 const double q = fabs(aaa / bbb);
                 const double vangle = acos(q);
-                atomicAdd(a.bins + 5 * nb + bi, vangle);
+                atomicAdd(bins + 5 * nb + bi, vangle);
                 // sin(acos q) = sqrt(1 - q^2) for q in [0, 1]: the same value
                 // as the reference's sin(vangle) (partlib.f90:1400) to rounding,
                 // without the register cost of a general double sine
                 const double sb = bbb * sqrt(fmax(0.0, 1.0 - q * q));
-                atomicAdd(a.bins + 6 * nb + bi, sb * sb / dnij0);
+                atomicAdd(bins + 6 * nb + bi, sb * sb / dnij0);
             }
         }
         if (dnij0 <= rcollb) {                        // 1411-1437
@@ -371,7 +371,35 @@ __device__ __forceinline__ void collide_pair(const CollArgs& a, int ia, const do
 // the collision tests are handed on (cell-order position of ih1, of the partner and
 // the neighbour-cell number), so the heavy, rarely taken branches do not stall
 // whole wavefronts here.
-__device__ __forceinline__ void collide_search(const CollArgs& a, int t, int lane, int& tested, int& touching)
+// The handed-on pairs of a wavefront are staged in its own LDS buffer of STAGE entries
+// and written out with one atomic on the global count per flush: one atomic per walk
+// step with an emitting lane (a droplet in a dense realisation has tens of pairs in
+// the statistics window) serialises the search on that count.
+constexpr int STAGE = 512;
+
+struct CandStage {
+    int2* buf;              // STAGE entries of this wavefront (LDS)
+    volatile int* cnt;      // entries held (LDS); written by one lane, read by all
+};
+
+// Hand the staged entries to the global list: the lanes active here (mask exec) copy
+// them to the slots one atomic reserved.  Entries past cand_cap are dropped and
+// counted; the host sees the overflow and runs the search again with a larger list.
+__device__ __forceinline__ void stage_flush(const CollArgs& a, const CandStage& st, int c, int lane,
+                                            unsigned long long exec)
+{
+    const int leader = __ffsll((long long)exec) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(a.ncand, c);
+    base = __shfl(base, leader, 64);
+    const int nact = __popcll(exec), rank = __popcll(exec & ((1ull << lane) - 1));
+    for (int i = rank; i < c; i += nact)
+        if (base + i < a.cand_cap) a.cand[base + i] = st.buf[i];
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ void collide_search(const CollArgs& a, int t, int lane, int& tested, int& touching,
+                                               const CandStage& st)
 {
     const int ia = a.sorted[t];
     const size_t C = (size_t)a.cap;
@@ -433,17 +461,19 @@ __device__ __forceinline__ void collide_search(const CollArgs& a, int t, int lan
         const bool near = !(di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) &&   // 1445
                           !(dnij0 > rcoll + dmove_a + dmax);
         const bool emit = stat || near;
-        // one atomic per wavefront for the slots of its emitting lanes
+        // the emitting lanes take consecutive entries of the wavefront's stage
         const unsigned long long m = __ballot(emit);
         if (m) {
-            const int leader = __ffsll((long long)m) - 1;
-            int base = 0;
-            if (lane == leader) base = atomicAdd(a.ncand, __popcll(m));
-            base = __shfl(base, leader, 64);
-            if (emit) {
-                const int slot = base + __popcll(m & ((1ull << lane) - 1));
-                if (slot < a.cand_cap) a.cand[slot] = make_int2(t, (int)((unsigned)p | ((unsigned)kc << 27)));
+            const int k = __popcll(m);
+            int c = *st.cnt;
+            if (c + k > STAGE) {
+                stage_flush(a, st, c, lane, __ballot(1));
+                c = 0;
             }
+            if (emit) st.buf[c + __popcll(m & ((1ull << lane) - 1))] = make_int2(t, (int)((unsigned)p | ((unsigned)kc << 27)));
+            __builtin_amdgcn_wave_barrier();
+            if (lane == __ffsll((long long)m) - 1) *st.cnt = c + k;
+            __builtin_amdgcn_wave_barrier();
         }
     }, reach_span(xa, cc, g.hc, reach));
 }
@@ -452,8 +482,18 @@ __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
 {
     const int t = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
+    __shared__ int2 stage_buf[CT / 64][STAGE];
+    __shared__ int stage_cnt[CT / 64];
+    const CandStage st{stage_buf[threadIdx.x >> 6], stage_cnt + (threadIdx.x >> 6)};
+    if (lane == 0) *st.cnt = 0;
+    __builtin_amdgcn_wave_barrier();
     int tested = 0, touching = 0;
-    if (t < a.n) collide_search(a, t, lane, tested, touching);
+    if (t < a.n) collide_search(a, t, lane, tested, touching, st);
+    // the whole wavefront is back together: hand on what its stage still holds
+    {
+        const int c = *st.cnt;
+        if (c) stage_flush(a, st, c, lane, __ballot(1));
+    }
     // one atomic per wavefront for the counters (one per lane would serialise a
     // million atomics on two addresses)
 #pragma unroll
@@ -471,8 +511,26 @@ __global__ void __launch_bounds__(CT) k_collide(CollArgs a)
 // search did and runs the statistics and collision tests.
 // The kernel is queued right behind the search and reads the search's count on the
 // device (the host checks it for overflow afterwards); a grid-stride loop over the pairs.
+// With LB the statistics bins (nset^2 NBIN of them, at most LB_MAXBIN) are summed in
+// LDS per work-group and added to the global bins once at the end: a realisation of one
+// or two droplet sizes puts all its pairs on 180 or 720 bins, and one global atomic per
+// pair and bin value serialises the kernel on them.
+constexpr int LB_MAXBIN = 4 * DDNS_NBIN;
+
+template <bool LB>
 __global__ void __launch_bounds__(256) k_collide_pairs(CollArgs a)
 {
+    extern __shared__ double lds_bins[];
+    const int nb = a.nset * a.nset * DDNS_NBIN;
+    double* bins = a.bins;
+    int* nirbin = a.nirbin;
+    if (LB) {
+        bins = lds_bins;
+        nirbin = (int*)(lds_bins + 7 * nb);
+        for (int i = threadIdx.x; i < 7 * nb; i += blockDim.x) bins[i] = 0.0;
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) nirbin[i] = 0;
+        __syncthreads();
+    }
     const int ncand = min(*a.ncand, a.cand_cap);
     long long instat = 0;
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < ncand; r += gridDim.x * blockDim.x) {
@@ -508,7 +566,8 @@ __global__ void __launch_bounds__(256) k_collide_pairs(CollArgs a)
             for (int c = 0; c < 3; ++c) xd[c] = xa[c] - xb[c];
             const double dnij0 = sqrt(xd[0] * xd[0] + xd[1] * xd[1] + xd[2] * xd[2]);
             int ncollip1 = 0;
-            collide_pair(a, ia, xa, ra, dmove_a, ib, cb, wy, wz, xb, xd, dnij0, di, dj, dk, instat, ncollip1);
+            collide_pair(a, ia, xa, ra, dmove_a, ib, cb, wy, wz, xb, xd, dnij0, di, dj, dk, instat, ncollip1,
+                         bins, nirbin);
             // ih1 counts once among the droplets that collided (1578-1583)
             if (ncollip1 != 0 && atomicExch(a.hit + ia, 1) == 0) {
                 atomicAdd(a.mult + ia, 1);
@@ -520,6 +579,19 @@ __global__ void __launch_bounds__(256) k_collide_pairs(CollArgs a)
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) instat += __shfl_xor(instat, o, 64);
     if ((threadIdx.x & 63) == 0 && instat) atomicAdd((unsigned long long*)(a.tested + 1), (unsigned long long)instat);
+    if (LB) {
+        __syncthreads();
+        // a bin with no pair in this work-group adds nothing (its count is 0)
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+            if (!nirbin[i]) continue;
+            atomicAdd(a.nirbin + i, nirbin[i]);
+#pragma unroll
+            for (int v = 0; v < 7; ++v) {
+                const double x = bins[v * nb + i];
+                if (x != 0.0) atomicAdd(a.bins + (size_t)v * nb + i, x);
+            }
+        }
+    }
 }
 
 // OVERLAP_DET sweep (2076-2157): count, per droplet as ih1, the overlapping pairs
@@ -687,7 +759,12 @@ int part_collide_search(PartBuffers& pb, const PartConst& pc, hipStream_t s)
     DDNS_HIP(hipMemcpyAsync(pb.found_host, a.ncand, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
     // the pair kernel, sized by the last step's count (a grid-stride loop covers more)
     const int nwg = std::max(256, std::min(4096, (int)((pb.last_ncand + pb.last_ncand / 4 + 255) / 256)));
-    hipLaunchKernelGGL(k_collide_pairs, dim3(nwg), dim3(256), 0, s, a);
+    if ((int)nb <= LB_MAXBIN)
+        // bins in LDS: two work-groups per CU, each folding its sums into the global bins once
+        hipLaunchKernelGGL(k_collide_pairs<true>, dim3(std::min(nwg, 512)), dim3(256), nb * (7 * sizeof(double) + sizeof(int)),
+                           s, a);
+    else
+        hipLaunchKernelGGL(k_collide_pairs<false>, dim3(nwg), dim3(256), 0, s, a);
     DDNS_HIP(hipGetLastError());
     return 0;
 }
