@@ -163,6 +163,72 @@ __device__ __forceinline__ void for_each_candidate(const int* __restrict__ cell_
     }
 }
 
+// The candidates of for_each_candidate, in the same order, with the ranges of one row of
+// three cells held at a time: the next row's ranges load while the current row runs, and
+// its first record once the row is done.  The 27 ranges and the nine unrolled rows of
+// for_each_candidate take about 50 registers; without them more wavefronts fit on a SIMD,
+// which is what a walk that waits on one record load per candidate needs.
+template <typename F>
+__device__ __forceinline__ void for_each_candidate_lean(const int* __restrict__ cell_start, int G, const int cc[3],
+                                                        const double4* __restrict__ pk, const int* __restrict__ sorted,
+                                                        F&& f, const CellSpan& span = CellSpan())
+{
+    auto wrap = [G](int q) { return q < 0 ? q + G : (q >= G ? q - G : q); };
+    // [s, e) of the cells x-1, x, x+1 of row r = 3 my + mz; empty where span leaves it out
+    auto load_row = [&](int r, int& s0, int& e0, int& s1, int& e1, int& s2, int& e2) {
+        const int my = r / 3, mz = r - 3 * my;
+        s0 = e0 = s1 = e1 = s2 = e2 = 0;
+        if (my - 1 < span.lo[1] || my - 1 > span.hi[1] || mz - 1 < span.lo[2] || mz - 1 > span.hi[2]) return;
+        const int qy = wrap(cc[1] + my - 1), qz = wrap(cc[2] + mz - 1);
+        if (span.lo[0] < 0) {
+            const int c = cell_code(wrap(cc[0] - 1), qy, qz);
+            s0 = cell_start[c]; e0 = cell_start[c + 1];
+        }
+        {
+            const int c = cell_code(cc[0], qy, qz);
+            s1 = cell_start[c]; e1 = cell_start[c + 1];
+        }
+        if (span.hi[0] > 0) {
+            const int c = cell_code(wrap(cc[0] + 1), qy, qz);
+            s2 = cell_start[c]; e2 = cell_start[c + 1];
+        }
+    };
+    int s0, e0, s1, e1, s2, e2;
+    load_row(0, s0, e0, s1, e1, s2, e2);
+#pragma unroll 1
+    for (int r = 0; r < 9; ++r) {
+        int n0 = 0, m0 = 0, n1 = 0, m1 = 0, n2 = 0, m2 = 0;
+        if (r + 1 < 9) load_row(r + 1, n0, m0, n1, m1, n2, m2);
+        int m = 0, p = s0, end = e0;
+        if (p >= end) { m = 1; p = s1; end = e1; }
+        if (p >= end) { m = 2; p = s2; end = e2; }
+        if (p < end) {
+            double4 next = pk[p];
+            int inext = sorted ? sorted[p] : 0;
+#pragma unroll 1
+            while (true) {
+                const int pc = p, kc = 9 * m + r;
+                const double4 rec = next;
+                const int ib = inext;
+                ++p;
+                while (p >= end && m < 2) {
+                    ++m;
+                    p = m == 1 ? s1 : s2;
+                    end = m == 1 ? e1 : e2;
+                }
+                const bool more = p < end;
+                if (more) {
+                    next = pk[p];
+                    if (sorted) inext = sorted[p];
+                }
+                f(pc, kc, rec, ib);
+                if (!more) break;
+            }
+        }
+        s0 = n0; e0 = m0; s1 = n1; e1 = m1; s2 = n2; e2 = m2;
+    }
+}
+
 // periodic shift (-1, 0, +1) of neighbour k along each axis for a lane in cell cc
 __device__ __forceinline__ void neighbour_shift(int k, int G, const int cc[3], int* wx, int* wy, int* wz)
 {
