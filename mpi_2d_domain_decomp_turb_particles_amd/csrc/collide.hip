@@ -432,7 +432,7 @@ __device__ __forceinline__ void collide_search(const CollArgs& a, int t, int lan
     // the walk covers the neighbour cells only: a reach beyond the cell width is reported,
     // and the search is repeated on wider cells
     if (reach + 1e-9 * g.hc > g.hc) atomicOr(a.far, 1);
-    for_each_candidate_lean(a.cell_start, g.G, cc, a.pk, a.sorted, [&](int p, int kc, const double4& cb, int ib) {
+    for_each_candidate(a.cell_start, g.G, cc, a.pk, a.sorted, [&](int p, int kc, const double4& cb, int ib) {
         if (p == t) return;
         int wx, wy, wz;
         neighbour_shift(kc, g.G, cc, &wx, &wy, &wz);
@@ -629,7 +629,7 @@ __global__ void __launch_bounds__(CT) k_overlap(OverArgs a)
     // and the visiting rules are replayed for the partners within that distance only
     const double reach = ra + __longlong_as_double((long long)*a.rbig);
     const double reach2 = reach * reach * (1. + 1e-9);
-    for_each_candidate_lean(a.cell_start, g.G, cc, a.pk, a.sorted, [&](int p, int kc, const double4& cb, int ib) {
+    for_each_candidate(a.cell_start, g.G, cc, a.pk, a.sorted, [&](int p, int kc, const double4& cb, int ib) {
         if (p == t) return;
         int wx, wy, wz;
         neighbour_shift(kc, g.G, cc, &wx, &wy, &wz);

@@ -81,97 +81,18 @@ __device__ __forceinline__ CellSpan reach_span(const double x[3], const int cc[3
 }
 
 // The candidates of the 27 neighbour cells (or of the part of them that span selects)
-// in 9 rows: for each (y, z) offset the cells x-1, x, x+1 as one sequence.  The ranges
-// of all the cells are loaded together first, and each row's first packed record is
-// loaded while the row before it runs; inside a row the next record is loaded while f
-// works on the current one.  So a lane waits for memory about once per walk instead of
-// twice per row.  The ranges stay in registers (no LDS); a wavefront runs, per row, as
-// long as its fullest lane.  f(p, kc, rec, ib) gets the sorted position, the cell
-// number k = 9 (dx+1) + 3 (dy+1) + (dz+1) of neighbour_shift, the record pk[p] and,
-// when sorted is given, the droplet index sorted[p].
+// in 9 rows: for each (y, z) offset the cells x-1, x, x+1 as one sequence.  One row's
+// three ranges are held at a time: the next row's ranges load while the current row runs,
+// and inside a row the next record loads while f works on the current one.  Holding all
+// 27 ranges (and unrolling the nine rows) took about 50 more registers, and a walk that
+// waits on about one record load per candidate gains more from the wavefronts those
+// registers allow per SIMD.  f(p, kc, rec, ib) gets the sorted position, the cell number
+// k = 9 (dx+1) + 3 (dy+1) + (dz+1) of neighbour_shift, the record pk[p] and, when sorted
+// is given, the droplet index sorted[p].
 template <typename F>
 __device__ __forceinline__ void for_each_candidate(const int* __restrict__ cell_start, int G, const int cc[3],
                                                    const double4* __restrict__ pk, const int* __restrict__ sorted,
                                                    F&& f, const CellSpan& span = CellSpan())
-{
-    int qy[3], qz[3], qx[3];
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-        int q = cc[0] + m - 1; qx[m] = q < 0 ? q + G : (q >= G ? q - G : q);
-        q = cc[1] + m - 1;     qy[m] = q < 0 ? q + G : (q >= G ? q - G : q);
-        q = cc[2] + m - 1;     qz[m] = q < 0 ? q + G : (q >= G ? q - G : q);
-    }
-    // [s, e) of cell k = 9 mx + r, row r = 3 my + mz; empty where span leaves it out
-    int s[27], e[27];
-#pragma unroll
-    for (int k = 0; k < 27; ++k) {
-        const int mx = k / 9, my = (k % 9) / 3, mz = k % 3;
-        s[k] = e[k] = 0;
-        if (mx - 1 >= span.lo[0] && mx - 1 <= span.hi[0] && my - 1 >= span.lo[1] && my - 1 <= span.hi[1] &&
-            mz - 1 >= span.lo[2] && mz - 1 <= span.hi[2]) {
-            const int cell = cell_code(qx[mx], qy[my], qz[mz]);
-            s[k] = cell_start[cell];
-            e[k] = cell_start[cell + 1];
-        }
-    }
-    // first candidate of row r: position, or -1 for an empty row
-    auto first = [&](int r) { return s[r] < e[r] ? s[r] : (s[r + 9] < e[r + 9] ? s[r + 9] : (s[r + 18] < e[r + 18] ? s[r + 18] : -1)); };
-    double4 pre = make_double4(0.0, 0.0, 0.0, 0.0);
-    int pre_ib = 0;
-    {
-        const int p0 = first(0);
-        if (p0 >= 0) {
-            pre = pk[p0];
-            if (sorted) pre_ib = sorted[p0];
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        double4 next = pre;
-        int inext = pre_ib;
-        if (r + 1 < 9) {
-            const int p1 = first(r + 1);
-            if (p1 >= 0) {
-                pre = pk[p1];
-                if (sorted) pre_ib = sorted[p1];
-            }
-        }
-        // cursor over the three ranges of the row
-        int m = 0, p = s[r], end = e[r];
-        if (p >= end) { m = 1; p = s[r + 9]; end = e[r + 9]; }
-        if (p >= end) { m = 2; p = s[r + 18]; end = e[r + 18]; }
-        if (p >= end) continue;
-#pragma unroll 1
-        while (true) {
-            const int pc = p, kc = 9 * m + r;
-            const double4 rec = next;
-            const int ib = inext;
-            ++p;
-            while (p >= end && m < 2) {
-                ++m;
-                p = m == 1 ? s[r + 9] : s[r + 18];
-                end = m == 1 ? e[r + 9] : e[r + 18];
-            }
-            const bool more = p < end;
-            if (more) {
-                next = pk[p];
-                if (sorted) inext = sorted[p];
-            }
-            f(pc, kc, rec, ib);
-            if (!more) break;
-        }
-    }
-}
-
-// The candidates of for_each_candidate, in the same order, with the ranges of one row of
-// three cells held at a time: the next row's ranges load while the current row runs, and
-// its first record once the row is done.  The 27 ranges and the nine unrolled rows of
-// for_each_candidate take about 50 registers; without them more wavefronts fit on a SIMD,
-// which is what a walk that waits on one record load per candidate needs.
-template <typename F>
-__device__ __forceinline__ void for_each_candidate_lean(const int* __restrict__ cell_start, int G, const int cc[3],
-                                                        const double4* __restrict__ pk, const int* __restrict__ sorted,
-                                                        F&& f, const CellSpan& span = CellSpan())
 {
     auto wrap = [G](int q) { return q < 0 ? q + G : (q >= G ? q - G : q); };
     // [s, e) of the cells x-1, x, x+1 of row r = 3 my + mz; empty where span leaves it out

@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(NT_NEI) k_nei(const double4* __restrict__ pk, 
         }
         const size_t ld = (size_t)n;
         bool overlap = false;
-        for_each_candidate_lean(cell_start, G, cc, pk, (const int*)nullptr, [&](int p, int kc, const double4& o, int) {
+        for_each_candidate(cell_start, G, cc, pk, (const int*)nullptr, [&](int p, int kc, const double4& o, int) {
             if (p == t) return;
             int wx, wy, wz;
             neighbour_shift(kc, G, cc, &wx, &wy, &wz);
