@@ -202,6 +202,33 @@ def test_collision_contact_overflow_matches_oracle(ddns, oracle_mod):
         assert np.allclose(ro[:, 8:], rg[:, 8:], rtol=0, atol=1e-13)
 
 
+def test_collision_candidate_overflow_matches_oracle(ddns, oracle_mod):
+    """6000 droplets of 300 um in a 64^3 domain: about 136k pairs inside the statistics
+    window (10 collision radii), more than the first list of handed-on pairs holds (65536
+    entries at this size), so the search drops what does not fit, reports the count, and
+    runs again with a list of that size.  One radius set: the pair kernel sums its 180
+    statistics bins per work-group in LDS.  The result is the oracle's."""
+    N = 64
+    n = 6000
+    p = _params(ddns, N=N, npart=n, nset=1, radius=[300.0], dt=0.01)
+    with oracle_mod.Oracle(p) as o, ddns.Solver(p) as s:
+        rad = o.scaling()["radius"][:1]
+        yp, dm, v0, vn, vp = _collision_inputs(n, 11, rad, dt=0.01)
+        o.collision_detect(yp, dm, v0, vn, vp)
+        s.collision_detect(yp, dm, v0, vn, vp)
+        co, cg = o.collision_counts(), s.collision_counts()
+        print(f"pairs in the statistics window {co.pairs_in_dstat} (GPU {cg.pairs_in_dstat}), "
+              f"collision pairs {co.npairs}", flush=True)
+        assert co.pairs_in_dstat > 1 << 16
+        assert co.pairs_in_dstat == cg.pairs_in_dstat
+        assert list(co.icc) == list(cg.icc) and list(co.nir) == list(cg.nir)
+        assert co.npairs > 0 and np.array_equal(o.collision_pairs(), s.collision_pairs())
+        bo, bg = o.collision_bins(), s.collision_bins()
+        assert np.array_equal(bo["nirbin"], bg["nirbin"])
+        for k in ("wrbin", "wxbin", "wybin", "wzbin", "urbin", "thetabin", "wrtanbin"):
+            assert np.allclose(bo[k], bg[k], rtol=1e-9, atol=1e-9), k
+
+
 def test_droplet_steps_match_oracle(ddns, oracle_mod):
     N = 32
     p = _params(ddns, N=N, npart=3000, nset=2, radius=[20.0, 40.0], forcing=1, dt=0.002, rnu=0.02, nonoverlap=1)
